@@ -1,0 +1,108 @@
+"""ctypes binding of libd2fem_hip.so (the C ABI declared in include/d2fem.h).
+
+The library is built in-tree by diff_2d_fem_amd.build (hipcc for gfx950). There is no CPU
+fallback: if the library is missing, every solve raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+LIB_NAME = "libd2fem_hip.so"
+LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
+
+MAX_COLLIDERS = 8
+P = C.c_void_p
+
+
+class Collider(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("pad_", C.c_int32)] + [
+        (k, C.c_double) for k in ("cx", "cy", "rx", "ry", "rref", "r", "mu")]
+
+
+_SCENE_FIELDS = (
+    [(k, C.c_int32) for k in ("B", "N", "T", "n_b", "n_e", "n_bc")]
+    + [(k, P) for k in ("x", "v", "xn", "xhat", "grad", "grad_prev", "p", "diag", "diag_el", "x0",
+                        "mass", "free_", "node2env", "node_start",
+                        "tri", "Bm", "area", "mu", "lam", "tri2env",
+                        "bnd", "b2e", "bnd_start", "ea", "eb", "ela", "elb", "e2env", "ca", "cb",
+                        "excl", "excl_off", "mu_node",
+                        "bc_idx", "bc_target", "bc_q",
+                        "gPg", "g_p", "g_Py", "y_p", "y_Py", "beta", "gTp", "pHp", "alpha", "p_inf",
+                        "clr_min", "dE", "dE0", "Ener", "Eprev", "clr_ipc", "q_inf", "sweep_beta", "tc",
+                        "active", "iters", "fext")]
+    + [("n_pe", C.c_int32), ("n_ch", C.c_int32)]
+    + [(k, P) for k in ("pe_a", "pe_b", "pe_ch", "ch2env", "ch_ref", "ch_p")]
+    + [("n_ey", C.c_int32), ("n_td", C.c_int32)]
+    + [(k, P) for k in ("ey_node", "ey_prv", "ey_nxt", "ey_tid", "td2env", "td_stiff", "td_L0",
+                        "td_T", "td_L", "td_tau", "td_keff", "td_dL")]
+    + [("n_an", C.c_int32), ("n_ag", C.c_int32)]
+    + [(k, P) for k in ("an_node", "an_grp", "ag2env", "an_w", "ag_Mu", "ag_La", "ag_q", "ag_q0",
+                        "ag_pq", "ag_gq", "ag_gq_prev", "ag_zq", "ag_Hx", "ag_Hy")]
+    + [("n_rn", C.c_int32), ("n_rg", C.c_int32)]
+    + [(k, P) for k in ("rn_node", "rn_grp", "rg2env", "rg_acc", "rg_gq", "rg_gq_prev", "rg_zq", "rg_H")]
+    + [("n_col", C.c_int32), ("pad_", C.c_int32), ("col", Collider * MAX_COLLIDERS)]
+)
+
+
+class Scene(C.Structure):
+    _fields_ = _SCENE_FIELDS
+
+
+class Params(C.Structure):
+    _fields_ = ([(k, C.c_double) for k in ("dt", "h2", "inv_dt", "drag", "dhat", "kappa", "epsv",
+                                           "eps", "tol_x", "near_factor", "alpha_max", "area_shrink",
+                                           "ccd_eta", "ls_tol", "cg_guard", "ipc_kappa", "ipc_mu",
+                                           "ipc_reach", "ipc_gap")]
+                + [(k, C.c_int32) for k in ("iter_max", "conv_check", "friction", "line_search",
+                                            "n_backtrack", "ipc_on", "ipc_per_body", "rigid_modes_on",
+                                            "precision", "pad_")])
+
+
+_lib = None
+
+
+class NativeError(RuntimeError):
+    """A call into the HIP library failed."""
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = Path(os.environ.get("D2FEM_LIB", LIB_PATH))
+    if not path.exists():
+        raise RuntimeError(f"{path} is missing: build it with `python -m diff_2d_fem_amd.build` "
+                           f"(hipcc, gfx950). There is no CPU fallback.")
+    L = C.CDLL(str(path))
+    L.d2f_abi_version.restype = C.c_int
+    L.d2f_last_error.restype = C.c_char_p
+    L.d2f_struct_sizes.argtypes = [C.POINTER(C.c_int64)]
+    L.d2f_step_plain.argtypes = [C.POINTER(Scene), C.POINTER(Params), P]
+    L.d2f_ipc_min_distance.argtypes = [C.POINTER(Scene), C.POINTER(Params), P]
+    if hasattr(L, "d2f_step_scene"):
+        L.d2f_step_scene.argtypes = [C.POINTER(Scene), C.POINTER(Params), P]
+    i32, f64 = C.c_int32, C.c_double
+    L.d2f_elastic_grad_diag_f64.argtypes = [P, P, P, P, P, P, i32, f64, i32, P, P, P]
+    L.d2f_elastic_pHp_f64.argtypes = [P, P, P, P, P, P, P, P, i32, f64, i32, P, P]
+    L.d2f_inversion_alpha_f64.argtypes = [P, P, P, P, i32, f64, P, P]
+    L.d2f_elastic_energy_f64.argtypes = [P, P, P, P, P, P, P, i32, f64, P, P]
+    L.d2f_elastic_hvp_f64.argtypes = [P, P, P, P, P, P, P, i32, f64, P, P]
+    sizes = (C.c_int64 * 3)()
+    L.d2f_struct_sizes(sizes)
+    want = (C.sizeof(Scene), C.sizeof(Params), C.sizeof(Collider))
+    if tuple(sizes) != want:
+        raise RuntimeError(f"{path}: struct sizes {tuple(sizes)} do not match the binding {want}; "
+                           f"rebuild the library")
+    if L.d2f_abi_version() != 1:
+        raise RuntimeError(f"{path}: ABI version {L.d2f_abi_version()}, expected 1")
+    _lib = L
+    return L
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = lib().d2f_last_error().decode(errors="replace")
+        raise NativeError(f"{what} failed ({rc}): {msg}")

@@ -1,0 +1,1386 @@
+// step_plain.hip — the per-kernel float64 path: one launch per reference kernel family, in
+// the order of PNCG2DSolver._iterate (reference implicit/solver.py:252-451), with atomics for
+// every scatter and per-environment reduction. It is the plain GPU mirror of the reference's
+// launch structure and the general path for scenes of any size.
+#include <stdarg.h>
+#include <string.h>
+
+#include <vector>
+
+#include "common.h"
+
+namespace d2f {
+
+static thread_local char g_err[512];
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+using S = d2f_scene;
+using Pm = d2f_params;
+
+constexpr int TPB = 256;
+
+#define GRID_LOOP(i, n)                                                                 \
+  for (int base_ = blockIdx.x * blockDim.x; base_ < (n); base_ += gridDim.x * blockDim.x) \
+    for (int i = base_ + threadIdx.x, once_ = 1; once_; once_ = 0)
+
+__device__ __forceinline__ double fr_of(const S &s, int i) { return s.free_[i] ? 1.0 : 0.0; }
+
+// ------------------------------------------------------------------------- step setup
+// reference implicit/kernels.py:42-68, 438-441
+__global__ void k_init_step(S s, double dt, double drag) {
+  GRID_LOOP(i, s.N) {
+    if (i < s.N) {
+      const V2<double> x = ld2(s.x, i);
+      st2(s.xn, i, x);
+      st2(s.xhat, i, x + (dt * drag) * ld2(s.v, i));
+    }
+  }
+}
+__global__ void k_apply_bc(S s) {
+  GRID_LOOP(b, s.n_bc) {
+    if (b < s.n_bc) {
+      const int n = s.bc_idx[b];
+      const V2<double> t = ld2(s.bc_target, b);
+      st2(s.x, n, t);
+      st2(s.xhat, n, t);
+    }
+  }
+}
+__global__ void k_reset_step(S s) {
+  GRID_LOOP(e, s.B) {
+    if (e < s.B) {
+      s.dE0[e] = 0.0;
+      s.active[e] = 1;
+      s.iters[e] = 0;
+    }
+  }
+}
+__global__ void k_commit_v(S s, double inv_dt) {
+  GRID_LOOP(i, s.N) {
+    if (i < s.N) st2(s.v, i, inv_dt * (ld2(s.x, i) - ld2(s.xn, i)));
+  }
+}
+
+// ------------------------------------------------------------------------- gradient + diagonal
+// reference implicit/kernels.py:73-82 (+ the external force, implicit/actuation.py:37-41)
+__global__ void k_grad_diag_mass(S s, double h2) {
+  GRID_LOOP(i, s.N) {
+    if (i < s.N) {
+      const V2<double> g = ld2(s.grad, i);
+      st2(s.grad_prev, i, g);
+      const double m = s.mass[i], fr = fr_of(s, i);
+      const V2<double> d = ld2(s.x, i) - ld2(s.xhat, i);
+      st2(s.grad, i, V2<double>{m * d.x * fr, m * d.y * fr});
+      st2(s.diag, i, V2<double>{m, m});
+    }
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ M2<R> load_B(const double *Bm, int t) {
+  const double4 b = reinterpret_cast<const double4 *>(Bm)[t];
+  return {R(b.x), R(b.y), R(b.z), R(b.w)};
+}
+
+// reference implicit/kernels.py:86-128
+__global__ void k_elastic_grad_diag(const double *x, const int32_t *tri, const double *Bm,
+                                    const double *area, const double *mu, const double *lam, int T,
+                                    double h2, int clamp, double *grad, double *diag) {
+  GRID_LOOP(t, T) {
+    if (t < T) {
+      const int i0 = tri[3 * t], i1 = tri[3 * t + 1], i2 = tri[3 * t + 2];
+      const V2<double> p0 = ld2(x, i0), p1 = ld2(x, i1), p2 = ld2(x, i2);
+      const M2<double> B = load_B<double>(Bm, t);
+      const M2<double> Ds = {p1.x - p0.x, p2.x - p0.x, p1.y - p0.y, p2.y - p0.y};
+      const double m_ = mu[t], l_ = lam[t], A = area[t];
+      const NH<double> h = nh_eval(mat_mul(Ds, B), m_, l_);
+      V2<double> g0, g1, g2, d0, d1, d2;
+      nh_grad(h, B, m_, A, g0, g1, g2);
+      nh_diag(h, B, m_, l_, A, d0, d1, d2);
+      if (clamp) {
+        d0 = {fmax(d0.x, 0.0), fmax(d0.y, 0.0)};
+        d1 = {fmax(d1.x, 0.0), fmax(d1.y, 0.0)};
+        d2 = {fmax(d2.x, 0.0), fmax(d2.y, 0.0)};
+      }
+      atomic_add2(grad, i0, h2 * g0);
+      atomic_add2(grad, i1, h2 * g1);
+      atomic_add2(grad, i2, h2 * g2);
+      atomic_add2(diag, i0, h2 * d0);
+      atomic_add2(diag, i1, h2 * d1);
+      atomic_add2(diag, i2, h2 * d2);
+    }
+  }
+}
+
+// Collider barrier + lagged friction of every boundary node (one thread owns its node, so no
+// atomics): core/sdf.py:141-154, 213-233; implicit/kernels.py:132-157, 661-690.
+__global__ void k_colliders_grad_diag(S s, Pm p) {
+  GRID_LOOP(b, s.n_b) {
+    if (b < s.n_b) {
+      const int n = s.bnd[b];
+      const V2<double> x = ld2(s.x, n);
+      V2<double> g = ld2(s.grad, n), dg = ld2(s.diag, n);
+      for (int k = 0; k < s.n_col; ++k) {
+        V2<double> gg, dd;
+        if (collider_grad_diag(to_collider(s.col[k]), x, p.dhat, p.kappa, p.h2, 1, gg, dd)) {
+          g = g + gg;
+          dg = dg + dd;
+        }
+      }
+      if (p.friction && s.free_[n]) {
+        const V2<double> xn = ld2(s.xn, n);
+        for (int k = 0; k < s.n_col; ++k) {
+          const Collider c = to_collider(s.col[k]);
+          FricFrame f;
+          if (!friction_frame(c, x, xn, p.dhat, p.kappa, p.inv_dt, f)) continue;
+          const double sc = p.h2 * c.mu * f.f_n / fmax(f.vtm, p.epsv);
+          g = g + sc * f.vt;
+          const double kt = p.h2 * c.mu * f.f_n * p.inv_dt / fmax(f.vtm, p.epsv);
+          const V2<double> th = {-f.n.y, f.n.x};
+          dg = dg + V2<double>{kt * th.x * th.x, kt * th.y * th.y};
+        }
+      }
+      st2(s.grad, n, g);
+      st2(s.diag, n, dg);
+    }
+  }
+}
+
+// grad -= h2 f (implicit/actuation.py:37-41)
+__global__ void k_extforce_grad(S s, double h2) {
+  GRID_LOOP(i, s.N) {
+    if (i < s.N) st2(s.grad, i, ld2(s.grad, i) - h2 * ld2(s.fext, i));
+  }
+}
+
+// ------------------------------------------------------------------------- IPC pairs
+struct PairCtx {
+  double dhat, kappa, mu, epsv, inv_dt, h2, reach, gap, floor_;
+  int per_body;
+};
+
+// One barrier-and-friction term (reference implicit/ipc_contact.py:131-171).
+__device__ inline void ipc_term(const S &s, int mode, const PairCtx &c, double coef, int ip, int ia,
+                                int ib, double wa, double wb, double d, V2<double> n, double ddd,
+                                int e, double mu, const double *pdir, double *grad, double *diag,
+                                const uint8_t *freev, double *pHp, double *E) {
+  const double bp = barrier_bp(d, c.dhat);
+  const double f_n = -c.kappa * bp;
+  const V2<double> tn = {-n.y, n.x};
+  const V2<double> u = (ld2(s.x, ip) - ld2(s.xn, ip)) - wa * (ld2(s.x, ia) - ld2(s.xn, ia)) -
+                       wb * (ld2(s.x, ib) - ld2(s.xn, ib));
+  const double ut = dot(u, tn);
+  const double vt = fabs(ut) * c.inv_dt;
+  const double kt = mu * f_n * c.inv_dt / fmax(vt, c.epsv);
+  if (mode == 0) {
+    const double bpp = barrier_bpp(d, c.dhat);
+    const V2<double> g = c.h2 * ((-f_n) * n + (mu * f_n * (ut * c.inv_dt / fmax(vt, c.epsv))) * tn);
+    const V2<double> hd = {c.h2 * (c.kappa * bpp * n.x * n.x + kt * tn.x * tn.x),
+                           c.h2 * (c.kappa * bpp * n.y * n.y + kt * tn.y * tn.y)};
+    const double fp = freev[ip] ? 1.0 : 0.0, fa = freev[ia] ? 1.0 : 0.0, fb = freev[ib] ? 1.0 : 0.0;
+    atomic_add2(grad, ip, (coef * fp) * g);
+    atomic_add2(diag, ip, coef * hd);
+    atomic_add2(grad, ia, (-coef * wa * fa) * g);
+    atomic_add2(diag, ia, (coef * wa * wa) * hd);
+    atomic_add2(grad, ib, (-coef * wb * fb) * g);
+    atomic_add2(diag, ib, (coef * wb * wb) * hd);
+  } else if (mode == 1) {
+    const double bpp = barrier_bpp(d, c.dhat);
+    const V2<double> dq = ld2(pdir, ip) - wa * ld2(pdir, ia) - wb * ld2(pdir, ib);
+    const double dn = dot(dq, n), dt_ = dot(dq, tn);
+    const double val = fmax(c.kappa * (bpp * dn * dn + bp * ddd), 0.0);
+    atomicAdd(pHp + e, coef * c.h2 * (val + kt * dt_ * dt_));
+  } else {
+    const double asl = c.epsv / c.inv_dt;
+    const double au = fabs(ut);
+    const double f0 = au < asl ? au * au / (2.0 * asl) : au - 0.5 * asl;
+    atomicAdd(E + e, coef * c.h2 * (c.kappa * barrier_b(d, c.dhat) + mu * f_n * f0));
+  }
+}
+
+// Every boundary node near one edge (reference implicit/ipc_contact.py:175-296), with exact
+// enumeration over the edge's environment in place of the hash-grid query. mode 0 grad+diag,
+// 1 pHp + clearance + step limit, 2 energy, 3 clearance + step limit along pdir.
+__device__ inline void ipc_edge(const S &s, int k, int mode, const PairCtx &c, const double *pdir,
+                                double *grad, double *diag, const uint8_t *freev, double *pHp,
+                                double *E, double *clr, double *alpha, double t_c, const double *mu_node) {
+  const int a = s.ea[k], b = s.eb[k], e = s.e2env[k];
+  const int start = s.bnd_start[e], n_be = s.bnd_start[e + 1] - start;
+  const int64_t eoff = s.excl_off[e];
+  const V2<double> xa = ld2(s.x, a), xb = ld2(s.x, b);
+  const V2<double> ab = xb - xa;
+  const double L2 = fmax(dot(ab, ab), 1.0e-30);
+  const double L = sqrt(L2);
+  const double dlo = 1.0e-9 * c.dhat, onset = 0.02 * c.dhat;
+  const int la = s.ela[k], lb = s.elb[k];
+  double amin = BIG, cmin = BIG;
+  for (int jl = 0; jl < n_be; ++jl) {
+    const int pn = s.bnd[start + jl];
+    if (pn == a || pn == b) continue;
+    const uint8_t *row = s.excl + eoff + (int64_t)jl * n_be;
+    if (row[la] || row[lb]) continue;
+    const V2<double> xp = ld2(s.x, pn);
+    const V2<double> w = xp - xa;
+    const double tt = dot(w, ab) / L2;
+    V2<double> dvec = w - tt * ab;
+    if (tt <= 0.0) dvec = w;
+    else if (tt >= 1.0) dvec = xp - xb;
+    double d = len(dvec);
+    if (!(d < c.reach)) continue;
+    if (mode == 1 || mode == 3) {
+      cmin = fmin(cmin, d);
+      double keep = fmax(c.gap * d, c.dhat - onset);
+      if (d < c.dhat) keep = d - fmin(2.0 * (c.dhat - d) + onset, 0.5 * d);
+      if (mode == 3) keep = fmin(d, fmax(c.gap * d, c.floor_));
+      amin = fmin(amin, accd(xp, xa, xb, ld2(pdir, pn), ld2(pdir, a), ld2(pdir, b), keep, t_c));
+    }
+    if (mode == 3) continue;
+    double mu_p = c.mu;
+    if (c.per_body) {
+      const double mi = mu_node[pn], mj = mu_node[a], sm = mi + mj;
+      mu_p = sm > 0.0 ? 2.0 * mi * mj / sm : 0.0;
+    }
+    if (tt > 0.0 && tt < 1.0 && d < c.dhat) {
+      d = fmax(d, dlo);
+      double ddd = 0.0;
+      if (mode == 1) {
+        const V2<double> de = ld2(pdir, b) - ld2(pdir, a);
+        const V2<double> dw = ld2(pdir, pn) - ld2(pdir, a);
+        const double c0 = cross(ab, w);
+        const double c1 = cross(de, w) + cross(ab, dw);
+        const double c2 = 2.0 * cross(de, dw);
+        const double L1 = dot(ab, de) / L;
+        const double Lq = (dot(de, de) - L1 * L1) / L;
+        const double s0 = c0 / L;
+        const double s1 = (c1 - s0 * L1) / L;
+        ddd = (c2 - 2.0 * s1 * L1 - s0 * Lq) / L;
+        if (c0 < 0.0) ddd = -ddd;
+      }
+      ipc_term(s, mode, c, 1.0, pn, a, b, 1.0 - tt, tt, d, (1.0 / d) * dvec, ddd, e, mu_p, pdir,
+               grad, diag, freev, pHp, E);
+    }
+    double coef_a = -s.ca[k];
+    if (tt <= 0.0) coef_a += 1.0;
+    double d_a = len(w);
+    if (coef_a != 0.0 && d_a < c.dhat) {
+      d_a = fmax(d_a, dlo);
+      const V2<double> n_a = (1.0 / d_a) * w;
+      double ddd = 0.0;
+      if (mode == 1) {
+        const V2<double> dw = ld2(pdir, pn) - ld2(pdir, a);
+        const double dn = dot(dw, n_a);
+        ddd = (dot(dw, dw) - dn * dn) / d_a;
+      }
+      ipc_term(s, mode, c, coef_a, pn, a, b, 1.0, 0.0, d_a, n_a, ddd, e, mu_p, pdir, grad, diag,
+               freev, pHp, E);
+    }
+    double coef_b = -s.cb[k];
+    if (tt >= 1.0) coef_b += 1.0;
+    const V2<double> wb_ = xp - xb;
+    double d_b = len(wb_);
+    if (coef_b != 0.0 && d_b < c.dhat) {
+      d_b = fmax(d_b, dlo);
+      const V2<double> n_b = (1.0 / d_b) * wb_;
+      double ddd = 0.0;
+      if (mode == 1) {
+        const V2<double> dw = ld2(pdir, pn) - ld2(pdir, b);
+        const double dn = dot(dw, n_b);
+        ddd = (dot(dw, dw) - dn * dn) / d_b;
+      }
+      ipc_term(s, mode, c, coef_b, pn, a, b, 0.0, 1.0, d_b, n_b, ddd, e, mu_p, pdir, grad, diag,
+               freev, pHp, E);
+    }
+  }
+  if (mode == 1 || mode == 3) {
+    if (cmin < BIG) atomic_min_d(clr + e, cmin);
+    if (amin < BIG) atomic_min_d(alpha + e, amin);
+  }
+}
+
+// t_c (the step the ACCD may stop at) is each environment's alpha at launch, read from
+// a snapshot so that every pair sees the same value.
+__global__ void k_ipc(S s, int mode, PairCtx c, const double *pdir, double *grad, double *diag,
+                      const uint8_t *freev, double *E, double *clr, double *alpha, const double *tc) {
+  GRID_LOOP(k, s.n_e) {
+    if (k < s.n_e)
+      ipc_edge(s, k, mode, c, pdir, grad, diag, freev, s.pHp, E, clr, alpha,
+               (mode == 1 || mode == 3) ? tc[s.e2env[k]] : 0.0, s.mu_node);
+  }
+}
+
+// alpha <= 0.45 reach / p_inf (reference implicit/ipc_contact.py:300-305)
+__global__ void k_far_cap(int B, const double *p_inf, double reach, double *alpha) {
+  GRID_LOOP(e, B) {
+    if (e < B && p_inf[e] > 0.0) alpha[e] = fmin(alpha[e], 0.45 * reach / p_inf[e]);
+  }
+}
+
+// Kinematic sweep (reference implicit/ipc_contact.py:308-340)
+__global__ void k_bc_motion(S s) {
+  GRID_LOOP(b, s.n_bc) {
+    const bool v = b < s.n_bc;
+    double l = 0.0;
+    int e = 0;
+    if (v) {
+      const int i = s.bc_idx[b];
+      const V2<double> d = ld2(s.bc_target, b) - ld2(s.x, i);
+      st2(s.bc_q, i, d);
+      l = len(d);
+      e = s.node2env[i];
+    }
+    env_max(s.q_inf, e, l, v);
+  }
+}
+__global__ void k_bc_advance(S s) {
+  GRID_LOOP(b, s.n_bc) {
+    if (b < s.n_bc) {
+      const int i = s.bc_idx[b];
+      const double t = s.sweep_beta[s.node2env[i]];
+      const V2<double> tg = ld2(s.bc_target, b);
+      const V2<double> xi = t >= 1.0 ? tg : ld2(s.x, i) + t * (tg - ld2(s.x, i));
+      st2(s.x, i, xi);
+      st2(s.xhat, i, xi);
+    }
+  }
+}
+__global__ void k_bc_wake(S s) {
+  GRID_LOOP(e, s.B) {
+    if (e < s.B && s.q_inf[e] > 0.0) s.active[e] = 1;
+  }
+}
+__global__ void k_fill(double *a, int n, double v) {
+  GRID_LOOP(i, n) {
+    if (i < n) a[i] = v;
+  }
+}
+
+// ------------------------------------------------------------------------- pressure, tendons
+// reference implicit/actuation.py:89-123
+__global__ void k_pressure(S s, int mode, double h2, double *E) {
+  GRID_LOOP(k, s.n_pe) {
+    if (k < s.n_pe) {
+      const int a = s.pe_a[k], b = s.pe_b[k], ch = s.pe_ch[k];
+      const double pr = s.ch_p[ch];
+      if (mode == 0) {
+        const V2<double> t = ld2(s.x, b) - ld2(s.x, a);
+        const V2<double> g = (-0.5 * h2 * pr) * V2<double>{t.y, -t.x};
+        atomic_add2(s.grad, a, g);
+        atomic_add2(s.grad, b, g);
+      } else if (mode == 1) {
+        const V2<double> pa = ld2(s.p, a), pb = ld2(s.p, b);
+        atomicAdd(s.pHp + s.ch2env[ch], -h2 * pr * cross(pa, pb));
+      } else {
+        const V2<double> ref = ld2(s.ch_ref, ch);
+        const V2<double> ra = ld2(s.x, a) - ref, rb = ld2(s.x, b) - ref;
+        atomicAdd(E + s.ch2env[ch], -h2 * pr * 0.5 * cross(ra, rb));
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ V2<double> seg_dir(V2<double> a, V2<double> b, double &il) {
+  const V2<double> t = b - a;
+  const double l = fmax(len(t), 1.0e-12);
+  il = 1.0 / l;
+  return {t.x / l, t.y / l};
+}
+// reference implicit/actuation.py:170-195
+__global__ void k_tendon_length(S s) {
+  GRID_LOOP(j, s.n_ey) {
+    if (j < s.n_ey && s.ey_nxt[j] >= 0)
+      atomicAdd(s.td_L + s.ey_tid[j], len(ld2(s.x, s.ey_node[s.ey_nxt[j]]) - ld2(s.x, s.ey_node[j])));
+  }
+}
+__global__ void k_tendon_state(S s) {
+  GRID_LOOP(k, s.n_td) {
+    if (k < s.n_td) {
+      if (s.td_stiff[k] > 0.0) {
+        const double ext = s.td_L[k] - s.td_L0[k];
+        s.td_tau[k] = ext > 0.0 ? s.td_stiff[k] * ext : 0.0;
+        s.td_keff[k] = ext > 0.0 ? s.td_stiff[k] : 0.0;
+      } else {
+        s.td_tau[k] = fmax(s.td_T[k], 0.0);
+        s.td_keff[k] = 0.0;
+      }
+    }
+  }
+}
+// reference implicit/actuation.py:205-226
+__global__ void k_tendon_grad(S s, double h2) {
+  GRID_LOOP(j, s.n_ey) {
+    if (j < s.n_ey) {
+      const int n = s.ey_node[j], k = s.ey_tid[j];
+      V2<double> gL = {0.0, 0.0}, geo = {0.0, 0.0};
+      const V2<double> xj = ld2(s.x, n);
+      double il;
+      if (s.ey_prv[j] >= 0) {
+        const V2<double> d = seg_dir(ld2(s.x, s.ey_node[s.ey_prv[j]]), xj, il);
+        gL = gL + d;
+        geo = geo + V2<double>{(1.0 - d.x * d.x) * il, (1.0 - d.y * d.y) * il};
+      }
+      if (s.ey_nxt[j] >= 0) {
+        const V2<double> d = seg_dir(xj, ld2(s.x, s.ey_node[s.ey_nxt[j]]), il);
+        gL = gL - d;
+        geo = geo + V2<double>{(1.0 - d.x * d.x) * il, (1.0 - d.y * d.y) * il};
+      }
+      const double tau = s.td_tau[k], ke = s.td_keff[k];
+      atomic_add2(s.grad, n, (h2 * tau) * gL);
+      atomic_add2(s.diag, n, V2<double>{h2 * (tau * geo.x + ke * gL.x * gL.x),
+                                        h2 * (tau * geo.y + ke * gL.y * gL.y)});
+    }
+  }
+}
+// reference implicit/actuation.py:229-255
+__global__ void k_tendon_pHp(S s, double h2) {
+  GRID_LOOP(j, s.n_ey) {
+    if (j < s.n_ey && s.ey_nxt[j] >= 0) {
+      const int k = s.ey_tid[j];
+      const int na = s.ey_node[j], nb = s.ey_node[s.ey_nxt[j]];
+      double il;
+      const V2<double> d = seg_dir(ld2(s.x, na), ld2(s.x, nb), il);
+      const V2<double> dp = ld2(s.p, nb) - ld2(s.p, na);
+      const double along = dot(dp, d);
+      atomicAdd(s.td_dL + k, along);
+      atomicAdd(s.pHp + s.td2env[k], h2 * s.td_tau[k] * (dot(dp, dp) - along * along) * il);
+    }
+  }
+}
+__global__ void k_tendon_taut(S s, double h2) {
+  GRID_LOOP(k, s.n_td) {
+    if (k < s.n_td) atomicAdd(s.pHp + s.td2env[k], h2 * s.td_keff[k] * s.td_dL[k] * s.td_dL[k]);
+  }
+}
+// reference implicit/actuation.py:258-268
+__global__ void k_tendon_energy(S s, double h2, double *E) {
+  GRID_LOOP(k, s.n_td) {
+    if (k < s.n_td) {
+      if (s.td_stiff[k] > 0.0) {
+        const double ext = s.td_L[k] - s.td_L0[k];
+        if (ext > 0.0) atomicAdd(E + s.td2env[k], 0.5 * h2 * s.td_stiff[k] * ext * ext);
+      } else {
+        atomicAdd(E + s.td2env[k], h2 * fmax(s.td_T[k], 0.0) * s.td_L[k]);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------- affine bodies
+// reference implicit/affine.py:61-269
+struct M6 {
+  double m[6][6];
+};
+
+__device__ inline void add_nh_block(M6 &H, double A11, double A12, double A21, double A22, double Mu,
+                                    double La, double h2) {
+  const int idx[4] = {1, 2, 4, 5};
+  const NH<double> h = nh_eval(M2<double>{A11, A12, A21, A22}, Mu, La);
+  for (int k = 0; k < 4; ++k) {
+    M2<double> E = {0.0, 0.0, 0.0, 0.0};
+    if (k == 0) E.a = 1.0;
+    else if (k == 1) E.b = 1.0;
+    else if (k == 2) E.c = 1.0;
+    else E.d = 1.0;
+    const M2<double> col = nh_d2_dir(h, E, Mu, La);
+    const double cv[4] = {col.a, col.b, col.c, col.d};
+    for (int r = 0; r < 4; ++r) H.m[idx[r]][idx[k]] += h2 * cv[r];
+  }
+  const double a = A11 + A22, b = A21 - A12;
+  const double n = sqrt(a * a + b * b);
+  if (n > 1.0e-12) {
+    const double s = 0.7071067811865476 / n;
+    const double t[4] = {-b * s, -a * s, a * s, -b * s};
+    const M2<double> Tm = {t[0], t[1], t[2], t[3]};
+    const double lam_t = nh_quad(h, Tm, Mu, La);
+    if (lam_t < 0.0)
+      for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) H.m[idx[r]][idx[c]] -= h2 * lam_t * t[r] * t[c];
+  }
+}
+
+__device__ inline void chol6(const M6 &H, M6 &L) {
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) L.m[i][j] = 0.0;
+  bool ok = true;
+  for (int j = 0; j < 6; ++j) {
+    double s = H.m[j][j];
+    for (int k = 0; k < j; ++k) s -= L.m[j][k] * L.m[j][k];
+    if (s <= 1.0e-9 * fabs(H.m[j][j]) || s <= 0.0) ok = false;
+    if (ok) {
+      const double d = sqrt(s);
+      L.m[j][j] = d;
+      for (int i = j + 1; i < 6; ++i) {
+        double v = H.m[i][j];
+        for (int k = 0; k < j; ++k) v -= L.m[i][k] * L.m[j][k];
+        L.m[i][j] = v / d;
+      }
+    }
+  }
+  if (!ok) {
+    for (int i = 0; i < 6; ++i)
+      for (int j = 0; j < 6; ++j) L.m[i][j] = 0.0;
+    for (int j = 0; j < 6; ++j) L.m[j][j] = sqrt(fmax(H.m[j][j], 1.0e-30));
+  }
+}
+
+__device__ inline void chol6_solve(const M6 &L, const double *b, double *x) {
+  double y[6];
+  for (int i = 0; i < 6; ++i) {
+    double s = b[i];
+    for (int k = 0; k < i; ++k) s -= L.m[i][k] * y[k];
+    y[i] = s / L.m[i][i];
+  }
+  for (int i = 5; i >= 0; --i) {
+    double s = y[i];
+    for (int k = i + 1; k < 6; ++k) s -= L.m[k][i] * x[k];
+    x[i] = s / L.m[i][i];
+  }
+}
+
+__global__ void k_affine_gather(S s) {
+  GRID_LOOP(k, s.n_an) {
+    if (k < s.n_an) {
+      const int i = s.an_node[k], b = s.an_grp[k];
+      const double w[3] = {s.an_w[3 * k], s.an_w[3 * k + 1], s.an_w[3 * k + 2]};
+      const V2<double> g = ld2(s.grad, i), d = ld2(s.diag, i);
+      for (int r = 0; r < 3; ++r) {
+        atomicAdd(s.ag_gq + 6 * b + r, g.x * w[r]);
+        atomicAdd(s.ag_gq + 6 * b + 3 + r, g.y * w[r]);
+        for (int c = 0; c < 3; ++c) {
+          atomicAdd(s.ag_Hx + 9 * b + 3 * r + c, d.x * w[r] * w[c]);
+          atomicAdd(s.ag_Hy + 9 * b + 3 * r + c, d.y * w[r] * w[c]);
+        }
+      }
+      st2(s.grad, i, V2<double>{0.0, 0.0});
+    }
+  }
+}
+
+__global__ void k_affine_reduce(S s, double h2) {
+  GRID_LOOP(b, s.n_ag) {
+    if (b < s.n_ag) {
+      const double *q = s.ag_q + 6 * b;
+      const double Mu = s.ag_Mu[b], La = s.ag_La[b];
+      const NH<double> h = nh_eval(M2<double>{q[1], q[2], q[4], q[5]}, Mu, La);
+      const double P[4] = {Mu * h.F.a + h.c * h.G.a, Mu * h.F.b + h.c * h.G.b,
+                           Mu * h.F.c + h.c * h.G.c, Mu * h.F.d + h.c * h.G.d};
+      double g[6], y[6], z[6], zy[6];
+      const double add[6] = {0.0, P[0], P[1], 0.0, P[2], P[3]};
+      for (int r = 0; r < 6; ++r) g[r] = s.ag_gq[6 * b + r] + h2 * add[r];
+      M6 H, L;
+      for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) H.m[r][c] = 0.0;
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+          H.m[r][c] = s.ag_Hx[9 * b + 3 * r + c];
+          H.m[r + 3][c + 3] = s.ag_Hy[9 * b + 3 * r + c];
+        }
+      add_nh_block(H, q[1], q[2], q[4], q[5], Mu, La, h2);
+      chol6(H, L);
+      for (int r = 0; r < 6; ++r) y[r] = g[r] - s.ag_gq_prev[6 * b + r];
+      chol6_solve(L, g, z);
+      chol6_solve(L, y, zy);
+      double gz = 0, gp = 0, gzy = 0, yp = 0, yzy = 0;
+      for (int r = 0; r < 6; ++r) {
+        const double pp = s.ag_pq[6 * b + r];
+        gz += g[r] * z[r];
+        gp += g[r] * pp;
+        gzy += g[r] * zy[r];
+        yp += y[r] * pp;
+        yzy += y[r] * zy[r];
+      }
+      const int e = s.ag2env[b];
+      atomicAdd(s.gPg + e, gz);
+      atomicAdd(s.g_p + e, gp);
+      atomicAdd(s.g_Py + e, gzy);
+      atomicAdd(s.y_p + e, yp);
+      atomicAdd(s.y_Py + e, yzy);
+      for (int r = 0; r < 6; ++r) {
+        s.ag_gq[6 * b + r] = g[r];
+        s.ag_zq[6 * b + r] = z[r];
+      }
+    }
+  }
+}
+
+// which = 0: pq = -zq + beta pq; 1: q += alpha pq (active); 2: q = q0 + alpha pq (active)
+__global__ void k_affine_update(S s, int which) {
+  GRID_LOOP(b, s.n_ag) {
+    if (b < s.n_ag) {
+      const int e = s.ag2env[b];
+      for (int r = 0; r < 6; ++r) {
+        const int k = 6 * b + r;
+        if (which == 0) s.ag_pq[k] = -s.ag_zq[k] + s.beta[e] * s.ag_pq[k];
+        else if (which == 1) { if (s.active[e]) s.ag_q[k] = s.ag_q[k] + s.alpha[e] * s.ag_pq[k]; }
+        else { if (s.active[e]) s.ag_q[k] = s.ag_q0[k] + s.alpha[e] * s.ag_pq[k]; }
+      }
+    }
+  }
+}
+
+__global__ void k_affine_expand(S s, const double *v6, double *out) {
+  GRID_LOOP(k, s.n_an) {
+    if (k < s.n_an) {
+      const double *w = s.an_w + 3 * k;
+      const double *v = v6 + 6 * s.an_grp[k];
+      st2(out, s.an_node[k], V2<double>{w[0] * v[0] + w[1] * v[1] + w[2] * v[2],
+                                        w[0] * v[3] + w[1] * v[4] + w[2] * v[5]});
+    }
+  }
+}
+
+__global__ void k_affine_pHp_energy(S s, int mode, double h2, double *E) {
+  GRID_LOOP(b, s.n_ag) {
+    if (b < s.n_ag) {
+      const double *q = s.ag_q + 6 * b;
+      const double Mu = s.ag_Mu[b], La = s.ag_La[b];
+      const NH<double> h = nh_eval(M2<double>{q[1], q[2], q[4], q[5]}, Mu, La);
+      if (mode == 1) {
+        const double *pq = s.ag_pq + 6 * b;
+        const double v = nh_quad(h, M2<double>{pq[1], pq[2], pq[4], pq[5]}, Mu, La);
+        atomicAdd(s.pHp + s.ag2env[b], h2 * fmax(v, 0.0));
+      } else {
+        atomicAdd(E + s.ag2env[b], h2 * nh_psi(h, Mu, La));
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------- rigid modes
+// reference implicit/affine.py:405-486
+__global__ void k_rigid_centroid(S s) {
+  GRID_LOOP(k, s.n_rn) {
+    if (k < s.n_rn) {
+      const int i = s.rn_node[k], b = s.rn_grp[k];
+      const double m = s.mass[i];
+      const V2<double> x = ld2(s.x, i);
+      atomicAdd(s.rg_acc + 3 * b, m * x.x);
+      atomicAdd(s.rg_acc + 3 * b + 1, m * x.y);
+      atomicAdd(s.rg_acc + 3 * b + 2, m);
+    }
+  }
+}
+__device__ __forceinline__ V2<double> rigid_arm(const S &s, int i, int b) {
+  const double *a = s.rg_acc + 3 * b;
+  const V2<double> x = ld2(s.x, i);
+  return {x.x - a[0] / a[2], x.y - a[1] / a[2]};
+}
+__global__ void k_rigid_gather(S s) {
+  GRID_LOOP(k, s.n_rn) {
+    if (k < s.n_rn) {
+      const int i = s.rn_node[k], b = s.rn_grp[k];
+      const V2<double> rr = rigid_arm(s, i, b);
+      const V2<double> g = ld2(s.grad, i), gp = ld2(s.grad_prev, i);
+      const double m = s.mass[i];
+      const V2<double> d = ld2(s.diag, i) - ld2(s.diag_el, i);
+      const double dx = fmax(d.x, 0.0) + m, dy = fmax(d.y, 0.0) + m;
+      double *gq = s.rg_gq + 3 * b, *gq0 = s.rg_gq_prev + 3 * b, *H = s.rg_H + 9 * b;
+      atomicAdd(gq, g.x);
+      atomicAdd(gq + 1, g.y);
+      atomicAdd(gq + 2, rr.x * g.y - rr.y * g.x);
+      atomicAdd(gq0, gp.x);
+      atomicAdd(gq0 + 1, gp.y);
+      atomicAdd(gq0 + 2, rr.x * gp.y - rr.y * gp.x);
+      atomicAdd(H + 0, dx);
+      atomicAdd(H + 2, -dx * rr.y);
+      atomicAdd(H + 4, dy);
+      atomicAdd(H + 5, dy * rr.x);
+      atomicAdd(H + 6, -dx * rr.y);
+      atomicAdd(H + 7, dy * rr.x);
+      atomicAdd(H + 8, dx * rr.y * rr.y + dy * rr.x * rr.x);
+    }
+  }
+}
+__device__ inline void inv33(const double *H, double *Hi) {
+  const double a = H[0], b = H[1], c = H[2], d = H[3], e = H[4], f = H[5], g = H[6], h = H[7], i = H[8];
+  const double A = e * i - f * h, Bc = -(d * i - f * g), C = d * h - e * g;
+  const double det = a * A + b * Bc + c * C;
+  const double id = 1.0 / det;
+  Hi[0] = A * id;
+  Hi[1] = -(b * i - c * h) * id;
+  Hi[2] = (b * f - c * e) * id;
+  Hi[3] = Bc * id;
+  Hi[4] = (a * i - c * g) * id;
+  Hi[5] = -(a * f - c * d) * id;
+  Hi[6] = C * id;
+  Hi[7] = -(a * h - b * g) * id;
+  Hi[8] = (a * e - b * d) * id;
+}
+__global__ void k_rigid_reduce(S s) {
+  GRID_LOOP(b, s.n_rg) {
+    if (b < s.n_rg) {
+      double Hi[9];
+      inv33(s.rg_H + 9 * b, Hi);
+      const double *g = s.rg_gq + 3 * b, *g0 = s.rg_gq_prev + 3 * b;
+      const double y[3] = {g[0] - g0[0], g[1] - g0[1], g[2] - g0[2]};
+      double z[3], zy[3];
+      for (int r = 0; r < 3; ++r) {
+        z[r] = Hi[3 * r] * g[0] + Hi[3 * r + 1] * g[1] + Hi[3 * r + 2] * g[2];
+        zy[r] = Hi[3 * r] * y[0] + Hi[3 * r + 1] * y[1] + Hi[3 * r + 2] * y[2];
+      }
+      const int e = s.rg2env[b];
+      atomicAdd(s.gPg + e, g[0] * z[0] + g[1] * z[1] + g[2] * z[2]);
+      atomicAdd(s.g_Py + e, g[0] * zy[0] + g[1] * zy[1] + g[2] * zy[2]);
+      atomicAdd(s.y_Py + e, y[0] * zy[0] + y[1] * zy[1] + y[2] * zy[2]);
+      for (int r = 0; r < 3; ++r) s.rg_zq[3 * b + r] = z[r];
+    }
+  }
+}
+__global__ void k_rigid_direction(S s) {
+  GRID_LOOP(k, s.n_rn) {
+    if (k < s.n_rn) {
+      const int i = s.rn_node[k], b = s.rn_grp[k];
+      const V2<double> rr = rigid_arm(s, i, b);
+      const double *z = s.rg_zq + 3 * b;
+      st2(s.p, i, ld2(s.p, i) - V2<double>{z[0] - z[2] * rr.y, z[1] + z[2] * rr.x});
+    }
+  }
+}
+
+// ------------------------------------------------------------------------- direction
+// reference implicit/kernels.py:161-163, 168-192
+__global__ void k_mask_dk(S s) {
+  GRID_LOOP(i, s.N) {
+    const bool v = i < s.N;
+    double a = 0, b = 0, c = 0, d = 0, f = 0;
+    int e = 0;
+    if (v) {
+      e = s.node2env[i];
+      const double fr = fr_of(s, i);
+      V2<double> g = ld2(s.grad, i);
+      g = {g.x * fr, g.y * fr};
+      st2(s.grad, i, g);
+      const V2<double> dh = ld2(s.diag, i);
+      const double Px = fr / fmax(dh.x, 1.0e-12), Py = fr / fmax(dh.y, 1.0e-12);
+      const V2<double> pp = ld2(s.p, i);
+      const V2<double> yy = g - ld2(s.grad_prev, i);
+      const double Pg0 = Px * g.x, Pg1 = Py * g.y, Py0 = Px * yy.x, Py1 = Py * yy.y;
+      a = g.x * Pg0 + g.y * Pg1;
+      b = g.x * pp.x + g.y * pp.y;
+      c = g.x * Py0 + g.y * Py1;
+      d = yy.x * pp.x + yy.y * pp.y;
+      f = yy.x * Py0 + yy.y * Py1;
+    }
+    env_add(s.gPg, e, a, v);
+    env_add(s.g_p, e, b, v);
+    env_add(s.g_Py, e, c, v);
+    env_add(s.y_p, e, d, v);
+    env_add(s.y_Py, e, f, v);
+  }
+}
+
+// reference implicit/kernels.py:196-217 and implicit/solver.py:77-93
+__global__ void k_finalize_beta(S s, int it, double guard, int use_hold) {
+  GRID_LOOP(e, s.B) {
+    if (e < s.B) {
+      double b = 0.0;
+      if (it > 0) {
+        const double ytp = s.y_p[e];
+        if (fabs(ytp) > 1.0e-30) b = s.g_Py[e] / ytp - s.y_Py[e] * s.g_p[e] / (ytp * ytp);
+      }
+      double t = -s.gPg[e] + b * s.g_p[e];
+      if (t >= 0.0) {
+        b = 0.0;
+        t = -s.gPg[e];
+      }
+      if (guard > 0.0 && it > 0) {
+        const double hold = use_hold ? s.q_inf[e] : 0.0;
+        if (hold <= 0.0 && s.y_p[e] < guard * s.alpha[e] * s.pHp[e]) {
+          b = 0.0;
+          t = -s.gPg[e];
+        }
+      }
+      s.beta[e] = b;
+      s.gTp[e] = t;
+    }
+  }
+}
+
+// reference implicit/kernels.py:221-230
+__global__ void k_update_p(S s) {
+  GRID_LOOP(i, s.N) {
+    if (i < s.N) {
+      const double fr = fr_of(s, i);
+      const double bb = s.beta[s.node2env[i]];
+      const V2<double> dh = ld2(s.diag, i), g = ld2(s.grad, i), pp = ld2(s.p, i);
+      const double px = -fr / fmax(dh.x, 1.0e-12) * g.x + bb * pp.x;
+      const double py = -fr / fmax(dh.y, 1.0e-12) * g.y + bb * pp.y;
+      st2(s.p, i, V2<double>{px * fr, py * fr});
+    }
+  }
+}
+
+// ------------------------------------------------------------------------- curvature, caps
+// reference implicit/kernels.py:235-255
+__global__ void k_pHp_reset(S s, double alpha_max) {
+  GRID_LOOP(e, s.B) {
+    if (e < s.B) {
+      s.pHp[e] = 0.0;
+      s.alpha[e] = alpha_max;
+      s.p_inf[e] = 0.0;
+      s.clr_min[e] = BIG;
+    }
+  }
+}
+__global__ void k_pHp_mass(S s) {
+  GRID_LOOP(i, s.N) {
+    const bool v = i < s.N;
+    double a = 0.0, l = 0.0;
+    int e = 0;
+    if (v) {
+      e = s.node2env[i];
+      const V2<double> pp = ld2(s.p, i);
+      a = s.mass[i] * dot(pp, pp);
+      l = len(pp);
+    }
+    env_add(s.pHp, e, a, v);
+    env_max(s.p_inf, e, l, v);
+  }
+}
+
+__global__ void k_elastic_pHp(const double *x, const double *p, const int32_t *tri, const double *Bm,
+                              const double *area, const double *mu, const double *lam,
+                              const int32_t *tri2env, int T, double h2, int clamp, double *pHp) {
+  GRID_LOOP(t, T) {
+    const bool v = t < T;
+    double val = 0.0;
+    int e = 0;
+    if (v) {
+      const int i0 = tri[3 * t], i1 = tri[3 * t + 1], i2 = tri[3 * t + 2];
+      const V2<double> p0 = ld2(x, i0), p1 = ld2(x, i1), p2 = ld2(x, i2);
+      const M2<double> B = load_B<double>(Bm, t);
+      const M2<double> Ds = {p1.x - p0.x, p2.x - p0.x, p1.y - p0.y, p2.y - p0.y};
+      const NH<double> h = nh_eval(mat_mul(Ds, B), mu[t], lam[t]);
+      const M2<double> dF = dF_of(ld2(p, i0), ld2(p, i1), ld2(p, i2), B);
+      val = area[t] * nh_quad(h, dF, mu[t], lam[t]);
+      if (clamp) val = fmax(val, 0.0);
+      val *= h2;
+      e = tri2env[t];
+    }
+    env_add(pHp, e, val, v);
+  }
+}
+
+__global__ void k_inversion_alpha(const double *x, const double *p, const int32_t *tri,
+                                  const int32_t *tri2env, int T, double shrink, double *alpha) {
+  GRID_LOOP(t, T) {
+    bool v = t < T;
+    double sv = BIG;
+    int e = 0;
+    if (v) {
+      const int i0 = tri[3 * t], i1 = tri[3 * t + 1], i2 = tri[3 * t + 2];
+      const V2<double> x0 = ld2(x, i0), p0 = ld2(p, i0);
+      sv = inversion_root(ld2(x, i1) - x0, ld2(x, i2) - x0, ld2(p, i1) - p0, ld2(p, i2) - p0, shrink);
+      e = tri2env[t];
+      v = sv < BIG;
+    }
+    env_min(alpha, e, sv, v);
+  }
+}
+
+__global__ void k_elastic_energy(const double *x, const int32_t *tri, const double *Bm,
+                                 const double *area, const double *mu, const double *lam,
+                                 const int32_t *tri2env, int T, double h2, double *E) {
+  GRID_LOOP(t, T) {
+    const bool v = t < T;
+    double val = 0.0;
+    int e = 0;
+    if (v) {
+      const int i0 = tri[3 * t], i1 = tri[3 * t + 1], i2 = tri[3 * t + 2];
+      const V2<double> p0 = ld2(x, i0), p1 = ld2(x, i1), p2 = ld2(x, i2);
+      const M2<double> B = load_B<double>(Bm, t);
+      const M2<double> Ds = {p1.x - p0.x, p2.x - p0.x, p1.y - p0.y, p2.y - p0.y};
+      const NH<double> h = nh_eval(mat_mul(Ds, B), mu[t], lam[t]);
+      val = h2 * area[t] * nh_psi(h, mu[t], lam[t]);
+      e = tri2env[t];
+    }
+    env_add(E, e, val, v);
+  }
+}
+
+__global__ void k_elastic_hvp(const double *x, const double *vv, const int32_t *tri, const double *Bm,
+                              const double *area, const double *mu, const double *lam, int T,
+                              double h2, double *y) {
+  GRID_LOOP(t, T) {
+    if (t < T) {
+      const int i0 = tri[3 * t], i1 = tri[3 * t + 1], i2 = tri[3 * t + 2];
+      const V2<double> p0 = ld2(x, i0), p1 = ld2(x, i1), p2 = ld2(x, i2);
+      const M2<double> B = load_B<double>(Bm, t);
+      const M2<double> Ds = {p1.x - p0.x, p2.x - p0.x, p1.y - p0.y, p2.y - p0.y};
+      const NH<double> h = nh_eval(mat_mul(Ds, B), mu[t], lam[t]);
+      const M2<double> dF = dF_of(ld2(vv, i0), ld2(vv, i1), ld2(vv, i2), B);
+      const M2<double> dP = nh_d2_dir(h, dF, mu[t], lam[t]);
+      const double A = area[t];
+      const V2<double> h1 = {A * (dP.a * B.a + dP.b * B.b), A * (dP.c * B.a + dP.d * B.b)};
+      const V2<double> hh2 = {A * (dP.a * B.c + dP.b * B.d), A * (dP.c * B.c + dP.d * B.d)};
+      atomic_add2(y, i0, h2 * V2<double>{-(h1.x + hh2.x), -(h1.y + hh2.y)});
+      atomic_add2(y, i1, h2 * h1);
+      atomic_add2(y, i2, h2 * hh2);
+    }
+  }
+}
+
+// Collider curvature + clearance, friction curvature (core/sdf.py:158-171, 237-255)
+__global__ void k_colliders_pHp(S s, Pm p) {
+  GRID_LOOP(b, s.n_b) {
+    const bool v = b < s.n_b;
+    double acc = 0.0, cl = BIG;
+    int e = 0;
+    if (v) {
+      const int n = s.bnd[b];
+      e = s.b2e[b];
+      const V2<double> x = ld2(s.x, n), pp = ld2(s.p, n);
+      for (int k = 0; k < s.n_col; ++k) {
+        double dist;
+        acc += collider_pHp(to_collider(s.col[k]), x, pp, p.dhat, p.kappa, p.h2, 1, dist);
+        cl = fmin(cl, dist);
+      }
+      if (p.friction && s.free_[n]) {
+        const V2<double> xn = ld2(s.xn, n);
+        for (int k = 0; k < s.n_col; ++k) {
+          const Collider c = to_collider(s.col[k]);
+          FricFrame f;
+          if (!friction_frame(c, x, xn, p.dhat, p.kappa, p.inv_dt, f)) continue;
+          const double kt = p.h2 * c.mu * f.f_n * p.inv_dt / fmax(f.vtm, p.epsv);
+          const double pt = dot(pp, V2<double>{-f.n.y, f.n.x});
+          acc += kt * pt * pt;
+        }
+      }
+    }
+    env_add(s.pHp, e, acc, v);
+    env_min(s.clr_min, e, cl, v && s.n_col > 0);
+  }
+}
+
+__global__ void k_colliders_ccd(S s, double eta) {
+  GRID_LOOP(b, s.n_b) {
+    bool v = b < s.n_b;
+    double a = BIG;
+    int e = 0;
+    if (v) {
+      const int n = s.bnd[b];
+      e = s.b2e[b];
+      const V2<double> x = ld2(s.x, n), pp = ld2(s.p, n);
+      for (int k = 0; k < s.n_col; ++k) a = fmin(a, collider_ccd(to_collider(s.col[k]), x, pp, eta));
+      v = a < BIG;
+    }
+    env_min(s.alpha, e, a, v);
+  }
+}
+
+__global__ void k_colliders_energy(S s, Pm p, double *E) {
+  GRID_LOOP(b, s.n_b) {
+    const bool v = b < s.n_b;
+    double acc = 0.0;
+    int e = 0;
+    if (v) {
+      const int n = s.bnd[b];
+      e = s.b2e[b];
+      const V2<double> x = ld2(s.x, n);
+      for (int k = 0; k < s.n_col; ++k) acc += collider_energy(to_collider(s.col[k]), x, p.dhat, p.kappa, p.h2);
+      if (p.friction && s.free_[n]) {
+        const V2<double> xn = ld2(s.xn, n);
+        for (int k = 0; k < s.n_col; ++k) {
+          const Collider c = to_collider(s.col[k]);
+          FricFrame f;
+          if (!friction_frame(c, x, xn, p.dhat, p.kappa, p.inv_dt, f)) continue;
+          const double ut = f.vtm / p.inv_dt, a = p.epsv / p.inv_dt;
+          const double f0 = ut < a ? ut * ut / (2.0 * a) : ut - 0.5 * a;
+          acc += p.h2 * c.mu * f.f_n * f0;
+        }
+      }
+    }
+    env_add(E, e, acc, v);
+  }
+}
+
+// inertia + external force energy (implicit/kernels.py:446-451, implicit/actuation.py:44-50)
+__global__ void k_node_energy(S s, double h2, double *E) {
+  GRID_LOOP(i, s.N) {
+    const bool v = i < s.N;
+    double val = 0.0;
+    int e = 0;
+    if (v) {
+      e = s.node2env[i];
+      const V2<double> d = ld2(s.x, i) - ld2(s.xhat, i);
+      val = 0.5 * s.mass[i] * dot(d, d);
+      if (s.fext) val += -h2 * dot(ld2(s.fext, i), ld2(s.x, i) - ld2(s.xn, i));
+    }
+    env_add(E, e, val, v);
+  }
+}
+
+// reference implicit/kernels.py:375-423
+__global__ void k_alpha_finalize(S s, Pm p, int it) {
+  GRID_LOOP(e, s.B) {
+    if (e < s.B && s.active[e] != 0) {
+      const double ph = s.pHp[e];
+      double a = s.alpha[e];
+      if (ph > 0.0) a = fmin(a, -s.gTp[e] / ph);
+      if (s.clr_min[e] < p.near_factor * p.dhat && s.p_inf[e] > 0.0) a = fmin(a, 0.5 * p.dhat / s.p_inf[e]);
+      if (a <= 0.0) {
+        s.active[e] = 0;
+        s.alpha[e] = 0.0;
+      } else {
+        s.alpha[e] = a;
+        if (p.tol_x > 0.0 && a * s.p_inf[e] < p.tol_x) {
+          s.active[e] = 0;
+          s.iters[e] += 1;
+        } else {
+          const double d = -a * s.gTp[e] - 0.5 * a * a * ph;
+          s.dE[e] = d;
+          if (it == 0) s.dE0[e] = fmax(d, 1.0e-300);
+          else if (d < p.eps * s.dE0[e]) s.active[e] = 0;
+          s.iters[e] += 1;
+        }
+      }
+    }
+  }
+}
+
+// x += alpha p (active), or x = x0 + alpha p for a line-search trial
+// (implicit/kernels.py:427-434, implicit/solver.py:52-60)
+__global__ void k_update_x(S s, const double *from) {
+  GRID_LOOP(i, s.N) {
+    if (i < s.N) {
+      const int e = s.node2env[i];
+      if (s.active[e]) st2(s.x, i, ld2(from, i) + s.alpha[e] * ld2(s.p, i));
+    }
+  }
+}
+
+// reference implicit/solver.py:64-73
+__global__ void k_ls_halve(S s, double tol) {
+  GRID_LOOP(e, s.B) {
+    if (e < s.B && s.active[e] && s.Ener[e] > s.Eprev[e] + tol * (fabs(s.Eprev[e]) + 1.0))
+      s.alpha[e] *= 0.5;
+  }
+}
+
+// ------------------------------------------------------------------------- host driver
+static inline int nblk(int n) { return n <= 0 ? 1 : (n + TPB - 1) / TPB; }
+
+#define LAUNCH(kern, n, ...)                                                          \
+  do {                                                                                \
+    if ((n) > 0) hipLaunchKernelGGL(kern, dim3(nblk(n)), dim3(TPB), 0, st, __VA_ARGS__); \
+  } while (0)
+
+struct Plain {
+  const S &s;
+  const Pm &p;
+  hipStream_t st;
+  PairCtx pc;
+  bool pending = false;
+  std::vector<double> hbuf;
+  std::vector<int> ibuf;
+
+  Plain(const S &s_, const Pm &p_, hipStream_t st_) : s(s_), p(p_), st(st_) {
+    pc = PairCtx{p.dhat, p.ipc_kappa, p.ipc_mu, p.epsv, p.inv_dt, p.h2, p.ipc_reach, p.ipc_gap, 0.0,
+                 p.ipc_per_body};
+  }
+
+  int ipc_pass(int mode, const double *pdir, double *grad, double *diag, const uint8_t *freev,
+               double *E, double *clr, double *alpha, double h2, double mu) {
+    PairCtx c = pc;
+    c.h2 = h2;
+    c.mu = mu;
+    c.floor_ = mode == 3 ? 0.25 * p.dhat : 0.0;
+    if (mode == 1 || mode == 3) {
+      // t_c snapshot: the ACCD of every pair stops at the alpha the pass started from
+      D2F_HIP(hipMemcpyAsync(s.tc, alpha, sizeof(double) * s.B, hipMemcpyDeviceToDevice, st));
+    }
+    LAUNCH(k_ipc, s.n_e, s, mode, c, pdir, grad, diag, freev, E, clr, alpha, s.tc);
+    return 0;
+  }
+
+  int ccd_sweep() {  // IPCContact.ccd along bc_q into sweep_beta (reference ipc_contact.py:432-439)
+    LAUNCH(k_fill, s.B, s.sweep_beta, s.B, 1.0);
+    LAUNCH(k_fill, s.B, s.clr_ipc, s.B, BIG);
+    int r = ipc_pass(3, s.bc_q, s.grad, s.diag, s.free_, s.Ener, s.clr_ipc, s.sweep_beta, p.h2, p.ipc_mu);
+    if (r) return r;
+    LAUNCH(k_far_cap, s.B, s.B, s.q_inf, p.ipc_reach, s.sweep_beta);
+    return 0;
+  }
+
+  int sweep() {
+    D2F_HIP(hipMemsetAsync(s.q_inf, 0, sizeof(double) * s.B, st));
+    LAUNCH(k_bc_motion, s.n_bc, s);
+    int r = ccd_sweep();
+    if (r) return r;
+    LAUNCH(k_bc_advance, s.n_bc, s);
+    return 0;
+  }
+
+  int poll_pending(bool &out) {
+    hbuf.resize(s.B);
+    D2F_HIP(hipMemcpyAsync(hbuf.data(), s.sweep_beta, sizeof(double) * s.B, hipMemcpyDeviceToHost, st));
+    D2F_HIP(hipStreamSynchronize(st));
+    out = false;
+    for (int e = 0; e < s.B; ++e)
+      if (!(hbuf[e] >= 1.0)) out = true;
+    return 0;
+  }
+
+  int grad_diag() {
+    LAUNCH(k_grad_diag_mass, s.N, s, p.h2);
+    LAUNCH(k_elastic_grad_diag, s.T, s.x, s.tri, s.Bm, s.area, s.mu, s.lam, s.T, p.h2, 1, s.grad, s.diag);
+    if (s.n_rg > 0)
+      D2F_HIP(hipMemcpyAsync(s.diag_el, s.diag, sizeof(double) * 2 * s.N, hipMemcpyDeviceToDevice, st));
+    if (s.n_col > 0) LAUNCH(k_colliders_grad_diag, s.n_b, s, p);
+    // terms, in the reference's order: external force, affine (none), IPC, pressure, tendon
+    if (s.fext) LAUNCH(k_extforce_grad, s.N, s, p.h2);
+    if (p.ipc_on) {
+      int r = ipc_pass(0, s.p, s.grad, s.diag, s.free_, s.Ener, s.clr_min, s.alpha, p.h2, p.ipc_mu);
+      if (r) return r;
+    }
+    if (s.n_pe > 0) LAUNCH(k_pressure, s.n_pe, s, 0, p.h2, s.Ener);
+    if (s.n_td > 0) {
+      D2F_HIP(hipMemsetAsync(s.td_L, 0, sizeof(double) * s.n_td, st));
+      LAUNCH(k_tendon_length, s.n_ey, s);
+      LAUNCH(k_tendon_state, s.n_td, s);
+      LAUNCH(k_tendon_grad, s.n_ey, s, p.h2);
+    }
+    // modes: affine gather, rigid gather
+    if (s.n_ag > 0) {
+      D2F_HIP(hipMemcpyAsync(s.ag_gq_prev, s.ag_gq, sizeof(double) * 6 * s.n_ag, hipMemcpyDeviceToDevice, st));
+      D2F_HIP(hipMemsetAsync(s.ag_gq, 0, sizeof(double) * 6 * s.n_ag, st));
+      D2F_HIP(hipMemsetAsync(s.ag_Hx, 0, sizeof(double) * 9 * s.n_ag, st));
+      D2F_HIP(hipMemsetAsync(s.ag_Hy, 0, sizeof(double) * 9 * s.n_ag, st));
+      LAUNCH(k_affine_gather, s.n_an, s);
+    }
+    if (s.n_rg > 0) {
+      D2F_HIP(hipMemsetAsync(s.rg_acc, 0, sizeof(double) * 3 * s.n_rg, st));
+      D2F_HIP(hipMemsetAsync(s.rg_gq, 0, sizeof(double) * 3 * s.n_rg, st));
+      D2F_HIP(hipMemsetAsync(s.rg_gq_prev, 0, sizeof(double) * 3 * s.n_rg, st));
+      D2F_HIP(hipMemsetAsync(s.rg_H, 0, sizeof(double) * 9 * s.n_rg, st));
+      LAUNCH(k_rigid_centroid, s.n_rn, s);
+      LAUNCH(k_rigid_gather, s.n_rn, s);
+    }
+    return 0;
+  }
+
+  int direction(int it) {
+    for (double *a : {s.gPg, s.g_p, s.g_Py, s.y_p, s.y_Py})
+      D2F_HIP(hipMemsetAsync(a, 0, sizeof(double) * s.B, st));
+    LAUNCH(k_mask_dk, s.N, s);
+    if (s.n_ag > 0) LAUNCH(k_affine_reduce, s.n_ag, s, p.h2);
+    if (s.n_rg > 0) LAUNCH(k_rigid_reduce, s.n_rg, s);
+    LAUNCH(k_finalize_beta, s.B, s, it, p.cg_guard, (p.ipc_on && pending) ? 1 : 0);
+    LAUNCH(k_update_p, s.N, s);
+    if (s.n_ag > 0) {
+      LAUNCH(k_affine_update, s.n_ag, s, 0);
+      LAUNCH(k_affine_expand, s.n_an, s, s.ag_pq, s.p);
+    }
+    if (s.n_rg > 0) LAUNCH(k_rigid_direction, s.n_rn, s);
+    return 0;
+  }
+
+  int curvature_caps(int it) {
+    LAUNCH(k_pHp_reset, s.B, s, p.alpha_max);
+    LAUNCH(k_pHp_mass, s.N, s);
+    LAUNCH(k_elastic_pHp, s.T, s.x, s.p, s.tri, s.Bm, s.area, s.mu, s.lam, s.tri2env, s.T, p.h2, 1, s.pHp);
+    if (s.n_col > 0) LAUNCH(k_colliders_pHp, s.n_b, s, p);
+    if (s.n_ag > 0) LAUNCH(k_affine_pHp_energy, s.n_ag, s, 1, p.h2, s.Ener);
+    if (p.ipc_on) {
+      LAUNCH(k_fill, s.B, s.clr_ipc, s.B, BIG);
+      int r = ipc_pass(1, s.p, s.grad, s.diag, s.free_, s.Ener, s.clr_ipc, s.alpha, p.h2, p.ipc_mu);
+      if (r) return r;
+      LAUNCH(k_far_cap, s.B, s.B, s.p_inf, p.ipc_reach, s.alpha);
+    }
+    if (s.n_pe > 0) LAUNCH(k_pressure, s.n_pe, s, 1, p.h2, s.Ener);
+    if (s.n_td > 0) {
+      D2F_HIP(hipMemsetAsync(s.td_dL, 0, sizeof(double) * s.n_td, st));
+      LAUNCH(k_tendon_pHp, s.n_ey, s, p.h2);
+      LAUNCH(k_tendon_taut, s.n_td, s, p.h2);
+    }
+    LAUNCH(k_inversion_alpha, s.T, s.x, s.p, s.tri, s.tri2env, s.T, p.area_shrink, s.alpha);
+    if (s.n_col > 0) LAUNCH(k_colliders_ccd, s.n_b, s, p.ccd_eta);
+    LAUNCH(k_alpha_finalize, s.B, s, p, it);
+    return 0;
+  }
+
+  int energy_into(double *E) {
+    D2F_HIP(hipMemsetAsync(E, 0, sizeof(double) * s.B, st));
+    LAUNCH(k_node_energy, s.N, s, p.h2, E);
+    LAUNCH(k_elastic_energy, s.T, s.x, s.tri, s.Bm, s.area, s.mu, s.lam, s.tri2env, s.T, p.h2, E);
+    if (s.n_col > 0) LAUNCH(k_colliders_energy, s.n_b, s, p, E);
+    if (s.n_ag > 0) LAUNCH(k_affine_pHp_energy, s.n_ag, s, 2, p.h2, E);
+    if (p.ipc_on) {
+      int r = ipc_pass(2, s.p, s.grad, s.diag, s.free_, E, s.clr_ipc, s.alpha, p.h2, p.ipc_mu);
+      if (r) return r;
+    }
+    if (s.n_pe > 0) LAUNCH(k_pressure, s.n_pe, s, 2, p.h2, E);
+    if (s.n_td > 0) {
+      D2F_HIP(hipMemsetAsync(s.td_L, 0, sizeof(double) * s.n_td, st));
+      LAUNCH(k_tendon_length, s.n_ey, s);
+      LAUNCH(k_tendon_energy, s.n_td, s, p.h2, E);
+    }
+    return 0;
+  }
+
+  int update() {
+    if (!p.line_search) {
+      LAUNCH(k_update_x, s.N, s, s.x);
+      if (s.n_ag > 0) {
+        LAUNCH(k_affine_update, s.n_ag, s, 1);
+        LAUNCH(k_affine_expand, s.n_an, s, s.ag_q, s.x);
+      }
+      return 0;
+    }
+    int r = energy_into(s.Eprev);
+    if (r) return r;
+    D2F_HIP(hipMemcpyAsync(s.x0, s.x, sizeof(double) * 2 * s.N, hipMemcpyDeviceToDevice, st));
+    if (s.n_ag > 0)
+      D2F_HIP(hipMemcpyAsync(s.ag_q0, s.ag_q, sizeof(double) * 6 * s.n_ag, hipMemcpyDeviceToDevice, st));
+    for (int bt = 0; bt <= p.n_backtrack; ++bt) {
+      LAUNCH(k_update_x, s.N, s, s.x0);
+      if (s.n_ag > 0) {
+        LAUNCH(k_affine_update, s.n_ag, s, 2);
+        LAUNCH(k_affine_expand, s.n_an, s, s.ag_q, s.x);
+      }
+      if (bt == p.n_backtrack) break;
+      r = energy_into(s.Ener);
+      if (r) return r;
+      LAUNCH(k_ls_halve, s.B, s, p.ls_tol);
+    }
+    return 0;
+  }
+
+  int run() {
+    LAUNCH(k_init_step, s.N, s, p.dt, p.drag);
+    pending = false;
+    if (!p.ipc_on) {
+      LAUNCH(k_apply_bc, s.n_bc, s);
+    } else if (s.n_bc > 0) {
+      int r = sweep();
+      if (r) return r;
+      r = poll_pending(pending);
+      if (r) return r;
+    }
+    LAUNCH(k_reset_step, s.B, s);
+    D2F_HIP(hipMemsetAsync(s.p, 0, sizeof(double) * 2 * s.N, st));
+    D2F_HIP(hipMemsetAsync(s.grad, 0, sizeof(double) * 2 * s.N, st));
+    for (int it = 0; it < p.iter_max; ++it) {
+      if (p.ipc_on && pending) {
+        int r = sweep();
+        if (r) return r;
+        LAUNCH(k_bc_wake, s.B, s);
+        if (it % 8 == 7) {
+          r = poll_pending(pending);
+          if (r) return r;
+        }
+      }
+      if (p.conv_check && it && it % p.conv_check == 0) {
+        ibuf.resize(s.B);
+        D2F_HIP(hipMemcpyAsync(ibuf.data(), s.active, sizeof(int) * s.B, hipMemcpyDeviceToHost, st));
+        D2F_HIP(hipStreamSynchronize(st));
+        long tot = 0;
+        for (int e = 0; e < s.B; ++e) tot += ibuf[e];
+        if (tot == 0) break;
+      }
+      int r = grad_diag();
+      if (!r) r = direction(it);
+      if (!r) r = curvature_caps(it);
+      if (!r) r = update();
+      if (r) return r;
+    }
+    LAUNCH(k_commit_v, s.N, s, p.inv_dt);
+    D2F_HIP(hipGetLastError());
+    return 0;
+  }
+};
+
+}  // namespace d2f
+
+using namespace d2f;
+
+extern "C" {
+
+int d2f_abi_version(void) { return D2F_ABI_VERSION; }
+const char *d2f_last_error(void) { return g_err; }
+void d2f_struct_sizes(int64_t *out) {
+  out[0] = (int64_t)sizeof(d2f_scene);
+  out[1] = (int64_t)sizeof(d2f_params);
+  out[2] = (int64_t)sizeof(d2f_collider);
+}
+
+int d2f_elastic_grad_diag_f64(const double *x, const int32_t *tri, const double *Bm, const double *area,
+                              const double *mu, const double *lam, int32_t T, double h2, int32_t clamp,
+                              double *grad, double *diag, void *stream) {
+  D2F_CHECK(T >= 0 && x && tri && Bm && area && mu && lam && grad && diag, "d2f_elastic_grad_diag_f64: bad argument");
+  hipStream_t st = (hipStream_t)stream;
+  LAUNCH(k_elastic_grad_diag, T, x, tri, Bm, area, mu, lam, T, h2, clamp, grad, diag);
+  D2F_HIP(hipGetLastError());
+  return 0;
+}
+
+int d2f_elastic_pHp_f64(const double *x, const double *p, const int32_t *tri, const double *Bm,
+                        const double *area, const double *mu, const double *lam, const int32_t *tri2env,
+                        int32_t T, double h2, int32_t clamp, double *pHp, void *stream) {
+  D2F_CHECK(T >= 0 && x && p && tri && Bm && area && mu && lam && tri2env && pHp, "d2f_elastic_pHp_f64: bad argument");
+  hipStream_t st = (hipStream_t)stream;
+  LAUNCH(k_elastic_pHp, T, x, p, tri, Bm, area, mu, lam, tri2env, T, h2, clamp, pHp);
+  D2F_HIP(hipGetLastError());
+  return 0;
+}
+
+int d2f_inversion_alpha_f64(const double *x, const double *p, const int32_t *tri, const int32_t *tri2env,
+                            int32_t T, double shrink, double *alpha, void *stream) {
+  D2F_CHECK(T >= 0 && x && p && tri && tri2env && alpha, "d2f_inversion_alpha_f64: bad argument");
+  hipStream_t st = (hipStream_t)stream;
+  LAUNCH(k_inversion_alpha, T, x, p, tri, tri2env, T, shrink, alpha);
+  D2F_HIP(hipGetLastError());
+  return 0;
+}
+
+int d2f_elastic_energy_f64(const double *x, const int32_t *tri, const double *Bm, const double *area,
+                           const double *mu, const double *lam, const int32_t *tri2env, int32_t T,
+                           double h2, double *E, void *stream) {
+  D2F_CHECK(T >= 0 && x && tri && Bm && area && mu && lam && tri2env && E, "d2f_elastic_energy_f64: bad argument");
+  hipStream_t st = (hipStream_t)stream;
+  LAUNCH(k_elastic_energy, T, x, tri, Bm, area, mu, lam, tri2env, T, h2, E);
+  D2F_HIP(hipGetLastError());
+  return 0;
+}
+
+int d2f_elastic_hvp_f64(const double *x, const double *v, const int32_t *tri, const double *Bm,
+                        const double *area, const double *mu, const double *lam, int32_t T, double h2,
+                        double *y, void *stream) {
+  D2F_CHECK(T >= 0 && x && v && tri && Bm && area && mu && lam && y, "d2f_elastic_hvp_f64: bad argument");
+  hipStream_t st = (hipStream_t)stream;
+  LAUNCH(k_elastic_hvp, T, x, v, tri, Bm, area, mu, lam, T, h2, y);
+  D2F_HIP(hipGetLastError());
+  return 0;
+}
+
+int d2f_ipc_min_distance(const d2f_scene *s, const d2f_params *p, void *stream) {
+  D2F_CHECK(s && p && p->ipc_on, "d2f_ipc_min_distance: needs a scene with IPC contact");
+  hipStream_t st = (hipStream_t)stream;
+  // IPCContact.min_distance (reference implicit/ipc_contact.py:441-447): a sweep along a zero
+  // direction leaves the smallest pair distance in clr_ipc.
+  D2F_HIP(hipMemsetAsync(s->x0, 0, sizeof(double) * 2 * s->N, st));
+  D2F_HIP(hipMemsetAsync(s->q_inf, 0, sizeof(double) * s->B, st));
+  Plain run(*s, *p, st);
+  LAUNCH(k_fill, s->B, s->sweep_beta, s->B, 1.0);
+  LAUNCH(k_fill, s->B, s->clr_ipc, s->B, BIG);
+  int r = run.ipc_pass(3, s->x0, s->grad, s->diag, s->free_, s->Ener, s->clr_ipc, s->sweep_beta, p->h2, p->ipc_mu);
+  if (r) return r;
+  D2F_HIP(hipGetLastError());
+  return 0;
+}
+
+int d2f_step_plain(const d2f_scene *s, const d2f_params *p, void *stream) {
+  D2F_CHECK(s && p, "d2f_step_plain: null scene or params");
+  D2F_CHECK(s->B > 0 && s->N > 0 && s->T >= 0 && s->n_col >= 0 && s->n_col <= D2F_MAX_COLLIDERS,
+            "d2f_step_plain: bad sizes (B=%d N=%d T=%d n_col=%d)", s->B, s->N, s->T, s->n_col);
+  D2F_CHECK(p->iter_max >= 0 && p->dt > 0.0, "d2f_step_plain: bad params");
+  Plain run(*s, *p, (hipStream_t)stream);
+  return run.run();
+}
+
+}  // extern "C"

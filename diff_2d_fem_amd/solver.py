@@ -1,0 +1,394 @@
+"""Device state of one batched implicit solve, in PyTorch-ROCm tensors, and the calls into
+the HIP library that advance it (reference implicit/solver.py:96-612, with the set-up of
+implicit/affine.py, implicit/ipc_contact.py and implicit/actuation.py).
+
+The host side builds index and constant arrays once (numpy, like the reference), then every
+step is one call into libd2fem_hip.so; positions stay on the GPU and are read back only on
+request (get_x).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+import torch
+from scipy.sparse import coo_matrix
+from scipy.sparse.csgraph import connected_components
+
+from . import _lib
+
+I32, F64, U8, I64 = torch.int32, torch.float64, torch.uint8, torch.int64
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        raise RuntimeError("no GPU: the implicit step runs only on the GPU (HIP); there is no CPU path")
+    return torch.device("cuda")
+
+
+class DeviceSolver:
+    """Everything the kernels read and write, for every environment of one Model."""
+
+    def __init__(self, me, cfg, driven, colliders=(), dt=1.0e-3, iter_max=150, eps=1e-4, tol_x=0.0,
+                 conv_check=0, near_factor=4.0, alpha_max=3.0, area_shrink=0.1, ccd_eta=0.9,
+                 friction=False, line_search=False, n_backtrack=4, ls_tol=1e-7, path="scene",
+                 precision="double"):
+        self.dev = _dev()
+        self.me, self.cfg = me, cfg
+        self.path, self.precision = path, precision
+        self.iter_max, self.eps, self.tol_x = int(iter_max), float(eps), float(tol_x)
+        self.conv_check = int(conv_check)
+        self.near_factor, self.alpha_max = float(near_factor), float(alpha_max)
+        self.area_shrink, self.ccd_eta = float(area_shrink), float(ccd_eta)
+        self.friction = bool(friction)
+        self.line_search, self.n_bt, self.ls_tol = bool(line_search), int(n_backtrack), float(ls_tol)
+        self.cg_guard = 0.0
+        self.ipc = None
+        self.affine = None
+        self.rigid = None
+        self.B = int(me.n_envs)
+        self.N = me.rest_pos.shape[0]
+        self.n_tri = len(me.rest_area)
+        self.n_b = len(me.boundary_nodes)
+        self.set_dt(dt)
+        t = self._t
+        z2 = lambda: torch.zeros((self.N, 2), dtype=F64, device=self.dev)
+        self.x = t(me.rest_pos, F64)
+        self.v, self.xn, self.xhat, self.x0 = z2(), z2(), z2(), z2()
+        self.grad, self.grad_prev, self.p, self.diag, self.diag_el = z2(), z2(), z2(), z2(), z2()
+        self.bc_q = z2()
+        self.mass_np = np.asarray(me.mass, np.float64)
+        self.mass = t(self.mass_np, F64)
+        self.tri = t(np.asarray(me.tri, np.int32), I32)
+        self.Bm = t(np.asarray(me.D_m_inv, np.float64).reshape(-1, 4), F64)
+        self.area = t(me.rest_area, F64)
+        self.mu_np = np.asarray(me.mu, np.float64).copy()
+        self.lam_np = np.asarray(me.lam, np.float64).copy()
+        self.mu = t(self.mu_np, F64)
+        self.lam = t(self.lam_np, F64)
+        self.node2env_np = np.asarray(me.node2env, np.int64)
+        self.node2env = t(self.node2env_np, I32)
+        self.tri2env = t(np.asarray(me.tri2env), I32)
+        self.node_start = t(np.asarray(me.node_starts), I32)
+        self.bnd = t(np.asarray(me.boundary_nodes), I32)
+        self.b2e = t(np.asarray(me.boundary2env), I32)
+        self.bnd_start = t(np.asarray(me.boundary_starts), I32)
+        self.free_np = np.ones((self.N, 2))
+        self.bc_idx_np = np.asarray(driven, np.int64)
+        self.free_np[self.bc_idx_np] = 0.0
+        self.free = t((self.free_np[:, 0] > 0.5).astype(np.uint8), U8)
+        self.bc_idx = t(self.bc_idx_np, I32)
+        self.n_bc = len(self.bc_idx_np)
+        self.bc_target = torch.zeros((max(self.n_bc, 1), 2), dtype=F64, device=self.dev)
+        zb = lambda: torch.zeros(self.B, dtype=F64, device=self.dev)
+        for k in ("gPg", "g_p", "g_Py", "y_p", "y_Py", "beta", "gTp", "pHp", "alpha", "p_inf",
+                  "clr_min", "dE", "dE0", "Ener", "Eprev", "clr_ipc", "q_inf", "sweep_beta", "tc"):
+            setattr(self, k, zb())
+        self.active = torch.zeros(self.B, dtype=I32, device=self.dev)
+        self.iters = torch.zeros(self.B, dtype=I32, device=self.dev)
+        self.colliders = [dict(c) for c in colliders]
+        if len(self.colliders) > _lib.MAX_COLLIDERS:
+            raise ValueError(f"at most {_lib.MAX_COLLIDERS} colliders are supported")
+        self.fext = None
+        self.pressure = None
+        self.tendon = None
+        self._empty_i = torch.zeros(1, dtype=I32, device=self.dev)
+        self._empty_f = torch.zeros(8, dtype=F64, device=self.dev)
+
+    def _t(self, a, dtype):
+        return torch.as_tensor(np.ascontiguousarray(a), device=self.dev).to(dtype).contiguous()
+
+    def set_dt(self, dt):
+        self.dt = float(dt)
+        self.h2 = self.dt * self.dt
+        self.inv_dt = 1.0 / self.dt
+        self.drag = math.exp(-self.dt * self.cfg.damping)
+
+    # ------------------------------------------------------------------ contact between bodies
+    def setup_ipc(self, mu, dhat, kappa, reach, gap=0.1, mu_node=None):
+        """IPC node-edge contact data (reference implicit/ipc_contact.py:353-386)."""
+        me = self.me
+        edges = np.asarray(me.boundary_edges, np.int64).reshape(-1, 2)
+        env = self.node2env_np[edges[:, 0]] if len(edges) else np.zeros(0, np.int64)
+        pos = np.full(self.N, -1, np.int64)
+        bstart = np.asarray(me.boundary_starts)
+        pos[np.asarray(me.boundary_nodes)] = np.arange(self.n_b) - bstart[np.asarray(me.boundary2env)]
+        m = np.bincount(edges.reshape(-1), minlength=self.N).astype(np.float64)
+        share = (m - 1.0) / np.maximum(m, 1.0)
+        t = self._t
+        self.ipc = dict(mu=float(mu), dhat=float(dhat), kappa=float(kappa), reach=float(reach),
+                        gap=float(gap), per_body=mu_node is not None, pending=False)
+        self.ea, self.eb = t(edges[:, 0], I32), t(edges[:, 1], I32)
+        self.ela, self.elb = t(pos[edges[:, 0]], I32), t(pos[edges[:, 1]], I32)
+        self.e2env = t(env, I32)
+        self.ca, self.cb = t(share[edges[:, 0]], F64), t(share[edges[:, 1]], F64)
+        self.excl = t(np.asarray(me.excl_flat, np.uint8), U8)
+        self.excl_off = t(np.asarray(me.excl_offset, np.int64), I64)
+        self.mu_node = None if mu_node is None else t(mu_node, F64)
+        self.n_e = len(edges)
+
+    def refresh_exclusion(self):
+        if self.ipc is not None:
+            self.excl = self._t(np.asarray(self.me.excl_flat, np.uint8), U8)
+
+    # ------------------------------------------------------------------ rigid bodies
+    def setup_affine(self, groups):
+        """Rigid regions as affine bodies (reference implicit/affine.py:282-328)."""
+        me = self.me
+        tri = np.asarray(me.tri)
+        X = np.asarray(me.rest_pos, np.float64)
+        area = np.asarray(me.rest_area, np.float64)
+        node, grp, w, Mu, La, env = [], [], [], [], [], []
+        owner = np.full(self.N, -1)
+        for b, g in enumerate(groups):
+            el = np.asarray(g["elems"], np.int64)
+            nd = np.unique(tri[el].reshape(-1))
+            if (owner[nd] >= 0).any():
+                raise ValueError("two rigid regions share a node; merge them or leave a gap of at "
+                                 "least one soft element between them")
+            if (self.free_np[nd] == 0).any():
+                raise ValueError("a rigid region contains kinematically constrained nodes. A body "
+                                 "whose motion is PRESCRIBED needs no rigid region")
+            owner[nd] = b
+            c = (self.mass_np[nd, None] * X[nd]).sum(0) / self.mass_np[nd].sum()
+            node += list(nd)
+            grp += [b] * len(nd)
+            w.append(np.c_[np.ones(len(nd)), X[nd] - c])
+            Mu.append(float((area[el] * np.asarray(me.mu, np.float64)[el]).sum()))
+            La.append(float((area[el] * np.asarray(me.lam, np.float64)[el]).sum()))
+            env.append(int(np.asarray(me.tri2env)[el[0]]))
+            self.mu_np[el] = 0.0
+            self.lam_np[el] = 0.0
+        self.mu = self._t(self.mu_np, F64)
+        self.lam = self._t(self.lam_np, F64)
+        n = len(groups)
+        t = self._t
+        a = dict(n=n, node_np=np.asarray(node, np.int64), grp_np=np.asarray(grp, np.int64),
+                 w_np=np.concatenate(w, 0), first=np.concatenate([[0], np.cumsum([len(v) for v in w])]),
+                 slaved=owner >= 0)
+        self.affine = a
+        self.an_node, self.an_grp = t(a["node_np"], I32), t(a["grp_np"], I32)
+        self.an_w = t(a["w_np"], F64)
+        self.ag_Mu, self.ag_La, self.ag2env = t(Mu, F64), t(La, F64), t(env, I32)
+        z = lambda k: torch.zeros((n, k), dtype=F64, device=self.dev)
+        self.ag_q, self.ag_q0, self.ag_pq, self.ag_gq, self.ag_gq_prev, self.ag_zq = (z(6) for _ in range(6))
+        self.ag_Hx, self.ag_Hy = z(9), z(9)
+        self.sync_affine(None)
+
+    def sync_affine(self, regions=None):
+        """q from the current node positions by least squares (reference implicit/affine.py:331-344)."""
+        a = self.affine
+        if a is None or (regions is not None and len(regions) == 0):
+            return
+        x = self.x.double().cpu().numpy()
+        q = np.zeros((a["n"], 6)) if regions is None else self.ag_q.cpu().numpy()
+        for b in (range(a["n"]) if regions is None else regions):
+            k = slice(a["first"][b], a["first"][b + 1])
+            sol = np.linalg.lstsq(a["w_np"][k], x[a["node_np"][k]], rcond=None)[0]
+            q[b, :3], q[b, 3:] = sol[:, 0], sol[:, 1]
+        self.ag_q.copy_(torch.as_tensor(q))
+        self.ag_pq.zero_()
+        self.ag_gq.zero_()
+        self.ag_gq_prev.zero_()
+        v = q[a["grp_np"]]
+        xs = np.stack([(a["w_np"] * v[:, 0:3]).sum(1), (a["w_np"] * v[:, 3:6]).sum(1)], 1)
+        self.x[torch.as_tensor(a["node_np"], device=self.dev)] = torch.as_tensor(xs, device=self.dev)
+
+    def affine_state(self):
+        q = self.ag_q.cpu().numpy()
+        return q[:, [0, 3]], np.stack([q[:, 1:3], q[:, 4:6]], 1)
+
+    def setup_rigid_modes(self, skip_nodes=None):
+        """Coarse space of every free soft piece (reference implicit/affine.py:503-537)."""
+        tri = np.asarray(self.me.tri)
+        a = np.concatenate([tri[:, 0], tri[:, 1], tri[:, 2]])
+        b = np.concatenate([tri[:, 1], tri[:, 2], tri[:, 0]])
+        ncomp, lab = connected_components(coo_matrix((np.ones(len(a)), (a, b)), shape=(self.N, self.N)),
+                                          directed=False)
+        held = np.zeros(ncomp, bool)
+        held[np.unique(lab[(self.free_np == 0).any(1)])] = True
+        if skip_nodes is not None and np.any(skip_nodes):
+            held[np.unique(lab[np.asarray(skip_nodes, bool)])] = True
+        size = np.bincount(lab, minlength=ncomp)
+        keep = np.where(~held & (size >= 3))[0]
+        if len(keep) == 0:
+            return
+        newid = np.full(ncomp, -1)
+        newid[keep] = np.arange(len(keep))
+        nodes = np.where(newid[lab] >= 0)[0]
+        grp = newid[lab[nodes]]
+        n = len(keep)
+        first = np.zeros(n, np.int64)
+        first[grp[::-1]] = nodes[::-1]
+        t = self._t
+        self.rigid = dict(n=n)
+        self.rn_node, self.rn_grp = t(nodes, I32), t(grp, I32)
+        self.rg2env = t(self.node2env_np[first], I32)
+        z = lambda k: torch.zeros((n, k), dtype=F64, device=self.dev)
+        self.rg_acc, self.rg_gq, self.rg_gq_prev, self.rg_zq, self.rg_H = z(3), z(3), z(3), z(3), z(9)
+
+    # ------------------------------------------------------------------ actuation
+    def set_fext(self, f):
+        f = np.asarray(f, np.float64)
+        if self.fext is None:
+            self.fext = self._t(f, F64)
+        else:
+            self.fext.copy_(torch.as_tensor(f))
+
+    def setup_pressure(self, ea, eb, ech, ch2env, cref):
+        t = self._t
+        self.pressure = dict(n_pe=len(ea), n_ch=len(ch2env))
+        self.pe_a, self.pe_b, self.pe_ch = t(ea, I32), t(eb, I32), t(ech, I32)
+        self.ch2env, self.ch_ref = t(ch2env, I32), t(np.asarray(cref, np.float64).reshape(-1, 2), F64)
+        self.ch_p = torch.zeros(len(ch2env), dtype=F64, device=self.dev)
+
+    def set_pressure(self, p):
+        self.ch_p.copy_(torch.as_tensor(np.asarray(p, np.float64)))
+
+    def setup_tendon(self, node, prv, nxt, tid, t2env, stiff, L0):
+        t = self._t
+        self.tendon = dict(n_ey=len(node), n_td=len(t2env))
+        self.ey_node, self.ey_prv, self.ey_nxt, self.ey_tid = (t(v, I32) for v in (node, prv, nxt, tid))
+        self.td2env = t(t2env, I32)
+        self.td_stiff = t(stiff, F64)
+        self.td_L0 = t(L0, F64)
+        n = len(t2env)
+        self.td_T, self.td_L, self.td_tau, self.td_keff, self.td_dL = (
+            torch.zeros(n, dtype=F64, device=self.dev) for _ in range(5))
+
+    def set_tendon(self, tension, length):
+        self.td_T.copy_(torch.as_tensor(np.asarray(tension, np.float64)))
+        self.td_L0.copy_(torch.as_tensor(np.asarray(length, np.float64)))
+
+    def clear_actuators(self):
+        self.pressure = None
+        self.tendon = None
+
+    # ------------------------------------------------------------------ ABI structs
+    def _ptr(self, tensor):
+        return None if tensor is None else C.c_void_p(tensor.data_ptr())
+
+    def scene_struct(self):
+        s = _lib.Scene()
+        s.B, s.N, s.T, s.n_b, s.n_bc = self.B, self.N, self.n_tri, self.n_b, self.n_bc
+        s.n_e = self.n_e if self.ipc is not None else 0
+        p = lambda name: C.c_void_p(getattr(self, name).data_ptr())
+        for k in ("x", "v", "xn", "xhat", "grad", "grad_prev", "p", "diag", "diag_el", "x0", "mass",
+                  "node2env", "node_start", "tri", "Bm", "area", "mu", "lam", "tri2env", "bnd", "b2e",
+                  "bnd_start", "bc_idx", "bc_target", "bc_q", "gPg", "g_p", "g_Py", "y_p", "y_Py", "beta",
+                  "gTp", "pHp", "alpha", "p_inf", "clr_min", "dE", "dE0", "Ener", "Eprev", "clr_ipc",
+                  "q_inf", "sweep_beta", "tc", "active", "iters"):
+            setattr(s, k, p(k))
+        s.free_ = p("free")
+        if self.ipc is not None:
+            for k in ("ea", "eb", "ela", "elb", "e2env", "ca", "cb", "excl", "excl_off"):
+                setattr(s, k, p(k))
+            s.mu_node = self._ptr(self.mu_node)
+        s.fext = self._ptr(self.fext)
+        if self.pressure is not None:
+            s.n_pe, s.n_ch = self.pressure["n_pe"], self.pressure["n_ch"]
+            for k in ("pe_a", "pe_b", "pe_ch", "ch2env", "ch_ref", "ch_p"):
+                setattr(s, k, p(k))
+        if self.tendon is not None:
+            s.n_ey, s.n_td = self.tendon["n_ey"], self.tendon["n_td"]
+            for k in ("ey_node", "ey_prv", "ey_nxt", "ey_tid", "td2env", "td_stiff", "td_L0", "td_T",
+                      "td_L", "td_tau", "td_keff", "td_dL"):
+                setattr(s, k, p(k))
+        if self.affine is not None:
+            s.n_an, s.n_ag = len(self.affine["node_np"]), self.affine["n"]
+            for k in ("an_node", "an_grp", "ag2env", "an_w", "ag_Mu", "ag_La", "ag_q", "ag_q0", "ag_pq",
+                      "ag_gq", "ag_gq_prev", "ag_zq", "ag_Hx", "ag_Hy"):
+                setattr(s, k, p(k))
+        if self.rigid is not None:
+            s.n_rn, s.n_rg = len(self.rn_node), self.rigid["n"]
+            for k in ("rn_node", "rn_grp", "rg2env", "rg_acc", "rg_gq", "rg_gq_prev", "rg_zq", "rg_H"):
+                setattr(s, k, p(k))
+        s.n_col = len(self.colliders)
+        for i, c in enumerate(self.colliders):
+            s.col[i].kind = int(c["kind"])
+            for k in ("cx", "cy", "rx", "ry", "rref", "r", "mu"):
+                setattr(s.col[i], k, float(c[k]))
+        return s
+
+    def params_struct(self):
+        q = _lib.Params()
+        q.dt, q.h2, q.inv_dt, q.drag = self.dt, self.h2, self.inv_dt, self.drag
+        q.dhat, q.kappa = float(self.cfg.ipc_d_hat), float(self.cfg.obstacle_stiffness)
+        q.epsv = float(self.cfg.obstacle_friction_epsv)
+        q.eps, q.tol_x, q.near_factor, q.alpha_max = self.eps, self.tol_x, self.near_factor, self.alpha_max
+        q.area_shrink, q.ccd_eta, q.ls_tol, q.cg_guard = self.area_shrink, self.ccd_eta, self.ls_tol, self.cg_guard
+        q.iter_max, q.conv_check = self.iter_max, self.conv_check
+        q.friction, q.line_search, q.n_backtrack = int(self.friction), int(self.line_search), self.n_bt
+        if self.ipc is not None:
+            q.ipc_on = 1
+            q.ipc_kappa, q.ipc_mu = self.ipc["kappa"], self.ipc["mu"]
+            q.ipc_reach, q.ipc_gap = self.ipc["reach"], self.ipc["gap"]
+            q.ipc_per_body = int(self.ipc["per_body"])
+        q.rigid_modes_on = int(self.rigid is not None)
+        q.precision = 1 if self.precision == "mixed" else 0
+        return q
+
+    # ------------------------------------------------------------------ stepping
+    def set_bc_targets(self, targets_full):
+        if self.n_bc:
+            t = np.asarray(targets_full, np.float64)[self.bc_idx_np]
+            self.bc_target.copy_(torch.as_tensor(t))
+
+    def step(self):
+        """One implicit step of every environment (reference implicit/solver.py:433-451)."""
+        L = _lib.lib()
+        s, q = self.scene_struct(), self.params_struct()
+        stream = C.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
+        if self.path == "plain":
+            _lib.check(L.d2f_step_plain(C.byref(s), C.byref(q), stream), "d2f_step_plain")
+        else:
+            if not hasattr(L, "d2f_step_scene"):
+                raise RuntimeError("this build of libd2fem_hip.so has no scene-persistent path")
+            _lib.check(L.d2f_step_scene(C.byref(s), C.byref(q), stream), "d2f_step_scene")
+
+    def min_distance(self):
+        """Smallest node-edge distance per environment (1e30 where none is within reach)."""
+        if self.ipc is None:
+            return np.full(self.B, 1.0e30)
+        L = _lib.lib()
+        s, q = self.scene_struct(), self.params_struct()
+        stream = C.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
+        _lib.check(L.d2f_ipc_min_distance(C.byref(s), C.byref(q), stream), "d2f_ipc_min_distance")
+        return self.clr_ipc.cpu().numpy().copy()
+
+    # ------------------------------------------------------------------ state I/O
+    def get_x(self):
+        return self.x.cpu().numpy().copy()
+
+    def get_v(self):
+        return self.v.cpu().numpy().copy()
+
+    def set_nodes(self, start, x, v=None):
+        n = len(x)
+        xt = torch.as_tensor(np.asarray(x, np.float64), device=self.dev)
+        self.x[start:start + n] = xt
+        self.xn[start:start + n] = xt
+        if v is not None:
+            self.v[start:start + n] = torch.as_tensor(np.asarray(v, np.float64), device=self.dev)
+
+    def set_v_rows(self, rows, v):
+        self.v[torch.as_tensor(np.asarray(rows), device=self.dev)] = torch.as_tensor(
+            np.asarray(v, np.float64), device=self.dev)
+
+    def reset_all(self, rest):
+        self.x.copy_(torch.as_tensor(np.asarray(rest, np.float64)))
+        self.v.zero_()
+        self.xn.zero_()
+        if self.affine is not None:
+            self.sync_affine(None)
+
+    def reset_envs(self, pick):
+        m = torch.as_tensor(np.asarray(pick)[self.node2env_np] != 0, device=self.dev)
+        rest = torch.as_tensor(np.asarray(self.me.rest_pos, np.float64), device=self.dev)
+        self.x[m] = rest[m]
+        self.xn[m] = rest[m]
+        self.v[m] = 0.0
+
+    def set_collider(self, index, c):
+        self.colliders[index].update(cx=c["cx"], cy=c["cy"])

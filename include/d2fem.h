@@ -1,0 +1,165 @@
+/* d2fem.h — C ABI of the MI355X implicit PNCG-IPC step (libd2fem_hip.so).
+ *
+ * The library replaces the device side of Diff_2D_FEM's ImplicitIntegrator.simulate(model, dt):
+ * the Warp kernel sequence PNCG2DSolver.step_targets / _iterate (implicit/solver.py:252-451)
+ * and the kernels it launches (implicit/kernels.py, core/sdf.py, implicit/ipc_contact.py,
+ * implicit/affine.py, implicit/actuation.py). The reference has no native FFI for this path —
+ * its Python calls Warp kernels directly — so each entry point below names the reference
+ * function or kernel family it stands in for.
+ *
+ * Conventions
+ *   - Every pointer in d2f_scene is DEVICE memory owned by the caller (the Python layer keeps
+ *     it in PyTorch-ROCm tensors). The library allocates nothing and keeps no state between
+ *     calls.
+ *   - 2-vectors are interleaved (x, y) float64 pairs; 2x2 matrices are row-major 4-tuples.
+ *   - Calls are asynchronous on `stream` (a hipStream_t passed as void*), except where the
+ *     comment says the call reads device results back (the per-kernel path's convergence poll).
+ *   - Return 0 on success, a negative D2F_E* code on failure; d2f_last_error() describes the
+ *     most recent failure of the calling thread. Nothing throws or aborts across the ABI.
+ */
+#ifndef D2FEM_H
+#define D2FEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define D2F_ABI_VERSION 1
+
+#define D2F_OK 0
+#define D2F_EINVAL (-1)   /* bad argument or inconsistent sizes */
+#define D2F_EHIP (-2)     /* a HIP runtime call failed */
+#define D2F_ECAP (-3)     /* a scene exceeds a compiled capacity of the scene-persistent path */
+
+#define D2F_MAX_COLLIDERS 8
+
+/* A rigid collider (reference sim_model.py:167-174; kinds core/objects.py:22-24):
+ * 0 disc, 1 ellipse, 2 rounded box, 3 rounded triangle, 4 half-space (rx, ry = unit normal). */
+typedef struct d2f_collider {
+  int32_t kind, pad_;
+  double cx, cy, rx, ry, rref, r, mu;
+} d2f_collider;
+
+/* The scene: mesh, state and term data of every environment, laid out flat with per-scene
+ * offsets (CSR). Sizes are element counts. */
+typedef struct d2f_scene {
+  int32_t B, N, T, n_b, n_e, n_bc;
+  /* nodes [N] */
+  double *x, *v, *xn, *xhat, *grad, *grad_prev, *p, *diag, *diag_el, *x0; /* 2N each */
+  double *mass;                       /* N */
+  const uint8_t *free_;               /* N: 1 free, 0 kinematically driven */
+  const int32_t *node2env;            /* N */
+  const int32_t *node_start;          /* B+1 */
+  /* triangles [T] (reference tri order) */
+  const int32_t *tri;                 /* 3T */
+  const double *Bm;                   /* 4T, D_m^-1 row-major */
+  const double *area, *mu, *lam;      /* T (mu, lam zeroed on rigid regions) */
+  const int32_t *tri2env;             /* T */
+  /* boundary nodes [n_b] and edges [n_e] */
+  const int32_t *bnd, *b2e;           /* n_b */
+  const int32_t *bnd_start;           /* B+1 */
+  const int32_t *ea, *eb, *ela, *elb, *e2env; /* n_e */
+  const double *ca, *cb;              /* n_e: vertex-correction shares */
+  const uint8_t *excl;                /* per-scene n_b_e^2 exclusion matrices */
+  const int64_t *excl_off;            /* B+1 */
+  const double *mu_node;              /* N per-body friction, or NULL */
+  /* kinematic nodes [n_bc] */
+  const int32_t *bc_idx;              /* n_bc */
+  double *bc_target;                  /* 2 n_bc */
+  double *bc_q;                       /* 2N sweep direction (zero off the kinematic nodes) */
+  /* per-environment scalars [B] (float64 accumulators) */
+  double *gPg, *g_p, *g_Py, *y_p, *y_Py, *beta, *gTp, *pHp, *alpha, *p_inf, *clr_min;
+  double *dE, *dE0, *Ener, *Eprev, *clr_ipc, *q_inf, *sweep_beta, *tc;
+  int32_t *active, *iters;
+  /* external force [N] (gravity + applied), or NULL */
+  const double *fext;                 /* 2N */
+  /* pressure edges [n_pe] */
+  int32_t n_pe, n_ch;
+  const int32_t *pe_a, *pe_b, *pe_ch, *ch2env; /* n_pe, n_pe, n_pe, n_ch */
+  const double *ch_ref;               /* 2 n_ch */
+  const double *ch_p;                 /* n_ch */
+  /* tendons: eyelets [n_ey], tendons [n_td] */
+  int32_t n_ey, n_td;
+  const int32_t *ey_node, *ey_prv, *ey_nxt, *ey_tid, *td2env;
+  const double *td_stiff, *td_L0, *td_T;
+  double *td_L, *td_tau, *td_keff, *td_dL;
+  /* affine bodies: slaved nodes [n_an], groups [n_ag] */
+  int32_t n_an, n_ag;
+  const int32_t *an_node, *an_grp, *ag2env;
+  const double *an_w;                 /* 3 n_an: (1, X - c) */
+  const double *ag_Mu, *ag_La;        /* n_ag */
+  double *ag_q, *ag_q0, *ag_pq, *ag_gq, *ag_gq_prev, *ag_zq; /* 6 n_ag */
+  double *ag_Hx, *ag_Hy;              /* 9 n_ag */
+  /* rigid-mode groups: member nodes [n_rn], groups [n_rg] */
+  int32_t n_rn, n_rg;
+  const int32_t *rn_node, *rn_grp, *rg2env;
+  double *rg_acc, *rg_gq, *rg_gq_prev, *rg_zq; /* 3 n_rg */
+  double *rg_H;                       /* 9 n_rg */
+  /* colliders */
+  int32_t n_col, pad_;
+  d2f_collider col[D2F_MAX_COLLIDERS];
+} d2f_scene;
+
+/* Solver options and step constants (reference PNCG2DSolver.__init__, implicit/solver.py:97-226,
+ * and ImplicitIntegrator._build, implicit/integrator.py:252-311). */
+typedef struct d2f_params {
+  double dt, h2, inv_dt, drag;
+  double dhat, kappa, epsv;                 /* collider barrier and friction smoothing */
+  double eps, tol_x, near_factor, alpha_max, area_shrink, ccd_eta;
+  double ls_tol, cg_guard;
+  double ipc_kappa, ipc_mu, ipc_reach, ipc_gap;
+  int32_t iter_max, conv_check, friction, line_search, n_backtrack;
+  int32_t ipc_on, ipc_per_body, rigid_modes_on, precision; /* precision: 0 float64, 1 mixed */
+  int32_t pad_;
+} d2f_params;
+
+/* ---- library ---------------------------------------------------------------------------- */
+int d2f_abi_version(void);
+/* Text of the calling thread's last failure (empty if none). */
+const char *d2f_last_error(void);
+/* sizeof(d2f_scene), sizeof(d2f_params), sizeof(d2f_collider) into out[0..2], so a binding can
+ * check its mirror of the structs. */
+void d2f_struct_sizes(int64_t *out);
+
+/* ---- per-kernel entry points (float64), one per reference kernel family ------------------ */
+/* grad += h2 dPsi/dx, diag += h2 max(diag d2Psi/dx2, 0) over all triangles.
+ * Replaces grad_diag_elastic_kernel (implicit/kernels.py:86-128). */
+int d2f_elastic_grad_diag_f64(const double *x, const int32_t *tri, const double *Bm,
+                              const double *area, const double *mu, const double *lam, int32_t T,
+                              double h2, int32_t clamp, double *grad, double *diag, void *stream);
+/* pHp[env] += h2 max(p^T d2Psi p, 0) per triangle. Replaces pHp_elastic_kernel
+ * (implicit/kernels.py:259-273). */
+int d2f_elastic_pHp_f64(const double *x, const double *p, const int32_t *tri, const double *Bm,
+                        const double *area, const double *mu, const double *lam,
+                        const int32_t *tri2env, int32_t T, double h2, int32_t clamp, double *pHp,
+                        void *stream);
+/* alpha[env] = min(alpha, smallest positive area-shrink root). Replaces inversion_alpha_kernel
+ * (implicit/kernels.py:304-341). */
+int d2f_inversion_alpha_f64(const double *x, const double *p, const int32_t *tri,
+                            const int32_t *tri2env, int32_t T, double shrink, double *alpha,
+                            void *stream);
+/* E[env] += h2 A Psi per triangle. Replaces energy_elastic_kernel (implicit/kernels.py:454-468). */
+int d2f_elastic_energy_f64(const double *x, const int32_t *tri, const double *Bm,
+                           const double *area, const double *mu, const double *lam,
+                           const int32_t *tri2env, int32_t T, double h2, double *E, void *stream);
+/* y += h2 (d2Psi/dx2) v, unclamped. Replaces hvp_elastic_kernel (implicit/adjoint.py:70-85). */
+int d2f_elastic_hvp_f64(const double *x, const double *v, const int32_t *tri, const double *Bm,
+                        const double *area, const double *mu, const double *lam, int32_t T,
+                        double h2, double *y, void *stream);
+
+/* ---- whole implicit steps ----------------------------------------------------------------- */
+/* One implicit step of every environment on the per-kernel path: one launch per reference
+ * kernel family, float64, atomics for scatters; polls convergence every conv_check iterations
+ * (a device->host read). Replaces PNCG2DSolver.step_targets (implicit/solver.py:433-451). */
+int d2f_step_plain(const d2f_scene *s, const d2f_params *prm, void *stream);
+/* Smallest node-edge distance per environment into s->clr_ipc (1e30 where no pair is within
+ * reach); uses s->x0 as scratch. Replaces IPCContact.min_distance (implicit/ipc_contact.py:441-447),
+ * the "bodies must start apart" check of ImplicitIntegrator._build. */
+int d2f_ipc_min_distance(const d2f_scene *s, const d2f_params *prm, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,106 @@
+"""Write the golden fixtures that pin the host scene layer's indexing.
+
+Run once, in the container that holds the reference checkout (its warp-free host files only:
+core/mesh.py and core/actuators.py, loaded by file path, plus tests/reference_static.py);
+the outputs are committed as small .npz files next to this script and are what the tests
+read. Nothing on the GPU machine runs this.
+
+    python tests/golden/make_fixtures.py [/path/to/reference]
+"""
+from __future__ import annotations
+
+import importlib.util
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+
+
+def _load(path, name):
+    spec = importlib.util.spec_from_file_location(name, path)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod                      # dataclasses look their module up here
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def scenes():
+    """The scenes whose meshes are pinned: name -> list of body dicts for build_scene."""
+    rng = np.random.default_rng(7)
+    blob = rng.random((9, 14)) > 0.35
+    blob[4, :] = True
+    ring = np.ones((7, 9), bool)
+    ring[2:5, 3:6] = False                      # a sealed cavity
+    notch = np.ones((6, 10), bool)
+    notch[3:, 4:6] = False                      # a slot open to the top
+    # a small jittered mesh body
+    xs, ys = np.meshgrid(np.linspace(0, 4e-3, 5), np.linspace(0, 2e-3, 3), indexing="ij")
+    V = np.stack([xs.ravel(), ys.ravel()], 1)
+    V[6] += [1.3e-4, -0.7e-4]
+    T = []
+    for i in range(4):
+        for j in range(2):
+            a = i * 3 + j
+            T += [[a, a + 1, a + 4], [a, a + 4, a + 3]]
+    T = np.asarray(T)
+    return {
+        # the strip of the quickstart: mask ones(6, 72), (H, W) with row 0 at the bottom
+        "strip": [dict(mask_xy=np.ones((6, 72), bool).T.copy(), size=(0.054, 0.0045),
+                       center=(0.0, 0.0030), young=0.5e6, nu=0.40, rho=1100.0, env=0)],
+        "blob": [dict(mask_xy=blob.T.copy(), size=(14e-3, 9e-3), center=(0.01, 0.02),
+                      young=2e5, nu=0.45, rho=1000.0, env=0)],
+        # three scenes, two bodies in scene 1 given out of scene order
+        "multi": [dict(mask_xy=ring.T.copy(), size=(9e-3, 7e-3), center=(0.0, 0.005),
+                       young=3e5, nu=0.45, rho=1100.0, env=1),
+                  dict(mask_xy=notch.T.copy(), size=(10e-3, 6e-3), center=(0.0, 0.004),
+                       young=1e6, nu=0.40, rho=900.0, env=0),
+                  dict(mask_xy=np.ones((4, 4), bool), size=(4e-3, 4e-3), center=(0.012, 0.003),
+                       young=1e9, nu=0.30, rho=600.0, env=1),
+                  dict(mask_xy=notch.T.copy(), size=(10e-3, 6e-3), center=(0.0, 0.004),
+                       young=1e6, nu=0.40, rho=900.0, env=2)],
+        "meshbody": [dict(vertices=V, triangles=T, excl_radius=2.5e-3, young=1e6, nu=0.4,
+                          rho=1000.0, env=0)],
+    }
+
+
+MESH_KEYS = ("rest_pos", "tri", "D_m_inv", "rest_area", "mu", "lam", "mass", "node2env",
+             "tri2env", "boundary_nodes", "boundary2env", "boundary_edges", "node_starts",
+             "tri_starts", "boundary_starts", "boundary_per_env", "excl_flat", "excl_offset",
+             "tri_quad", "body_node_range", "body_tri_range")
+
+
+def main(ref_root):
+    ref = Path(ref_root)
+    mesh = _load(ref / "core" / "mesh.py", "ref_core_mesh")
+    act = _load(ref / "core" / "actuators.py", "ref_core_actuators")
+    out = {}
+    for name, bodies in scenes().items():
+        me = mesh.build_scene([dict(b) for b in bodies])
+        for k in MESH_KEYS:
+            out[f"{name}/{k}"] = np.asarray(getattr(me, k))
+    np.savez_compressed(HERE / "mesh_fixtures.npz", **out)
+
+    # chamber edges of the ring's cavity and the exterior edges of the notch (pixel bodies)
+    ring = scenes()["multi"][0]["mask_xy"].T
+    lab, ids = act.enclosed_voids(ring)
+    me = mesh.build_scene([dict(scenes()["multi"][0], env=0)])
+    remap = me.grid_to_live_node_per_body[0]
+    ny = ring.shape[0]
+
+    def node_of(ix, iy):
+        loc = int(remap[(ny + 1) * ix + iy])
+        return -1 if loc < 0 else loc
+    ea, eb = act.chamber_edges(ring, lab, ids[0], node_of)
+    xa, xb = act.exterior_edges(ring, lab, ids, node_of)
+    pts = np.array([[-3e-3, 2e-3], [-1e-3, 2.2e-3], [1.1e-3, 2.4e-3], [4e-3, 2.6e-3]])
+    path = act.snap_path(pts, me.rest_pos, np.arange(me.rest_pos.shape[0]))
+    np.savez_compressed(HERE / "actuator_fixtures.npz", labels=lab, ids=np.asarray(ids),
+                        chamber_a=ea, chamber_b=eb, exterior_a=xa, exterior_b=xb,
+                        tendon_points=pts, tendon_path=path)
+    print("wrote", HERE / "mesh_fixtures.npz", "and", HERE / "actuator_fixtures.npz")
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

@@ -1,0 +1,52 @@
+"""Scene builders shared by the tests: small versions of the benchmark configurations."""
+import numpy as np
+
+from diff_2d_fem_amd import Model
+
+
+def strip(n_env=1, E=0.5e6, nu=0.40, rho=1100.0, mu=0.8, shape=(6, 72), size=(0.054, 0.0045),
+          center=(0.0, 0.0030), gravity=True):
+    """The quickstart strip lying on the ground (reference README.md:38-58), n_env copies."""
+    m = Model()
+    m.add_ground(0.0, mu=mu)
+    for i in range(n_env):
+        m.add_soft_body(np.ones(shape, np.uint8), physical_size=size, center=center, E=E, nu=nu,
+                        density=rho, env_id=i)
+    m.finalize()
+    if gravity:
+        m.gravity = (0.0, -9.81)
+    return m
+
+
+def grip(n_env=1, px=0.5e-3, block=8, pad=(8, 4), friction=0.8):
+    """Two soft pads either side of a rigid block on the ground, pads' back faces driven
+    (a small version of the pinch-and-lift scene, reference examples/pinch_and_lift.py:29-66)."""
+    m = Model()
+    m.body_friction = friction
+    m.add_ground(0.0, mu=0.5)
+    pads = []
+    for i in range(n_env):
+        m.add_affine_body(np.ones((block, block), np.uint8), physical_size=(block * px, block * px),
+                          center=(0.0, block * px / 2 + 0.9 * px / 8), density=600.0, env_id=i)
+        ph, pw = pad
+        pads.append([m.add_soft_body(np.ones((ph, pw), np.uint8), physical_size=(pw * px, ph * px),
+                                     center=(s * (block * px / 2 + 2 * px + pw * px / 2), ph * px / 2 + 0.8e-3),
+                                     E=1.0e6, nu=0.40, density=1100.0, env_id=i) for s in (-1, 1)])
+    m.finalize()
+    m.gravity = (0.0, -9.81)
+    for pair in pads:
+        for s, p in zip((-1, 1), pair):
+            X = m.rest_x(p)
+            back = X[:, 0].min() if s < 0 else X[:, 0].max()
+            m.set_kinematic_constraint(p, np.where(np.abs(X[:, 0] - back) < 1e-9)[0])
+    return m, pads
+
+
+def close_jaws(m, pads, k, n_close=10, squeeze=2.25, px=0.5e-3, lift=0.0):
+    a = min(k / n_close, 1.0)
+    for pair in pads:
+        for s, p in zip((-1, 1), pair):
+            X = m.rest_x(p)
+            X[:, 0] -= s * squeeze * px * a
+            X[:, 1] += lift
+            m.set_kinematic_target(p, X)
