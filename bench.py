@@ -1,0 +1,176 @@
+"""Throughput of the batched implicit step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3] [--path scene|plain]
+                    [--precision mixed|double] [--no-plain] [--no-cpu]
+
+The workload is the contact-heavy configuration: 256 scenes of two soft pads on kinematic
+jaws gripping and lifting a rigid block (diff_2d_fem_amd.workloads.pads_lift_block). W
+untimed steps bring the scenes into the gripped state, then K steps are timed, bracketed by
+torch.cuda.synchronize(). A "step" is one ImplicitIntegrator.simulate of every scene,
+including the host command that sets the jaws' targets.
+
+Printed: one JSON line. `value` is scene-steps/s of the requested path and precision;
+`plain_fp64` is the per-kernel float64 path on the same workload and steps (the
+denominator of the batched-vs-plain ratio); `pos_err_vs_fp64` is the largest node-position
+difference against the float64 run after the same steps, over the block size; `roofline`
+is the dominant kernel's algorithmic bytes over its measured duration against HBM peak;
+`cpu_baseline` times the float64 CPU oracle on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--config", default="c3")
+    ap.add_argument("--envs", type=int, default=None)
+    ap.add_argument("--path", default=None)
+    ap.add_argument("--precision", default=None)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    return ap.parse_args()
+
+
+def build(config, n_env):
+    from diff_2d_fem_amd import workloads as W
+    fn = W.CONFIGS[config]
+    return fn() if n_env is None else fn(n_env)
+
+
+def run_gpu(args, path, precision, steps, warmup):
+    import torch
+    from diff_2d_fem_amd import ImplicitIntegrator
+    w = build(args.config, args.envs)
+    m = w["model"]
+    integ = ImplicitIntegrator(path=path, precision=precision)
+    step = 0
+    for _ in range(warmup):
+        step += 1
+        w["command"](step)
+        integ.simulate(m, w["dt"])
+    torch.cuda.synchronize()
+    acc = torch.zeros(m.n_envs, dtype=torch.int64, device="cuda")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step += 1
+        w["command"](step)
+        integ.simulate(m, w["dt"])
+        acc += m._solver.iters                      # on the stream: no host sync in the loop
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    st = integ.stats(m)
+    iters = int(acc.sum().item())
+    return dict(t=t, m=m, w=w, integ=integ, stats=st, x=m.get_x(), iters=iters)
+
+
+def elastic_roofline(res):
+    """Time the float64 elastic gradient+diagonal kernel on the run's final state with HIP
+    events on the launch stream; algorithmic bytes = 68 T + 48 N per launch."""
+    import ctypes as C
+    import torch
+    from diff_2d_fem_amd import _lib
+    s = res["m"]._solver
+    L = _lib.lib()
+    stream = torch.cuda.current_stream()
+    h = C.c_void_p(stream.cuda_stream)
+    g = torch.zeros_like(s.x)
+    d = torch.zeros_like(s.x)
+    P = lambda t: C.c_void_p(t.data_ptr())
+    args = (P(s.x), P(s.tri), P(s.Bm), P(s.area), P(s.mu), P(s.lam), s.n_tri, s.h2, 1, P(g), P(d), h)
+    for _ in range(3):
+        L.d2f_elastic_grad_diag_f64(*args)
+    n = 20
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(n):
+        L.d2f_elastic_grad_diag_f64(*args)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    dur = e0.elapsed_time(e1) / n * 1e-3
+    bytes_ = 68.0 * s.n_tri + 48.0 * s.N
+    ach = bytes_ / dur / 1e9
+    return dict(kernel="k_elastic_grad_diag (float64)", bound="hbm", achieved=round(ach, 1),
+                peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
+                avg_us=round(dur * 1e6, 2), bytes_per_launch=int(bytes_))
+
+
+def cpu_baseline(args, steps):
+    """The float64 numpy oracle on one scene of the same workload, from its first step."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle.pncg import OracleIntegrator
+    from diff_2d_fem_amd import workloads as W
+    w = W.CONFIGS[args.config](1)
+    m = w["model"]
+    oi = OracleIntegrator()
+    t0 = time.perf_counter()
+    for k in range(1, steps + 1):
+        w["command"](k)
+        oi.simulate(m, w["dt"])
+    t = time.perf_counter() - t0
+    return dict(value=round(steps / t, 4), unit="scene-steps/s", cores=1, kind="port",
+                sample=f"float64 numpy oracle, 1 scene of {args.config}, steps 1-{steps} "
+                       f"({t:.1f} s, {int(oi.stats()['iters'].sum())} PNCG iterations)")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
+              f"environments shard across GPUs with no communication, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from diff_2d_fem_amd import _lib
+    has_scene = hasattr(_lib.lib(), "d2f_step_scene")
+    path = args.path or ("scene" if has_scene else "plain")
+    precision = args.precision or ("mixed" if path == "scene" else "double")
+    main_run = run_gpu(args, path, precision, args.steps, args.warmup)
+    m = main_run["m"]
+    B = m.n_envs
+    value = B * args.steps / main_run["t"]
+    out = {
+        "metric": "scene-steps/s batched mixed vs plain FP64 at fixed error; % HBM peak",
+        "value": round(value, 2), "unit": "scene-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(main_run["t"] / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if precision == "mixed" else "f64", "data": "synthetic",
+        "config": {"workload": main_run["w"]["name"], "scenes": B, "nodes": m.n_nodes,
+                   "triangles": m.n_elements, "dt": main_run["w"]["dt"], "path": path,
+                   "precision": precision, "parallelism": "one workgroup per scene" if path == "scene"
+                   else "one launch per kernel family"},
+        "pncg_iters_per_s": round(main_run["iters"] / main_run["t"], 1),
+        "max_iter": int(main_run["stats"]["max_iter"]),
+        "converged_frac": round(float(np.mean(main_run["stats"]["converged"])), 4),
+    }
+    ref_x = None
+    if not args.no_plain and not (path == "plain" and precision == "double"):
+        plain = run_gpu(args, "plain", "double", args.steps, args.warmup)
+        pv = B * args.steps / plain["t"]
+        out["plain_fp64"] = dict(value=round(pv, 2), ms_per_step=round(plain["t"] / args.steps * 1e3, 3))
+        out["speedup_vs_plain_fp64"] = round(value / pv, 3)
+        ref_x = plain["x"]
+        roof_src = plain
+    else:
+        roof_src = main_run
+    if ref_x is not None:
+        out["pos_err_vs_fp64"] = float(np.abs(main_run["x"] - ref_x).max() / main_run["w"]["size"])
+    out["roofline"] = elastic_roofline(roof_src)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

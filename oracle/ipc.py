@@ -40,7 +40,8 @@ def accd(p, a, b, dp, da, db, g, t_c):
         hit_c = live & (tn >= t_c)
         out[hit_c] = t_c[hit_c]
         live &= ~hit_c
-        dn = seg_dist(p + tn[:, None] * dp, a + tn[:, None] * da, b + tn[:, None] * db)
+        with np.errstate(invalid="ignore", over="ignore"):
+            dn = seg_dist(p + tn[:, None] * dp, a + tn[:, None] * da, b + tn[:, None] * db)
         if k > 0:
             stop = live & (dn < g)
             out[stop] = t[stop]
