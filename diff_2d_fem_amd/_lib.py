@@ -13,6 +13,7 @@ LIB_NAME = "libd2fem_hip.so"
 LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
 
 MAX_COLLIDERS = 8
+MAX_COLORS = 24
 P = C.c_void_p
 
 
@@ -43,6 +44,10 @@ _SCENE_FIELDS = (
     + [("n_rn", C.c_int32), ("n_rg", C.c_int32)]
     + [(k, P) for k in ("rn_node", "rn_grp", "rg2env", "rg_acc", "rg_gq", "rg_gq_prev", "rg_zq", "rg_H")]
     + [("n_col", C.c_int32), ("pad_", C.c_int32), ("col", Collider * MAX_COLLIDERS)]
+    + [(k, P) for k in ("sc_tri_start", "sc_cs", "sc_tri", "sc_Bm", "sc_Amu", "sc_Alam", "sc_Bm32",
+                        "sc_Amu32", "sc_Alam32", "bloc", "e_start", "bc_start", "ag_start", "an_start",
+                        "rg_start", "rn_start", "pe_start", "td_start", "ey_start")]
+    + [("max_nb", C.c_int32), ("max_cells", C.c_int32)]
 )
 
 
@@ -57,7 +62,7 @@ class Params(C.Structure):
                                            "ipc_reach", "ipc_gap")]
                 + [(k, C.c_int32) for k in ("iter_max", "conv_check", "friction", "line_search",
                                             "n_backtrack", "ipc_on", "ipc_per_body", "rigid_modes_on",
-                                            "precision", "pad_")])
+                                            "precision", "block_size")])
 
 
 _lib = None

@@ -457,4 +457,93 @@ __device__ inline double accd(V2<double> p, V2<double> a, V2<double> b, V2<doubl
   return t;
 }
 
+
+// ------------------------------------------------------------------------------- rigid bodies
+// Affine-body 6x6 block algebra (reference implicit/affine.py:61-147) and the 3x3 inverse of
+// the rigid-mode coarse block (reference implicit/affine.py:465).
+struct M66 {
+  double m[6][6];
+};
+
+__device__ inline void affine_block(M66 &H, double A11, double A12, double A21, double A22, double Mu,
+                                    double La, double h2) {
+  const int idx[4] = {1, 2, 4, 5};
+  const NH<double> h = nh_eval(M2<double>{A11, A12, A21, A22}, Mu, La);
+  for (int k = 0; k < 4; ++k) {
+    M2<double> E = {0.0, 0.0, 0.0, 0.0};
+    if (k == 0) E.a = 1.0;
+    else if (k == 1) E.b = 1.0;
+    else if (k == 2) E.c = 1.0;
+    else E.d = 1.0;
+    const M2<double> col = nh_d2_dir(h, E, Mu, La);
+    const double cv[4] = {col.a, col.b, col.c, col.d};
+    for (int r = 0; r < 4; ++r) H.m[idx[r]][idx[k]] += h2 * cv[r];
+  }
+  const double a = A11 + A22, b = A21 - A12;
+  const double n = sqrt(a * a + b * b);
+  if (n > 1.0e-12) {
+    const double s = 0.7071067811865476 / n;
+    const double t[4] = {-b * s, -a * s, a * s, -b * s};
+    const M2<double> Tm = {t[0], t[1], t[2], t[3]};
+    const double lam_t = nh_quad(h, Tm, Mu, La);
+    if (lam_t < 0.0)
+      for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) H.m[idx[r]][idx[c]] -= h2 * lam_t * t[r] * t[c];
+  }
+}
+
+__device__ inline void cholesky6(const M66 &H, M66 &L) {
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) L.m[i][j] = 0.0;
+  bool ok = true;
+  for (int j = 0; j < 6; ++j) {
+    double s = H.m[j][j];
+    for (int k = 0; k < j; ++k) s -= L.m[j][k] * L.m[j][k];
+    if (s <= 1.0e-9 * fabs(H.m[j][j]) || s <= 0.0) ok = false;
+    if (ok) {
+      const double d = sqrt(s);
+      L.m[j][j] = d;
+      for (int i = j + 1; i < 6; ++i) {
+        double v = H.m[i][j];
+        for (int k = 0; k < j; ++k) v -= L.m[i][k] * L.m[j][k];
+        L.m[i][j] = v / d;
+      }
+    }
+  }
+  if (!ok) {
+    for (int i = 0; i < 6; ++i)
+      for (int j = 0; j < 6; ++j) L.m[i][j] = 0.0;
+    for (int j = 0; j < 6; ++j) L.m[j][j] = sqrt(fmax(H.m[j][j], 1.0e-30));
+  }
+}
+
+__device__ inline void cholesky6_solve(const M66 &L, const double *b, double *x) {
+  double y[6];
+  for (int i = 0; i < 6; ++i) {
+    double s = b[i];
+    for (int k = 0; k < i; ++k) s -= L.m[i][k] * y[k];
+    y[i] = s / L.m[i][i];
+  }
+  for (int i = 5; i >= 0; --i) {
+    double s = y[i];
+    for (int k = i + 1; k < 6; ++k) s -= L.m[k][i] * x[k];
+    x[i] = s / L.m[i][i];
+  }
+}
+
+__device__ inline void invert33(const double *H, double *Hi) {
+  const double a = H[0], b = H[1], c = H[2], d = H[3], e = H[4], f = H[5], g = H[6], h = H[7], i = H[8];
+  const double A = e * i - f * h, Bc = -(d * i - f * g), C = d * h - e * g;
+  const double det = a * A + b * Bc + c * C;
+  const double id = 1.0 / det;
+  Hi[0] = A * id;
+  Hi[1] = -(b * i - c * h) * id;
+  Hi[2] = (b * f - c * e) * id;
+  Hi[3] = Bc * id;
+  Hi[4] = (a * i - c * g) * id;
+  Hi[5] = -(a * f - c * d) * id;
+  Hi[6] = C * id;
+  Hi[7] = -(a * h - b * g) * id;
+  Hi[8] = (a * e - b * d) * id;
+}
 }  // namespace d2f

@@ -472,76 +472,6 @@ __global__ void k_tendon_energy(S s, double h2, double *E) {
 
 // ------------------------------------------------------------------------- affine bodies
 // reference implicit/affine.py:61-269
-struct M6 {
-  double m[6][6];
-};
-
-__device__ inline void add_nh_block(M6 &H, double A11, double A12, double A21, double A22, double Mu,
-                                    double La, double h2) {
-  const int idx[4] = {1, 2, 4, 5};
-  const NH<double> h = nh_eval(M2<double>{A11, A12, A21, A22}, Mu, La);
-  for (int k = 0; k < 4; ++k) {
-    M2<double> E = {0.0, 0.0, 0.0, 0.0};
-    if (k == 0) E.a = 1.0;
-    else if (k == 1) E.b = 1.0;
-    else if (k == 2) E.c = 1.0;
-    else E.d = 1.0;
-    const M2<double> col = nh_d2_dir(h, E, Mu, La);
-    const double cv[4] = {col.a, col.b, col.c, col.d};
-    for (int r = 0; r < 4; ++r) H.m[idx[r]][idx[k]] += h2 * cv[r];
-  }
-  const double a = A11 + A22, b = A21 - A12;
-  const double n = sqrt(a * a + b * b);
-  if (n > 1.0e-12) {
-    const double s = 0.7071067811865476 / n;
-    const double t[4] = {-b * s, -a * s, a * s, -b * s};
-    const M2<double> Tm = {t[0], t[1], t[2], t[3]};
-    const double lam_t = nh_quad(h, Tm, Mu, La);
-    if (lam_t < 0.0)
-      for (int r = 0; r < 4; ++r)
-        for (int c = 0; c < 4; ++c) H.m[idx[r]][idx[c]] -= h2 * lam_t * t[r] * t[c];
-  }
-}
-
-__device__ inline void chol6(const M6 &H, M6 &L) {
-  for (int i = 0; i < 6; ++i)
-    for (int j = 0; j < 6; ++j) L.m[i][j] = 0.0;
-  bool ok = true;
-  for (int j = 0; j < 6; ++j) {
-    double s = H.m[j][j];
-    for (int k = 0; k < j; ++k) s -= L.m[j][k] * L.m[j][k];
-    if (s <= 1.0e-9 * fabs(H.m[j][j]) || s <= 0.0) ok = false;
-    if (ok) {
-      const double d = sqrt(s);
-      L.m[j][j] = d;
-      for (int i = j + 1; i < 6; ++i) {
-        double v = H.m[i][j];
-        for (int k = 0; k < j; ++k) v -= L.m[i][k] * L.m[j][k];
-        L.m[i][j] = v / d;
-      }
-    }
-  }
-  if (!ok) {
-    for (int i = 0; i < 6; ++i)
-      for (int j = 0; j < 6; ++j) L.m[i][j] = 0.0;
-    for (int j = 0; j < 6; ++j) L.m[j][j] = sqrt(fmax(H.m[j][j], 1.0e-30));
-  }
-}
-
-__device__ inline void chol6_solve(const M6 &L, const double *b, double *x) {
-  double y[6];
-  for (int i = 0; i < 6; ++i) {
-    double s = b[i];
-    for (int k = 0; k < i; ++k) s -= L.m[i][k] * y[k];
-    y[i] = s / L.m[i][i];
-  }
-  for (int i = 5; i >= 0; --i) {
-    double s = y[i];
-    for (int k = i + 1; k < 6; ++k) s -= L.m[k][i] * x[k];
-    x[i] = s / L.m[i][i];
-  }
-}
-
 __global__ void k_affine_gather(S s) {
   GRID_LOOP(k, s.n_an) {
     if (k < s.n_an) {
@@ -572,7 +502,7 @@ __global__ void k_affine_reduce(S s, double h2) {
       double g[6], y[6], z[6], zy[6];
       const double add[6] = {0.0, P[0], P[1], 0.0, P[2], P[3]};
       for (int r = 0; r < 6; ++r) g[r] = s.ag_gq[6 * b + r] + h2 * add[r];
-      M6 H, L;
+      M66 H, L;
       for (int r = 0; r < 6; ++r)
         for (int c = 0; c < 6; ++c) H.m[r][c] = 0.0;
       for (int r = 0; r < 3; ++r)
@@ -580,11 +510,11 @@ __global__ void k_affine_reduce(S s, double h2) {
           H.m[r][c] = s.ag_Hx[9 * b + 3 * r + c];
           H.m[r + 3][c + 3] = s.ag_Hy[9 * b + 3 * r + c];
         }
-      add_nh_block(H, q[1], q[2], q[4], q[5], Mu, La, h2);
-      chol6(H, L);
+      affine_block(H, q[1], q[2], q[4], q[5], Mu, La, h2);
+      cholesky6(H, L);
       for (int r = 0; r < 6; ++r) y[r] = g[r] - s.ag_gq_prev[6 * b + r];
-      chol6_solve(L, g, z);
-      chol6_solve(L, y, zy);
+      cholesky6_solve(L, g, z);
+      cholesky6_solve(L, y, zy);
       double gz = 0, gp = 0, gzy = 0, yp = 0, yzy = 0;
       for (int r = 0; r < 6; ++r) {
         const double pp = s.ag_pq[6 * b + r];
@@ -696,26 +626,11 @@ __global__ void k_rigid_gather(S s) {
     }
   }
 }
-__device__ inline void inv33(const double *H, double *Hi) {
-  const double a = H[0], b = H[1], c = H[2], d = H[3], e = H[4], f = H[5], g = H[6], h = H[7], i = H[8];
-  const double A = e * i - f * h, Bc = -(d * i - f * g), C = d * h - e * g;
-  const double det = a * A + b * Bc + c * C;
-  const double id = 1.0 / det;
-  Hi[0] = A * id;
-  Hi[1] = -(b * i - c * h) * id;
-  Hi[2] = (b * f - c * e) * id;
-  Hi[3] = Bc * id;
-  Hi[4] = (a * i - c * g) * id;
-  Hi[5] = -(a * f - c * d) * id;
-  Hi[6] = C * id;
-  Hi[7] = -(a * h - b * g) * id;
-  Hi[8] = (a * e - b * d) * id;
-}
 __global__ void k_rigid_reduce(S s) {
   GRID_LOOP(b, s.n_rg) {
     if (b < s.n_rg) {
       double Hi[9];
-      inv33(s.rg_H + 9 * b, Hi);
+      invert33(s.rg_H + 9 * b, Hi);
       const double *g = s.rg_gq + 3 * b, *g0 = s.rg_gq_prev + 3 * b;
       const double y[3] = {g[0] - g0[0], g[1] - g0[1], g[2] - g0[2]};
       double z[3], zy[3];

@@ -1,0 +1,1200 @@
+// step_scene.hip — the scene-persistent path: one workgroup owns one environment and runs the
+// whole implicit step (every PNCG iteration of reference implicit/solver.py:252-451) inside one
+// launch. Per-environment scalars live in LDS; reductions are fixed-order block reductions; the
+// elastic scatter is colour-by-colour read-modify-write (no atomics, deterministic); contact
+// terms accumulate per boundary node in LDS; the node-edge broadphase is a per-scene uniform
+// grid built in LDS. An environment leaves the loop at the first convergence check it is
+// converged at — the state it leaves is the one the reference's global loop would leave it in,
+// because a converged environment's positions no longer change (implicit/kernels.py:375-434).
+//
+// Template R is the element arithmetic: double (float64 path) or float (mixed: float32 element
+// stress and curvature from float64 positions; float64 state, dot products and energies).
+#include "common.h"
+
+namespace d2f {
+
+using S = d2f_scene;
+using Pm = d2f_params;
+constexpr int MAXC = D2F_MAX_COLORS;
+constexpr int RED_K = 20;  // widest block reduction
+
+// ----------------------------------------------------------------------------- LDS state
+struct SceneScalars {
+  double gPg, g_p, g_Py, y_p, y_Py, beta, gTp, pHp, alpha, p_inf, clr_min, dE, dE0;
+  double clr_ipc, q_inf, sweep_beta, Eprev, Ener, tc;
+  double gx0, gy0, gh;          // broadphase grid origin and cell size
+  int gnx, gny, active, iters, pending, stop;
+};
+
+template <int TPB>
+struct Block {
+  static constexpr int NW = TPB / WAVE;
+  double *red;  // [NW * RED_K]
+
+  template <int K>
+  __device__ void sum(double (&v)[K]) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+    const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
+    __syncthreads();
+    if (l == 0)
+#pragma unroll
+      for (int k = 0; k < K; ++k) red[w * K + k] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double r = 0.0;
+      for (int i = 0; i < NW; ++i) r += red[i * K + k];
+      v[k] = r;
+    }
+  }
+  // component-wise minimum
+  template <int K>
+  __device__ void min(double (&v)[K]) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = wave_min(v[k]);
+    const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
+    __syncthreads();
+    if (l == 0)
+#pragma unroll
+      for (int k = 0; k < K; ++k) red[w * K + k] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double r = red[k];
+      for (int i = 1; i < NW; ++i) r = fmin(r, red[i * K + k]);
+      v[k] = r;
+    }
+  }
+  __device__ double sum1(double a) {
+    double v[1] = {a};
+    sum<1>(v);
+    return v[0];
+  }
+  __device__ double min1(double a) {
+    double v[1] = {a};
+    min<1>(v);
+    return v[0];
+  }
+  __device__ double max1(double a) { return -min1(-a); }
+  // exclusive prefix sum of one int per thread
+  __device__ int scan_excl(int v, int &total) {
+    int incl = v;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+      const int t = __shfl_up(incl, o, WAVE);
+      if ((int)(threadIdx.x % WAVE) >= o) incl += t;
+    }
+    int *ired = reinterpret_cast<int *>(red);
+    const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
+    __syncthreads();
+    if (l == WAVE - 1) ired[w] = incl;
+    __syncthreads();
+    int off = 0;
+    total = 0;
+    for (int i = 0; i < NW; ++i) {
+      if (i < w) off += ired[i];
+      total += ired[i];
+    }
+    return off + incl - v;
+  }
+};
+
+// ----------------------------------------------------------------------------- element access
+template <typename R>
+struct Elem {
+  int i0, i1, i2;
+  M2<R> B;
+  R Amu, Alam;
+};
+
+template <typename R>
+__device__ __forceinline__ Elem<R> load_elem(const S &s, int t);
+
+template <>
+__device__ __forceinline__ Elem<double> load_elem<double>(const S &s, int t) {
+  Elem<double> el;
+  el.i0 = s.sc_tri[3 * t];
+  el.i1 = s.sc_tri[3 * t + 1];
+  el.i2 = s.sc_tri[3 * t + 2];
+  const double4 b = reinterpret_cast<const double4 *>(s.sc_Bm)[t];
+  el.B = {b.x, b.y, b.z, b.w};
+  el.Amu = s.sc_Amu[t];
+  el.Alam = s.sc_Alam[t];
+  return el;
+}
+template <>
+__device__ __forceinline__ Elem<float> load_elem<float>(const S &s, int t) {
+  Elem<float> el;
+  el.i0 = s.sc_tri[3 * t];
+  el.i1 = s.sc_tri[3 * t + 1];
+  el.i2 = s.sc_tri[3 * t + 2];
+  const float4 b = reinterpret_cast<const float4 *>(s.sc_Bm32)[t];
+  el.B = {b.x, b.y, b.z, b.w};
+  el.Amu = s.sc_Amu32[t];
+  el.Alam = s.sc_Alam32[t];
+  return el;
+}
+
+// F of an element from float64 positions: edge vectors formed in float64, then rounded.
+template <typename R>
+__device__ __forceinline__ NH<R> elem_F(const Elem<R> &el, V2<double> p0, V2<double> p1, V2<double> p2) {
+  const M2<R> Ds = {R(p1.x - p0.x), R(p2.x - p0.x), R(p1.y - p0.y), R(p2.y - p0.y)};
+  return nh_eval(mat_mul(Ds, el.B), el.Amu, el.Alam);
+}
+
+template <typename R>
+__device__ __forceinline__ V2<double> up(V2<R> v) {
+  return {double(v.x), double(v.y)};
+}
+
+// ----------------------------------------------------------------------------- the kernel
+template <typename R, int TPB>
+struct Step {
+  const S &s;
+  const Pm &p;
+  SceneScalars &sc;
+  Block<TPB> blk;
+  double *bpos;   // 2 nb: boundary positions
+  double *bacc;   // 4 nb: boundary accumulators (grad x, grad y, diag x, diag y)
+  int *cstart;    // max_cells + 1
+  int *citem;     // nb
+  int *ccur;      // max_cells
+  int e, tid;
+  int n0, n1, b0, nb, k0, k1, t0, t1, c0, bc0, bc1, g0, g1, r0, r1, pe0, pe1, td0, td1;
+  int64_t eoff;
+
+  __device__ Step(const S &s_, const Pm &p_, SceneScalars &sc_, double *lds)
+      : s(s_), p(p_), sc(sc_) {
+    e = blockIdx.x;
+    tid = threadIdx.x;
+    blk.red = lds;
+    bpos = lds + Block<TPB>::NW * RED_K;
+    bacc = bpos + 2 * s.max_nb;
+    cstart = reinterpret_cast<int *>(bacc + 4 * s.max_nb);
+    citem = cstart + s.max_cells + 1;
+    ccur = citem + s.max_nb;
+    n0 = s.node_start[e];
+    n1 = s.node_start[e + 1];
+    b0 = s.bnd_start[e];
+    nb = s.bnd_start[e + 1] - b0;
+    k0 = s.n_e > 0 ? s.e_start[e] : 0;
+    k1 = s.n_e > 0 ? s.e_start[e + 1] : 0;
+    t0 = s.sc_tri_start[e];
+    t1 = s.sc_tri_start[e + 1];
+    c0 = e * (MAXC + 1);
+    bc0 = s.n_bc > 0 ? s.bc_start[e] : 0;
+    bc1 = s.n_bc > 0 ? s.bc_start[e + 1] : 0;
+    g0 = s.n_ag > 0 ? s.ag_start[e] : 0;
+    g1 = s.n_ag > 0 ? s.ag_start[e + 1] : 0;
+    r0 = s.n_rg > 0 ? s.rg_start[e] : 0;
+    r1 = s.n_rg > 0 ? s.rg_start[e + 1] : 0;
+    pe0 = s.n_pe > 0 ? s.pe_start[e] : 0;
+    pe1 = s.n_pe > 0 ? s.pe_start[e + 1] : 0;
+    td0 = s.n_td > 0 ? s.td_start[e] : 0;
+    td1 = s.n_td > 0 ? s.td_start[e + 1] : 0;
+    eoff = s.n_e > 0 ? s.excl_off[e] : 0;
+  }
+
+  __device__ double fr(int i) const { return s.free_[i] ? 1.0 : 0.0; }
+
+  // ------------------------------------------------------------------ IPC broadphase grid
+  // Boundary positions into LDS and a uniform grid over them (cell >= reach, so a pair within
+  // reach is found among the cells the edge's box +- reach overlaps).
+  __device__ void build_grid() {
+    double bb[4] = {BIG, BIG, BIG, BIG};  // min x, min y, -max x, -max y
+    for (int j = tid; j < nb; j += TPB) {
+      const V2<double> x = ld2(s.x, s.bnd[b0 + j]);
+      bpos[2 * j] = x.x;
+      bpos[2 * j + 1] = x.y;
+      bb[0] = fmin(bb[0], x.x);
+      bb[1] = fmin(bb[1], x.y);
+      bb[2] = fmin(bb[2], -x.x);
+      bb[3] = fmin(bb[3], -x.y);
+    }
+    blk.template min<4>(bb);
+    if (tid == 0) {
+      double h = p.ipc_reach;
+      const double wx = -bb[2] - bb[0], wy = -bb[3] - bb[1];
+      int nx = 1, ny = 1;
+      if (wx >= 0.0 && wy >= 0.0 && wx < 1.0e12 && wy < 1.0e12) {  // finite positions
+        for (int k = 0; k < 400; ++k) {
+          nx = (int)fmin(wx / h + 1.0, 1.0e9);
+          ny = (int)fmin(wy / h + 1.0, 1.0e9);
+          if ((long)nx * ny <= s.max_cells) break;
+          h *= 1.25;
+        }
+        if ((long)nx * ny > s.max_cells) nx = ny = 1;
+      }
+      sc.gx0 = bb[0];
+      sc.gy0 = bb[1];
+      sc.gh = h;
+      sc.gnx = nx;
+      sc.gny = ny;
+    }
+    __syncthreads();
+    const int nc = sc.gnx * sc.gny;
+    for (int c = tid; c <= nc; c += TPB) cstart[c] = 0;
+    __syncthreads();
+    for (int j = tid; j < nb; j += TPB) atomicAdd(&cstart[cell_of(bpos[2 * j], bpos[2 * j + 1])], 1);
+    __syncthreads();
+    // exclusive scan of the counts, in chunks per thread
+    const int chunk = (nc + TPB - 1) / TPB;
+    const int lo = tid * chunk, hi = min(lo + chunk, nc);
+    int loc = 0;
+    for (int c = lo; c < hi; ++c) loc += cstart[c];
+    int total;
+    int off = blk.scan_excl(loc, total);
+    __syncthreads();
+    for (int c = lo; c < hi; ++c) {
+      const int v = cstart[c];
+      cstart[c] = off;
+      off += v;
+    }
+    if (tid == 0) cstart[nc] = total;
+    __syncthreads();
+    // place items by an atomic cursor, then sort each cell so that the candidate order (and
+    // with it every per-edge partial sum) does not depend on the placement order
+    for (int c = tid; c < nc; c += TPB) ccur[c] = cstart[c];
+    __syncthreads();
+    for (int j = tid; j < nb; j += TPB) {
+      const int c = cell_of(bpos[2 * j], bpos[2 * j + 1]);
+      citem[atomicAdd(&ccur[c], 1)] = j;
+    }
+    __syncthreads();
+    for (int c = tid; c < nc; c += TPB)
+      for (int q = cstart[c] + 1; q < cstart[c + 1]; ++q) {
+        const int v = citem[q];
+        int r = q - 1;
+        while (r >= cstart[c] && citem[r] > v) {
+          citem[r + 1] = citem[r];
+          --r;
+        }
+        citem[r + 1] = v;
+      }
+    __syncthreads();
+  }
+
+  __device__ __forceinline__ int cell_of(double x, double y) const {
+    int ix = (int)((x - sc.gx0) / sc.gh);
+    int iy = (int)((y - sc.gy0) / sc.gh);
+    ix = ix < 0 ? 0 : (ix >= sc.gnx ? sc.gnx - 1 : ix);
+    iy = iy < 0 ? 0 : (iy >= sc.gny ? sc.gny - 1 : iy);
+    return iy * sc.gnx + ix;
+  }
+
+  // ------------------------------------------------------------------ IPC pair terms
+  // One barrier-and-friction term (reference implicit/ipc_contact.py:131-171); jp, ja, jb are
+  // scene-local boundary indices. mode 0 accumulates into bacc (LDS atomics), 1 returns the
+  // p^T H p share, 2 the energy share.
+  __device__ double term(int mode, double coef, int jp, int ja, int jb, int ip, int ia, int ib,
+                         double wa, double wb, double d, V2<double> n, double ddd, double mu,
+                         const double *pdir, double h2) {
+    const double dhat = p.dhat, kappa = p.ipc_kappa;
+    const double bp = barrier_bp(d, dhat);
+    const double f_n = -kappa * bp;
+    const V2<double> tn = {-n.y, n.x};
+    const V2<double> u = (ld2(s.x, ip) - ld2(s.xn, ip)) - wa * (ld2(s.x, ia) - ld2(s.xn, ia)) -
+                         wb * (ld2(s.x, ib) - ld2(s.xn, ib));
+    const double ut = dot(u, tn);
+    const double vt = fabs(ut) * p.inv_dt;
+    const double kt = mu * f_n * p.inv_dt / fmax(vt, p.epsv);
+    if (mode == 0) {
+      const double bpp = barrier_bpp(d, dhat);
+      const V2<double> g = h2 * ((-f_n) * n + (mu * f_n * (ut * p.inv_dt / fmax(vt, p.epsv))) * tn);
+      const V2<double> hd = {h2 * (kappa * bpp * n.x * n.x + kt * tn.x * tn.x),
+                             h2 * (kappa * bpp * n.y * n.y + kt * tn.y * tn.y)};
+      const double fp = fr(ip), fa = fr(ia), fb = fr(ib);
+      double *A = bacc + 4 * jp;
+      atomicAdd(A, coef * fp * g.x);
+      atomicAdd(A + 1, coef * fp * g.y);
+      atomicAdd(A + 2, coef * hd.x);
+      atomicAdd(A + 3, coef * hd.y);
+      if (wa != 0.0) {
+        double *Ba = bacc + 4 * ja;
+        atomicAdd(Ba, -coef * wa * fa * g.x);
+        atomicAdd(Ba + 1, -coef * wa * fa * g.y);
+        atomicAdd(Ba + 2, coef * wa * wa * hd.x);
+        atomicAdd(Ba + 3, coef * wa * wa * hd.y);
+      }
+      if (wb != 0.0) {
+        double *Bb = bacc + 4 * jb;
+        atomicAdd(Bb, -coef * wb * fb * g.x);
+        atomicAdd(Bb + 1, -coef * wb * fb * g.y);
+        atomicAdd(Bb + 2, coef * wb * wb * hd.x);
+        atomicAdd(Bb + 3, coef * wb * wb * hd.y);
+      }
+      return 0.0;
+    }
+    if (mode == 1) {
+      const double bpp = barrier_bpp(d, dhat);
+      const V2<double> dq = ld2(pdir, ip) - wa * ld2(pdir, ia) - wb * ld2(pdir, ib);
+      const double dn = dot(dq, n), dt_ = dot(dq, tn);
+      const double val = fmax(kappa * (bpp * dn * dn + bp * ddd), 0.0);
+      return coef * h2 * (val + kt * dt_ * dt_);
+    }
+    const double asl = p.epsv / p.inv_dt;
+    const double au = fabs(ut);
+    const double f0 = au < asl ? au * au / (2.0 * asl) : au - 0.5 * asl;
+    return coef * h2 * (kappa * barrier_b(d, dhat) + mu * f_n * f0);
+  }
+
+  // Every boundary node near each of the scene's edges (reference implicit/ipc_contact.py:175-296)
+  // using the grid built at the current positions. Returns (sum, clearance min, step min).
+  __device__ void ipc_pass(int mode, const double *pdir, double t_c, double h2, double &psum,
+                           double &cmin, double &amin) {
+    psum = 0.0;
+    cmin = BIG;
+    amin = BIG;
+    const double dhat = p.dhat, reach = p.ipc_reach;
+    const double dlo = 1.0e-9 * dhat, onset = 0.02 * dhat;
+    const double floor_ = mode == 3 ? 0.25 * dhat : 0.0;
+    const uint8_t *ex = s.excl + eoff;
+    for (int k = k0 + tid; k < k1; k += TPB) {
+      const int a = s.ea[k], b = s.eb[k], la = s.ela[k], lb = s.elb[k];
+      const V2<double> xa = {bpos[2 * la], bpos[2 * la + 1]}, xb = {bpos[2 * lb], bpos[2 * lb + 1]};
+      const V2<double> ab = xb - xa;
+      const double L2 = fmax(dot(ab, ab), 1.0e-30);
+      const double L = sqrt(L2);
+      int ix0 = (int)((fmin(xa.x, xb.x) - reach - sc.gx0) / sc.gh);
+      int ix1 = (int)((fmax(xa.x, xb.x) + reach - sc.gx0) / sc.gh);
+      int iy0 = (int)((fmin(xa.y, xb.y) - reach - sc.gy0) / sc.gh);
+      int iy1 = (int)((fmax(xa.y, xb.y) + reach - sc.gy0) / sc.gh);
+      ix0 = max(ix0, 0);
+      iy0 = max(iy0, 0);
+      ix1 = min(ix1, sc.gnx - 1);
+      iy1 = min(iy1, sc.gny - 1);
+      double mu_a = p.ipc_mu;
+      if (p.ipc_per_body && s.mu_node) mu_a = s.mu_node[a];
+      for (int iy = iy0; iy <= iy1; ++iy)
+        for (int ix = ix0; ix <= ix1; ++ix) {
+          const int c = iy * sc.gnx + ix;
+          for (int q = cstart[c]; q < cstart[c + 1]; ++q) {
+            const int jp = citem[q];
+            if (jp == la || jp == lb) continue;
+            const uint8_t *row = ex + (int64_t)jp * nb;
+            if (row[la] || row[lb]) continue;
+            const V2<double> xp = {bpos[2 * jp], bpos[2 * jp + 1]};
+            const V2<double> w = xp - xa;
+            const double tt = dot(w, ab) / L2;
+            V2<double> dvec = w - tt * ab;
+            if (tt <= 0.0) dvec = w;
+            else if (tt >= 1.0) dvec = xp - xb;
+            double d = len(dvec);
+            if (!(d < reach)) continue;
+            const int pn = s.bnd[b0 + jp];
+            if (mode == 1 || mode == 3) {
+              cmin = fmin(cmin, d);
+              double keep = fmax(p.ipc_gap * d, dhat - onset);
+              if (d < dhat) keep = d - fmin(2.0 * (dhat - d) + onset, 0.5 * d);
+              if (mode == 3) keep = fmin(d, fmax(p.ipc_gap * d, floor_));
+              amin = fmin(amin, accd(xp, xa, xb, ld2(pdir, pn), ld2(pdir, a), ld2(pdir, b), keep, t_c));
+              if (mode == 3) continue;
+            }
+            double mu_p = p.ipc_mu;
+            if (p.ipc_per_body && s.mu_node) {
+              const double mi = s.mu_node[pn], sm = mi + mu_a;
+              mu_p = sm > 0.0 ? 2.0 * mi * mu_a / sm : 0.0;
+            }
+            if (tt > 0.0 && tt < 1.0 && d < dhat) {
+              d = fmax(d, dlo);
+              double ddd = 0.0;
+              if (mode == 1) {
+                const V2<double> de = ld2(pdir, b) - ld2(pdir, a);
+                const V2<double> dw = ld2(pdir, pn) - ld2(pdir, a);
+                const double c0_ = cross(ab, w);
+                const double c1_ = cross(de, w) + cross(ab, dw);
+                const double c2_ = 2.0 * cross(de, dw);
+                const double L1 = dot(ab, de) / L;
+                const double Lq = (dot(de, de) - L1 * L1) / L;
+                const double s0 = c0_ / L;
+                const double s1 = (c1_ - s0 * L1) / L;
+                ddd = (c2_ - 2.0 * s1 * L1 - s0 * Lq) / L;
+                if (c0_ < 0.0) ddd = -ddd;
+              }
+              psum += term(mode, 1.0, jp, la, lb, pn, a, b, 1.0 - tt, tt, d, (1.0 / d) * dvec, ddd,
+                           mu_p, pdir, h2);
+            }
+            double coef_a = -s.ca[k];
+            if (tt <= 0.0) coef_a += 1.0;
+            double d_a = len(w);
+            if (coef_a != 0.0 && d_a < dhat) {
+              d_a = fmax(d_a, dlo);
+              const V2<double> n_a = (1.0 / d_a) * w;
+              double ddd = 0.0;
+              if (mode == 1) {
+                const V2<double> dw = ld2(pdir, pn) - ld2(pdir, a);
+                const double dn = dot(dw, n_a);
+                ddd = (dot(dw, dw) - dn * dn) / d_a;
+              }
+              psum += term(mode, coef_a, jp, la, lb, pn, a, b, 1.0, 0.0, d_a, n_a, ddd, mu_p, pdir, h2);
+            }
+            double coef_b = -s.cb[k];
+            if (tt >= 1.0) coef_b += 1.0;
+            const V2<double> wb_ = xp - xb;
+            double d_b = len(wb_);
+            if (coef_b != 0.0 && d_b < dhat) {
+              d_b = fmax(d_b, dlo);
+              const V2<double> n_b = (1.0 / d_b) * wb_;
+              double ddd = 0.0;
+              if (mode == 1) {
+                const V2<double> dw = ld2(pdir, pn) - ld2(pdir, b);
+                const double dn = dot(dw, n_b);
+                ddd = (dot(dw, dw) - dn * dn) / d_b;
+              }
+              psum += term(mode, coef_b, jp, la, lb, pn, a, b, 0.0, 1.0, d_b, n_b, ddd, mu_p, pdir, h2);
+            }
+          }
+        }
+    }
+  }
+
+  // ------------------------------------------------------------------ kinematic sweep
+  // reference implicit/ipc_contact.py:308-331, 456-483
+  __device__ void sweep() {
+    double q = 0.0;
+    for (int b = bc0 + tid; b < bc1; b += TPB) {
+      const int i = s.bc_idx[b];
+      const V2<double> d = ld2(s.bc_target, b) - ld2(s.x, i);
+      st2(s.bc_q, i, d);
+      q = fmax(q, len(d));
+    }
+    q = blk.max1(q);
+    if (tid == 0) sc.q_inf = q;
+    build_grid();
+    double ps, cm, am;
+    ipc_pass(3, s.bc_q, 1.0, p.h2, ps, cm, am);
+    double v[2] = {cm, am};
+    blk.template min<2>(v);
+    if (tid == 0) {
+      double beta = fmin(1.0, v[1]);
+      if (q > 0.0) beta = fmin(beta, 0.45 * p.ipc_reach / q);
+      sc.sweep_beta = beta;
+      sc.clr_ipc = v[0];
+    }
+    __syncthreads();
+    const double t = sc.sweep_beta;
+    for (int b = bc0 + tid; b < bc1; b += TPB) {
+      const int i = s.bc_idx[b];
+      const V2<double> tg = ld2(s.bc_target, b);
+      const V2<double> xi = t >= 1.0 ? tg : ld2(s.x, i) + t * (tg - ld2(s.x, i));
+      st2(s.x, i, xi);
+      st2(s.xhat, i, xi);
+    }
+    __syncthreads();
+  }
+
+  // ------------------------------------------------------------------ gradient + diagonal
+  __device__ void grad_diag() {
+    const double h2 = p.h2;
+    for (int i = n0 + tid; i < n1; i += TPB) {
+      const V2<double> g = ld2(s.grad, i);
+      st2(s.grad_prev, i, g);
+      const double m = s.mass[i], f = fr(i);
+      const V2<double> d = ld2(s.x, i) - ld2(s.xhat, i);
+      V2<double> gn = {m * d.x * f, m * d.y * f};
+      if (s.fext) gn = gn - h2 * ld2(s.fext, i);
+      st2(s.grad, i, gn);
+      st2(s.diag, i, V2<double>{m, m});
+    }
+    for (int j = tid; j < 4 * nb; j += TPB) bacc[j] = 0.0;
+    __syncthreads();
+    // elastic, colour by colour (reference implicit/kernels.py:86-128)
+    for (int c = 0; c < MAXC; ++c) {
+      const int ta = s.sc_cs[c0 + c], tb = s.sc_cs[c0 + c + 1];
+      if (ta == tb) continue;
+      for (int t = ta + tid; t < tb; t += TPB) {
+        const Elem<R> el = load_elem<R>(s, t);
+        const NH<R> h = elem_F(el, ld2(s.x, el.i0), ld2(s.x, el.i1), ld2(s.x, el.i2));
+        V2<R> g0, g1, g2, d0, d1, d2;
+        nh_grad(h, el.B, el.Amu, R(1), g0, g1, g2);
+        nh_diag(h, el.B, el.Amu, el.Alam, R(1), d0, d1, d2);
+        st2(s.grad, el.i0, ld2(s.grad, el.i0) + h2 * up(g0));
+        st2(s.grad, el.i1, ld2(s.grad, el.i1) + h2 * up(g1));
+        st2(s.grad, el.i2, ld2(s.grad, el.i2) + h2 * up(g2));
+        st2(s.diag, el.i0, ld2(s.diag, el.i0) + h2 * V2<double>{fmax(double(d0.x), 0.0), fmax(double(d0.y), 0.0)});
+        st2(s.diag, el.i1, ld2(s.diag, el.i1) + h2 * V2<double>{fmax(double(d1.x), 0.0), fmax(double(d1.y), 0.0)});
+        st2(s.diag, el.i2, ld2(s.diag, el.i2) + h2 * V2<double>{fmax(double(d2.x), 0.0), fmax(double(d2.y), 0.0)});
+      }
+      __syncthreads();
+    }
+    // rigid modes: the diagonal as it stands after elasticity (reference affine.py:541-542)
+    if (r1 > r0) {
+      for (int g = r0; g < r1; ++g)
+        for (int k = s.rn_start[g] + tid; k < s.rn_start[g + 1]; k += TPB) {
+          const int i = s.rn_node[k];
+          st2(s.diag_el, i, ld2(s.diag, i));
+        }
+    }
+    // contact between bodies and pressure: into the boundary accumulators
+    if (p.ipc_on && k1 > k0) {
+      build_grid();
+      double ps, cm, am;
+      ipc_pass(0, s.p, 0.0, h2, ps, cm, am);
+    }
+    for (int k = pe0 + tid; k < pe1; k += TPB) {
+      const int a = s.pe_a[k], b = s.pe_b[k];
+      const double pr = s.ch_p[s.pe_ch[k]];
+      const V2<double> tv = ld2(s.x, b) - ld2(s.x, a);
+      const V2<double> g = (-0.5 * h2 * pr) * V2<double>{tv.y, -tv.x};
+      const int ja = s.bloc[a], jb = s.bloc[b];
+      atomicAdd(bacc + 4 * ja, g.x);
+      atomicAdd(bacc + 4 * ja + 1, g.y);
+      atomicAdd(bacc + 4 * jb, g.x);
+      atomicAdd(bacc + 4 * jb + 1, g.y);
+    }
+    __syncthreads();
+    // colliders, friction and the accumulated contact terms, one thread per boundary node
+    for (int j = tid; j < nb; j += TPB) {
+      const int n = s.bnd[b0 + j];
+      const V2<double> x = ld2(s.x, n);
+      V2<double> g = ld2(s.grad, n), dg = ld2(s.diag, n);
+      for (int k = 0; k < s.n_col; ++k) {
+        V2<double> gg, dd;
+        if (collider_grad_diag(to_collider(s.col[k]), x, p.dhat, p.kappa, h2, 1, gg, dd)) {
+          g = g + gg;
+          dg = dg + dd;
+        }
+      }
+      if (p.friction && s.free_[n]) {
+        const V2<double> xn = ld2(s.xn, n);
+        for (int k = 0; k < s.n_col; ++k) {
+          const Collider c = to_collider(s.col[k]);
+          FricFrame f;
+          if (!friction_frame(c, x, xn, p.dhat, p.kappa, p.inv_dt, f)) continue;
+          g = g + (h2 * c.mu * f.f_n / fmax(f.vtm, p.epsv)) * f.vt;
+          const double kt = h2 * c.mu * f.f_n * p.inv_dt / fmax(f.vtm, p.epsv);
+          dg = dg + V2<double>{kt * f.n.y * f.n.y, kt * f.n.x * f.n.x};
+        }
+      }
+      g = g + V2<double>{bacc[4 * j], bacc[4 * j + 1]};
+      dg = dg + V2<double>{bacc[4 * j + 2], bacc[4 * j + 3]};
+      st2(s.grad, n, g);
+      st2(s.diag, n, dg);
+    }
+    __syncthreads();
+    // tendons: one thread walks the scene's cables (reference implicit/actuation.py:170-226)
+    if (td1 > td0 && tid == 0) {
+      for (int k = td0; k < td1; ++k) {
+        const int ja = s.ey_start[k], jz = s.ey_start[k + 1];
+        double L = 0.0;
+        for (int j = ja; j + 1 < jz; ++j) L += len(ld2(s.x, s.ey_node[j + 1]) - ld2(s.x, s.ey_node[j]));
+        s.td_L[k] = L;
+        double tau, ke;
+        if (s.td_stiff[k] > 0.0) {
+          const double ext = L - s.td_L0[k];
+          tau = ext > 0.0 ? s.td_stiff[k] * ext : 0.0;
+          ke = ext > 0.0 ? s.td_stiff[k] : 0.0;
+        } else {
+          tau = fmax(s.td_T[k], 0.0);
+          ke = 0.0;
+        }
+        s.td_tau[k] = tau;
+        s.td_keff[k] = ke;
+        for (int j = ja; j < jz; ++j) {
+          const int n = s.ey_node[j];
+          V2<double> gL = {0.0, 0.0}, geo = {0.0, 0.0};
+          const V2<double> xj = ld2(s.x, n);
+          if (j > ja) {
+            const V2<double> t = xj - ld2(s.x, s.ey_node[j - 1]);
+            const double l = fmax(len(t), 1.0e-12), il = 1.0 / l;
+            const V2<double> d = {t.x / l, t.y / l};
+            gL = gL + d;
+            geo = geo + V2<double>{(1.0 - d.x * d.x) * il, (1.0 - d.y * d.y) * il};
+          }
+          if (j + 1 < jz) {
+            const V2<double> t = ld2(s.x, s.ey_node[j + 1]) - xj;
+            const double l = fmax(len(t), 1.0e-12), il = 1.0 / l;
+            const V2<double> d = {t.x / l, t.y / l};
+            gL = gL - d;
+            geo = geo + V2<double>{(1.0 - d.x * d.x) * il, (1.0 - d.y * d.y) * il};
+          }
+          st2(s.grad, n, ld2(s.grad, n) + (h2 * tau) * gL);
+          st2(s.diag, n, ld2(s.diag, n) + V2<double>{h2 * (tau * geo.x + ke * gL.x * gL.x),
+                                                      h2 * (tau * geo.y + ke * gL.y * gL.y)});
+        }
+      }
+    }
+    __syncthreads();
+    // affine bodies: J^T g, J^T D J per group, then the slaved nodes leave the nodal iteration
+    // (reference implicit/affine.py:157-175)
+    for (int g = g0; g < g1; ++g) {
+      double v[18];
+#pragma unroll
+      for (int k = 0; k < 18; ++k) v[k] = 0.0;
+      for (int k = s.an_start[g] + tid; k < s.an_start[g + 1]; k += TPB) {
+        const int i = s.an_node[k];
+        const double w0 = s.an_w[3 * k], w1 = s.an_w[3 * k + 1], w2 = s.an_w[3 * k + 2];
+        const V2<double> gg = ld2(s.grad, i), dd = ld2(s.diag, i);
+        const double w[3] = {w0, w1, w2};
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          v[r] += gg.x * w[r];
+          v[3 + r] += gg.y * w[r];
+        }
+        const double ww[6] = {w0 * w0, w0 * w1, w0 * w2, w1 * w1, w1 * w2, w2 * w2};
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+          v[6 + r] += dd.x * ww[r];
+          v[12 + r] += dd.y * ww[r];
+        }
+      }
+      blk.template sum<18>(v);
+      if (tid == 0) {
+        for (int r = 0; r < 6; ++r) {
+          s.ag_gq_prev[6 * g + r] = s.ag_gq[6 * g + r];
+          s.ag_gq[6 * g + r] = v[r];
+        }
+        const int map[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+        for (int r = 0; r < 9; ++r) {
+          s.ag_Hx[9 * g + r] = v[6 + map[r]];
+          s.ag_Hy[9 * g + r] = v[12 + map[r]];
+        }
+      }
+      for (int k = s.an_start[g] + tid; k < s.an_start[g + 1]; k += TPB)
+        st2(s.grad, s.an_node[k], V2<double>{0.0, 0.0});
+    }
+    // rigid-mode coarse space (reference implicit/affine.py:405-453)
+    for (int g = r0; g < r1; ++g) {
+      double a[3] = {0.0, 0.0, 0.0};
+      for (int k = s.rn_start[g] + tid; k < s.rn_start[g + 1]; k += TPB) {
+        const int i = s.rn_node[k];
+        const double m = s.mass[i];
+        const V2<double> x = ld2(s.x, i);
+        a[0] += m * x.x;
+        a[1] += m * x.y;
+        a[2] += m;
+      }
+      blk.template sum<3>(a);
+      const double cx = a[0] / a[2], cy = a[1] / a[2];
+      double v[12];
+#pragma unroll
+      for (int k = 0; k < 12; ++k) v[k] = 0.0;
+      for (int k = s.rn_start[g] + tid; k < s.rn_start[g + 1]; k += TPB) {
+        const int i = s.rn_node[k];
+        const V2<double> x = ld2(s.x, i);
+        const V2<double> rr = {x.x - cx, x.y - cy};
+        const V2<double> gg = ld2(s.grad, i), gp = ld2(s.grad_prev, i);
+        const double m = s.mass[i];
+        const V2<double> d = ld2(s.diag, i) - ld2(s.diag_el, i);
+        const double dx = fmax(d.x, 0.0) + m, dy = fmax(d.y, 0.0) + m;
+        v[0] += gg.x;
+        v[1] += gg.y;
+        v[2] += rr.x * gg.y - rr.y * gg.x;
+        v[3] += gp.x;
+        v[4] += gp.y;
+        v[5] += rr.x * gp.y - rr.y * gp.x;
+        v[6] += dx;
+        v[7] += -dx * rr.y;
+        v[8] += dy;
+        v[9] += dy * rr.x;
+        v[10] += dx * rr.y * rr.y + dy * rr.x * rr.x;
+      }
+      blk.template sum<12>(v);
+      if (tid == 0) {
+        s.rg_acc[3 * g] = a[0];
+        s.rg_acc[3 * g + 1] = a[1];
+        s.rg_acc[3 * g + 2] = a[2];
+        for (int r = 0; r < 3; ++r) {
+          s.rg_gq[3 * g + r] = v[r];
+          s.rg_gq_prev[3 * g + r] = v[3 + r];
+        }
+        double *H = s.rg_H + 9 * g;
+        H[0] = v[6]; H[1] = 0.0; H[2] = v[7];
+        H[3] = 0.0; H[4] = v[8]; H[5] = v[9];
+        H[6] = v[7]; H[7] = v[9]; H[8] = v[10];
+      }
+    }
+    __syncthreads();
+  }
+
+  // ------------------------------------------------------------------ Dai-Kou direction
+  __device__ void direction(int it) {
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int i = n0 + tid; i < n1; i += TPB) {
+      const double f = fr(i);
+      V2<double> g = ld2(s.grad, i);
+      g = {g.x * f, g.y * f};
+      st2(s.grad, i, g);
+      const V2<double> dh = ld2(s.diag, i);
+      const double Px = f / fmax(dh.x, 1.0e-12), Py = f / fmax(dh.y, 1.0e-12);
+      const V2<double> pp = ld2(s.p, i);
+      const V2<double> yy = g - ld2(s.grad_prev, i);
+      const double Pg0 = Px * g.x, Pg1 = Py * g.y, Py0 = Px * yy.x, Py1 = Py * yy.y;
+      v[0] += g.x * Pg0 + g.y * Pg1;
+      v[1] += g.x * pp.x + g.y * pp.y;
+      v[2] += g.x * Py0 + g.y * Py1;
+      v[3] += yy.x * pp.x + yy.y * pp.y;
+      v[4] += yy.x * Py0 + yy.y * Py1;
+    }
+    blk.template sum<5>(v);
+    if (tid == 0) {
+      for (int g = g0; g < g1; ++g) affine_reduce(g, v);
+      for (int g = r0; g < r1; ++g) rigid_reduce(g, v);
+      sc.gPg = v[0];
+      sc.g_p = v[1];
+      sc.g_Py = v[2];
+      sc.y_p = v[3];
+      sc.y_Py = v[4];
+      double b = 0.0;
+      if (it > 0) {
+        const double ytp = v[3];
+        if (fabs(ytp) > 1.0e-30) b = v[2] / ytp - v[4] * v[1] / (ytp * ytp);
+      }
+      double t = -v[0] + b * v[1];
+      if (t >= 0.0) {
+        b = 0.0;
+        t = -v[0];
+      }
+      if (p.cg_guard > 0.0 && it > 0) {
+        const double hold = (p.ipc_on && sc.pending) ? sc.q_inf : 0.0;
+        if (hold <= 0.0 && v[3] < p.cg_guard * sc.alpha * sc.pHp) {
+          b = 0.0;
+          t = -v[0];
+        }
+      }
+      sc.beta = b;
+      sc.gTp = t;
+      for (int g = g0; g < g1; ++g)
+        for (int r = 0; r < 6; ++r) s.ag_pq[6 * g + r] = -s.ag_zq[6 * g + r] + b * s.ag_pq[6 * g + r];
+    }
+    __syncthreads();
+    const double bb = sc.beta;
+    for (int i = n0 + tid; i < n1; i += TPB) {
+      const double f = fr(i);
+      const V2<double> dh = ld2(s.diag, i), g = ld2(s.grad, i), pp = ld2(s.p, i);
+      const double px = -f / fmax(dh.x, 1.0e-12) * g.x + bb * pp.x;
+      const double py = -f / fmax(dh.y, 1.0e-12) * g.y + bb * pp.y;
+      st2(s.p, i, V2<double>{px * f, py * f});
+    }
+    __syncthreads();
+    for (int g = g0; g < g1; ++g) expand(g, s.ag_pq, s.p);
+    for (int g = r0; g < r1; ++g) {
+      const double *a = s.rg_acc + 3 * g, *z = s.rg_zq + 3 * g;
+      const double cx = a[0] / a[2], cy = a[1] / a[2];
+      for (int k = s.rn_start[g] + tid; k < s.rn_start[g + 1]; k += TPB) {
+        const int i = s.rn_node[k];
+        const V2<double> x = ld2(s.x, i);
+        const V2<double> rr = {x.x - cx, x.y - cy};
+        st2(s.p, i, ld2(s.p, i) - V2<double>{z[0] - z[2] * rr.y, z[1] + z[2] * rr.x});
+      }
+    }
+    __syncthreads();
+  }
+
+  __device__ void expand(int g, const double *v6, double *out) {
+    const double *v = v6 + 6 * g;
+    for (int k = s.an_start[g] + tid; k < s.an_start[g + 1]; k += TPB) {
+      const double *w = s.an_w + 3 * k;
+      st2(out, s.an_node[k], V2<double>{w[0] * v[0] + w[1] * v[1] + w[2] * v[2],
+                                        w[0] * v[3] + w[1] * v[4] + w[2] * v[5]});
+    }
+  }
+
+  // 6x6 block preconditioner of one rigid region (reference implicit/affine.py:179-205)
+  __device__ void affine_reduce(int b, double (&v)[5]) {
+    const double *q = s.ag_q + 6 * b;
+    const double Mu = s.ag_Mu[b], La = s.ag_La[b];
+    const NH<double> h = nh_eval(M2<double>{q[1], q[2], q[4], q[5]}, Mu, La);
+    const double P[4] = {Mu * h.F.a + h.c * h.G.a, Mu * h.F.b + h.c * h.G.b, Mu * h.F.c + h.c * h.G.c,
+                         Mu * h.F.d + h.c * h.G.d};
+    const double add[6] = {0.0, P[0], P[1], 0.0, P[2], P[3]};
+    double g[6], y[6], z[6], zy[6];
+    for (int r = 0; r < 6; ++r) g[r] = s.ag_gq[6 * b + r] + p.h2 * add[r];
+    M66 H, L;
+    for (int r = 0; r < 6; ++r)
+      for (int c = 0; c < 6; ++c) H.m[r][c] = 0.0;
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        H.m[r][c] = s.ag_Hx[9 * b + 3 * r + c];
+        H.m[r + 3][c + 3] = s.ag_Hy[9 * b + 3 * r + c];
+      }
+    affine_block(H, q[1], q[2], q[4], q[5], Mu, La, p.h2);
+    cholesky6(H, L);
+    for (int r = 0; r < 6; ++r) y[r] = g[r] - s.ag_gq_prev[6 * b + r];
+    cholesky6_solve(L, g, z);
+    cholesky6_solve(L, y, zy);
+    for (int r = 0; r < 6; ++r) {
+      const double pp = s.ag_pq[6 * b + r];
+      v[0] += g[r] * z[r];
+      v[1] += g[r] * pp;
+      v[2] += g[r] * zy[r];
+      v[3] += y[r] * pp;
+      v[4] += y[r] * zy[r];
+    }
+    for (int r = 0; r < 6; ++r) {
+      s.ag_gq[6 * b + r] = g[r];
+      s.ag_zq[6 * b + r] = z[r];
+    }
+  }
+
+  // reference implicit/affine.py:457-473
+  __device__ void rigid_reduce(int b, double (&v)[5]) {
+    double Hi[9];
+    invert33(s.rg_H + 9 * b, Hi);
+    const double *g = s.rg_gq + 3 * b, *g0_ = s.rg_gq_prev + 3 * b;
+    const double y[3] = {g[0] - g0_[0], g[1] - g0_[1], g[2] - g0_[2]};
+    double z[3], zy[3];
+    for (int r = 0; r < 3; ++r) {
+      z[r] = Hi[3 * r] * g[0] + Hi[3 * r + 1] * g[1] + Hi[3 * r + 2] * g[2];
+      zy[r] = Hi[3 * r] * y[0] + Hi[3 * r + 1] * y[1] + Hi[3 * r + 2] * y[2];
+    }
+    v[0] += g[0] * z[0] + g[1] * z[1] + g[2] * z[2];
+    v[2] += g[0] * zy[0] + g[1] * zy[1] + g[2] * zy[2];
+    v[4] += y[0] * zy[0] + y[1] * zy[1] + y[2] * zy[2];
+    for (int r = 0; r < 3; ++r) s.rg_zq[3 * b + r] = z[r];
+  }
+
+  // ------------------------------------------------------------------ curvature and caps
+  __device__ void curvature_caps(int it) {
+    const double h2 = p.h2;
+    // [0] pHp, [1] p_inf (as -max), [2] alpha min, [3] clr_min
+    double ph = 0.0, pinf = 0.0, amin = BIG, cmin = BIG;
+    for (int i = n0 + tid; i < n1; i += TPB) {
+      const V2<double> pp = ld2(s.p, i);
+      ph += s.mass[i] * dot(pp, pp);
+      pinf = fmax(pinf, len(pp));
+    }
+    // elastic curvature of the soft elements, inversion cap over all elements
+    const int tsoft = s.sc_cs[c0 + MAXC];
+    for (int t = t0 + tid; t < t1; t += TPB) {
+      const int i0 = s.sc_tri[3 * t], i1 = s.sc_tri[3 * t + 1], i2 = s.sc_tri[3 * t + 2];
+      const V2<double> x0 = ld2(s.x, i0), x1 = ld2(s.x, i1), x2 = ld2(s.x, i2);
+      const V2<double> q0 = ld2(s.p, i0), q1 = ld2(s.p, i1), q2 = ld2(s.p, i2);
+      if (t < tsoft) {
+        const Elem<R> el = load_elem<R>(s, t);
+        const NH<R> h = elem_F(el, x0, x1, x2);
+        const M2<R> dDs = {R(q1.x - q0.x), R(q2.x - q0.x), R(q1.y - q0.y), R(q2.y - q0.y)};
+        const M2<R> dF = mat_mul(dDs, el.B);
+        ph += h2 * fmax(double(nh_quad(h, dF, el.Amu, el.Alam)), 0.0);
+      }
+      amin = fmin(amin, inversion_root(x1 - x0, x2 - x0, q1 - q0, q2 - q0, p.area_shrink));
+    }
+    // colliders: curvature, clearance, friction curvature, step cap
+    for (int j = tid; j < nb; j += TPB) {
+      const int n = s.bnd[b0 + j];
+      const V2<double> x = ld2(s.x, n), pp = ld2(s.p, n);
+      for (int k = 0; k < s.n_col; ++k) {
+        const Collider c = to_collider(s.col[k]);
+        double dist;
+        ph += collider_pHp(c, x, pp, p.dhat, p.kappa, h2, 1, dist);
+        cmin = fmin(cmin, dist);
+      }
+      if (p.friction && s.free_[n]) {
+        const V2<double> xn = ld2(s.xn, n);
+        for (int k = 0; k < s.n_col; ++k) {
+          const Collider c = to_collider(s.col[k]);
+          FricFrame f;
+          if (!friction_frame(c, x, xn, p.dhat, p.kappa, p.inv_dt, f)) continue;
+          const double kt = h2 * c.mu * f.f_n * p.inv_dt / fmax(f.vtm, p.epsv);
+          const double pt = dot(pp, V2<double>{-f.n.y, f.n.x});
+          ph += kt * pt * pt;
+        }
+      }
+      for (int k = 0; k < s.n_col; ++k) amin = fmin(amin, collider_ccd(to_collider(s.col[k]), x, pp, p.ccd_eta));
+    }
+    // pressure (reference implicit/actuation.py:102-110)
+    for (int k = pe0 + tid; k < pe1; k += TPB) {
+      const V2<double> pa = ld2(s.p, s.pe_a[k]), pb = ld2(s.p, s.pe_b[k]);
+      ph += -h2 * s.ch_p[s.pe_ch[k]] * cross(pa, pb);
+    }
+    // contact between bodies: p^T H p, clearance and the ACCD step limit from alpha_max
+    double ipc_c = BIG;
+    if (p.ipc_on && k1 > k0) {
+      double ps, cm, am;
+      ipc_pass(1, s.p, p.alpha_max, h2, ps, cm, am);
+      ph += ps;
+      ipc_c = cm;
+      amin = fmin(amin, am);
+    }
+    double vs[1] = {ph};
+    blk.template sum<1>(vs);
+    double vm[4] = {-pinf, amin, cmin, ipc_c};
+    blk.template min<4>(vm);
+    if (tid == 0) {
+      double pH = vs[0];
+      const double p_inf = -vm[0];
+      // rigid regions' own curvature and the tendons' (single thread, in order)
+      for (int g = g0; g < g1; ++g) {
+        const double *q = s.ag_q + 6 * g, *pq = s.ag_pq + 6 * g;
+        const double Mu = s.ag_Mu[g], La = s.ag_La[g];
+        const NH<double> h = nh_eval(M2<double>{q[1], q[2], q[4], q[5]}, Mu, La);
+        pH += h2 * fmax(nh_quad(h, M2<double>{pq[1], pq[2], pq[4], pq[5]}, Mu, La), 0.0);
+      }
+      for (int k = td0; k < td1; ++k) {
+        double dL = 0.0;
+        for (int j = s.ey_start[k]; j + 1 < s.ey_start[k + 1]; ++j) {
+          const int na = s.ey_node[j], nbd = s.ey_node[j + 1];
+          const V2<double> t = ld2(s.x, nbd) - ld2(s.x, na);
+          const double l = fmax(len(t), 1.0e-12), il = 1.0 / l;
+          const V2<double> d = {t.x / l, t.y / l};
+          const V2<double> dp = ld2(s.p, nbd) - ld2(s.p, na);
+          const double along = dot(dp, d);
+          dL += along;
+          pH += h2 * s.td_tau[k] * (dot(dp, dp) - along * along) * il;
+        }
+        pH += h2 * s.td_keff[k] * dL * dL;
+      }
+      double a = fmin(p.alpha_max, vm[1]);
+      if (p.ipc_on && p_inf > 0.0) a = fmin(a, 0.45 * p.ipc_reach / p_inf);
+      sc.pHp = pH;
+      sc.p_inf = p_inf;
+      sc.clr_min = vm[2];
+      sc.clr_ipc = vm[3];
+      // alpha_finalize (reference implicit/kernels.py:375-423)
+      if (sc.active) {
+        if (pH > 0.0) a = fmin(a, -sc.gTp / pH);
+        if (vm[2] < p.near_factor * p.dhat && p_inf > 0.0) a = fmin(a, 0.5 * p.dhat / p_inf);
+        if (a <= 0.0) {
+          sc.active = 0;
+          a = 0.0;
+        } else if (p.tol_x > 0.0 && a * p_inf < p.tol_x) {
+          sc.active = 0;
+          sc.iters += 1;
+        } else {
+          const double d = -a * sc.gTp - 0.5 * a * a * pH;
+          sc.dE = d;
+          if (it == 0) sc.dE0 = fmax(d, 1.0e-300);
+          else if (d < p.eps * sc.dE0) sc.active = 0;
+          sc.iters += 1;
+        }
+      }
+      sc.alpha = a;
+    }
+    __syncthreads();
+  }
+
+  // ------------------------------------------------------------------ update and line search
+  __device__ void set_x(const double *from, double a) {
+    for (int i = n0 + tid; i < n1; i += TPB) st2(s.x, i, ld2(from, i) + a * ld2(s.p, i));
+  }
+
+  __device__ void affine_to(int which, double a) {  // 1: q += a pq ; 2: q = q0 + a pq
+    if (tid == 0)
+      for (int g = g0; g < g1; ++g)
+        for (int r = 0; r < 6; ++r) {
+          const int k = 6 * g + r;
+          s.ag_q[k] = (which == 1 ? s.ag_q[k] : s.ag_q0[k]) + a * s.ag_pq[k];
+        }
+    __syncthreads();
+    for (int g = g0; g < g1; ++g) expand(g, s.ag_q, s.x);
+    __syncthreads();
+  }
+
+  // total incremental potential of the scene (reference implicit/solver.py:453-498)
+  __device__ double energy() {
+    const double h2 = p.h2;
+    double E = 0.0;
+    for (int i = n0 + tid; i < n1; i += TPB) {
+      const V2<double> d = ld2(s.x, i) - ld2(s.xhat, i);
+      E += 0.5 * s.mass[i] * dot(d, d);
+      if (s.fext) E += -h2 * dot(ld2(s.fext, i), ld2(s.x, i) - ld2(s.xn, i));
+    }
+    const int tsoft = s.sc_cs[c0 + MAXC];
+    for (int t = t0 + tid; t < tsoft; t += TPB) {
+      const Elem<R> el = load_elem<R>(s, t);
+      const NH<R> h = elem_F(el, ld2(s.x, el.i0), ld2(s.x, el.i1), ld2(s.x, el.i2));
+      E += h2 * double(nh_psi(h, el.Amu, el.Alam));
+    }
+    for (int j = tid; j < nb; j += TPB) {
+      const int n = s.bnd[b0 + j];
+      const V2<double> x = ld2(s.x, n);
+      for (int k = 0; k < s.n_col; ++k) E += collider_energy(to_collider(s.col[k]), x, p.dhat, p.kappa, h2);
+      if (p.friction && s.free_[n]) {
+        const V2<double> xn = ld2(s.xn, n);
+        for (int k = 0; k < s.n_col; ++k) {
+          const Collider c = to_collider(s.col[k]);
+          FricFrame f;
+          if (!friction_frame(c, x, xn, p.dhat, p.kappa, p.inv_dt, f)) continue;
+          const double ut = f.vtm / p.inv_dt, a = p.epsv / p.inv_dt;
+          E += h2 * c.mu * f.f_n * (ut < a ? ut * ut / (2.0 * a) : ut - 0.5 * a);
+        }
+      }
+    }
+    for (int k = pe0 + tid; k < pe1; k += TPB) {
+      const int ch = s.pe_ch[k];
+      const V2<double> ref = ld2(s.ch_ref, ch);
+      E += -h2 * s.ch_p[ch] * 0.5 * cross(ld2(s.x, s.pe_a[k]) - ref, ld2(s.x, s.pe_b[k]) - ref);
+    }
+    if (p.ipc_on && k1 > k0) {
+      build_grid();
+      double ps, cm, am;
+      ipc_pass(2, s.p, 0.0, h2, ps, cm, am);
+      E += ps;
+    }
+    E = blk.sum1(E);
+    if (tid == 0) {
+      for (int g = g0; g < g1; ++g) {
+        const double *q = s.ag_q + 6 * g;
+        const NH<double> h = nh_eval(M2<double>{q[1], q[2], q[4], q[5]}, s.ag_Mu[g], s.ag_La[g]);
+        E += h2 * nh_psi(h, s.ag_Mu[g], s.ag_La[g]);
+      }
+      for (int k = td0; k < td1; ++k) {
+        double L = 0.0;
+        for (int j = s.ey_start[k]; j + 1 < s.ey_start[k + 1]; ++j)
+          L += len(ld2(s.x, s.ey_node[j + 1]) - ld2(s.x, s.ey_node[j]));
+        if (s.td_stiff[k] > 0.0) {
+          const double ext = L - s.td_L0[k];
+          if (ext > 0.0) E += 0.5 * h2 * s.td_stiff[k] * ext * ext;
+        } else {
+          E += h2 * fmax(s.td_T[k], 0.0) * L;
+        }
+      }
+      sc.Ener = E;
+    }
+    __syncthreads();
+    return sc.Ener;
+  }
+
+  __device__ void update() {
+    if (!sc.active) return;  // uniform: sc is shared
+    const double a = sc.alpha;
+    if (!p.line_search) {
+      set_x(s.x, a);
+      __syncthreads();
+      if (g1 > g0) affine_to(1, a);
+      return;
+    }
+    // energy backtracking gate (reference implicit/solver.py:403-425)
+    const double E0 = energy();
+    for (int i = n0 + tid; i < n1; i += TPB) st2(s.x0, i, ld2(s.x, i));
+    if (tid == 0)
+      for (int k = 6 * g0; k < 6 * g1; ++k) s.ag_q0[k] = s.ag_q[k];
+    __syncthreads();
+    for (int bt = 0; bt < p.n_backtrack; ++bt) {
+      set_x(s.x0, sc.alpha);
+      __syncthreads();
+      if (g1 > g0) affine_to(2, sc.alpha);
+      const double E1 = energy();
+      if (tid == 0 && E1 > E0 + p.ls_tol * (fabs(E0) + 1.0)) sc.alpha *= 0.5;
+      __syncthreads();
+    }
+    set_x(s.x0, sc.alpha);
+    __syncthreads();
+    if (g1 > g0) affine_to(2, sc.alpha);
+  }
+
+  // ------------------------------------------------------------------ the step
+  __device__ void run() {
+    for (int i = n0 + tid; i < n1; i += TPB) {
+      const V2<double> x = ld2(s.x, i);
+      st2(s.xn, i, x);
+      st2(s.xhat, i, x + (p.dt * p.drag) * ld2(s.v, i));
+    }
+    if (tid == 0) {
+      sc.pending = 0;
+      sc.q_inf = 0.0;
+      sc.sweep_beta = 1.0;
+      sc.alpha = 0.0;
+      sc.pHp = 0.0;
+    }
+    __syncthreads();
+    if (!p.ipc_on) {
+      for (int b = bc0 + tid; b < bc1; b += TPB) {
+        const int n = s.bc_idx[b];
+        const V2<double> t = ld2(s.bc_target, b);
+        st2(s.x, n, t);
+        st2(s.xhat, n, t);
+      }
+    } else if (bc1 > bc0) {
+      sweep();
+      if (tid == 0) sc.pending = !(sc.sweep_beta >= 1.0);
+    }
+    if (tid == 0) {
+      sc.dE0 = 0.0;
+      sc.active = 1;
+      sc.iters = 0;
+    }
+    for (int i = n0 + tid; i < n1; i += TPB) {
+      st2(s.p, i, V2<double>{0.0, 0.0});
+      st2(s.grad, i, V2<double>{0.0, 0.0});
+    }
+    __syncthreads();
+    for (int it = 0; it < p.iter_max; ++it) {
+      if (p.ipc_on && sc.pending) {
+        sweep();
+        if (tid == 0) {
+          if (sc.q_inf > 0.0) sc.active = 1;
+          if (it % 8 == 7) sc.pending = !(sc.sweep_beta >= 1.0);
+        }
+        __syncthreads();
+      }
+      if (p.conv_check && it && it % p.conv_check == 0 && !sc.active) break;
+      grad_diag();
+      direction(it);
+      curvature_caps(it);
+      update();
+      __syncthreads();
+    }
+    for (int i = n0 + tid; i < n1; i += TPB) st2(s.v, i, p.inv_dt * (ld2(s.x, i) - ld2(s.xn, i)));
+    if (tid == 0) {
+      s.active[e] = sc.active;
+      s.iters[e] = sc.iters;
+      s.alpha[e] = sc.alpha;
+      s.pHp[e] = sc.pHp;
+      s.p_inf[e] = sc.p_inf;
+      s.clr_min[e] = sc.clr_min;
+      s.clr_ipc[e] = sc.clr_ipc;
+      s.dE[e] = sc.dE;
+      s.dE0[e] = sc.dE0;
+      s.beta[e] = sc.beta;
+      s.gTp[e] = sc.gTp;
+      s.gPg[e] = sc.gPg;
+      s.g_p[e] = sc.g_p;
+      s.g_Py[e] = sc.g_Py;
+      s.y_p[e] = sc.y_p;
+      s.y_Py[e] = sc.y_Py;
+      s.q_inf[e] = sc.q_inf;
+      s.sweep_beta[e] = sc.sweep_beta;
+    }
+  }
+};
+
+template <typename R, int TPB>
+__global__ __launch_bounds__(TPB) void k_scene(S s, Pm p) {
+  extern __shared__ double lds[];
+  __shared__ SceneScalars sc;
+  Step<R, TPB> st(s, p, sc, lds);
+  st.run();
+}
+
+template <typename R, int TPB>
+static int launch(const S &s, const Pm &p, hipStream_t stream) {
+  const size_t lds = sizeof(double) * (Block<TPB>::NW * RED_K + 6 * (size_t)s.max_nb) +
+                     sizeof(int) * (2 * (size_t)s.max_cells + 1 + s.max_nb);
+  if (lds > 150 * 1024) {
+    set_error("d2f_step_scene: %zu bytes of LDS per scene (max_nb=%d, max_cells=%d) exceed the 150 KiB "
+              "budget; use the per-kernel path for scenes this large", lds, s.max_nb, s.max_cells);
+    return D2F_ECAP;
+  }
+  if (lds > 64 * 1024)
+    D2F_HIP(hipFuncSetAttribute((const void *)k_scene<R, TPB>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL((k_scene<R, TPB>), dim3(s.B), dim3(TPB), lds, stream, s, p);
+  D2F_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace d2f
+
+using namespace d2f;
+
+extern "C" int d2f_step_scene(const d2f_scene *s, const d2f_params *p, void *stream) {
+  D2F_CHECK(s && p, "d2f_step_scene: null scene or params");
+  D2F_CHECK(s->B > 0 && s->N > 0 && s->sc_tri && s->sc_cs && s->sc_tri_start,
+            "d2f_step_scene: the scene has no per-scene layout");
+  D2F_CHECK(s->max_cells >= 1 && s->max_nb >= 0, "d2f_step_scene: bad grid capacity");
+  D2F_CHECK(!p->ipc_on || (s->bloc && s->e_start), "d2f_step_scene: IPC needs bloc and e_start");
+  hipStream_t st = (hipStream_t)stream;
+  const int tpb = p->block_size > 0 ? p->block_size : 256;
+  if (p->precision == 1) {
+    D2F_CHECK(s->sc_Bm32 && s->sc_Amu32 && s->sc_Alam32, "d2f_step_scene: mixed precision needs float32 constants");
+    if (tpb == 512) return launch<float, 512>(*s, *p, st);
+    if (tpb == 1024) return launch<float, 1024>(*s, *p, st);
+    return launch<float, 256>(*s, *p, st);
+  }
+  D2F_CHECK(s->sc_Bm && s->sc_Amu && s->sc_Alam, "d2f_step_scene: float64 constants missing");
+  if (tpb == 512) return launch<double, 512>(*s, *p, st);
+  if (tpb == 1024) return launch<double, 1024>(*s, *p, st);
+  return launch<double, 256>(*s, *p, st);
+}

@@ -69,17 +69,22 @@ class DeviceSolver:
         self.lam = t(self.lam_np, F64)
         self.node2env_np = np.asarray(me.node2env, np.int64)
         self.node2env = t(self.node2env_np, I32)
+        self.block_size = 256
+        self._layout = None
         self.tri2env = t(np.asarray(me.tri2env), I32)
         self.node_start = t(np.asarray(me.node_starts), I32)
         self.bnd = t(np.asarray(me.boundary_nodes), I32)
         self.b2e = t(np.asarray(me.boundary2env), I32)
         self.bnd_start = t(np.asarray(me.boundary_starts), I32)
         self.free_np = np.ones((self.N, 2))
-        self.bc_idx_np = np.asarray(driven, np.int64)
+        drv = np.asarray(driven, np.int64)
+        # kinematic nodes grouped by environment (each one's target follows it)
+        self.bc_idx_np = drv[np.argsort(self.node2env_np[drv], kind="stable")] if len(drv) else drv
         self.free_np[self.bc_idx_np] = 0.0
         self.free = t((self.free_np[:, 0] > 0.5).astype(np.uint8), U8)
         self.bc_idx = t(self.bc_idx_np, I32)
         self.n_bc = len(self.bc_idx_np)
+        self.bc_start = t(np.searchsorted(self.node2env_np[self.bc_idx_np], np.arange(self.B + 1)), I32)
         self.bc_target = torch.zeros((max(self.n_bc, 1), 2), dtype=F64, device=self.dev)
         zb = lambda: torch.zeros(self.B, dtype=F64, device=self.dev)
         for k in ("gPg", "g_p", "g_Py", "y_p", "y_Py", "beta", "gTp", "pHp", "alpha", "p_inf",
@@ -141,7 +146,11 @@ class DeviceSolver:
         area = np.asarray(me.rest_area, np.float64)
         node, grp, w, Mu, La, env = [], [], [], [], [], []
         owner = np.full(self.N, -1)
-        for b, g in enumerate(groups):
+        genv = [int(np.asarray(me.tri2env)[np.asarray(g["elems"], np.int64)[0]]) for g in groups]
+        order = np.argsort(genv, kind="stable")          # groups by environment
+        self.ag_order = order
+        for b, gi in enumerate(order):
+            g = groups[gi]
             el = np.asarray(g["elems"], np.int64)
             nd = np.unique(tri[el].reshape(-1))
             if (owner[nd] >= 0).any():
@@ -165,22 +174,30 @@ class DeviceSolver:
         n = len(groups)
         t = self._t
         a = dict(n=n, node_np=np.asarray(node, np.int64), grp_np=np.asarray(grp, np.int64),
+                 elems=np.concatenate([np.asarray(g["elems"], np.int64) for g in groups]),
                  w_np=np.concatenate(w, 0), first=np.concatenate([[0], np.cumsum([len(v) for v in w])]),
                  slaved=owner >= 0)
         self.affine = a
         self.an_node, self.an_grp = t(a["node_np"], I32), t(a["grp_np"], I32)
         self.an_w = t(a["w_np"], F64)
         self.ag_Mu, self.ag_La, self.ag2env = t(Mu, F64), t(La, F64), t(env, I32)
+        self.ag_start = t(np.searchsorted(env, np.arange(self.B + 1)), I32)
+        self.an_start = t(a["first"], I32)
         z = lambda k: torch.zeros((n, k), dtype=F64, device=self.dev)
         self.ag_q, self.ag_q0, self.ag_pq, self.ag_gq, self.ag_gq_prev, self.ag_zq = (z(6) for _ in range(6))
         self.ag_Hx, self.ag_Hy = z(9), z(9)
         self.sync_affine(None)
 
     def sync_affine(self, regions=None):
-        """q from the current node positions by least squares (reference implicit/affine.py:331-344)."""
+        """q from the current node positions by least squares (reference implicit/affine.py:331-344).
+        regions: registration indices of the regions to re-sync (None: all)."""
         a = self.affine
         if a is None or (regions is not None and len(regions) == 0):
             return
+        if regions is not None:
+            pos = np.empty(len(self.ag_order), np.int64)
+            pos[self.ag_order] = np.arange(len(self.ag_order))
+            regions = [int(pos[r]) for r in regions]
         x = self.x.double().cpu().numpy()
         q = np.zeros((a["n"], 6)) if regions is None else self.ag_q.cpu().numpy()
         for b in (range(a["n"]) if regions is None else regions):
@@ -196,7 +213,10 @@ class DeviceSolver:
         self.x[torch.as_tensor(a["node_np"], device=self.dev)] = torch.as_tensor(xs, device=self.dev)
 
     def affine_state(self):
-        q = self.ag_q.cpu().numpy()
+        """(t, A) per rigid region in registration order."""
+        qs = self.ag_q.cpu().numpy()
+        q = np.empty_like(qs)
+        q[self.ag_order] = qs
         return q[:, [0, 3]], np.stack([q[:, 1:3], q[:, 4:6]], 1)
 
     def setup_rigid_modes(self, skip_nodes=None):
@@ -221,10 +241,21 @@ class DeviceSolver:
         n = len(keep)
         first = np.zeros(n, np.int64)
         first[grp[::-1]] = nodes[::-1]
+        # groups by environment, each group's nodes contiguous (in node order)
+        genv = self.node2env_np[first]
+        gorder = np.argsort(genv, kind="stable")
+        rank = np.empty(n, np.int64)
+        rank[gorder] = np.arange(n)
+        grp = rank[grp]
+        first = first[gorder]
+        o = np.lexsort((nodes, grp))
+        nodes, grp = nodes[o], grp[o]
         t = self._t
         self.rigid = dict(n=n)
         self.rn_node, self.rn_grp = t(nodes, I32), t(grp, I32)
         self.rg2env = t(self.node2env_np[first], I32)
+        self.rg_start = t(np.searchsorted(self.node2env_np[first], np.arange(self.B + 1)), I32)
+        self.rn_start = t(np.searchsorted(grp, np.arange(n + 1)), I32)
         z = lambda k: torch.zeros((n, k), dtype=F64, device=self.dev)
         self.rg_acc, self.rg_gq, self.rg_gq_prev, self.rg_zq, self.rg_H = z(3), z(3), z(3), z(3), z(9)
 
@@ -237,33 +268,130 @@ class DeviceSolver:
             self.fext.copy_(torch.as_tensor(f))
 
     def setup_pressure(self, ea, eb, ech, ch2env, cref):
+        """Pressure chambers, grouped by environment (chamber order is kept within one)."""
         t = self._t
+        ea, eb, ech = (np.asarray(v, np.int64) for v in (ea, eb, ech))
+        ch2env = np.asarray(ch2env, np.int64)
+        corder = np.argsort(ch2env, kind="stable")
+        rank = np.empty(len(corder), np.int64)
+        rank[corder] = np.arange(len(corder))
+        ech = rank[ech]
+        o = np.argsort(ech, kind="stable")
+        ea, eb, ech = ea[o], eb[o], ech[o]
+        self.ch_order = corder
         self.pressure = dict(n_pe=len(ea), n_ch=len(ch2env))
         self.pe_a, self.pe_b, self.pe_ch = t(ea, I32), t(eb, I32), t(ech, I32)
-        self.ch2env, self.ch_ref = t(ch2env, I32), t(np.asarray(cref, np.float64).reshape(-1, 2), F64)
+        self.ch2env = t(ch2env[corder], I32)
+        self.ch_ref = t(np.asarray(cref, np.float64).reshape(-1, 2)[corder], F64)
+        self.pe_start = t(np.searchsorted(ch2env[corder][ech], np.arange(self.B + 1)), I32)
         self.ch_p = torch.zeros(len(ch2env), dtype=F64, device=self.dev)
+        self._layout = None
 
     def set_pressure(self, p):
-        self.ch_p.copy_(torch.as_tensor(np.asarray(p, np.float64)))
+        self.ch_p.copy_(torch.as_tensor(np.asarray(p, np.float64)[self.ch_order]))
 
-    def setup_tendon(self, node, prv, nxt, tid, t2env, stiff, L0):
+    def setup_tendon(self, paths, t2env, stiff, L0):
+        """Tendons (eyelet node lists), grouped by environment."""
         t = self._t
+        t2env = np.asarray(t2env, np.int64)
+        torder = np.argsort(t2env, kind="stable")
+        self.td_order = torder
+        node, prv, nxt, tid = [], [], [], []
+        for i, k in enumerate(torder):
+            o, n = len(node), len(paths[k])
+            node += list(paths[k])
+            tid += [i] * n
+            prv += [-1] + list(range(o, o + n - 1))
+            nxt += list(range(o + 1, o + n)) + [-1]
         self.tendon = dict(n_ey=len(node), n_td=len(t2env))
         self.ey_node, self.ey_prv, self.ey_nxt, self.ey_tid = (t(v, I32) for v in (node, prv, nxt, tid))
-        self.td2env = t(t2env, I32)
-        self.td_stiff = t(stiff, F64)
-        self.td_L0 = t(L0, F64)
+        self.ey_start = t(np.searchsorted(np.asarray(tid), np.arange(len(t2env) + 1)), I32)
+        self.td2env = t(t2env[torder], I32)
+        self.td_start = t(np.searchsorted(t2env[torder], np.arange(self.B + 1)), I32)
+        self.td_stiff = t(np.asarray(stiff, np.float64)[torder], F64)
+        self.td_L0 = t(np.asarray(L0, np.float64)[torder], F64)
         n = len(t2env)
         self.td_T, self.td_L, self.td_tau, self.td_keff, self.td_dL = (
             torch.zeros(n, dtype=F64, device=self.dev) for _ in range(5))
 
     def set_tendon(self, tension, length):
-        self.td_T.copy_(torch.as_tensor(np.asarray(tension, np.float64)))
-        self.td_L0.copy_(torch.as_tensor(np.asarray(length, np.float64)))
+        self.td_T.copy_(torch.as_tensor(np.asarray(tension, np.float64)[self.td_order]))
+        self.td_L0.copy_(torch.as_tensor(np.asarray(length, np.float64)[self.td_order]))
 
     def clear_actuators(self):
         self.pressure = None
         self.tendon = None
+
+    # ------------------------------------------------------------------ scene-persistent layout
+    def build_layout(self, max_cells=4096):
+        """Per-scene triangle order (soft triangles by colour, then rigid ones), area-weighted
+        element constants in float64 and float32, and the per-scene offsets the
+        scene-persistent kernel reads."""
+        from . import layout as LY
+        me = self.me
+        tri = np.asarray(me.tri, np.int64)
+        T = len(tri)
+        color = np.zeros(T, np.int64)
+        tq = np.asarray(me.tri_quad)
+        cache = {}
+        for h in range(len(me.body_tri_range)):
+            t0, t1 = (int(v) for v in me.body_tri_range[h])
+            if t1 == t0:
+                continue
+            n0 = int(me.body_node_range[h][0])
+            loc = tri[t0:t1] - n0
+            if tq[t0] >= 0:
+                color[t0:t1] = LY.color_pixel_body(tq[t0:t1], int(me.body_grid[h][1]))
+            else:
+                key = (loc.shape, loc.tobytes())
+                if key not in cache:
+                    cache[key] = LY.color_mesh(loc, int(me.body_node_range[h][1]) - n0)
+                color[t0:t1] = cache[key]
+        rigid = np.zeros(T, bool)
+        if self.affine is not None:
+            rigid[self.affine["elems"]] = True
+        tenv = np.asarray(me.tri2env, np.int64)
+        M = LY.MAX_COLORS + 1
+        key = tenv * M + np.where(rigid, LY.MAX_COLORS, color)
+        perm = np.lexsort((np.arange(T), key))
+        ks = key[perm]
+        t = self._t
+        self.sc_perm = perm
+        self.sc_tri_start = t(np.searchsorted(ks, np.arange(self.B + 1) * M), I32)
+        self.sc_cs = t(np.searchsorted(ks, (np.arange(self.B)[:, None] * M + np.arange(M)[None, :]).reshape(-1)), I32)
+        self.sc_tri = t(tri[perm].astype(np.int32), I32)
+        Bm = np.asarray(me.D_m_inv, np.float64).reshape(-1, 4)[perm]
+        area = np.asarray(me.rest_area, np.float64)[perm]
+        Amu, Alam = area * self.mu_np[perm], area * self.lam_np[perm]
+        self.sc_Bm, self.sc_Amu, self.sc_Alam = t(Bm, F64), t(Amu, F64), t(Alam, F64)
+        self.sc_Bm32, self.sc_Amu32 = t(Bm.astype(np.float32), torch.float32), t(Amu.astype(np.float32), torch.float32)
+        self.sc_Alam32 = t(Alam.astype(np.float32), torch.float32)
+        bl = np.full(self.N, -1, np.int64)
+        bn = np.asarray(me.boundary_nodes, np.int64)
+        bl[bn] = np.arange(len(bn)) - np.asarray(me.boundary_starts)[np.asarray(me.boundary2env)]
+        self.bloc = t(bl, I32)
+        if self.ipc is not None:
+            self.e_start = t(np.searchsorted(self.e2env.cpu().numpy(), np.arange(self.B + 1)), I32)
+        self.max_nb = int(np.max(np.asarray(me.boundary_per_env))) if self.B else 0
+        if self.pressure is not None:
+            ends = np.concatenate([self.pe_a.cpu().numpy(), self.pe_b.cpu().numpy()])
+            if (bl[ends] < 0).any():
+                raise ValueError("a pressure edge has an end that is not a boundary node")
+        self.max_cells = int(max_cells)
+        self._layout = True
+
+    def refresh_moduli(self):
+        """Re-upload element moduli after mu_np / lam_np changed (both layouts)."""
+        self.mu = self._t(self.mu_np, F64)
+        self.lam = self._t(self.lam_np, F64)
+        if self._layout is not None:
+            perm = self.sc_perm
+            area = np.asarray(self.me.rest_area, np.float64)[perm]
+            Amu, Alam = area * self.mu_np[perm], area * self.lam_np[perm]
+            self.sc_Amu.copy_(torch.as_tensor(Amu))
+            self.sc_Alam.copy_(torch.as_tensor(Alam))
+            self.sc_Amu32.copy_(torch.as_tensor(Amu.astype(np.float32)))
+            self.sc_Alam32.copy_(torch.as_tensor(Alam.astype(np.float32)))
 
     # ------------------------------------------------------------------ ABI structs
     def _ptr(self, tensor):
@@ -304,6 +432,21 @@ class DeviceSolver:
             s.n_rn, s.n_rg = len(self.rn_node), self.rigid["n"]
             for k in ("rn_node", "rn_grp", "rg2env", "rg_acc", "rg_gq", "rg_gq_prev", "rg_zq", "rg_H"):
                 setattr(s, k, p(k))
+        if self._layout is not None:
+            for k in ("sc_tri_start", "sc_cs", "sc_tri", "sc_Bm", "sc_Amu", "sc_Alam", "sc_Bm32",
+                      "sc_Amu32", "sc_Alam32", "bloc", "bc_start"):
+                setattr(s, k, p(k))
+            if self.ipc is not None:
+                s.e_start = p("e_start")
+            if self.affine is not None:
+                s.ag_start, s.an_start = p("ag_start"), p("an_start")
+            if self.rigid is not None:
+                s.rg_start, s.rn_start = p("rg_start"), p("rn_start")
+            if self.pressure is not None:
+                s.pe_start = p("pe_start")
+            if self.tendon is not None:
+                s.td_start, s.ey_start = p("td_start"), p("ey_start")
+            s.max_nb, s.max_cells = self.max_nb, self.max_cells
         s.n_col = len(self.colliders)
         for i, c in enumerate(self.colliders):
             s.col[i].kind = int(c["kind"])
@@ -327,6 +470,7 @@ class DeviceSolver:
             q.ipc_per_body = int(self.ipc["per_body"])
         q.rigid_modes_on = int(self.rigid is not None)
         q.precision = 1 if self.precision == "mixed" else 0
+        q.block_size = int(self.block_size)
         return q
 
     # ------------------------------------------------------------------ stepping
@@ -338,6 +482,8 @@ class DeviceSolver:
     def step(self):
         """One implicit step of every environment (reference implicit/solver.py:433-451)."""
         L = _lib.lib()
+        if self.path == "scene" and self._layout is None:
+            self.build_layout()
         s, q = self.scene_struct(), self.params_struct()
         stream = C.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
         if self.path == "plain":

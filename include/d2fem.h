@@ -34,6 +34,7 @@ extern "C" {
 #define D2F_ECAP (-3)     /* a scene exceeds a compiled capacity of the scene-persistent path */
 
 #define D2F_MAX_COLLIDERS 8
+#define D2F_MAX_COLORS 24
 
 /* A rigid collider (reference sim_model.py:167-174; kinds core/objects.py:22-24):
  * 0 disc, 1 ellipse, 2 rounded box, 3 rounded triangle, 4 half-space (rx, ry = unit normal). */
@@ -100,6 +101,24 @@ typedef struct d2f_scene {
   /* colliders */
   int32_t n_col, pad_;
   d2f_collider col[D2F_MAX_COLLIDERS];
+  /* ---- scene-persistent path: per-scene layout (all grouped by environment) ---- */
+  /* triangles of scene e, soft ones first by colour (no two triangles of one colour share a
+   * node), then the rigid-region ones: colour c of scene e is [sc_cs[e*(D2F_MAX_COLORS+1)+c],
+   * sc_cs[e*(D2F_MAX_COLORS+1)+c+1]), the rigid tail starts at entry D2F_MAX_COLORS and ends at
+   * sc_tri_start[e+1]. Element constants are premultiplied by the rest area. */
+  const int32_t *sc_tri_start;        /* B+1 */
+  const int32_t *sc_cs;               /* B*(D2F_MAX_COLORS+1) */
+  const int32_t *sc_tri;              /* 3T */
+  const double *sc_Bm, *sc_Amu, *sc_Alam;   /* 4T, T, T (float64 path) */
+  const float *sc_Bm32, *sc_Amu32, *sc_Alam32; /* 4T, T, T (mixed path) */
+  const int32_t *bloc;                /* N: index among its scene's boundary nodes, or -1 */
+  const int32_t *e_start;             /* B+1 boundary edges per scene */
+  const int32_t *bc_start;            /* B+1 (bc_idx grouped by environment) */
+  const int32_t *ag_start, *an_start; /* B+1 groups per scene, n_ag+1 slaved nodes per group */
+  const int32_t *rg_start, *rn_start; /* B+1, n_rg+1 */
+  const int32_t *pe_start;            /* B+1 pressure edges per scene */
+  const int32_t *td_start, *ey_start; /* B+1 tendons per scene, n_td+1 eyelets per tendon */
+  int32_t max_nb, max_cells;          /* largest per-scene boundary count; grid cell cap */
 } d2f_scene;
 
 /* Solver options and step constants (reference PNCG2DSolver.__init__, implicit/solver.py:97-226,
@@ -112,7 +131,7 @@ typedef struct d2f_params {
   double ipc_kappa, ipc_mu, ipc_reach, ipc_gap;
   int32_t iter_max, conv_check, friction, line_search, n_backtrack;
   int32_t ipc_on, ipc_per_body, rigid_modes_on, precision; /* precision: 0 float64, 1 mixed */
-  int32_t pad_;
+  int32_t block_size;                        /* scene path: threads per workgroup (256/512/1024) */
 } d2f_params;
 
 /* ---- library ---------------------------------------------------------------------------- */
@@ -158,6 +177,13 @@ int d2f_step_plain(const d2f_scene *s, const d2f_params *prm, void *stream);
  * reach); uses s->x0 as scratch. Replaces IPCContact.min_distance (implicit/ipc_contact.py:441-447),
  * the "bodies must start apart" check of ImplicitIntegrator._build. */
 int d2f_ipc_min_distance(const d2f_scene *s, const d2f_params *prm, void *stream);
+
+/* The same step on the scene-persistent path: one workgroup per environment runs the whole
+ * iteration loop in one launch, with no host synchronisation; prm->precision 0 = float64,
+ * 1 = mixed (float32 element arithmetic, float64 state and reductions); prm->block_size picks
+ * the workgroup size. Needs the sc_* layout fields. Returns D2F_ECAP when a scene's boundary
+ * does not fit the LDS budget (use d2f_step_plain for it). */
+int d2f_step_scene(const d2f_scene *s, const d2f_params *prm, void *stream);
 
 #ifdef __cplusplus
 }

@@ -99,7 +99,25 @@ def main(ref_root):
     np.savez_compressed(HERE / "actuator_fixtures.npz", labels=lab, ids=np.asarray(ids),
                         chamber_a=ea, chamber_b=eb, exterior_a=xa, exterior_b=xb,
                         tendon_points=pts, tendon_path=path)
-    print("wrote", HERE / "mesh_fixtures.npz", "and", HERE / "actuator_fixtures.npz")
+    # the reference's independent float64 Newton oracle (tests/reference_static.py): elastic
+    # energy and gradient on a randomly deformed blob, and the static equilibrium of a small
+    # cantilever (left column fixed) under a dead load
+    rs = _load(ref / "tests" / "reference_static.py", "ref_static")
+    me = mesh.build_scene([dict(b) for b in scenes()["blob"]])
+    rng = np.random.default_rng(11)
+    x = me.rest_pos + 2e-4 * rng.standard_normal(me.rest_pos.shape)
+    E, g, _ = rs.elastic(x, np.asarray(me.tri), np.asarray(me.D_m_inv), np.asarray(me.rest_area),
+                         np.asarray(me.mu), np.asarray(me.lam), hess=False)
+    cant = [dict(mask_xy=np.ones((4, 20), bool).T.copy(), size=(0.02, 0.004), center=(0.0, 0.0),
+                 young=1e5, nu=0.4, rho=1000.0, env=0)]
+    mc = mesh.build_scene(cant)
+    fixed = np.nonzero(np.abs(mc.rest_pos[:, 0] - mc.rest_pos[:, 0].min()) < 1e-12)[0]
+    fext = np.zeros_like(mc.rest_pos)
+    fext[:, 1] = -np.asarray(mc.mass) * 9.81 * 20.0
+    xs, res = rs.solve_static(mc, fixed, fext=fext, ramps=4, tol=1e-12)
+    np.savez_compressed(HERE / "static_fixtures.npz", blob_x=x, blob_E=np.float64(E), blob_grad=g,
+                        cant_fixed=fixed, cant_fext=fext, cant_x=xs, cant_residual=np.float64(res))
+    print("wrote", HERE / "mesh_fixtures.npz", HERE / "actuator_fixtures.npz", HERE / "static_fixtures.npz")
 
 
 if __name__ == "__main__":

@@ -1,4 +1,5 @@
-"""GPU parity of the per-kernel float64 path against the float64 CPU oracle.
+"""GPU parity of both device paths (per-kernel "plain" and scene-persistent "scene") against
+the float64 CPU oracle.
 
 Per-kernel entry points are called through the C ABI on identical inputs and must agree to
 1e-12 relative (float64, different summation order only). Whole steps are run with a fixed
@@ -100,14 +101,14 @@ def test_elastic_curvature_energy_caps_hvp(deformed):
     assert _rel(y.cpu().numpy(), yo) < 1e-12
 
 
-def _run_pair(make, n_steps, dt, path, kw, each=None):
+def _run_pair(make, n_steps, dt, path, kw, each=None, precision="double"):
     mo, mg = make(), make()
     if isinstance(mo, tuple):
         (mo, po), (mg, pg) = mo, mg
     else:
         po = pg = None
     oi = OracleIntegrator(**kw)
-    gi = ImplicitIntegrator(path=path, precision="double", **kw)
+    gi = ImplicitIntegrator(path=path, precision=precision, **kw)
     for k in range(1, n_steps + 1):
         if each is not None:
             each(mo, po, k)
@@ -117,7 +118,7 @@ def _run_pair(make, n_steps, dt, path, kw, each=None):
     return oi, gi, mo, mg
 
 
-@pytest.mark.parametrize("path", ["plain"])
+@pytest.mark.parametrize("path", ["plain", "scene"])
 def test_strip_steps_fixed_iterations(path):
     """Strip on the ground: gravity, ground barrier + friction, IPC self-contact, rigid modes."""
     kw = dict(max_iter=40, eps=0.0, conv_check=0)
@@ -131,7 +132,7 @@ def test_strip_steps_fixed_iterations(path):
     assert (st["iters"] == oi.stats()["iters"]).all()
 
 
-@pytest.mark.parametrize("path", ["plain"])
+@pytest.mark.parametrize("path", ["plain", "scene"])
 def test_strip_steps_converged(path):
     """Default tolerances (eps = 1e-10, convergence poll every 8 iterations)."""
     oi, gi, mo, mg = _run_pair(lambda: strip(n_env=2), 8, 2e-3, path, {})
@@ -141,7 +142,7 @@ def test_strip_steps_converged(path):
     assert err < 1e-7
 
 
-@pytest.mark.parametrize("path", ["plain"])
+@pytest.mark.parametrize("path", ["plain", "scene"])
 def test_grip_steps_fixed_iterations(path):
     """Pads closing on a rigid block: IPC with friction between bodies, affine body, kinematic
     sweep, collider contact."""
@@ -154,3 +155,24 @@ def test_grip_steps_fixed_iterations(path):
     err = np.abs(mg.get_x() - oi.solver.x).max() / 8e-3
     print(f"grip {path}: max |dx| / size = {err:.3e}")
     assert err < 1e-9
+
+
+def test_scene_mixed_strip_close_to_float64():
+    """Mixed precision (float32 element arithmetic) on the strip: positions within 1e-5 of the
+    body size of the float64 oracle after the same fixed iterations."""
+    kw = dict(max_iter=40, eps=0.0, conv_check=0)
+    oi, gi, mo, mg = _run_pair(lambda: strip(n_env=2), 6, 2e-3, "scene", kw, precision="mixed")
+    err = np.abs(mg.get_x() - oi.solver.x).max() / 0.054
+    print(f"strip mixed: max |dx| / size = {err:.3e}")
+    assert err < 1e-5
+
+
+def test_scene_mixed_grip_close_to_float64():
+    kw = dict(max_iter=60, eps=0.0, conv_check=0)
+
+    def each(m, pads, k):
+        close_jaws(m, pads, k, n_close=8)
+    oi, gi, mo, mg = _run_pair(lambda: grip(n_env=2), 12, 1e-3, "scene", kw, each, precision="mixed")
+    err = np.abs(mg.get_x() - oi.solver.x).max() / 8e-3
+    print(f"grip mixed: max |dx| / size = {err:.3e}")
+    assert err < 1e-5
