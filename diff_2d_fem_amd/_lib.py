@@ -48,6 +48,7 @@ _SCENE_FIELDS = (
                         "sc_Amu32", "sc_Alam32", "bloc", "e_start", "bc_start", "ag_start", "an_start",
                         "rg_start", "rn_start", "pe_start", "td_start", "ey_start")]
     + [("max_nb", C.c_int32), ("max_cells", C.c_int32)]
+    + [("excl_bits", P), ("excl_bits_off", P), ("excl_bits_lds", C.c_int32), ("max_pairs", C.c_int32)]
 )
 
 

@@ -24,6 +24,7 @@ struct SceneScalars {
   double clr_ipc, q_inf, sweep_beta, Eprev, Ener, tc;
   double gx0, gy0, gh;          // broadphase grid origin and cell size
   int gnx, gny, active, iters, pending, stop;
+  int npairs, plist, excl_lds;  // candidate pairs of the last build, list valid, bitset in LDS
 };
 
 template <int TPB>
@@ -160,6 +161,8 @@ struct Step {
   int *cstart;    // max_cells + 1
   int *citem;     // nb
   int *ccur;      // max_cells
+  uint32_t *xbits;  // exclusion bitset (excl_bits_lds words)
+  int *plist;     // max_pairs packed (edge - k0) << 16 | jp
   int e, tid;
   int n0, n1, b0, nb, k0, k1, t0, t1, c0, bc0, bc1, g0, g1, r0, r1, pe0, pe1, td0, td1;
   int64_t eoff;
@@ -174,6 +177,8 @@ struct Step {
     cstart = reinterpret_cast<int *>(bacc + 4 * s.max_nb);
     citem = cstart + s.max_cells + 1;
     ccur = citem + s.max_nb;
+    xbits = reinterpret_cast<uint32_t *>(ccur + s.max_cells);
+    plist = reinterpret_cast<int *>(xbits + s.excl_bits_lds);
     n0 = s.node_start[e];
     n1 = s.node_start[e + 1];
     b0 = s.bnd_start[e];
@@ -339,114 +344,166 @@ struct Step {
     return coef * h2 * (kappa * barrier_b(d, dhat) + mu * f_n * f0);
   }
 
-  // Every boundary node near each of the scene's edges (reference implicit/ipc_contact.py:175-296)
-  // using the grid built at the current positions. Returns (sum, clearance min, step min).
+  // Is boundary node jp excluded against boundary node j of this scene (mesh neighbours,
+  // nodes of one rigid piece)? From the LDS bitset when it was staged, else the byte matrix.
+  __device__ __forceinline__ bool excluded(int jp, int j) const {
+    if (sc.excl_lds) {
+      const unsigned idx = (unsigned)jp * (unsigned)nb + (unsigned)j;
+      return (xbits[idx >> 5] >> (idx & 31u)) & 1u;
+    }
+    return s.excl[eoff + (int64_t)jp * nb + j] != 0;
+  }
+
+  // One node-edge pair (reference implicit/ipc_contact.py:175-296, the body of the query loop):
+  // edge k (global), scene-local boundary node jp that is neither end nor excluded.
+  __device__ void pair_eval(int mode, int k, int jp, const double *pdir, double t_c, double h2,
+                            double &psum, double &cmin, double &amin) {
+    const double dhat = p.dhat, reach = p.ipc_reach;
+    const double dlo = 1.0e-9 * dhat, onset = 0.02 * dhat;
+    const int a = s.ea[k], b = s.eb[k], la = s.ela[k], lb = s.elb[k];
+    const V2<double> xa = {bpos[2 * la], bpos[2 * la + 1]}, xb = {bpos[2 * lb], bpos[2 * lb + 1]};
+    const V2<double> ab = xb - xa;
+    const double L2 = fmax(dot(ab, ab), 1.0e-30);
+    const double L = sqrt(L2);
+    const V2<double> xp = {bpos[2 * jp], bpos[2 * jp + 1]};
+    const V2<double> w = xp - xa;
+    const double tt = dot(w, ab) / L2;
+    V2<double> dvec = w - tt * ab;
+    if (tt <= 0.0) dvec = w;
+    else if (tt >= 1.0) dvec = xp - xb;
+    double d = len(dvec);
+    if (!(d < reach)) return;
+    const int pn = s.bnd[b0 + jp];
+    if (mode == 1 || mode == 3) {
+      cmin = fmin(cmin, d);
+      double keep = fmax(p.ipc_gap * d, dhat - onset);
+      if (d < dhat) keep = d - fmin(2.0 * (dhat - d) + onset, 0.5 * d);
+      if (mode == 3) keep = fmin(d, fmax(p.ipc_gap * d, 0.25 * dhat));
+      amin = fmin(amin, accd(xp, xa, xb, ld2(pdir, pn), ld2(pdir, a), ld2(pdir, b), keep, t_c));
+      if (mode == 3) return;
+    }
+    double mu_p = p.ipc_mu;
+    if (p.ipc_per_body && s.mu_node) {
+      const double mi = s.mu_node[pn], mj = s.mu_node[a], sm = mi + mj;
+      mu_p = sm > 0.0 ? 2.0 * mi * mj / sm : 0.0;
+    }
+    if (tt > 0.0 && tt < 1.0 && d < dhat) {
+      d = fmax(d, dlo);
+      double ddd = 0.0;
+      if (mode == 1) {
+        const V2<double> de = ld2(pdir, b) - ld2(pdir, a);
+        const V2<double> dw = ld2(pdir, pn) - ld2(pdir, a);
+        const double c0_ = cross(ab, w);
+        const double c1_ = cross(de, w) + cross(ab, dw);
+        const double c2_ = 2.0 * cross(de, dw);
+        const double L1 = dot(ab, de) / L;
+        const double Lq = (dot(de, de) - L1 * L1) / L;
+        const double s0 = c0_ / L;
+        const double s1 = (c1_ - s0 * L1) / L;
+        ddd = (c2_ - 2.0 * s1 * L1 - s0 * Lq) / L;
+        if (c0_ < 0.0) ddd = -ddd;
+      }
+      psum += term(mode, 1.0, jp, la, lb, pn, a, b, 1.0 - tt, tt, d, (1.0 / d) * dvec, ddd, mu_p, pdir, h2);
+    }
+    double coef_a = -s.ca[k];
+    if (tt <= 0.0) coef_a += 1.0;
+    double d_a = len(w);
+    if (coef_a != 0.0 && d_a < dhat) {
+      d_a = fmax(d_a, dlo);
+      const V2<double> n_a = (1.0 / d_a) * w;
+      double ddd = 0.0;
+      if (mode == 1) {
+        const V2<double> dw = ld2(pdir, pn) - ld2(pdir, a);
+        const double dn = dot(dw, n_a);
+        ddd = (dot(dw, dw) - dn * dn) / d_a;
+      }
+      psum += term(mode, coef_a, jp, la, lb, pn, a, b, 1.0, 0.0, d_a, n_a, ddd, mu_p, pdir, h2);
+    }
+    double coef_b = -s.cb[k];
+    if (tt >= 1.0) coef_b += 1.0;
+    const V2<double> wb_ = xp - xb;
+    double d_b = len(wb_);
+    if (coef_b != 0.0 && d_b < dhat) {
+      d_b = fmax(d_b, dlo);
+      const V2<double> n_b = (1.0 / d_b) * wb_;
+      double ddd = 0.0;
+      if (mode == 1) {
+        const V2<double> dw = ld2(pdir, pn) - ld2(pdir, b);
+        const double dn = dot(dw, n_b);
+        ddd = (dot(dw, dw) - dn * dn) / d_b;
+      }
+      psum += term(mode, coef_b, jp, la, lb, pn, a, b, 0.0, 1.0, d_b, n_b, ddd, mu_p, pdir, h2);
+    }
+  }
+
+  // Candidates of edge k from the grid: calls f(jp) for every boundary node in the cells the
+  // edge's box +- reach overlaps that is neither end of the edge nor excluded against one.
+  template <typename F>
+  __device__ void candidates(int k, F &&f) {
+    const double reach = p.ipc_reach;
+    const int la = s.ela[k], lb = s.elb[k];
+    const double ax = bpos[2 * la], ay = bpos[2 * la + 1], bx = bpos[2 * lb], by = bpos[2 * lb + 1];
+    int ix0 = (int)((fmin(ax, bx) - reach - sc.gx0) / sc.gh);
+    int ix1 = (int)((fmax(ax, bx) + reach - sc.gx0) / sc.gh);
+    int iy0 = (int)((fmin(ay, by) - reach - sc.gy0) / sc.gh);
+    int iy1 = (int)((fmax(ay, by) + reach - sc.gy0) / sc.gh);
+    ix0 = max(ix0, 0);
+    iy0 = max(iy0, 0);
+    ix1 = min(ix1, sc.gnx - 1);
+    iy1 = min(iy1, sc.gny - 1);
+    for (int iy = iy0; iy <= iy1; ++iy)
+      for (int ix = ix0; ix <= ix1; ++ix) {
+        const int c = iy * sc.gnx + ix;
+        for (int q = cstart[c]; q < cstart[c + 1]; ++q) {
+          const int jp = citem[q];
+          if (jp == la || jp == lb || excluded(jp, la) || excluded(jp, lb)) continue;
+          f(jp);
+        }
+      }
+  }
+
+  // The pairs within reach at the positions the grid was built from, into the LDS list; the
+  // list is used by every pass at these positions unless it overflowed.
+  __device__ void build_pairs() {
+    if (tid == 0) sc.npairs = 0;
+    __syncthreads();
+    const bool fits = s.max_pairs > 0 && k1 - k0 < 32768;  // edge index packed in 15 bits
+    if (fits) {
+      const double reach = p.ipc_reach;
+      for (int k = k0 + tid; k < k1; k += TPB) {
+        const int la = s.ela[k], lb = s.elb[k];
+        const V2<double> xa = {bpos[2 * la], bpos[2 * la + 1]}, xb = {bpos[2 * lb], bpos[2 * lb + 1]};
+        candidates(k, [&](int jp) {
+          const V2<double> xp = {bpos[2 * jp], bpos[2 * jp + 1]};
+          if (seg_dist(xp, xa, xb) < reach) {
+            const int q = atomicAdd(&sc.npairs, 1);
+            if (q < s.max_pairs) plist[q] = ((k - k0) << 16) | jp;
+          }
+        });
+      }
+    }
+    __syncthreads();
+    if (tid == 0) sc.plist = fits && sc.npairs <= s.max_pairs;
+    __syncthreads();
+  }
+
+  // One pass over the pairs (reference implicit/ipc_contact.py:175-296): mode 0 gradient +
+  // diagonal into bacc, 1 p^T H p + clearance + step limit, 2 energy, 3 clearance + step limit.
   __device__ void ipc_pass(int mode, const double *pdir, double t_c, double h2, double &psum,
                            double &cmin, double &amin) {
     psum = 0.0;
     cmin = BIG;
     amin = BIG;
-    const double dhat = p.dhat, reach = p.ipc_reach;
-    const double dlo = 1.0e-9 * dhat, onset = 0.02 * dhat;
-    const double floor_ = mode == 3 ? 0.25 * dhat : 0.0;
-    const uint8_t *ex = s.excl + eoff;
-    for (int k = k0 + tid; k < k1; k += TPB) {
-      const int a = s.ea[k], b = s.eb[k], la = s.ela[k], lb = s.elb[k];
-      const V2<double> xa = {bpos[2 * la], bpos[2 * la + 1]}, xb = {bpos[2 * lb], bpos[2 * lb + 1]};
-      const V2<double> ab = xb - xa;
-      const double L2 = fmax(dot(ab, ab), 1.0e-30);
-      const double L = sqrt(L2);
-      int ix0 = (int)((fmin(xa.x, xb.x) - reach - sc.gx0) / sc.gh);
-      int ix1 = (int)((fmax(xa.x, xb.x) + reach - sc.gx0) / sc.gh);
-      int iy0 = (int)((fmin(xa.y, xb.y) - reach - sc.gy0) / sc.gh);
-      int iy1 = (int)((fmax(xa.y, xb.y) + reach - sc.gy0) / sc.gh);
-      ix0 = max(ix0, 0);
-      iy0 = max(iy0, 0);
-      ix1 = min(ix1, sc.gnx - 1);
-      iy1 = min(iy1, sc.gny - 1);
-      double mu_a = p.ipc_mu;
-      if (p.ipc_per_body && s.mu_node) mu_a = s.mu_node[a];
-      for (int iy = iy0; iy <= iy1; ++iy)
-        for (int ix = ix0; ix <= ix1; ++ix) {
-          const int c = iy * sc.gnx + ix;
-          for (int q = cstart[c]; q < cstart[c + 1]; ++q) {
-            const int jp = citem[q];
-            if (jp == la || jp == lb) continue;
-            const uint8_t *row = ex + (int64_t)jp * nb;
-            if (row[la] || row[lb]) continue;
-            const V2<double> xp = {bpos[2 * jp], bpos[2 * jp + 1]};
-            const V2<double> w = xp - xa;
-            const double tt = dot(w, ab) / L2;
-            V2<double> dvec = w - tt * ab;
-            if (tt <= 0.0) dvec = w;
-            else if (tt >= 1.0) dvec = xp - xb;
-            double d = len(dvec);
-            if (!(d < reach)) continue;
-            const int pn = s.bnd[b0 + jp];
-            if (mode == 1 || mode == 3) {
-              cmin = fmin(cmin, d);
-              double keep = fmax(p.ipc_gap * d, dhat - onset);
-              if (d < dhat) keep = d - fmin(2.0 * (dhat - d) + onset, 0.5 * d);
-              if (mode == 3) keep = fmin(d, fmax(p.ipc_gap * d, floor_));
-              amin = fmin(amin, accd(xp, xa, xb, ld2(pdir, pn), ld2(pdir, a), ld2(pdir, b), keep, t_c));
-              if (mode == 3) continue;
-            }
-            double mu_p = p.ipc_mu;
-            if (p.ipc_per_body && s.mu_node) {
-              const double mi = s.mu_node[pn], sm = mi + mu_a;
-              mu_p = sm > 0.0 ? 2.0 * mi * mu_a / sm : 0.0;
-            }
-            if (tt > 0.0 && tt < 1.0 && d < dhat) {
-              d = fmax(d, dlo);
-              double ddd = 0.0;
-              if (mode == 1) {
-                const V2<double> de = ld2(pdir, b) - ld2(pdir, a);
-                const V2<double> dw = ld2(pdir, pn) - ld2(pdir, a);
-                const double c0_ = cross(ab, w);
-                const double c1_ = cross(de, w) + cross(ab, dw);
-                const double c2_ = 2.0 * cross(de, dw);
-                const double L1 = dot(ab, de) / L;
-                const double Lq = (dot(de, de) - L1 * L1) / L;
-                const double s0 = c0_ / L;
-                const double s1 = (c1_ - s0 * L1) / L;
-                ddd = (c2_ - 2.0 * s1 * L1 - s0 * Lq) / L;
-                if (c0_ < 0.0) ddd = -ddd;
-              }
-              psum += term(mode, 1.0, jp, la, lb, pn, a, b, 1.0 - tt, tt, d, (1.0 / d) * dvec, ddd,
-                           mu_p, pdir, h2);
-            }
-            double coef_a = -s.ca[k];
-            if (tt <= 0.0) coef_a += 1.0;
-            double d_a = len(w);
-            if (coef_a != 0.0 && d_a < dhat) {
-              d_a = fmax(d_a, dlo);
-              const V2<double> n_a = (1.0 / d_a) * w;
-              double ddd = 0.0;
-              if (mode == 1) {
-                const V2<double> dw = ld2(pdir, pn) - ld2(pdir, a);
-                const double dn = dot(dw, n_a);
-                ddd = (dot(dw, dw) - dn * dn) / d_a;
-              }
-              psum += term(mode, coef_a, jp, la, lb, pn, a, b, 1.0, 0.0, d_a, n_a, ddd, mu_p, pdir, h2);
-            }
-            double coef_b = -s.cb[k];
-            if (tt >= 1.0) coef_b += 1.0;
-            const V2<double> wb_ = xp - xb;
-            double d_b = len(wb_);
-            if (coef_b != 0.0 && d_b < dhat) {
-              d_b = fmax(d_b, dlo);
-              const V2<double> n_b = (1.0 / d_b) * wb_;
-              double ddd = 0.0;
-              if (mode == 1) {
-                const V2<double> dw = ld2(pdir, pn) - ld2(pdir, b);
-                const double dn = dot(dw, n_b);
-                ddd = (dot(dw, dw) - dn * dn) / d_b;
-              }
-              psum += term(mode, coef_b, jp, la, lb, pn, a, b, 0.0, 1.0, d_b, n_b, ddd, mu_p, pdir, h2);
-            }
-          }
-        }
+    if (sc.plist) {
+      const int np = sc.npairs;
+      for (int q = tid; q < np; q += TPB) {
+        const int v = plist[q];
+        pair_eval(mode, k0 + (v >> 16), v & 0xffff, pdir, t_c, h2, psum, cmin, amin);
+      }
+      return;
     }
+    for (int k = k0 + tid; k < k1; k += TPB)
+      candidates(k, [&](int jp) { pair_eval(mode, k, jp, pdir, t_c, h2, psum, cmin, amin); });
   }
 
   // ------------------------------------------------------------------ kinematic sweep
@@ -462,6 +519,7 @@ struct Step {
     q = blk.max1(q);
     if (tid == 0) sc.q_inf = q;
     build_grid();
+    build_pairs();
     double ps, cm, am;
     ipc_pass(3, s.bc_q, 1.0, p.h2, ps, cm, am);
     double v[2] = {cm, am};
@@ -529,6 +587,7 @@ struct Step {
     // contact between bodies and pressure: into the boundary accumulators
     if (p.ipc_on && k1 > k0) {
       build_grid();
+      build_pairs();
       double ps, cm, am;
       ipc_pass(0, s.p, 0.0, h2, ps, cm, am);
     }
@@ -1018,6 +1077,7 @@ struct Step {
     }
     if (p.ipc_on && k1 > k0) {
       build_grid();
+      build_pairs();
       double ps, cm, am;
       ipc_pass(2, s.p, 0.0, h2, ps, cm, am);
       E += ps;
@@ -1076,6 +1136,20 @@ struct Step {
 
   // ------------------------------------------------------------------ the step
   __device__ void run() {
+    // exclusion bitset of this scene into LDS when it fits
+    if (tid == 0) {
+      sc.excl_lds = 0;
+      sc.plist = 0;
+    }
+    __syncthreads();
+    if (p.ipc_on && k1 > k0 && s.excl_bits) {
+      const int64_t w0 = s.excl_bits_off[e], nw = s.excl_bits_off[e + 1] - w0;
+      if (nw <= s.excl_bits_lds) {
+        for (int64_t i = tid; i < nw; i += TPB) xbits[i] = s.excl_bits[w0 + i];
+        if (tid == 0) sc.excl_lds = 1;
+      }
+    }
+    __syncthreads();
     for (int i = n0 + tid; i < n1; i += TPB) {
       const V2<double> x = ld2(s.x, i);
       st2(s.xn, i, x);
@@ -1161,7 +1235,8 @@ __global__ __launch_bounds__(TPB) void k_scene(S s, Pm p) {
 template <typename R, int TPB>
 static int launch(const S &s, const Pm &p, hipStream_t stream) {
   const size_t lds = sizeof(double) * (Block<TPB>::NW * RED_K + 6 * (size_t)s.max_nb) +
-                     sizeof(int) * (2 * (size_t)s.max_cells + 1 + s.max_nb);
+                     sizeof(int) * (2 * (size_t)s.max_cells + 1 + s.max_nb + (size_t)s.excl_bits_lds +
+                                    (size_t)s.max_pairs);
   if (lds > 150 * 1024) {
     set_error("d2f_step_scene: %zu bytes of LDS per scene (max_nb=%d, max_cells=%d) exceed the 150 KiB "
               "budget; use the per-kernel path for scenes this large", lds, s.max_nb, s.max_cells);
@@ -1184,6 +1259,11 @@ extern "C" int d2f_step_scene(const d2f_scene *s, const d2f_params *p, void *str
   D2F_CHECK(s->B > 0 && s->N > 0 && s->sc_tri && s->sc_cs && s->sc_tri_start,
             "d2f_step_scene: the scene has no per-scene layout");
   D2F_CHECK(s->max_cells >= 1 && s->max_nb >= 0, "d2f_step_scene: bad grid capacity");
+  D2F_CHECK(s->excl_bits_lds >= 0 && s->max_pairs >= 0, "d2f_step_scene: negative LDS capacity");
+  D2F_CHECK(s->excl_bits_lds == 0 || (s->excl_bits && s->excl_bits_off),
+            "d2f_step_scene: excl_bits_lds set without the packed exclusion bitset");
+  D2F_CHECK(s->max_pairs == 0 || s->max_nb < 65536,
+            "d2f_step_scene: the pair list packs boundary indices in 16 bits (max_nb < 65536)");
   D2F_CHECK(!p->ipc_on || (s->bloc && s->e_start), "d2f_step_scene: IPC needs bloc and e_start");
   hipStream_t st = (hipStream_t)stream;
   const int tpb = p->block_size > 0 ? p->block_size : 256;

@@ -136,6 +136,7 @@ class DeviceSolver:
     def refresh_exclusion(self):
         if self.ipc is not None:
             self.excl = self._t(np.asarray(self.me.excl_flat, np.uint8), U8)
+            self._layout = None
 
     # ------------------------------------------------------------------ rigid bodies
     def setup_affine(self, groups):
@@ -378,7 +379,39 @@ class DeviceSolver:
             if (bl[ends] < 0).any():
                 raise ValueError("a pressure edge has an end that is not a boundary node")
         self.max_cells = int(max_cells)
+        self._pack_contact_lists()
         self._layout = True
+
+    LDS_BUDGET = 150 * 1024          # bytes of LDS one scene's workgroup may take
+    EXCL_LDS_MAX = 8192              # largest exclusion bitset staged in LDS, 32-bit words
+
+    def _pack_contact_lists(self):
+        """Each scene's boundary exclusion matrix as a bitset (bit jp * n_b_e + j), and the
+        capacity of the per-scene candidate-pair list, sized to the LDS left over."""
+        self.excl_bits_lds, self.max_pairs = 0, 0
+        self.excl_bits = self.excl_bits_off = None
+        if self.ipc is None or self.n_e == 0:
+            return
+        me = self.me
+        flat = np.asarray(me.excl_flat, np.uint8)
+        off = np.asarray(me.excl_offset, np.int64)
+        nbe = np.asarray(me.boundary_per_env, np.int64)
+        words, woff = [], [0]
+        for e in range(self.B):
+            bits = np.packbits(flat[off[e]:off[e] + nbe[e] * nbe[e]] != 0, bitorder="little")
+            bits = np.concatenate([bits, np.zeros((-len(bits)) % 4, np.uint8)])
+            words.append(bits.view("<u4"))
+            woff.append(woff[-1] + len(bits) // 4)
+        self.excl_bits = self._t(np.concatenate(words).view(np.int32), I32)
+        self.excl_bits_off = self._t(np.asarray(woff, np.int64), I64)
+        wmax = int(np.max(np.diff(woff)))
+        base = 8 * (16 * 20 + 6 * self.max_nb) + 4 * (2 * self.max_cells + 1 + self.max_nb)
+        if wmax <= self.EXCL_LDS_MAX and base + 4 * wmax <= self.LDS_BUDGET:
+            self.excl_bits_lds = wmax
+        left = (self.LDS_BUDGET - base - 4 * self.excl_bits_lds) // 4
+        e_per = np.bincount(self.e2env.cpu().numpy(), minlength=self.B)
+        if self.max_nb < 65536 and e_per.max(initial=0) < 32768:
+            self.max_pairs = int(max(0, min(left, 64 * self.max_nb)))
 
     def refresh_moduli(self):
         """Re-upload element moduli after mu_np / lam_np changed (both layouts)."""
@@ -447,6 +480,9 @@ class DeviceSolver:
             if self.tendon is not None:
                 s.td_start, s.ey_start = p("td_start"), p("ey_start")
             s.max_nb, s.max_cells = self.max_nb, self.max_cells
+            if self.excl_bits is not None:
+                s.excl_bits, s.excl_bits_off = p("excl_bits"), p("excl_bits_off")
+            s.excl_bits_lds, s.max_pairs = self.excl_bits_lds, self.max_pairs
         s.n_col = len(self.colliders)
         for i, c in enumerate(self.colliders):
             s.col[i].kind = int(c["kind"])

@@ -119,6 +119,11 @@ typedef struct d2f_scene {
   const int32_t *pe_start;            /* B+1 pressure edges per scene */
   const int32_t *td_start, *ey_start; /* B+1 tendons per scene, n_td+1 eyelets per tendon */
   int32_t max_nb, max_cells;          /* largest per-scene boundary count; grid cell cap */
+  /* per-scene exclusion matrices bit-packed (bit jp*n_b_e + j of scene e's words), staged in LDS
+   * when excl_bits_lds is set; pair-list capacity per scene (0: traverse the grid each pass) */
+  const uint32_t *excl_bits;          /* sum_e ceil(n_b_e^2 / 32) words */
+  const int64_t *excl_bits_off;       /* B+1 word offsets */
+  int32_t excl_bits_lds, max_pairs;   /* words of LDS for the bitset (0: read bytes); pairs cap */
 } d2f_scene;
 
 /* Solver options and step constants (reference PNCG2DSolver.__init__, implicit/solver.py:97-226,

@@ -6,7 +6,7 @@ import pytest
 from diff_2d_fem_amd import Model, geometry as G
 from diff_2d_fem_amd.mesh import build_scene, check_mesh, mask_to_mesh
 from diff_2d_fem_amd import layout as LY
-from tests.golden.make_fixtures import scenes, MESH_KEYS
+from tests.golden.fixture_scenes import scenes, MESH_KEYS
 
 FX = np.load("tests/golden/mesh_fixtures.npz")
 AX = np.load("tests/golden/actuator_fixtures.npz")

@@ -62,7 +62,7 @@ def _elastic_g(me, x, clamp=1, h2=1.0):
 
 # ---------------------------------------------------------------------------- elastic
 def test_elastic_matches_reference_newton_oracle():
-    from tests.golden.make_fixtures import scenes
+    from tests.golden.fixture_scenes import scenes
     from diff_2d_fem_amd.mesh import build_scene
     f = np.load(GOLD + "static_fixtures.npz")
     me = build_scene([dict(b) for b in scenes()["blob"]])
