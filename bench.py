@@ -11,10 +11,13 @@ including the host command that sets the jaws' targets.
 
 Printed: one JSON line. `value` is scene-steps/s of the requested path and precision;
 `plain_fp64` is the per-kernel float64 path on the same workload and steps (the
-denominator of the batched-vs-plain ratio); `pos_err_vs_fp64` is the largest node-position
-difference against the float64 run after the same steps, over the block size; `roofline`
-is the dominant kernel's algorithmic bytes over its measured duration against HBM peak;
-`cpu_baseline` times the float64 CPU oracle on a bounded sample of the same workload.
+denominator of the batched-vs-plain ratio); `pos_err_1step` is, after the timed steps, the
+node-position difference between one step of the measured precision and one float64 step
+from the identical state, over each scene's block size; `roofline` is the dominant kernel
+(the scene path's one step kernel, or the plain path's element kernel) with its algorithmic
+bytes over its event-timed duration against HBM peak; `element_roofline` is the float64
+elastic gradient+diagonal element kernel alone; `cpu_baseline` times the float64 CPU oracle
+on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -40,7 +43,7 @@ def parse():
     ap.add_argument("--precision", default=None)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=5)
     return ap.parse_args()
 
 
@@ -51,6 +54,8 @@ def build(config, n_env):
 
 
 def run_gpu(args, path, precision, steps, warmup):
+    """W warm-up steps, then K timed steps. HIP events recorded on the launch stream right
+    around each solver launch give the step kernel's own duration."""
     import torch
     from diff_2d_fem_amd import ImplicitIntegrator
     w = build(args.config, args.envs)
@@ -62,18 +67,92 @@ def run_gpu(args, path, precision, steps, warmup):
         w["command"](step)
         integ.simulate(m, w["dt"])
     torch.cuda.synchronize()
+    s = m._solver
+    events = []
+    launch = s.step
+
+    def timed_launch():
+        st = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        launch()
+        e1.record(st)
+        events.append((e0, e1))
     acc = torch.zeros(m.n_envs, dtype=torch.int64, device="cuda")
+    s.step = timed_launch
     t0 = time.perf_counter()
     for _ in range(steps):
         step += 1
         w["command"](step)
         integ.simulate(m, w["dt"])
-        acc += m._solver.iters                      # on the stream: no host sync in the loop
+        acc += s.iters                              # on the stream: no host sync in the loop
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
+    del s.step
+    launch_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in events]))
     st = integ.stats(m)
     iters = int(acc.sum().item())
-    return dict(t=t, m=m, w=w, integ=integ, stats=st, x=m.get_x(), iters=iters)
+    return dict(t=t, m=m, w=w, integ=integ, stats=st, iters=iters, step=step, launch_ms=launch_ms)
+
+
+def _snapshot(s):
+    import torch
+    return {k: v.clone() for k, v in vars(s).items() if torch.is_tensor(v)}
+
+
+def _restore(s, snap):
+    for k, v in snap.items():
+        getattr(s, k).copy_(v)
+
+
+def one_step_error(res):
+    """Position error of one step of the measured precision against one float64 step of the
+    same path from the identical state (same commands, float64 defaults: eps 1e-10), over
+    each scene's block size (or the workload's body size)."""
+    import torch
+    m, w, s = res["m"], res["w"], res["m"]._solver
+    snap, t_model = _snapshot(s), m.time
+    prec, eps = s.precision, s.eps
+    k = res["step"] + 1
+    w["command"](k)
+    res["integ"].simulate(m, w["dt"])
+    x_meas, it_meas = s.get_x(), s.iters.cpu().numpy().copy()
+    _restore(s, snap)
+    m.time = t_model
+    s.precision, s.eps = "double", 1e-10
+    w["command"](k)
+    res["integ"].simulate(m, w["dt"])
+    torch.cuda.synchronize()
+    x_ref, it_ref = s.get_x(), s.iters.cpu().numpy().copy()
+    _restore(s, snap)
+    m.time = t_model
+    s.precision, s.eps = prec, eps
+    env = np.asarray(m.me.node2env)
+    err = np.linalg.norm(x_meas - x_ref, axis=1)
+    per_env = np.zeros(m.n_envs)
+    np.maximum.at(per_env, env, err)
+    size = np.asarray(w.get("sides", np.full(m.n_envs, w["size"])), np.float64)
+    rel = per_env / size
+    return dict(max=float(rel.max()), mean=float(rel.mean()), iters_measured=float(it_meas.mean()),
+                iters_fp64=float(it_ref.mean()), reference="float64 " + s.path + " path, eps 1e-10",
+                step=k)
+
+
+def scene_roofline(res, precision):
+    """The step kernel of the scene path (one launch per step): algorithmic bytes per launch
+    = sum over scenes of iterations x per-iteration bytes, where one iteration reads each
+    scene's element constants twice (gradient+diagonal and curvature: T (12 + 7 s_e)) and
+    touches node state for 272 bytes per node; s_e = 4 in mixed, 8 in float64."""
+    s = res["m"]._solver
+    se = 4 if precision == "mixed" else 8
+    it_per_env = res["iters"] / res["m"].n_envs / res["n_steps"]
+    bytes_ = it_per_env * (2.0 * (12 + 7 * se) * s.n_tri + 272.0 * s.N)
+    dur = res["launch_ms"] * 1e-3
+    ach = bytes_ / dur / 1e9
+    return dict(kernel=f"k_scene ({precision})", bound="hbm", achieved=round(ach, 1),
+                peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
+                avg_us=round(dur * 1e6, 1), bytes_per_launch=int(bytes_),
+                iters_per_scene_per_step=round(it_per_env, 2))
 
 
 def elastic_roofline(res):
@@ -154,19 +233,19 @@ def main():
         "max_iter": int(main_run["stats"]["max_iter"]),
         "converged_frac": round(float(np.mean(main_run["stats"]["converged"])), 4),
     }
-    ref_x = None
+    main_run["n_steps"] = args.steps
+    if path == "scene":
+        out["roofline"] = scene_roofline(main_run, precision)
+    out["element_roofline"] = elastic_roofline(main_run)
+    if path == "plain":
+        out["roofline"] = dict(out["element_roofline"])
+    if not (path == "scene" and precision == "double"):
+        out["pos_err_1step"] = one_step_error(main_run)
     if not args.no_plain and not (path == "plain" and precision == "double"):
         plain = run_gpu(args, "plain", "double", args.steps, args.warmup)
         pv = B * args.steps / plain["t"]
         out["plain_fp64"] = dict(value=round(pv, 2), ms_per_step=round(plain["t"] / args.steps * 1e3, 3))
         out["speedup_vs_plain_fp64"] = round(value / pv, 3)
-        ref_x = plain["x"]
-        roof_src = plain
-    else:
-        roof_src = main_run
-    if ref_x is not None:
-        out["pos_err_vs_fp64"] = float(np.abs(main_run["x"] - ref_x).max() / main_run["w"]["size"])
-    out["roofline"] = elastic_roofline(roof_src)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, args.cpu_steps)
     print(json.dumps(out), flush=True)

@@ -79,13 +79,22 @@ def random_blobs(n_env=1024, seed=0, px=0.75e-3):
                 name="random_blobs")
 
 
-def pads_lift_block(n_env=256, seed=0, close_speed=1.0e-3, lift_speed=20e-6):
+def pads_lift_block(n_env=256, seed=0, n_close=10, n_lift=200, scene_ids=None):
+    """Two soft pads on kinematic jaws grip a rigid block and lift it, one scene per
+    environment (the protocol of reference examples/pinch_and_lift.py:23-24, 51-56, at larger
+    sizes): the jaws close from 2 px apart to a quarter pixel past touching over n_close
+    steps, then rise by half the block's side over n_lift steps and hold.
+
+    scene_ids: the scene indices to build (default range(n_env)); scene i draws its block
+    side from np.random.default_rng([seed, i]) and becomes the environment at its position
+    in the list."""
     m = Model()
     m.body_friction = 0.6
     m.add_ground(0.0, mu=0.6)
     pads, blocks, pxs = [], [], []
-    for i in range(n_env):
-        rng = np.random.default_rng([seed, i])
+    ids = list(range(n_env)) if scene_ids is None else [int(v) for v in scene_ids]
+    for i, sid in enumerate(ids):
+        rng = np.random.default_rng([seed, sid])
         side = float(rng.uniform(12e-3, 20e-3))
         px = side / 16.0
         pxs.append(px)
@@ -109,18 +118,20 @@ def pads_lift_block(n_env=256, seed=0, close_speed=1.0e-3, lift_speed=20e-6):
         rest.append(r)
     close = np.array([2.25 * px for px in pxs])             # to a quarter pixel past touching
 
+    lift = np.array([8.0 * px for px in pxs])
+
     def command(step):
         for i, (pair, xs) in enumerate(zip(pads, rest)):
-            c = min(close_speed * step, close[i])
-            n_close = int(np.ceil(close[i] / close_speed))
-            up = lift_speed * max(step - n_close, 0)
+            c = close[i] * min(step / n_close, 1.0)
+            up = lift[i] * min(max(step - n_close, 0) / n_lift, 1.0)
             for s, p, X in zip((-1, 1), pair, xs):
                 Y = X.copy()
                 Y[:, 0] -= s * c
                 Y[:, 1] += up
                 m.set_kinematic_target(p, Y)
     return dict(model=m, dt=1e-3, command=command, bodies=blocks, pads=pads,
-                size=float(np.mean([16 * px for px in pxs])), name="pads_lift_block")
+                size=float(np.mean([16 * px for px in pxs])),
+                sides=np.array([16 * px for px in pxs]), name="pads_lift_block")
 
 
 CONFIGS = {"c1": strip_on_ground, "c2": random_blobs, "c3": pads_lift_block}
