@@ -138,6 +138,22 @@ def one_step_error(res):
                 step=k)
 
 
+def measured_traffic(kernel, res):
+    """HBM bytes per launch of `kernel` on this workload from the committed rocprofv3 PMC
+    summary (profiles/c3_traffic.json: FETCH_SIZE and WRITE_SIZE scaled by a copy kernel's
+    readings), or None when there is no measurement for it."""
+    import json
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "c3_traffic.json")
+    try:
+        d = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    if d.get("workload") != res["w"]["name"] or d.get("scenes") != res["m"].n_envs:
+        return None
+    k = d["kernels"].get(kernel)
+    return None if k is None else int(k["traffic_bytes"])
+
+
 def scene_roofline(res, precision):
     """The step kernel of the scene path (one launch per step): algorithmic bytes per launch
     = sum over scenes of iterations x per-iteration bytes, where one iteration reads each
@@ -149,8 +165,10 @@ def scene_roofline(res, precision):
     bytes_ = it_per_env * (2.0 * (12 + 7 * se) * s.n_tri + 272.0 * s.N)
     dur = res["launch_ms"] * 1e-3
     ach = bytes_ / dur / 1e9
+    name = "d2f::k_scene<float, 256>" if precision == "mixed" else "d2f::k_scene<double, 256>"
     return dict(kernel=f"k_scene ({precision})", bound="hbm", achieved=round(ach, 1),
-                peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
+                peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
+                traffic=measured_traffic(name, res),
                 avg_us=round(dur * 1e6, 1), bytes_per_launch=int(bytes_),
                 iters_per_scene_per_step=round(it_per_env, 2))
 
@@ -182,7 +200,8 @@ def elastic_roofline(res):
     bytes_ = 68.0 * s.n_tri + 48.0 * s.N
     ach = bytes_ / dur / 1e9
     return dict(kernel="k_elastic_grad_diag (float64)", bound="hbm", achieved=round(ach, 1),
-                peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
+                peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
+                traffic=measured_traffic("d2f::k_elastic_grad_diag", res),
                 avg_us=round(dur * 1e6, 2), bytes_per_launch=int(bytes_))
 
 

@@ -50,3 +50,19 @@ def close_jaws(m, pads, k, n_close=10, squeeze=2.25, px=0.5e-3, lift=0.0):
             X[:, 0] -= s * squeeze * px * a
             X[:, 1] += lift
             m.set_kinematic_target(p, X)
+
+
+def cantilever():
+    """The reference Newton oracle's cantilever (tests/golden/static_fixtures.npz, written by
+    the reference's tests/reference_static.py): a 4 x 20 px beam, left column held, a dead
+    load of 20 g on every node. Returns the model and the fixture (cant_x is the static
+    equilibrium to a 1e-12 residual)."""
+    import os
+    f = np.load(os.path.join(os.path.dirname(__file__), "golden", "static_fixtures.npz"))
+    m = Model()
+    h = m.add_soft_body(np.ones((4, 20), np.uint8), physical_size=(0.02, 0.004), center=(0.0, 0.0),
+                        E=1e5, nu=0.4, density=1000.0)
+    m.finalize()
+    m.set_kinematic_constraint(h, f["cant_fixed"])
+    m.set_external_force(h, f["cant_fext"])
+    return m, f

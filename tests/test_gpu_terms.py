@@ -261,3 +261,18 @@ def test_reset_and_set_state(path):
         err = np.abs(m.get_x()[env == e] - f.get_x()[env == e]).max()
         print(f"{path}: env {e} reset/set vs fresh: {err:.3e}")
         assert err < 1e-12
+
+
+@pytest.mark.parametrize("path,precision,tol", [("plain", "double", 1e-8), ("scene", "double", 1e-8),
+                                                ("scene", "mixed", 1e-5)])
+def test_cantilever_reaches_reference_static_equilibrium(path, precision, tol):
+    """The held, loaded beam stepped with a one-second step size reaches the static
+    equilibrium of the reference's Newton oracle (golden fixture)."""
+    from tests.scenes import cantilever
+    m, f = cantilever()
+    gi = ImplicitIntegrator(path=path, precision=precision, max_iter=400, eps=0.0, conv_check=0)
+    for _ in range(5):
+        gi.simulate(m, 1.0)
+    err = np.abs(m.get_x() - f["cant_x"]).max() / 0.02
+    print(f"{path} {precision}: |x - x_static| / length = {err:.3e}")
+    assert err < tol

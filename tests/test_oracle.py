@@ -32,7 +32,7 @@ from oracle import terms as T
 from oracle.bodies import chol6, chol6_solve, reduced_hessian, Pressure, Tendon
 from oracle.ipc import IPCContact, accd, seg_dist
 from oracle.pncg import OracleIntegrator
-from tests.scenes import strip
+from tests.scenes import strip, cantilever
 
 GOLD = "tests/golden/"
 
@@ -70,6 +70,19 @@ def test_elastic_matches_reference_newton_oracle():
     assert abs(_elastic_E(me, x) / float(f["blob_E"]) - 1.0) < 1e-12
     g, _ = _elastic_g(me, x)
     assert np.abs(g - f["blob_grad"]).max() < 1e-10 * np.abs(f["blob_grad"]).max()
+
+
+def test_cantilever_reaches_reference_static_equilibrium():
+    """Implicit steps with a one-second step size drive the held, loaded beam to the static
+    equilibrium the reference's Newton oracle computed (tip deflection ~0.7 of the length)."""
+    m, f = cantilever()
+    fixed = np.nonzero(np.abs(m.me.rest_pos[:, 0] - m.me.rest_pos[:, 0].min()) < 1e-12)[0]
+    assert np.array_equal(fixed, f["cant_fixed"])
+    oi = OracleIntegrator(max_iter=400, eps=0.0, conv_check=0)
+    for _ in range(5):
+        oi.simulate(m, 1.0)
+    assert np.abs(oi.solver.x - f["cant_x"]).max() / 0.02 < 1e-8
+    assert np.abs(f["cant_x"] - m.me.rest_pos).max() / 0.02 > 0.5
 
 
 def test_elastic_gradient_is_energy_derivative():
