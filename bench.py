@@ -165,7 +165,7 @@ def scene_roofline(res, precision):
     bytes_ = it_per_env * (2.0 * (12 + 7 * se) * s.n_tri + 272.0 * s.N)
     dur = res["launch_ms"] * 1e-3
     ach = bytes_ / dur / 1e9
-    name = "d2f::k_scene<float, 256>" if precision == "mixed" else "d2f::k_scene<double, 256>"
+    name = f"d2f::k_scene<{'float' if precision == 'mixed' else 'double'}, {s.block_size}>"
     return dict(kernel=f"k_scene ({precision})", bound="hbm", achieved=round(ach, 1),
                 peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
                 traffic=measured_traffic(name, res),

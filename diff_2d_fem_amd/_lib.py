@@ -49,6 +49,7 @@ _SCENE_FIELDS = (
                         "rg_start", "rn_start", "pe_start", "td_start", "ey_start")]
     + [("max_nb", C.c_int32), ("max_cells", C.c_int32)]
     + [("excl_bits", P), ("excl_bits_off", P), ("excl_bits_lds", C.c_int32), ("max_pairs", C.c_int32)]
+    + [("max_nn", C.c_int32), ("nacc_lds", C.c_int32)]
 )
 
 
@@ -60,7 +61,7 @@ class Params(C.Structure):
     _fields_ = ([(k, C.c_double) for k in ("dt", "h2", "inv_dt", "drag", "dhat", "kappa", "epsv",
                                            "eps", "tol_x", "near_factor", "alpha_max", "area_shrink",
                                            "ccd_eta", "ls_tol", "cg_guard", "ipc_kappa", "ipc_mu",
-                                           "ipc_reach", "ipc_gap")]
+                                           "ipc_reach", "ipc_gap", "ipc_skin")]
                 + [(k, C.c_int32) for k in ("iter_max", "conv_check", "friction", "line_search",
                                             "n_backtrack", "ipc_on", "ipc_per_body", "rigid_modes_on",
                                             "precision", "block_size")])

@@ -158,6 +158,8 @@ struct Step {
   Block<TPB> blk;
   double *bpos;   // 2 nb: boundary positions
   double *bacc;   // 4 nb: boundary accumulators (grad x, grad y, diag x, diag y)
+  double *bref;   // 2 nb: boundary positions the grid and pair list were built from
+  double *nacc;   // 4 nn: element gradient and diagonal per node (when nacc_lds)
   int *cstart;    // max_cells + 1
   int *citem;     // nb
   int *ccur;      // max_cells
@@ -174,7 +176,9 @@ struct Step {
     blk.red = lds;
     bpos = lds + Block<TPB>::NW * RED_K;
     bacc = bpos + 2 * s.max_nb;
-    cstart = reinterpret_cast<int *>(bacc + 4 * s.max_nb);
+    bref = bacc + 4 * s.max_nb;
+    nacc = bref + 2 * s.max_nb;
+    cstart = reinterpret_cast<int *>(nacc + (s.nacc_lds ? 4 * s.max_nn : 0));
     citem = cstart + s.max_cells + 1;
     ccur = citem + s.max_nb;
     xbits = reinterpret_cast<uint32_t *>(ccur + s.max_cells);
@@ -204,14 +208,34 @@ struct Step {
   __device__ double fr(int i) const { return s.free_[i] ? 1.0 : 0.0; }
 
   // ------------------------------------------------------------------ IPC broadphase grid
-  // Boundary positions into LDS and a uniform grid over them (cell >= reach, so a pair within
-  // reach is found among the cells the edge's box +- reach overlaps).
-  __device__ void build_grid() {
-    double bb[4] = {BIG, BIG, BIG, BIG};  // min x, min y, -max x, -max y
+  // The boundary positions in LDS (current) are compared with those the pair list was built
+  // from; when some node has moved more than half the skin, or there is no valid list, the
+  // grid and the list are rebuilt. A pair within reach now was within reach + skin at the
+  // build (each end moved at most skin / 2), so the list holds every pair a pass needs.
+  __device__ void contact_prepare() {
+    const bool have = sc.plist;
+    double dmax = 0.0;
     for (int j = tid; j < nb; j += TPB) {
       const V2<double> x = ld2(s.x, s.bnd[b0 + j]);
       bpos[2 * j] = x.x;
       bpos[2 * j + 1] = x.y;
+      if (have) dmax = fmax(dmax, len(V2<double>{x.x - bref[2 * j], x.y - bref[2 * j + 1]}));
+    }
+    dmax = blk.max1(dmax);
+    if (!have || !(2.0 * dmax <= p.ipc_skin)) {
+      build_grid();
+      build_pairs();
+    }
+  }
+
+  // A uniform grid over the boundary positions in LDS (cell >= reach + skin, so a pair within
+  // that distance is found among the cells the edge's box +- reach + skin overlaps).
+  __device__ void build_grid() {
+    double bb[4] = {BIG, BIG, BIG, BIG};  // min x, min y, -max x, -max y
+    for (int j = tid; j < nb; j += TPB) {
+      const V2<double> x = {bpos[2 * j], bpos[2 * j + 1]};
+      bref[2 * j] = x.x;
+      bref[2 * j + 1] = x.y;
       bb[0] = fmin(bb[0], x.x);
       bb[1] = fmin(bb[1], x.y);
       bb[2] = fmin(bb[2], -x.x);
@@ -219,7 +243,7 @@ struct Step {
     }
     blk.template min<4>(bb);
     if (tid == 0) {
-      double h = p.ipc_reach;
+      double h = p.ipc_reach + p.ipc_skin;
       const double wx = -bb[2] - bb[0], wy = -bb[3] - bb[1];
       int nx = 1, ny = 1;
       if (wx >= 0.0 && wy >= 0.0 && wx < 1.0e12 && wy < 1.0e12) {  // finite positions
@@ -440,7 +464,7 @@ struct Step {
   // edge's box +- reach overlaps that is neither end of the edge nor excluded against one.
   template <typename F>
   __device__ void candidates(int k, F &&f) {
-    const double reach = p.ipc_reach;
+    const double reach = p.ipc_reach + p.ipc_skin;
     const int la = s.ela[k], lb = s.elb[k];
     const double ax = bpos[2 * la], ay = bpos[2 * la + 1], bx = bpos[2 * lb], by = bpos[2 * lb + 1];
     int ix0 = (int)((fmin(ax, bx) - reach - sc.gx0) / sc.gh);
@@ -469,7 +493,7 @@ struct Step {
     __syncthreads();
     const bool fits = s.max_pairs > 0 && k1 - k0 < 32768;  // edge index packed in 15 bits
     if (fits) {
-      const double reach = p.ipc_reach;
+      const double reach = p.ipc_reach + p.ipc_skin;
       for (int k = k0 + tid; k < k1; k += TPB) {
         const int la = s.ela[k], lb = s.elb[k];
         const V2<double> xa = {bpos[2 * la], bpos[2 * la + 1]}, xb = {bpos[2 * lb], bpos[2 * lb + 1]};
@@ -518,8 +542,7 @@ struct Step {
     }
     q = blk.max1(q);
     if (tid == 0) sc.q_inf = q;
-    build_grid();
-    build_pairs();
+    contact_prepare();
     double ps, cm, am;
     ipc_pass(3, s.bc_q, 1.0, p.h2, ps, cm, am);
     double v[2] = {cm, am};
@@ -544,6 +567,60 @@ struct Step {
 
   // ------------------------------------------------------------------ gradient + diagonal
   __device__ void grad_diag() {
+    if (s.nacc_lds && n1 - n0 <= s.max_nn) {
+      grad_diag_lds();
+    } else {
+      grad_diag_colored();
+    }
+    grad_diag_rest();
+  }
+
+  // Elastic gradient and diagonal (reference implicit/kernels.py:86-128) summed per node in
+  // LDS with LDS atomics, all soft triangles in one pass; then one pass over the nodes writes
+  // grad_prev, grad = inertia - h^2 f_ext + h^2 elastic, diag = mass + h^2 elastic
+  // (reference grad_diag_mass_kernel, implicit/kernels.py:73-82).
+  __device__ void grad_diag_lds() {
+    const double h2 = p.h2;
+    const int nn = n1 - n0;
+    for (int j = tid; j < 4 * nn; j += TPB) nacc[j] = 0.0;
+    for (int j = tid; j < 4 * nb; j += TPB) bacc[j] = 0.0;
+    __syncthreads();
+    const int ta = s.sc_cs[c0], tb = s.sc_cs[c0 + MAXC];
+    for (int t = ta + tid; t < tb; t += TPB) {
+      const Elem<R> el = load_elem<R>(s, t);
+      const NH<R> h = elem_F(el, ld2(s.x, el.i0), ld2(s.x, el.i1), ld2(s.x, el.i2));
+      V2<R> g[3], d[3];
+      nh_grad(h, el.B, el.Amu, R(1), g[0], g[1], g[2]);
+      nh_diag(h, el.B, el.Amu, el.Alam, R(1), d[0], d[1], d[2]);
+      const int li[3] = {el.i0 - n0, el.i1 - n0, el.i2 - n0};
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        double *a = nacc + 4 * li[q];
+        atomicAdd(a, double(g[q].x));
+        atomicAdd(a + 1, double(g[q].y));
+        atomicAdd(a + 2, fmax(double(d[q].x), 0.0));
+        atomicAdd(a + 3, fmax(double(d[q].y), 0.0));
+      }
+    }
+    __syncthreads();
+    for (int i = n0 + tid; i < n1; i += TPB) {
+      const V2<double> g = ld2(s.grad, i);
+      st2(s.grad_prev, i, g);
+      const double m = s.mass[i], f = fr(i);
+      const V2<double> d = ld2(s.x, i) - ld2(s.xhat, i);
+      const double *a = nacc + 4 * (i - n0);
+      V2<double> gn = {m * d.x * f, m * d.y * f};
+      if (s.fext) gn = gn - h2 * ld2(s.fext, i);
+      gn = gn + h2 * V2<double>{a[0], a[1]};
+      st2(s.grad, i, gn);
+      st2(s.diag, i, V2<double>{m + h2 * a[2], m + h2 * a[3]});
+    }
+    __syncthreads();
+  }
+
+  // The same, colour by colour with plain read-modify-write of the node arrays (no two
+  // triangles of a colour share a node), for scenes whose nodes do not fit in LDS.
+  __device__ void grad_diag_colored() {
     const double h2 = p.h2;
     for (int i = n0 + tid; i < n1; i += TPB) {
       const V2<double> g = ld2(s.grad, i);
@@ -576,6 +653,12 @@ struct Step {
       }
       __syncthreads();
     }
+  }
+
+  // Everything after elasticity: rigid-mode diagonal snapshot, contact, pressure, colliders,
+  // friction, tendons, affine and rigid-mode gathers.
+  __device__ void grad_diag_rest() {
+    const double h2 = p.h2;
     // rigid modes: the diagonal as it stands after elasticity (reference affine.py:541-542)
     if (r1 > r0) {
       for (int g = r0; g < r1; ++g)
@@ -586,8 +669,7 @@ struct Step {
     }
     // contact between bodies and pressure: into the boundary accumulators
     if (p.ipc_on && k1 > k0) {
-      build_grid();
-      build_pairs();
+      contact_prepare();
       double ps, cm, am;
       ipc_pass(0, s.p, 0.0, h2, ps, cm, am);
     }
@@ -1076,8 +1158,7 @@ struct Step {
       E += -h2 * s.ch_p[ch] * 0.5 * cross(ld2(s.x, s.pe_a[k]) - ref, ld2(s.x, s.pe_b[k]) - ref);
     }
     if (p.ipc_on && k1 > k0) {
-      build_grid();
-      build_pairs();
+      contact_prepare();
       double ps, cm, am;
       ipc_pass(2, s.p, 0.0, h2, ps, cm, am);
       E += ps;
@@ -1228,13 +1309,24 @@ template <typename R, int TPB>
 __global__ __launch_bounds__(TPB) void k_scene(S s, Pm p) {
   extern __shared__ double lds[];
   __shared__ SceneScalars sc;
-  Step<R, TPB> st(s, p, sc, lds);
+  // The arguments are read through references everywhere below; a reference to a by-value
+  // kernel argument makes the compiler copy it into every lane's private memory, so they are
+  // copied once into LDS instead.
+  __shared__ S s_lds;
+  __shared__ Pm p_lds;
+  if (threadIdx.x == 0) {
+    s_lds = s;
+    p_lds = p;
+  }
+  __syncthreads();
+  Step<R, TPB> st(s_lds, p_lds, sc, lds);
   st.run();
 }
 
 template <typename R, int TPB>
 static int launch(const S &s, const Pm &p, hipStream_t stream) {
-  const size_t lds = sizeof(double) * (Block<TPB>::NW * RED_K + 6 * (size_t)s.max_nb) +
+  const size_t lds = sizeof(double) * (Block<TPB>::NW * RED_K + 8 * (size_t)s.max_nb +
+                                       (s.nacc_lds ? 4 * (size_t)s.max_nn : 0)) +
                      sizeof(int) * (2 * (size_t)s.max_cells + 1 + s.max_nb + (size_t)s.excl_bits_lds +
                                     (size_t)s.max_pairs);
   if (lds > 150 * 1024) {

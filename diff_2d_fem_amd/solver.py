@@ -69,7 +69,7 @@ class DeviceSolver:
         self.lam = t(self.lam_np, F64)
         self.node2env_np = np.asarray(me.node2env, np.int64)
         self.node2env = t(self.node2env_np, I32)
-        self.block_size = 256
+        self.block_size = 512              # scene path: 2 waves per SIMD, no register spills
         self._layout = None
         self.tri2env = t(np.asarray(me.tri2env), I32)
         self.node_start = t(np.asarray(me.node_starts), I32)
@@ -324,7 +324,7 @@ class DeviceSolver:
         self.tendon = None
 
     # ------------------------------------------------------------------ scene-persistent layout
-    def build_layout(self, max_cells=4096):
+    def build_layout(self, max_cells=1024):
         """Per-scene triangle order (soft triangles by colour, then rigid ones), area-weighted
         element constants in float64 and float32, and the per-scene offsets the
         scene-persistent kernel reads."""
@@ -379,17 +379,26 @@ class DeviceSolver:
             if (bl[ends] < 0).any():
                 raise ValueError("a pressure edge has an end that is not a boundary node")
         self.max_cells = int(max_cells)
-        self._pack_contact_lists()
+        self._plan_lds()
         self._layout = True
 
     LDS_BUDGET = 150 * 1024          # bytes of LDS one scene's workgroup may take
     EXCL_LDS_MAX = 8192              # largest exclusion bitset staged in LDS, 32-bit words
+    SKIN = 0.5                       # pair-list margin, in units of the contact reach
 
-    def _pack_contact_lists(self):
-        """Each scene's boundary exclusion matrix as a bitset (bit jp * n_b_e + j), and the
-        capacity of the per-scene candidate-pair list, sized to the LDS left over."""
-        self.excl_bits_lds, self.max_pairs = 0, 0
+    def _plan_lds(self):
+        """Split each scene workgroup's LDS budget: fixed scratch (reductions, boundary
+        positions and accumulators, grid), then per-node element accumulators, then the
+        bit-packed exclusion matrix, then the candidate-pair list (bit jp * n_b_e + j of a
+        scene's bitset is its exclusion of boundary nodes jp and j)."""
+        self.excl_bits_lds, self.max_pairs, self.nacc_lds = 0, 0, 0
         self.excl_bits = self.excl_bits_off = None
+        nn = np.diff(np.asarray(self.me.node_starts, np.int64))
+        self.max_nn = int(nn.max(initial=0))
+        base = 8 * (16 * 20 + 8 * self.max_nb) + 4 * (2 * self.max_cells + 1 + self.max_nb)
+        if base + 32 * self.max_nn <= self.LDS_BUDGET:
+            self.nacc_lds = 1
+            base += 32 * self.max_nn
         if self.ipc is None or self.n_e == 0:
             return
         me = self.me
@@ -405,7 +414,6 @@ class DeviceSolver:
         self.excl_bits = self._t(np.concatenate(words).view(np.int32), I32)
         self.excl_bits_off = self._t(np.asarray(woff, np.int64), I64)
         wmax = int(np.max(np.diff(woff)))
-        base = 8 * (16 * 20 + 6 * self.max_nb) + 4 * (2 * self.max_cells + 1 + self.max_nb)
         if wmax <= self.EXCL_LDS_MAX and base + 4 * wmax <= self.LDS_BUDGET:
             self.excl_bits_lds = wmax
         left = (self.LDS_BUDGET - base - 4 * self.excl_bits_lds) // 4
@@ -483,6 +491,7 @@ class DeviceSolver:
             if self.excl_bits is not None:
                 s.excl_bits, s.excl_bits_off = p("excl_bits"), p("excl_bits_off")
             s.excl_bits_lds, s.max_pairs = self.excl_bits_lds, self.max_pairs
+            s.max_nn, s.nacc_lds = self.max_nn, self.nacc_lds
         s.n_col = len(self.colliders)
         for i, c in enumerate(self.colliders):
             s.col[i].kind = int(c["kind"])
@@ -503,6 +512,7 @@ class DeviceSolver:
             q.ipc_on = 1
             q.ipc_kappa, q.ipc_mu = self.ipc["kappa"], self.ipc["mu"]
             q.ipc_reach, q.ipc_gap = self.ipc["reach"], self.ipc["gap"]
+            q.ipc_skin = self.SKIN * self.ipc["reach"]
             q.ipc_per_body = int(self.ipc["per_body"])
         q.rigid_modes_on = int(self.rigid is not None)
         q.precision = 1 if self.precision == "mixed" else 0

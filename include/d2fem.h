@@ -124,6 +124,9 @@ typedef struct d2f_scene {
   const uint32_t *excl_bits;          /* sum_e ceil(n_b_e^2 / 32) words */
   const int64_t *excl_bits_off;       /* B+1 word offsets */
   int32_t excl_bits_lds, max_pairs;   /* words of LDS for the bitset (0: read bytes); pairs cap */
+  /* largest per-scene node count; nacc_lds = 1: the element pass accumulates every node's
+   * gradient and diagonal in LDS (32 bytes per node), 0: colour-by-colour read-modify-write */
+  int32_t max_nn, nacc_lds;
 } d2f_scene;
 
 /* Solver options and step constants (reference PNCG2DSolver.__init__, implicit/solver.py:97-226,
@@ -134,6 +137,7 @@ typedef struct d2f_params {
   double eps, tol_x, near_factor, alpha_max, area_shrink, ccd_eta;
   double ls_tol, cg_guard;
   double ipc_kappa, ipc_mu, ipc_reach, ipc_gap;
+  double ipc_skin;                          /* scene path: pair-list margin beyond the reach */
   int32_t iter_max, conv_check, friction, line_search, n_backtrack;
   int32_t ipc_on, ipc_per_body, rigid_modes_on, precision; /* precision: 0 float64, 1 mixed */
   int32_t block_size;                        /* scene path: threads per workgroup (256/512/1024) */

