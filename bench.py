@@ -174,8 +174,9 @@ def scene_roofline(res, precision):
 
 
 def elastic_roofline(res):
-    """Time the float64 elastic gradient+diagonal kernel on the run's final state with HIP
-    events on the launch stream; algorithmic bytes = 68 T + 48 N per launch."""
+    """Time the float64 elastic gradient+diagonal kernel (tiled, with the solver's node
+    ownership) on the run's final state with HIP events on the launch stream; algorithmic
+    bytes = 68 T + 48 N per launch."""
     import ctypes as C
     import torch
     from diff_2d_fem_amd import _lib
@@ -186,7 +187,8 @@ def elastic_roofline(res):
     g = torch.zeros_like(s.x)
     d = torch.zeros_like(s.x)
     P = lambda t: C.c_void_p(t.data_ptr())
-    args = (P(s.x), P(s.tri), P(s.Bm), P(s.area), P(s.mu), P(s.lam), s.n_tri, s.h2, 1, P(g), P(d), h)
+    args = (P(s.x), P(s.tri), P(s.Bm), P(s.area), P(s.mu), P(s.lam), s.n_tri, s.h2, 1, P(s.el_owner),
+            P(s.el_range), s.el_window, P(g), P(d), h)
     for _ in range(3):
         L.d2f_elastic_grad_diag_f64(*args)
     n = 20

@@ -50,6 +50,7 @@ _SCENE_FIELDS = (
     + [("max_nb", C.c_int32), ("max_cells", C.c_int32)]
     + [("excl_bits", P), ("excl_bits_off", P), ("excl_bits_lds", C.c_int32), ("max_pairs", C.c_int32)]
     + [("max_nn", C.c_int32), ("nacc_lds", C.c_int32)]
+    + [("el_owner", P), ("el_range", P), ("el_window", C.c_int32), ("pad2_", C.c_int32)]
 )
 
 
@@ -92,7 +93,7 @@ def lib():
     if hasattr(L, "d2f_step_scene"):
         L.d2f_step_scene.argtypes = [C.POINTER(Scene), C.POINTER(Params), P]
     i32, f64 = C.c_int32, C.c_double
-    L.d2f_elastic_grad_diag_f64.argtypes = [P, P, P, P, P, P, i32, f64, i32, P, P, P]
+    L.d2f_elastic_grad_diag_f64.argtypes = [P, P, P, P, P, P, i32, f64, i32, P, P, i32, P, P, P]
     L.d2f_elastic_pHp_f64.argtypes = [P, P, P, P, P, P, P, P, i32, f64, i32, P, P]
     L.d2f_inversion_alpha_f64.argtypes = [P, P, P, P, i32, f64, P, P]
     L.d2f_elastic_energy_f64.argtypes = [P, P, P, P, P, P, P, i32, f64, P, P]

@@ -7,6 +7,8 @@
 
 #include <vector>
 
+#include <climits>
+
 #include "common.h"
 
 namespace d2f {
@@ -88,34 +90,97 @@ __device__ __forceinline__ M2<R> load_B(const double *Bm, int t) {
   return {R(b.x), R(b.y), R(b.z), R(b.w)};
 }
 
-// reference implicit/kernels.py:86-128
-__global__ void k_elastic_grad_diag(const double *x, const int32_t *tri, const double *Bm,
-                                    const double *area, const double *mu, const double *lam, int T,
-                                    double h2, int clamp, double *grad, double *diag) {
-  GRID_LOOP(t, T) {
-    if (t < T) {
-      const int i0 = tri[3 * t], i1 = tri[3 * t + 1], i2 = tri[3 * t + 2];
-      const V2<double> p0 = ld2(x, i0), p1 = ld2(x, i1), p2 = ld2(x, i2);
-      const M2<double> B = load_B<double>(Bm, t);
-      const M2<double> Ds = {p1.x - p0.x, p2.x - p0.x, p1.y - p0.y, p2.y - p0.y};
-      const double m_ = mu[t], l_ = lam[t], A = area[t];
-      const NH<double> h = nh_eval(mat_mul(Ds, B), m_, l_);
-      V2<double> g0, g1, g2, d0, d1, d2;
-      nh_grad(h, B, m_, A, g0, g1, g2);
-      nh_diag(h, B, m_, l_, A, d0, d1, d2);
-      if (clamp) {
-        d0 = {fmax(d0.x, 0.0), fmax(d0.y, 0.0)};
-        d1 = {fmax(d1.x, 0.0), fmax(d1.y, 0.0)};
-        d2 = {fmax(d2.x, 0.0), fmax(d2.y, 0.0)};
+// Elastic gradient and diagonal (reference implicit/kernels.py:86-128), tiled: workgroup c
+// takes triangles [c * D2F_ELEM_CHUNK, (c + 1) * D2F_ELEM_CHUNK), ELEM_PER_THREAD per thread,
+// sums their node contributions in LDS over the chunk's node range
+// [chunk_range[2c], chunk_range[2c+1]], then adds each node once: plain read-modify-write where
+// owner[n] == c (every triangle at n is in this chunk, so no other workgroup writes n), an
+// atomic otherwise. Without the tiling arrays, or when a chunk's range is wider than `window`,
+// every contribution is an atomic.
+constexpr int ELEM_TPB = 512;
+constexpr int ELEM_PER_THREAD = D2F_ELEM_CHUNK / ELEM_TPB;
+
+__global__ __launch_bounds__(ELEM_TPB) void k_elastic_grad_diag(
+    const double *x, const int32_t *tri, const double *Bm, const double *area, const double *mu,
+    const double *lam, int T, double h2, int clamp, const int32_t *owner, const int32_t *chunk_range,
+    int window, double *grad, double *diag) {
+  extern __shared__ double acc[];  // 4 * window: grad x, grad y, diag x, diag y
+  // Workgroups are dealt to the 8 XCDs round-robin; give each XCD a contiguous run of chunks
+  // so that the halo nodes neighbouring chunks share stay in one L2.
+  const int nbk = gridDim.x, q8 = nbk / 8, r8 = nbk % 8, xcd = blockIdx.x % 8;
+  const int c = xcd * q8 + min(xcd, r8) + blockIdx.x / 8;
+  const int base = c * D2F_ELEM_CHUNK + threadIdx.x;
+  const bool tiling = owner != nullptr && chunk_range != nullptr;
+  const int lo = tiling ? chunk_range[2 * c] : 0, hi = tiling ? chunk_range[2 * c + 1] : -1;
+  const bool tiled = tiling && hi >= lo && hi - lo < window;
+  if (tiled) {
+    for (int j = threadIdx.x; j < 4 * (hi - lo + 1); j += ELEM_TPB) acc[j] = 0.0;
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < ELEM_PER_THREAD; ++k) {
+    const int t = base + k * ELEM_TPB;
+    if (t >= T) continue;
+    const int ii[3] = {tri[3 * t], tri[3 * t + 1], tri[3 * t + 2]};
+    const V2<double> p0 = ld2(x, ii[0]), p1 = ld2(x, ii[1]), p2 = ld2(x, ii[2]);
+    const M2<double> B = load_B<double>(Bm, t);
+    const double m_ = mu[t], l_ = lam[t], A = area[t];
+    const M2<double> Ds = {p1.x - p0.x, p2.x - p0.x, p1.y - p0.y, p2.y - p0.y};
+    const NH<double> h = nh_eval(mat_mul(Ds, B), m_, l_);
+    V2<double> g[3], d[3];
+    nh_grad(h, B, m_, A, g[0], g[1], g[2]);
+    nh_diag(h, B, m_, l_, A, d[0], d[1], d[2]);
+    if (clamp)
+#pragma unroll
+      for (int v = 0; v < 3; ++v) d[v] = {fmax(d[v].x, 0.0), fmax(d[v].y, 0.0)};
+    if (tiled) {
+#pragma unroll
+      for (int v = 0; v < 3; ++v) {
+        double *a = acc + 4 * (ii[v] - lo);
+        atomicAdd(a, g[v].x);
+        atomicAdd(a + 1, g[v].y);
+        atomicAdd(a + 2, d[v].x);
+        atomicAdd(a + 3, d[v].y);
       }
-      atomic_add2(grad, i0, h2 * g0);
-      atomic_add2(grad, i1, h2 * g1);
-      atomic_add2(grad, i2, h2 * g2);
-      atomic_add2(diag, i0, h2 * d0);
-      atomic_add2(diag, i1, h2 * d1);
-      atomic_add2(diag, i2, h2 * d2);
+    } else {
+#pragma unroll
+      for (int v = 0; v < 3; ++v) {
+        atomic_add2(grad, ii[v], h2 * g[v]);
+        atomic_add2(diag, ii[v], h2 * d[v]);
+      }
     }
   }
+  if (!tiled) return;
+  __syncthreads();
+  for (int f = 0; lo + (int)threadIdx.x + f * ELEM_TPB <= hi; ++f) {
+    const int n = lo + threadIdx.x + f * ELEM_TPB;
+    const double *a = acc + 4 * (n - lo);
+    const double gx = a[0], gy = a[1], dx = a[2], dy = a[3];
+    if (gx == 0.0 && gy == 0.0 && dx == 0.0 && dy == 0.0) continue;  // not a node of this chunk
+    const V2<double> gg = {h2 * gx, h2 * gy}, dd = {h2 * dx, h2 * dy};
+    if (owner[n] == c) {
+      st2(grad, n, ld2(grad, n) + gg);
+      st2(diag, n, ld2(diag, n) + dd);
+    } else {
+      atomic_add2(grad, n, gg);
+      atomic_add2(diag, n, dd);
+    }
+  }
+}
+
+static int launch_elastic_grad_diag(const double *x, const int32_t *tri, const double *Bm,
+                                    const double *area, const double *mu, const double *lam, int T,
+                                    double h2, int clamp, const int32_t *owner,
+                                    const int32_t *chunk_range, int window, double *grad,
+                                    double *diag, hipStream_t st) {
+  if (T == 0) return 0;
+  const int w = owner && chunk_range ? window : 0;
+  D2F_CHECK(w >= 0 && w <= D2F_ELEM_WINDOW_MAX, "elastic grad/diag: window out of range");
+  const int blocks = (T + D2F_ELEM_CHUNK - 1) / D2F_ELEM_CHUNK;
+  hipLaunchKernelGGL(k_elastic_grad_diag, dim3(blocks), dim3(ELEM_TPB), sizeof(double) * 4 * (size_t)w, st,
+                     x, tri, Bm, area, mu, lam, T, h2, clamp, owner, chunk_range, w, grad, diag);
+  D2F_HIP(hipGetLastError());
+  return 0;
 }
 
 // Collider barrier + lagged friction of every boundary node (one thread owns its node, so no
@@ -207,9 +272,13 @@ __device__ inline void ipc_term(const S &s, int mode, const PairCtx &c, double c
 // Every boundary node near one edge (reference implicit/ipc_contact.py:175-296), with exact
 // enumeration over the edge's environment in place of the hash-grid query. mode 0 grad+diag,
 // 1 pHp + clearance + step limit, 2 energy, 3 clearance + step limit along pdir.
+// bp, bn: when not null, the environment's boundary positions and node ids staged in LDS
+// (indexed by local boundary index); the distance filter then runs before the exclusion
+// lookup (both are pure filters, so the pair set is the same).
 __device__ inline void ipc_edge(const S &s, int k, int mode, const PairCtx &c, const double *pdir,
                                 double *grad, double *diag, const uint8_t *freev, double *pHp,
-                                double *E, double *clr, double *alpha, double t_c, const double *mu_node) {
+                                double *E, double *clr, double *alpha, double t_c, const double *mu_node,
+                                const double *bp = nullptr, const int *bn = nullptr) {
   const int a = s.ea[k], b = s.eb[k], e = s.e2env[k];
   const int start = s.bnd_start[e], n_be = s.bnd_start[e + 1] - start;
   const int64_t eoff = s.excl_off[e];
@@ -221,11 +290,11 @@ __device__ inline void ipc_edge(const S &s, int k, int mode, const PairCtx &c, c
   const int la = s.ela[k], lb = s.elb[k];
   double amin = BIG, cmin = BIG;
   for (int jl = 0; jl < n_be; ++jl) {
-    const int pn = s.bnd[start + jl];
+    const int pn = bn ? bn[jl] : s.bnd[start + jl];
     if (pn == a || pn == b) continue;
     const uint8_t *row = s.excl + eoff + (int64_t)jl * n_be;
-    if (row[la] || row[lb]) continue;
-    const V2<double> xp = ld2(s.x, pn);
+    if (!bp && (row[la] || row[lb])) continue;
+    const V2<double> xp = bp ? V2<double>{bp[2 * jl], bp[2 * jl + 1]} : ld2(s.x, pn);
     const V2<double> w = xp - xa;
     const double tt = dot(w, ab) / L2;
     V2<double> dvec = w - tt * ab;
@@ -233,6 +302,7 @@ __device__ inline void ipc_edge(const S &s, int k, int mode, const PairCtx &c, c
     else if (tt >= 1.0) dvec = xp - xb;
     double d = len(dvec);
     if (!(d < c.reach)) continue;
+    if (bp && (row[la] || row[lb])) continue;
     if (mode == 1 || mode == 3) {
       cmin = fmin(cmin, d);
       double keep = fmax(c.gap * d, c.dhat - onset);
@@ -312,6 +382,30 @@ __global__ void k_ipc(S s, int mode, PairCtx c, const double *pdir, double *grad
       ipc_edge(s, k, mode, c, pdir, grad, diag, freev, s.pHp, E, clr, alpha,
                (mode == 1 || mode == 3) ? tc[s.e2env[k]] : 0.0, s.mu_node);
   }
+}
+
+// The same pass with one workgroup per environment: its boundary positions and node ids are
+// staged in LDS (24 bytes per boundary node) and its edges are spread over the threads.
+__global__ __launch_bounds__(256) void k_ipc_env(S s, int mode, PairCtx c, const double *pdir,
+                                                 double *grad, double *diag, const uint8_t *freev,
+                                                 double *E, double *clr, double *alpha,
+                                                 const double *tc) {
+  extern __shared__ double sh[];
+  double *bp = sh;
+  int *bn = reinterpret_cast<int *>(sh + 2 * s.max_nb);
+  const int e = blockIdx.x;
+  const int start = s.bnd_start[e], n_be = s.bnd_start[e + 1] - start;
+  for (int j = threadIdx.x; j < n_be; j += blockDim.x) {
+    const int n = s.bnd[start + j];
+    const V2<double> x = ld2(s.x, n);
+    bn[j] = n;
+    bp[2 * j] = x.x;
+    bp[2 * j + 1] = x.y;
+  }
+  __syncthreads();
+  const double t_c = (mode == 1 || mode == 3) ? tc[e] : 0.0;
+  for (int k = s.e_start[e] + threadIdx.x; k < s.e_start[e + 1]; k += blockDim.x)
+    ipc_edge(s, k, mode, c, pdir, grad, diag, freev, s.pHp, E, clr, alpha, t_c, s.mu_node, bp, bn);
 }
 
 // alpha <= 0.45 reach / p_inf (reference implicit/ipc_contact.py:300-305)
@@ -472,22 +566,30 @@ __global__ void k_tendon_energy(S s, double h2, double *E) {
 
 // ------------------------------------------------------------------------- affine bodies
 // reference implicit/affine.py:61-269
+// The slaved nodes are grouped by body, so a wave's lanes almost always share a group: each
+// sum is reduced across the wave first and lands as one atomic per wave (env_add).
 __global__ void k_affine_gather(S s) {
   GRID_LOOP(k, s.n_an) {
-    if (k < s.n_an) {
-      const int i = s.an_node[k], b = s.an_grp[k];
-      const double w[3] = {s.an_w[3 * k], s.an_w[3 * k + 1], s.an_w[3 * k + 2]};
-      const V2<double> g = ld2(s.grad, i), d = ld2(s.diag, i);
-      for (int r = 0; r < 3; ++r) {
-        atomicAdd(s.ag_gq + 6 * b + r, g.x * w[r]);
-        atomicAdd(s.ag_gq + 6 * b + 3 + r, g.y * w[r]);
-        for (int c = 0; c < 3; ++c) {
-          atomicAdd(s.ag_Hx + 9 * b + 3 * r + c, d.x * w[r] * w[c]);
-          atomicAdd(s.ag_Hy + 9 * b + 3 * r + c, d.y * w[r] * w[c]);
-        }
-      }
-      st2(s.grad, i, V2<double>{0.0, 0.0});
+    const bool ok = k < s.n_an;
+    const int i = ok ? s.an_node[k] : 0, b = ok ? s.an_grp[k] : -1;
+    double w[3] = {0.0, 0.0, 0.0};
+    V2<double> g = {0.0, 0.0}, d = {0.0, 0.0};
+    if (ok) {
+      w[0] = s.an_w[3 * k];
+      w[1] = s.an_w[3 * k + 1];
+      w[2] = s.an_w[3 * k + 2];
+      g = ld2(s.grad, i);
+      d = ld2(s.diag, i);
     }
+    for (int r = 0; r < 3; ++r) {
+      env_add(s.ag_gq + r, 6 * b, g.x * w[r], ok);
+      env_add(s.ag_gq + 3 + r, 6 * b, g.y * w[r], ok);
+      for (int c = 0; c < 3; ++c) {
+        env_add(s.ag_Hx + 3 * r + c, 9 * b, d.x * w[r] * w[c], ok);
+        env_add(s.ag_Hy + 3 * r + c, 9 * b, d.y * w[r] * w[c], ok);
+      }
+    }
+    if (ok) st2(s.grad, i, V2<double>{0.0, 0.0});
   }
 }
 
@@ -585,14 +687,13 @@ __global__ void k_affine_pHp_energy(S s, int mode, double h2, double *E) {
 // reference implicit/affine.py:405-486
 __global__ void k_rigid_centroid(S s) {
   GRID_LOOP(k, s.n_rn) {
-    if (k < s.n_rn) {
-      const int i = s.rn_node[k], b = s.rn_grp[k];
-      const double m = s.mass[i];
-      const V2<double> x = ld2(s.x, i);
-      atomicAdd(s.rg_acc + 3 * b, m * x.x);
-      atomicAdd(s.rg_acc + 3 * b + 1, m * x.y);
-      atomicAdd(s.rg_acc + 3 * b + 2, m);
-    }
+    const bool ok = k < s.n_rn;
+    const int i = ok ? s.rn_node[k] : 0, b = ok ? s.rn_grp[k] : -1;
+    const double m = ok ? s.mass[i] : 0.0;
+    const V2<double> x = ok ? ld2(s.x, i) : V2<double>{0.0, 0.0};
+    env_add(s.rg_acc, 3 * b, m * x.x, ok);
+    env_add(s.rg_acc + 1, 3 * b, m * x.y, ok);
+    env_add(s.rg_acc + 2, 3 * b, m, ok);
   }
 }
 __device__ __forceinline__ V2<double> rigid_arm(const S &s, int i, int b) {
@@ -602,28 +703,32 @@ __device__ __forceinline__ V2<double> rigid_arm(const S &s, int i, int b) {
 }
 __global__ void k_rigid_gather(S s) {
   GRID_LOOP(k, s.n_rn) {
-    if (k < s.n_rn) {
-      const int i = s.rn_node[k], b = s.rn_grp[k];
-      const V2<double> rr = rigid_arm(s, i, b);
-      const V2<double> g = ld2(s.grad, i), gp = ld2(s.grad_prev, i);
+    const bool ok = k < s.n_rn;
+    const int i = ok ? s.rn_node[k] : 0, b = ok ? s.rn_grp[k] : -1;
+    V2<double> rr = {0.0, 0.0}, g = {0.0, 0.0}, gp = {0.0, 0.0};
+    double dx = 0.0, dy = 0.0;
+    if (ok) {
+      rr = rigid_arm(s, i, b);
+      g = ld2(s.grad, i);
+      gp = ld2(s.grad_prev, i);
       const double m = s.mass[i];
       const V2<double> d = ld2(s.diag, i) - ld2(s.diag_el, i);
-      const double dx = fmax(d.x, 0.0) + m, dy = fmax(d.y, 0.0) + m;
-      double *gq = s.rg_gq + 3 * b, *gq0 = s.rg_gq_prev + 3 * b, *H = s.rg_H + 9 * b;
-      atomicAdd(gq, g.x);
-      atomicAdd(gq + 1, g.y);
-      atomicAdd(gq + 2, rr.x * g.y - rr.y * g.x);
-      atomicAdd(gq0, gp.x);
-      atomicAdd(gq0 + 1, gp.y);
-      atomicAdd(gq0 + 2, rr.x * gp.y - rr.y * gp.x);
-      atomicAdd(H + 0, dx);
-      atomicAdd(H + 2, -dx * rr.y);
-      atomicAdd(H + 4, dy);
-      atomicAdd(H + 5, dy * rr.x);
-      atomicAdd(H + 6, -dx * rr.y);
-      atomicAdd(H + 7, dy * rr.x);
-      atomicAdd(H + 8, dx * rr.y * rr.y + dy * rr.x * rr.x);
+      dx = fmax(d.x, 0.0) + m;
+      dy = fmax(d.y, 0.0) + m;
     }
+    env_add(s.rg_gq, 3 * b, g.x, ok);
+    env_add(s.rg_gq + 1, 3 * b, g.y, ok);
+    env_add(s.rg_gq + 2, 3 * b, rr.x * g.y - rr.y * g.x, ok);
+    env_add(s.rg_gq_prev, 3 * b, gp.x, ok);
+    env_add(s.rg_gq_prev + 1, 3 * b, gp.y, ok);
+    env_add(s.rg_gq_prev + 2, 3 * b, rr.x * gp.y - rr.y * gp.x, ok);
+    env_add(s.rg_H, 9 * b, dx, ok);
+    env_add(s.rg_H + 2, 9 * b, -dx * rr.y, ok);
+    env_add(s.rg_H + 4, 9 * b, dy, ok);
+    env_add(s.rg_H + 5, 9 * b, dy * rr.x, ok);
+    env_add(s.rg_H + 6, 9 * b, -dx * rr.y, ok);
+    env_add(s.rg_H + 7, 9 * b, dy * rr.x, ok);
+    env_add(s.rg_H + 8, 9 * b, dx * rr.y * rr.y + dy * rr.x * rr.x, ok);
   }
 }
 __global__ void k_rigid_reduce(S s) {
@@ -1009,7 +1114,14 @@ struct Plain {
       // t_c snapshot: the ACCD of every pair stops at the alpha the pass started from
       D2F_HIP(hipMemcpyAsync(s.tc, alpha, sizeof(double) * s.B, hipMemcpyDeviceToDevice, st));
     }
-    LAUNCH(k_ipc, s.n_e, s, mode, c, pdir, grad, diag, freev, E, clr, alpha, s.tc);
+    const size_t lds = (2 * sizeof(double) + sizeof(int)) * (size_t)s.max_nb;
+    if (s.e_start && s.max_nb > 0 && lds <= 48 * 1024) {
+      hipLaunchKernelGGL(k_ipc_env, dim3(s.B), dim3(256), lds, st, s, mode, c, pdir, grad, diag, freev, E,
+                         clr, alpha, s.tc);
+      D2F_HIP(hipGetLastError());
+    } else {
+      LAUNCH(k_ipc, s.n_e, s, mode, c, pdir, grad, diag, freev, E, clr, alpha, s.tc);
+    }
     return 0;
   }
 
@@ -1043,7 +1155,11 @@ struct Plain {
 
   int grad_diag() {
     LAUNCH(k_grad_diag_mass, s.N, s, p.h2);
-    LAUNCH(k_elastic_grad_diag, s.T, s.x, s.tri, s.Bm, s.area, s.mu, s.lam, s.T, p.h2, 1, s.grad, s.diag);
+    {
+      const int rc = launch_elastic_grad_diag(s.x, s.tri, s.Bm, s.area, s.mu, s.lam, s.T, p.h2, 1,
+                                              s.el_owner, s.el_range, s.el_window, s.grad, s.diag, st);
+      if (rc) return rc;
+    }
     if (s.n_rg > 0)
       D2F_HIP(hipMemcpyAsync(s.diag_el, s.diag, sizeof(double) * 2 * s.N, hipMemcpyDeviceToDevice, st));
     if (s.n_col > 0) LAUNCH(k_colliders_grad_diag, s.n_b, s, p);
@@ -1226,12 +1342,11 @@ void d2f_struct_sizes(int64_t *out) {
 
 int d2f_elastic_grad_diag_f64(const double *x, const int32_t *tri, const double *Bm, const double *area,
                               const double *mu, const double *lam, int32_t T, double h2, int32_t clamp,
+                              const int32_t *owner, const int32_t *chunk_range, int32_t window,
                               double *grad, double *diag, void *stream) {
   D2F_CHECK(T >= 0 && x && tri && Bm && area && mu && lam && grad && diag, "d2f_elastic_grad_diag_f64: bad argument");
-  hipStream_t st = (hipStream_t)stream;
-  LAUNCH(k_elastic_grad_diag, T, x, tri, Bm, area, mu, lam, T, h2, clamp, grad, diag);
-  D2F_HIP(hipGetLastError());
-  return 0;
+  return launch_elastic_grad_diag(x, tri, Bm, area, mu, lam, T, h2, clamp, owner, chunk_range, window,
+                                  grad, diag, (hipStream_t)stream);
 }
 
 int d2f_elastic_pHp_f64(const double *x, const double *p, const int32_t *tri, const double *Bm,

@@ -65,3 +65,33 @@ def check_coloring(tri, color):
 def starts_of(keys, n):
     """CSR offsets of items grouped by key in 0..n-1 (keys must be sorted)."""
     return np.searchsorted(np.asarray(keys), np.arange(n + 1), side="left").astype(np.int32)
+
+
+ELEM_CHUNK = 1024          # triangles per workgroup of the tiled element kernel (D2F_ELEM_CHUNK)
+ELEM_WINDOW_MAX = 4096     # largest node window a chunk accumulates in LDS
+
+
+def element_owners(tri, n_nodes, chunk=ELEM_CHUNK):
+    """Node ownership for the tiled element kernel. Triangle t belongs to chunk t // chunk;
+    owner[n] is that chunk when every triangle at node n belongs to it (the chunk may then
+    add into the node with a plain read-modify-write), else -1 (atomics). Also returns each
+    chunk's node range (smallest and largest node index of its triangles, interleaved) and
+    the window: the largest range size, capped at ELEM_WINDOW_MAX (a chunk with a wider
+    range falls back to atomics)."""
+    tri = np.asarray(tri, np.int64).reshape(-1, 3)
+    T = len(tri)
+    if T == 0:
+        return np.full(n_nodes, -1, np.int32), np.zeros(0, np.int32), 1
+    c = np.repeat(np.arange(T, dtype=np.int64) // chunk, 3)
+    flat = tri.reshape(-1)
+    lo = np.full(n_nodes, np.iinfo(np.int64).max)
+    hi = np.full(n_nodes, -1, np.int64)
+    np.minimum.at(lo, flat, c)
+    np.maximum.at(hi, flat, c)
+    owner = np.where(lo == hi, hi, -1).astype(np.int32)
+    starts = np.arange(0, T, chunk)
+    wmin = np.minimum.reduceat(tri.min(1), starts)
+    wmax = np.maximum.reduceat(tri.max(1), starts)
+    rng = np.stack([wmin, wmax], 1).reshape(-1).astype(np.int32)
+    window = int(min((wmax - wmin + 1).max(), ELEM_WINDOW_MAX))
+    return owner, rng, window

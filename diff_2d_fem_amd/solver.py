@@ -72,6 +72,9 @@ class DeviceSolver:
         self.block_size = 512              # scene path: 2 waves per SIMD, no register spills
         self._layout = None
         self.tri2env = t(np.asarray(me.tri2env), I32)
+        from .layout import element_owners
+        own, rng, self.el_window = element_owners(me.tri, self.N)
+        self.el_owner, self.el_range = t(own, I32), t(rng, I32)
         self.node_start = t(np.asarray(me.node_starts), I32)
         self.bnd = t(np.asarray(me.boundary_nodes), I32)
         self.b2e = t(np.asarray(me.boundary2env), I32)
@@ -132,6 +135,10 @@ class DeviceSolver:
         self.excl_off = t(np.asarray(me.excl_offset, np.int64), I64)
         self.mu_node = None if mu_node is None else t(mu_node, F64)
         self.n_e = len(edges)
+        if len(env) and (np.diff(env) < 0).any():
+            raise RuntimeError("boundary edges are not grouped by environment")
+        self.e_start = t(np.searchsorted(env, np.arange(self.B + 1)), I32)
+        self.max_nb = int(np.max(np.asarray(me.boundary_per_env))) if self.B else 0
 
     def refresh_exclusion(self):
         if self.ipc is not None:
@@ -450,10 +457,12 @@ class DeviceSolver:
                   "q_inf", "sweep_beta", "tc", "active", "iters"):
             setattr(s, k, p(k))
         s.free_ = p("free")
+        s.el_owner, s.el_range, s.el_window = p("el_owner"), p("el_range"), self.el_window
         if self.ipc is not None:
             for k in ("ea", "eb", "ela", "elb", "e2env", "ca", "cb", "excl", "excl_off"):
                 setattr(s, k, p(k))
             s.mu_node = self._ptr(self.mu_node)
+            s.e_start, s.max_nb = p("e_start"), self.max_nb
         s.fext = self._ptr(self.fext)
         if self.pressure is not None:
             s.n_pe, s.n_ch = self.pressure["n_pe"], self.pressure["n_ch"]

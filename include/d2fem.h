@@ -35,6 +35,8 @@ extern "C" {
 
 #define D2F_MAX_COLLIDERS 8
 #define D2F_MAX_COLORS 24
+#define D2F_ELEM_CHUNK 1024       /* triangles per workgroup of the tiled element kernel */
+#define D2F_ELEM_WINDOW_MAX 4096  /* largest node window one chunk sums in LDS */
 
 /* A rigid collider (reference sim_model.py:167-174; kinds core/objects.py:22-24):
  * 0 disc, 1 ellipse, 2 rounded box, 3 rounded triangle, 4 half-space (rx, ry = unit normal). */
@@ -127,6 +129,10 @@ typedef struct d2f_scene {
   /* largest per-scene node count; nacc_lds = 1: the element pass accumulates every node's
    * gradient and diagonal in LDS (32 bytes per node), 0: colour-by-colour read-modify-write */
   int32_t max_nn, nacc_lds;
+  /* per-kernel path: node ownership of the tiled element kernel (see d2f_elastic_grad_diag_f64;
+   * NULL: atomics only) and its node window */
+  const int32_t *el_owner, *el_range;
+  int32_t el_window, pad2_;
 } d2f_scene;
 
 /* Solver options and step constants (reference PNCG2DSolver.__init__, implicit/solver.py:97-226,
@@ -153,10 +159,17 @@ void d2f_struct_sizes(int64_t *out);
 
 /* ---- per-kernel entry points (float64), one per reference kernel family ------------------ */
 /* grad += h2 dPsi/dx, diag += h2 max(diag d2Psi/dx2, 0) over all triangles.
- * Replaces grad_diag_elastic_kernel (implicit/kernels.py:86-128). */
+ * Replaces grad_diag_elastic_kernel (implicit/kernels.py:86-128). Tiled: triangle t belongs to
+ * chunk t / D2F_ELEM_CHUNK; owner[n] = c when every triangle at node n is in chunk c (else -1);
+ * chunk_range[2c], chunk_range[2c+1] = smallest and largest node index of chunk c's triangles;
+ * window = the largest range size (<= D2F_ELEM_WINDOW_MAX). Chunk c sums its contributions in
+ * LDS and adds them once per node: plain read-modify-write for the nodes it owns, atomics for
+ * the rest. owner or chunk_range NULL: one atomic per contribution. */
 int d2f_elastic_grad_diag_f64(const double *x, const int32_t *tri, const double *Bm,
                               const double *area, const double *mu, const double *lam, int32_t T,
-                              double h2, int32_t clamp, double *grad, double *diag, void *stream);
+                              double h2, int32_t clamp, const int32_t *owner,
+                              const int32_t *chunk_range, int32_t window, double *grad,
+                              double *diag, void *stream);
 /* pHp[env] += h2 max(p^T d2Psi p, 0) per triangle. Replaces pHp_elastic_kernel
  * (implicit/kernels.py:259-273). */
 int d2f_elastic_pHp_f64(const double *x, const double *p, const int32_t *tri, const double *Bm,

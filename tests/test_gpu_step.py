@@ -13,6 +13,7 @@ import pytest
 import torch
 
 from diff_2d_fem_amd import ImplicitIntegrator, _lib
+from diff_2d_fem_amd.layout import element_owners
 from oracle import terms as OT
 from oracle.pncg import OracleIntegrator
 from tests.scenes import strip, grip, close_jaws
@@ -52,15 +53,23 @@ def test_elastic_grad_diag(deformed):
     tri = torch.tensor(me.tri, dtype=torch.int32, device=d)
     Bm = torch.tensor(me.D_m_inv.reshape(-1, 4), device=d)
     A, mu, lam = (torch.tensor(v, device=d) for v in (me.rest_area, me.mu, me.lam))
-    for clamp in (1, 0):
-        g = torch.zeros_like(xt)
-        dg = torch.zeros_like(xt)
-        _lib.check(L.d2f_elastic_grad_diag_f64(_p(xt), _p(tri), _p(Bm), _p(A), _p(mu), _p(lam), T,
-                                               4e-6, clamp, _p(g), _p(dg), _stream()), "grad_diag")
-        go, do = np.zeros_like(x), np.zeros_like(x)
-        OT.elastic_grad_diag(x, me.tri, me.D_m_inv, me.rest_area, me.mu, me.lam, 4e-6, clamp, go, do)
-        assert _rel(g.cpu().numpy(), go) < 1e-12
-        assert _rel(dg.cpu().numpy(), do) < 1e-12
+    own, rng, window = element_owners(me.tri, len(x))
+    assert window > 1 and (own >= 0).any() and (own < 0).any()
+    ot, rt = torch.tensor(own, device=d), torch.tensor(rng, device=d)
+    # atomics only; tiled with ownership; tiled with a window too small for any chunk (fallback)
+    for owner, win in ((None, 0), (ot, window), (ot, 8)):
+        for clamp in (1, 0):
+            # the kernel accumulates: start from values at the scale of the elastic terms
+            g0 = torch.tensor(1e-7 * np.random.default_rng(5).normal(size=x.shape), device=d)
+            g, dg = g0.clone(), g0.clone()
+            _lib.check(L.d2f_elastic_grad_diag_f64(_p(xt), _p(tri), _p(Bm), _p(A), _p(mu), _p(lam), T,
+                                                   4e-6, clamp, None if owner is None else _p(owner),
+                                                   None if owner is None else _p(rt), win,
+                                                   _p(g), _p(dg), _stream()), "grad_diag")
+            go, do = g0.cpu().numpy().copy(), g0.cpu().numpy().copy()
+            OT.elastic_grad_diag(x, me.tri, me.D_m_inv, me.rest_area, me.mu, me.lam, 4e-6, clamp, go, do)
+            assert _rel(g.cpu().numpy(), go) < 1e-12
+            assert _rel(dg.cpu().numpy(), do) < 1e-12
 
 
 def test_elastic_curvature_energy_caps_hvp(deformed):

@@ -91,3 +91,26 @@ def test_coloring_is_valid(name):
 def test_model_refuses_cpu_device():
     with pytest.raises(ValueError, match="GPU"):
         Model(device="cpu")
+
+
+def test_element_ownership_layout():
+    """Tiled element kernel layout: owner[n] is the one chunk holding every triangle at n (else
+    -1); the ranges bound each chunk's node indices; checked by brute force."""
+    from diff_2d_fem_amd.layout import element_owners
+    me = build_scene([dict(b) for b in scenes()["multi"]])
+    tri = np.asarray(me.tri)
+    N = me.rest_pos.shape[0]
+    own, rng, window = element_owners(tri, N, chunk=37)
+    chunks_at = [set() for _ in range(N)]
+    for t, tr in enumerate(tri):
+        for n in tr:
+            chunks_at[n].add(t // 37)
+    for n in range(N):
+        want = next(iter(chunks_at[n])) if len(chunks_at[n]) == 1 else -1
+        assert own[n] == want
+    nch = (len(tri) + 36) // 37
+    assert len(rng) == 2 * nch
+    for c in range(nch):
+        sub = tri[c * 37:(c + 1) * 37]
+        assert rng[2 * c] == sub.min() and rng[2 * c + 1] == sub.max()
+    assert window == int((rng[1::2] - rng[0::2] + 1).max())
