@@ -98,6 +98,11 @@ def lib():
     L.d2f_inversion_alpha_f64.argtypes = [P, P, P, P, i32, f64, P, P]
     L.d2f_elastic_energy_f64.argtypes = [P, P, P, P, P, P, P, i32, f64, P, P]
     L.d2f_elastic_hvp_f64.argtypes = [P, P, P, P, P, P, P, i32, f64, P, P]
+    L.d2f_collider_force.argtypes = [C.POINTER(Scene), C.POINTER(Params), i32, i32, P, P]
+    L.d2f_contact_force.argtypes = [C.POINTER(Scene), C.POINTER(Params), i32, i32, i32, P, P, P, P]
+    L.d2f_collider_hvp_f64.argtypes = [P, P, P, i32, P, i32, i32, f64, f64, f64, P, P]
+    L.d2f_collider_grad_diag_f64.argtypes = [P, P, i32, P, i32, i32, f64, f64, f64, i32, P, P, P]
+    L.d2f_elastic_drho_f64.argtypes = [P, P, P, P, P, P, P, i32, f64, f64, P, P]
     sizes = (C.c_int64 * 3)()
     L.d2f_struct_sizes(sizes)
     want = (C.sizeof(Scene), C.sizeof(Params), C.sizeof(Collider))

@@ -1,0 +1,326 @@
+"""Adjoint of the implicit step: dJ/d(density) through the implicit function theorem
+(reference implicit/adjoint.py:1-399, AdjointPNCG2D).
+
+Density enters as a per-element multiplier of the full-density moduli,
+mu_e = E(rho_e) mu0_e, lam_e = E(rho_e) lam0_e with E(rho) = Emin + rho (1 - Emin). A
+trajectory x_0 .. x_T is differentiated by the damped backward-Euler recursion
+
+    H_k lam_k = dJ/dx_k + (1 + drag) M lam_{k+1} - drag M lam_{k+2}
+    dJ/drho_e = sum_k -(1 - Emin) lam_k . [h^2 f_unit_e(x_k)]
+
+with H_k = M + h^2 (K_elastic + K_contact) at x_k, restricted to the free degrees of freedom,
+the contact active set frozen at x_k. The operator holds inertia, elasticity and the
+colliders' barriers; friction, contact between bodies, pressure, tendons and rigid regions
+have no tangent in it (transient_adjoint refuses a model with any of the last four, as the
+reference does).
+
+Everything runs on the GPU: the Hessian-vector products and dJ/drho are HIP kernels
+(d2f_elastic_hvp_f64, d2f_collider_hvp_f64, d2f_elastic_drho_f64), and the preconditioned
+MINRES iteration keeps its vectors in device memory; only its scalars come to the host.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+
+F64 = torch.float64
+
+
+class AdjointPNCG2D:
+    """dJ/drho of a DeviceSolver's implicit steps (reference implicit/adjoint.py:181-399)."""
+
+    def __init__(self, solver, Emin=1e-3):
+        self.s = solver
+        self.dev = solver.dev
+        me = solver.me
+        self.Emin = float(Emin)
+        self.mu0 = torch.as_tensor(np.asarray(me.mu, np.float64), device=self.dev)
+        self.lam0 = torch.as_tensor(np.asarray(me.lam, np.float64), device=self.dev)
+        self.N, self.n_tri = solver.N, solver.n_tri
+        self.h2 = float(solver.h2)
+        self.free_np = np.asarray(solver.free_np, np.float64)          # (N, 2): 1 free, 0 held
+        self._free = torch.as_tensor(self.free_np, device=self.dev)
+        self.refresh_mass()
+        self._y = torch.zeros((self.N, 2), dtype=F64, device=self.dev)
+        self._drho = torch.zeros(self.n_tri, dtype=F64, device=self.dev)
+        cols = solver.colliders
+        self._cols = (_lib.Collider * max(1, len(cols)))()
+        for i, c in enumerate(cols):
+            self._cols[i].kind = int(c["kind"])
+            for k in ("cx", "cy", "rx", "ry", "rref", "r", "mu"):
+                setattr(self._cols[i], k, float(c[k]))
+        self._n_col = len(cols)
+
+    # ---- plumbing
+    def _t(self, a):
+        if torch.is_tensor(a):
+            return a.to(device=self.dev, dtype=F64).contiguous()
+        return torch.as_tensor(np.asarray(a, np.float64), device=self.dev).contiguous()
+
+    @staticmethod
+    def _p(t):
+        return C.c_void_p(t.data_ptr())
+
+    def _stream(self):
+        return C.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
+
+    def refresh_mass(self):
+        """Re-read the solver's nodal mass (call after changing solver.mass)."""
+        self.mass_np = np.asarray(self.s.mass.cpu().numpy(), np.float64)
+        self._mass2 = self.s.mass.to(F64)[:, None]
+        return self.mass_np
+
+    # ---- density (reference implicit/adjoint.py:41-50, 225-228)
+    def set_density(self, rho_e):
+        rho = np.asarray(rho_e, np.float64).reshape(-1)
+        if rho.shape[0] != self.n_tri:
+            raise ValueError(f"density has {rho.shape[0]} entries, the mesh {self.n_tri} elements")
+        E = self.Emin + rho * (1.0 - self.Emin)
+        self.s.mu_np = E * np.asarray(self.s.me.mu, np.float64)
+        self.s.lam_np = E * np.asarray(self.s.me.lam, np.float64)
+        self.s.refresh_moduli()
+
+    # ---- operator H v = M v + h^2 (K_elastic + K_contact) v (reference :231-256)
+    def _hvp(self, xw, vw, use_disc=False, use_sdf=False, include_mass=True):
+        L = _lib.lib()
+        s = self.s
+        y = self._y
+        y.zero_()
+        _lib.check(L.d2f_elastic_hvp_f64(self._p(xw), self._p(vw), self._p(s.tri), self._p(s.Bm),
+                                         self._p(s.area), self._p(s.mu), self._p(s.lam), self.n_tri,
+                                         self.h2, self._p(y), self._stream()), "d2f_elastic_hvp_f64")
+        which = 2 if (use_disc and use_sdf) else (0 if use_disc else 1)
+        if (use_disc or use_sdf) and self._n_col:
+            _lib.check(L.d2f_collider_hvp_f64(self._p(xw), self._p(vw), self._p(s.bnd), s.n_b, self._cols,
+                                              self._n_col, which, float(s.cfg.ipc_d_hat),
+                                              float(s.cfg.obstacle_stiffness), self.h2, self._p(y),
+                                              self._stream()), "d2f_collider_hvp_f64")
+        return self._mass2 * vw + y if include_mass else y.clone()
+
+    def apply_H(self, xstar, v, use_self=False, use_disc=False, use_sdf=False, include_mass=True):
+        """H v at x* as an (N, 2) array (the reference's matrix-free product)."""
+        return self._hvp(self._t(xstar), self._t(v), use_disc, use_sdf, include_mass).cpu().numpy()
+
+    def spd_probe(self, xstar, use_self=False, include_mass=True, n=16, seed=0):
+        """min and max Rayleigh quotient p^T H p / p^T p over random free directions."""
+        rng = np.random.default_rng(seed)
+        xw = self._t(xstar)
+        vals = []
+        for _ in range(n):
+            p = self._t(rng.standard_normal((self.N, 2))) * self._free
+            Hp = self._hvp(xw, p, include_mass=include_mass) * self._free
+            vals.append(float((p * Hp).sum() / (p * p).sum()))
+        return min(vals), max(vals)
+
+    # ---- Jacobi preconditioner diag(M + h^2 (K_elastic + contact)) (reference :356-372)
+    def _diag_precond(self, xw, use_disc=False, use_sdf=False):
+        L = _lib.lib()
+        s = self.s
+        g = torch.zeros((self.N, 2), dtype=F64, device=self.dev)
+        d = self._mass2.expand(self.N, 2).clone()
+        _lib.check(L.d2f_elastic_grad_diag_f64(self._p(xw), self._p(s.tri), self._p(s.Bm), self._p(s.area),
+                                               self._p(s.mu), self._p(s.lam), self.n_tri, self.h2, 1,
+                                               self._p(s.el_owner), self._p(s.el_range), s.el_window,
+                                               self._p(g), self._p(d), self._stream()), "elastic grad/diag")
+        if (use_disc or use_sdf) and self._n_col:
+            which = 2 if (use_disc and use_sdf) else (0 if use_disc else 1)
+            _lib.check(L.d2f_collider_grad_diag_f64(self._p(xw), self._p(s.bnd), s.n_b, self._cols,
+                                                    self._n_col, which, float(s.cfg.ipc_d_hat),
+                                                    float(s.cfg.obstacle_stiffness), self.h2, 1,
+                                                    self._p(g), self._p(d), self._stream()),
+                       "collider grad/diag")
+        return torch.clamp(d, min=1e-30)
+
+    # ---- Krylov solves on the free degrees of freedom
+    def _minres(self, op, b, dinv, rtol, maxiter):
+        """Preconditioned MINRES (Paige and Saunders, SIAM J. Numer. Anal. 12, 1975) for the
+        symmetric, possibly indefinite op; the preconditioner is the diagonal dinv (zero on
+        held degrees of freedom, so every iterate stays on the free ones). Stops when the
+        preconditioned residual norm falls to rtol times that of b. Returns (x, info, iters):
+        info 0 converged, 1 iteration limit, -1 breakdown (a non-positive preconditioned
+        norm)."""
+        x = torch.zeros_like(b)
+        r1 = b.clone()
+        y = dinv * r1
+        beta1 = float((r1 * y).sum())
+        if beta1 < 0.0:
+            return x, -1, 0
+        beta1 = math.sqrt(beta1)
+        if beta1 == 0.0:
+            return x, 0, 0
+        r2 = r1.clone()
+        w = torch.zeros_like(b)
+        w2 = torch.zeros_like(b)
+        oldb, beta, dbar, epsln, phibar, cs, sn = 0.0, beta1, 0.0, 0.0, beta1, -1.0, 0.0
+        tiny = 1e-300
+        for it in range(1, maxiter + 1):
+            v = y / beta
+            y = op(v)
+            if it >= 2:
+                y = y - (beta / oldb) * r1
+            alfa = float((v * y).sum())
+            y = y - (alfa / beta) * r2
+            r1, r2 = r2, y
+            y = dinv * r2
+            oldb = beta
+            bb = float((r2 * y).sum())
+            if bb < 0.0:
+                return x, -1, it
+            beta = math.sqrt(bb)
+            oldeps = epsln
+            delta = cs * dbar + sn * alfa
+            gbar = sn * dbar - cs * alfa
+            epsln = sn * beta
+            dbar = -cs * beta
+            gamma = max(math.hypot(gbar, beta), tiny)
+            cs, sn = gbar / gamma, beta / gamma
+            phi = cs * phibar
+            phibar = sn * phibar
+            w1, w2 = w2, w
+            w = (v - oldeps * w1 - delta * w2) / gamma
+            x = x + phi * w
+            if phibar <= rtol * beta1:
+                return x, 0, it
+            if beta == 0.0:
+                return x, 0, it
+        return x, 1, maxiter
+
+    def adjoint_solve(self, xk, rhs, use_disc=False, use_self=False, use_sdf=False, rtol=1e-9,
+                      maxiter=5000):
+        """One adjoint solve H_k lam = rhs on the free degrees of freedom, MINRES with the Jacobi
+        preconditioner (reference implicit/adjoint.py:375-404). Records minres_calls,
+        minres_fail, minres_last_info (and minres_resid when minres_track_resid is set)."""
+        xw = self._t(xk)
+        b = self._t(rhs) * self._free
+        dinv = self._free / self._diag_precond(xw, use_disc, use_sdf)
+
+        def op(v):
+            return self._hvp(xw, v, use_disc, use_sdf) * self._free
+        lam, info, its = self._minres(op, b, dinv, float(rtol), int(maxiter))
+        self.minres_calls = getattr(self, "minres_calls", 0) + 1
+        self.minres_iters = its
+        if info != 0:
+            self.minres_fail = getattr(self, "minres_fail", 0) + 1
+            self.minres_last_info = int(info)
+        if getattr(self, "minres_track_resid", False):
+            nb = float(torch.linalg.vector_norm(b))
+            r = float(torch.linalg.vector_norm(op(lam) - b)) / max(nb, 1e-300)
+            self.minres_resid = max(getattr(self, "minres_resid", 0.0), r)
+        return lam.cpu().numpy()
+
+    def adjoint_cg(self, xstar, b, diagH, use_self=False, include_mass=True, iters=8000, tol=1e-9):
+        """Jacobi-preconditioned CG on the free degrees of freedom (for a positive definite H);
+        records negative curvature it meets in _cg_info (reference :258-277)."""
+        xw = self._t(xstar)
+        free = self._free
+        pre = free / torch.clamp(self._t(diagH), min=1e-30)
+        lam = torch.zeros((self.N, 2), dtype=F64, device=self.dev)
+        r = self._t(b) * free
+        z = pre * r
+        p = z.clone()
+        rz = float((r * z).sum())
+        r0 = float(torch.linalg.vector_norm(r))
+        neg, it = 0, 0
+        for it in range(1, iters + 1):
+            Ap = self._hvp(xw, p * free, include_mass=include_mass) * free
+            pAp = float((p * Ap).sum())
+            if pAp <= 0.0:
+                neg += 1
+            a = rz / pAp
+            lam = lam + a * p
+            r = r - a * Ap
+            if float(torch.linalg.vector_norm(r)) <= tol * r0 + 1e-30:
+                break
+            z = pre * r
+            rz2 = float((r * z).sum())
+            p = z + (rz2 / rz) * p
+            rz = rz2
+        self._cg_info = dict(iters=it, rel_resid=float(torch.linalg.vector_norm(r)) / (r0 + 1e-30),
+                             neg_curv=neg)
+        return lam.cpu().numpy()
+
+    # ---- dJ/drho of one adjoint vector (reference :165-178, 280-285)
+    def _drho_dev(self, xw, lamw):
+        L = _lib.lib()
+        s = self.s
+        self._drho.zero_()
+        _lib.check(L.d2f_elastic_drho_f64(self._p(xw), self._p(lamw), self._p(s.tri), self._p(s.Bm),
+                                          self._p(s.area), self._p(self.mu0), self._p(self.lam0),
+                                          self.n_tri, self.h2, self.Emin, self._p(self._drho),
+                                          self._stream()), "d2f_elastic_drho_f64")
+        return self._drho
+
+    def drho(self, xstar, lam):
+        return self._drho_dev(self._t(xstar), self._t(lam)).cpu().numpy().copy()
+
+    # ---- objectives on the elastic force (reference :287-326)
+    def elastic_force(self, x):
+        """Elastic force on every node at positions x, (N, 2), with the current density."""
+        L = _lib.lib()
+        s = self.s
+        xw = self._t(x)
+        g = torch.zeros((self.N, 2), dtype=F64, device=self.dev)
+        d = torch.zeros((self.N, 2), dtype=F64, device=self.dev)
+        _lib.check(L.d2f_elastic_grad_diag_f64(self._p(xw), self._p(s.tri), self._p(s.Bm), self._p(s.area),
+                                               self._p(s.mu), self._p(s.lam), self.n_tri, 1.0, 1,
+                                               self._p(s.el_owner), self._p(s.el_range), s.el_window,
+                                               self._p(g), self._p(d), self._stream()), "elastic grad/diag")
+        return -g.cpu().numpy()
+
+    def force_objective(self, x, nodes, direction):
+        """J = direction . (elastic force summed over `nodes`) at x; returns (J, dJ/dx, explicit
+        dJ/drho). Use transient_adjoint(..., rtol=1e-14) with the dJ/dx seed, then add the
+        explicit part."""
+        nodes = np.asarray(nodes, np.int64)
+        d = np.asarray(direction, np.float64)
+        w = np.zeros((self.N, 2))
+        w[nodes] = d
+        J = float((self.elastic_force(x)[nodes] * d).sum())
+        dJdx = -self.apply_H(x, w, include_mass=False) / self.h2
+        return J, dJdx, self.drho(x, w / self.h2)
+
+    # ---- the trajectory (reference :373-399)
+    def _check_terms(self):
+        s = self.s
+        missing = []
+        if s.ipc is not None:
+            missing.append("IPCContact")
+        if s.pressure is not None:
+            missing.append("Pressure")
+        if s.tendon is not None:
+            missing.append("Tendon")
+        if s.affine is not None:
+            missing.append("AffineBodies")
+        if missing:
+            raise NotImplementedError(
+                f"the adjoint operator has no tangent for {', '.join(sorted(missing))}: the gradient "
+                f"would be of a different system. Differentiate without these actuators."
+                + (" For contact between bodies, use ImplicitIntegrator(self_collision=False)."
+                   if "IPCContact" in missing else ""))
+
+    def transient_adjoint(self, xs, seed, drag=None, use_disc=False, use_self=False, use_sdf=False,
+                          **minres_kw):
+        """xs = [x_0 .. x_T] checkpointed states; seed = {k: dJ/dx_k (N, 2)}. Returns dJ/drho
+        (n_tri,) accumulated over the trajectory."""
+        self._check_terms()
+        T = len(xs) - 1
+        dr = float(self.s.drag) if drag is None else float(drag)
+        self.refresh_mass()
+        m2 = self._mass2
+        zero = torch.zeros((self.N, 2), dtype=F64, device=self.dev)
+        lam = {T + 1: zero, T + 2: zero}
+        g = torch.zeros(self.n_tri, dtype=F64, device=self.dev)
+        for k in range(T, 0, -1):
+            rhs = (1.0 + dr) * m2 * lam[k + 1] - dr * m2 * lam[k + 2]
+            if k in seed:
+                rhs = rhs + self._t(seed[k])
+            lam[k] = self._t(self.adjoint_solve(xs[k], rhs, use_disc=use_disc, use_self=use_self,
+                                                use_sdf=use_sdf, **minres_kw))
+            g += self._drho_dev(self._t(xs[k]), lam[k])
+            lam.pop(k + 2, None)
+        return g.cpu().numpy()

@@ -17,7 +17,7 @@ ARCH = os.environ.get("D2FEM_ARCH", "gfx950")
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
-    # keep the compiler from emitting packed FP32 instructions (see the repository notes)
+    # no packed FP32 (v_pk_*_f32) instructions: some of their forms have given wrong results
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics",               # hardware float64 atomic add
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",

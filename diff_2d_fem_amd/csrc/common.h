@@ -87,7 +87,7 @@ __device__ __forceinline__ void env_max(double *arr, int env, double v, bool val
   }
 }
 
-__device__ __forceinline__ Collider to_collider(const d2f_collider &c) {
+__host__ __device__ __forceinline__ Collider to_collider(const d2f_collider &c) {
   return Collider{c.kind, c.cx, c.cy, c.rx, c.ry, c.rref, c.r, c.mu};
 }
 

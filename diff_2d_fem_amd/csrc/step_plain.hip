@@ -1330,6 +1330,47 @@ struct Plain {
 
 using namespace d2f;
 
+// Reaction on the colliders per environment (reference disc_force_kernel, implicit/kernels.py:
+// 517-530; sdf_force_kernel, core/sdf.py:185-191): +kappa b' grad phi from every node in the
+// barrier; with friction, the reference's friction reactions (disc: +mu f_n t, implicit/
+// kernels.py:724-743; other shapes: -mu f_n t, core/sdf.py:259-271), t = v_t / max(|v_t|, epsv).
+// index < 0: every collider.
+__global__ void k_collider_force(S s, Pm p, int index, int friction, double *F) {
+  GRID_LOOP(b, s.n_b) {
+    const bool ok = b < s.n_b;
+    int e = -1;
+    V2<double> f = {0.0, 0.0};
+    if (ok) {
+      const int n = s.bnd[b];
+      e = s.b2e[b];
+      const V2<double> x = ld2(s.x, n);
+      for (int k = 0; k < s.n_col; ++k) {
+        if (index >= 0 && k != index) continue;
+        const Collider c = to_collider(s.col[k]);
+        if (c.kind == 0) {
+          const V2<double> diff = {x.x - c.cx, x.y - c.cy};
+          const double dc = len(diff), d = dc - c.r;
+          if (d <= 0.0 || d >= p.dhat) continue;
+          f = f + (p.kappa * barrier_bp(d, p.dhat) / dc) * diff;
+        } else {
+          const SdfHit h = sdf_eval(c, x);
+          if (h.d <= 0.0 || h.d >= p.dhat) continue;
+          f = f + (p.kappa * barrier_bp(h.d, p.dhat)) * h.g;
+        }
+        if (friction && p.friction) {
+          FricFrame fr;
+          if (friction_frame(c, x, ld2(s.xn, n), p.dhat, p.kappa, p.inv_dt, fr)) {
+            const double sgn = c.kind == 0 ? 1.0 : -1.0;
+            f = f + (sgn * c.mu * fr.f_n / fmax(fr.vtm, p.epsv)) * fr.vt;
+          }
+        }
+      }
+    }
+    env_add(F, 2 * e, f.x, ok);
+    env_add(F + 1, 2 * e, f.y, ok);
+  }
+}
+
 extern "C" {
 
 int d2f_abi_version(void) { return D2F_ABI_VERSION; }
@@ -1411,6 +1452,46 @@ int d2f_step_plain(const d2f_scene *s, const d2f_params *p, void *stream) {
   D2F_CHECK(p->iter_max >= 0 && p->dt > 0.0, "d2f_step_plain: bad params");
   Plain run(*s, *p, (hipStream_t)stream);
   return run.run();
+}
+
+int d2f_collider_force(const d2f_scene *s, const d2f_params *p, int32_t index, int32_t friction, double *F,
+                       void *stream) {
+  D2F_CHECK(s && p && F, "d2f_collider_force: null argument");
+  D2F_CHECK(index < s->n_col, "d2f_collider_force: collider %d of %d", index, s->n_col);
+  hipStream_t st = (hipStream_t)stream;
+  LAUNCH(k_fill, 2 * s->B, F, 2 * s->B, 0.0);
+  if (s->n_b > 0 && s->n_col > 0) LAUNCH(k_collider_force, s->n_b, *s, *p, index, friction, F);
+  D2F_HIP(hipGetLastError());
+  return 0;
+}
+
+int d2f_contact_force(const d2f_scene *s, const d2f_params *p, int32_t colliders, int32_t bodies,
+                      int32_t friction, const uint8_t *all_free, double *g, double *dg, void *stream) {
+  D2F_CHECK(s && p && all_free && g && dg, "d2f_contact_force: null argument");
+  hipStream_t st = (hipStream_t)stream;
+  LAUNCH(k_fill, 2 * s->N, g, 2 * s->N, 0.0);
+  LAUNCH(k_fill, 2 * s->N, dg, 2 * s->N, 0.0);
+  // the gradient passes with h^2 = 1, nothing masked, into g (reference implicit/solver.py:554-602)
+  S s1 = *s;
+  s1.grad = g;
+  s1.diag = dg;
+  s1.free_ = all_free;
+  Pm p1 = *p;
+  p1.h2 = 1.0;
+  p1.friction = friction ? p->friction : 0;
+  if (colliders && s->n_b > 0 && s->n_col > 0) LAUNCH(k_colliders_grad_diag, s->n_b, s1, p1);
+  if (bodies && p->ipc_on && s->n_e > 0) {
+    Pm p2 = p1;
+    if (!friction) {
+      p2.ipc_mu = 0.0;
+      p2.ipc_per_body = 0;
+    }
+    Plain run(s1, p2, st);
+    const int r = run.ipc_pass(0, s->p, g, dg, all_free, s->Ener, s->clr_min, s->alpha, 1.0, p2.ipc_mu);
+    if (r) return r;
+  }
+  D2F_HIP(hipGetLastError());
+  return 0;
 }
 
 }  // extern "C"

@@ -558,6 +558,37 @@ class DeviceSolver:
         _lib.check(L.d2f_ipc_min_distance(C.byref(s), C.byref(q), stream), "d2f_ipc_min_distance")
         return self.clr_ipc.cpu().numpy().copy()
 
+    # ------------------------------------------------------------------ contact read-outs
+    def _stream(self):
+        return C.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
+
+    def contact_force(self, friction=True, colliders=True, bodies=True):
+        """Force that contact exerts ON each node, (N, 2) (reference implicit/solver.py:554-602)."""
+        if getattr(self, "_fc", None) is None:
+            self._fc = torch.zeros((self.N, 2), dtype=F64, device=self.dev)
+            self._fc_diag = torch.zeros((self.N, 2), dtype=F64, device=self.dev)
+            self._all_free = torch.ones(self.N, dtype=U8, device=self.dev)
+        L = _lib.lib()
+        s, q = self.scene_struct(), self.params_struct()
+        _lib.check(L.d2f_contact_force(C.byref(s), C.byref(q), int(bool(colliders)), int(bool(bodies)),
+                                       int(bool(friction)), self._ptr(self._all_free), self._ptr(self._fc),
+                                       self._ptr(self._fc_diag), self._stream()), "d2f_contact_force")
+        return -self._fc.cpu().numpy()
+
+    def collider_force(self, index=None, include_friction=False):
+        """Reaction on the colliders per environment, (B, 2): on collider `index`, or on all of
+        them together (reference implicit/solver.py:504-532, 604-612)."""
+        if getattr(self, "_fcol", None) is None:
+            self._fcol = torch.zeros((self.B, 2), dtype=F64, device=self.dev)
+        if index is not None and not 0 <= int(index) < len(self.colliders):
+            raise IndexError(f"collider {index} of {len(self.colliders)}")
+        L = _lib.lib()
+        s, q = self.scene_struct(), self.params_struct()
+        _lib.check(L.d2f_collider_force(C.byref(s), C.byref(q), -1 if index is None else int(index),
+                                        int(bool(include_friction)), self._ptr(self._fcol), self._stream()),
+                   "d2f_collider_force")
+        return self._fcol.cpu().numpy().copy()
+
     # ------------------------------------------------------------------ state I/O
     def get_x(self):
         return self.x.cpu().numpy().copy()

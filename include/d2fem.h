@@ -207,6 +207,40 @@ int d2f_ipc_min_distance(const d2f_scene *s, const d2f_params *prm, void *stream
  * does not fit the LDS budget (use d2f_step_plain for it). */
 int d2f_step_scene(const d2f_scene *s, const d2f_params *prm, void *stream);
 
+/* ---- read-outs ------------------------------------------------------------------------------ */
+/* Reaction on the colliders per environment into F[2B]: collider `index`, or all (index < 0);
+ * the barrier, plus friction when `friction` (and prm->friction) is set. Replaces
+ * PNCG2DSolver.collider_force (implicit/solver.py:504-532, 604-612). */
+int d2f_collider_force(const d2f_scene *s, const d2f_params *prm, int32_t index, int32_t friction,
+                       double *F, void *stream);
+/* MINUS the contact force on every node into g[2N] (dg[2N] receives the passes' diagonal):
+ * colliders' barrier (+friction), IPC contact between bodies (+friction), with h^2 = 1 and
+ * all_free[N] (all ones) in place of the free mask, so kinematic nodes are included. Replaces
+ * PNCG2DSolver.contact_force (implicit/solver.py:554-602). */
+int d2f_contact_force(const d2f_scene *s, const d2f_params *prm, int32_t colliders, int32_t bodies,
+                      int32_t friction, const uint8_t *all_free, double *g, double *dg, void *stream);
+
+/* ---- adjoint (float64) ----------------------------------------------------------------------- */
+/* y += h2 H_contact v at x for the selected colliders (which: 0 discs, 1 the other shapes through
+ * their level sets, 2 all), barrier gate frozen at x, unclamped. Replaces chvp_disc_kernel and
+ * chvp_sdf_kernel (implicit/adjoint.py:89-112). */
+int d2f_collider_hvp_f64(const double *x, const double *v, const int32_t *bnd, int32_t n_b,
+                         const d2f_collider *cols, int32_t n_col, int32_t which, double dhat,
+                         double kappa, double h2, double *y, void *stream);
+/* grad += h2 kappa b' grad phi, diag += h2 curvature of the selected colliders (the contact
+ * part of the adjoint's Jacobi preconditioner). Replaces grad_diag_disc_kernel
+ * (implicit/kernels.py:132-157) and grad_diag_sdf_kernel (core/sdf.py:141-154) as the adjoint
+ * calls them (implicit/adjoint.py:356-370). */
+int d2f_collider_grad_diag_f64(const double *x, const int32_t *bnd, int32_t n_b,
+                               const d2f_collider *cols, int32_t n_col, int32_t which, double dhat,
+                               double kappa, double h2, int32_t clamp, double *g, double *dg,
+                               void *stream);
+/* out[t] += -(1 - Emin) h2 lam . f_t(x) with f_t the element's full-density elastic force (mu0,
+ * lam0). Replaces drho_kernel (implicit/adjoint.py:165-178). */
+int d2f_elastic_drho_f64(const double *x, const double *lam, const int32_t *tri, const double *Bm,
+                         const double *area, const double *mu0, const double *lam0, int32_t T,
+                         double h2, double Emin, double *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,3 +66,30 @@ def cantilever():
     m.set_kinematic_constraint(h, f["cant_fixed"])
     m.set_external_force(h, f["cant_fext"])
     return m, f
+
+
+def pressed_block(mode):
+    """The reference adjoint test's scene (tests/test_adjoint.py:35-50): a 10 x 6 px soft block,
+    top row held and driven (press 3 mm, shear 2 mm over the steps) onto a disc ("disc"), a
+    rounded box ("box"), or nothing within reach ("elastic"). Returns (model, handle, rest
+    positions of the body, the held top nodes)."""
+    m = Model()
+    oy = -1.0 if mode == "elastic" else 0.0335
+    if mode == "box":
+        m.add_object("box", center=(0.05, oy), rx=0.006, ry=0.006, corner_radius=0.002, mu=0.0)
+    else:
+        m.add_disc(center=(0.05, oy), radius=0.006, mu=0.0)
+    h = m.add_soft_body(np.ones((10, 6), np.uint8), physical_size=(0.012, 0.024), center=(0.05, 0.052),
+                        E=1.0e6, nu=0.40, density=1100.0)
+    m.finalize()
+    x0 = m.get_body_x(h)
+    top = np.where(x0[:, 1] > x0[:, 1].max() - 1e-5)[0]
+    m.set_kinematic_constraint(h, top)
+    return m, h, x0, top
+
+
+def press_targets(x0, top, t, T):
+    x = x0.copy()
+    x[top, 1] = x0[top, 1] - 0.003 * t / T
+    x[top, 0] = x0[top, 0] + 0.002 * t / T
+    return x

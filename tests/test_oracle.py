@@ -18,6 +18,8 @@ term, a wrong sign or index, a transposed operand) would fail:
   pressure, tendon             gradient vs FD of energy, p^T H p vs FD of gradient
   6x6 Cholesky, reduced        numpy.linalg.solve (< 1e-10); PSD under compression and one zero
   Hessian                      eigenvalue at a pure rotation (tests/test_rigid.py:62-94)
+  adjoint dJ/drho               central differences of converged rollouts, along the gradient
+                               (< 1e-4 relative) and a random direction
   whole step                   exact backward-Euler free fall of a rigid body and of a soft body
                                with the rigid-mode coarse space; batch == alone bitwise;
                                static equilibrium of the Newton oracle (golden fixture)
@@ -32,7 +34,7 @@ from oracle import terms as T
 from oracle.bodies import chol6, chol6_solve, reduced_hessian, Pressure, Tendon
 from oracle.ipc import IPCContact, accd, seg_dist
 from oracle.pncg import OracleIntegrator
-from tests.scenes import strip, cantilever
+from tests.scenes import strip, cantilever, pressed_block, press_targets
 
 GOLD = "tests/golden/"
 
@@ -566,3 +568,48 @@ def test_contact_parameters_are_scaled_to_the_scene():
     m.finalize()
     assert np.isclose(m.dhat, reach) and np.isclose(m.kappa, 3.2e4) and np.isclose(m.self_stiffness, 2.0e5)
     assert np.isclose(m.body_kappa, 0.1 * 1.0e6 * 0.5e-3 / reach)
+
+
+# ---------------------------------------------------------------------------- adjoint
+@pytest.mark.parametrize("mode", ["elastic", "disc"])
+def test_adjoint_matches_finite_differences(mode):
+    """The oracle's adjoint dJ/drho of J = c . x_T against central differences of converged
+    rollouts (the protocol of reference tests/test_adjoint.py, float64 so that a small step
+    can be used): along the gradient and along a random direction."""
+    from oracle.adjoint import OracleAdjoint
+    T, DT = 6, 2.0e-4
+    m, h, x0, top = pressed_block(mode)
+    oi = OracleIntegrator(max_iter=3000, eps=1e-14, tol_x=1e-10, conv_check=8, self_collision=False,
+                          friction=False)
+    oi.attach(m, DT)
+    adj = OracleAdjoint(oi.solver, m.me)
+    rng = np.random.default_rng(0)
+    rho0 = 0.45 + 0.5 * rng.random(adj.n_tri)
+    low = np.where(x0[:, 1] < np.median(x0[:, 1]))[0]
+    c = np.zeros((m.n_nodes, 2))
+    c[low, 0] = 1.0 / len(low)
+
+    def rollout(rho):
+        adj.set_density(rho)
+        s = oi.solver
+        s.x = m.me.rest_pos.astype(np.float64).copy()
+        s.v[:] = 0.0
+        xs = [s.x.copy()]
+        for t in range(1, T + 1):
+            m.set_kinematic_target(h, press_targets(x0, top, t, T))
+            oi.simulate(m, DT)
+            xs.append(s.x.copy())
+        return xs
+    xs = rollout(rho0)
+    if mode == "disc":
+        assert oi.solver.clr_min.min() < oi.solver.dhat          # in contact at the end
+    g = adj.transient_adjoint(xs, {T: c}, oi.solver.drag, use_disc=(mode == "disc"))
+    assert adj.info == 0
+    J = lambda rho: float((c * rollout(rho)[-1]).sum())
+    hfd = 0.002
+    d = g / np.abs(g).max()                      # along the gradient: relative error
+    fd = (J(rho0 + hfd * d) - J(rho0 - hfd * d)) / (2 * hfd)
+    assert abs(g @ d - fd) < 1e-4 * abs(fd), (mode, g @ d, fd)
+    d = rng.standard_normal(adj.n_tri)           # any direction: error over |g| |d|
+    fd = (J(rho0 + hfd * d) - J(rho0 - hfd * d)) / (2 * hfd)
+    assert abs(g @ d - fd) < 1e-5 * np.linalg.norm(g) * np.linalg.norm(d), (mode, g @ d, fd)
