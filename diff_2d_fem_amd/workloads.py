@@ -13,8 +13,17 @@ same whether it is built alone or inside a batch.
   pads_lift_block   two soft pads 48 x 16 px (E 0.3 MPa, nu 0.45, rho 1100) on kinematic jaws
                     either side of a rigid 16 x 16 px block (rho 600) of side U[12, 20] mm
                     (pixel = side / 16) resting on the ground; IPC contact between bodies with
-                    mu 0.6, ground mu 0.6, gravity, dt 1 ms. The jaws close at 1 mm per step to
-                    a quarter pixel past touching, then lift at 20 um per step.
+                    mu 0.6, ground mu 0.6, gravity, dt 1 ms. The jaws close from 2 px apart to a
+                    quarter pixel past touching over 10 steps, then lift half the block side
+                    over 200 steps.
+  pneumatic_fingers a 12 x 96 px finger (pixel 0.25 mm, E 0.5 MPa, nu 0.4, rho 1100) with 6-10
+                    sealed rectangular chambers (3-6 x 4-8 px, 1 px apart and from the border),
+                    left column held, pressure U[0, 40] kPa per chamber, gravity, dt 2 ms, and a
+                    random target point for the tip.
+  star_meshes       a Delaunay mesh body on a random star outline r = R (1 + sum_k a_k cos(k t +
+                    phi_k)), k <= 5, with a target triangle count U[2k, 20k]; 1-3 disjoint
+                    round rigid regions; 2 tendons through interior waypoints, tension U[0, 50]
+                    N/m; ground mu 0.6, gravity, dt 2 ms.
 """
 from __future__ import annotations
 
@@ -134,4 +143,132 @@ def pads_lift_block(n_env=256, seed=0, n_close=10, n_lift=200, scene_ids=None):
                 sides=np.array([16 * px for px in pxs]), name="pads_lift_block")
 
 
-CONFIGS = {"c1": strip_on_ground, "c2": random_blobs, "c3": pads_lift_block}
+def _chambers(rng, H=12, W=96):
+    """6-10 non-overlapping rectangles (3-6 x 4-8 px), each kept 1 px from the border and from
+    the others, as a list of (row, col, h, w)."""
+    want = int(rng.integers(6, 11))
+    taken = np.zeros((H, W), bool)                 # chamber pixels grown by one
+    rects = []
+    for _ in range(2000):
+        if len(rects) == want:
+            break
+        h, w = int(rng.integers(3, 7)), int(rng.integers(4, 9))
+        r, c = int(rng.integers(1, H - h)), int(rng.integers(1, W - w))
+        if r + h > H - 1 or c + w > W - 1 or taken[r - 1:r + h + 1, c - 1:c + w + 1].any():
+            continue
+        taken[r - 1:r + h + 1, c - 1:c + w + 1] = True
+        rects.append((r, c, h, w))
+    return rects
+
+
+def pneumatic_fingers(n_env=512, seed=0, px=0.25e-3):
+    m = Model()
+    hs, pressures, targets = [], [], []
+    for i in range(n_env):
+        rng = np.random.default_rng([seed, i])
+        mask = np.ones((12, 96), np.uint8)
+        for r, c, h, w in _chambers(rng):
+            mask[r:r + h, c:c + w] = 0
+        hs.append(m.add_soft_body(mask, physical_size=(96 * px, 12 * px), center=(0.0, 0.02),
+                                  E=0.5e6, nu=0.4, density=1100.0, env_id=i))
+        pressures.append(rng)
+    m.finalize()
+    m.gravity = (0.0, -9.81)
+    chambers = []
+    for i, (h, rng) in enumerate(zip(hs, pressures)):
+        X = m.rest_x(h)
+        m.set_kinematic_constraint(h, np.nonzero(np.abs(X[:, 0] - X[:, 0].min()) < 1e-9)[0])
+        ch = m.add_pressure_chambers(h)
+        for c in ch:
+            m.set_pressure(c, float(rng.uniform(0.0, 40e3)))
+        chambers.append(ch)
+        tip = X[np.argmax(X[:, 0])]
+        targets.append(tip + rng.uniform(-1.0, 1.0, 2) * 4e-3)
+    return dict(model=m, dt=2e-3, command=lambda step: None, bodies=hs, chambers=chambers,
+                targets=np.asarray(targets), size=96 * px, name="pneumatic_fingers")
+
+
+def _star_mesh(rng, R, n_tri):
+    """A Delaunay mesh of a random star outline with about n_tri triangles: boundary points
+    every h along the outline, a jittered triangular lattice inside, triangles kept when
+    their centroid is inside; vertices and triangles sorted for locality."""
+    from scipy.spatial import Delaunay
+    K = 5
+    a = rng.uniform(-0.08, 0.08, K) / np.arange(1, K + 1)
+    ph = rng.uniform(0.0, 2 * np.pi, K)
+
+    def radius(t):
+        return R * (1.0 + sum(a[k] * np.cos((k + 1) * t + ph[k]) for k in range(K)))
+    area = 0.5 * np.mean(radius(np.linspace(0.0, 2 * np.pi, 4096, endpoint=False)) ** 2) * 2 * np.pi
+    h = np.sqrt(2.0 * area / (np.sqrt(3.0) / 2.0 * n_tri))     # two triangles per lattice point
+    t = np.linspace(0.0, 2 * np.pi, 8192, endpoint=False)
+    P = np.stack([radius(t) * np.cos(t), radius(t) * np.sin(t)], 1)
+    seg = np.linalg.norm(np.roll(P, -1, 0) - P, axis=1)
+    arc = np.concatenate([[0.0], np.cumsum(seg)])
+    nb = max(12, int(round(arc[-1] / h)))
+    tb = np.interp(np.linspace(0.0, arc[-1], nb, endpoint=False), arc, np.concatenate([t, [2 * np.pi]]))
+    B = np.stack([radius(tb) * np.cos(tb), radius(tb) * np.sin(tb)], 1)
+    xs = np.arange(-1.3 * R, 1.3 * R, h)
+    ys = np.arange(-1.3 * R, 1.3 * R, h * np.sqrt(3.0) / 2.0)
+    gx, gy = np.meshgrid(xs, ys)
+    gx = gx + 0.5 * h * (np.arange(len(ys))[:, None] % 2)
+    G = np.stack([gx.ravel(), gy.ravel()], 1) + rng.uniform(-0.1, 0.1, (gx.size, 2)) * h
+    rg, tg = np.linalg.norm(G, axis=1), np.arctan2(G[:, 1], G[:, 0])
+    G = G[rg < radius(tg) - 0.7 * h]
+    V = np.concatenate([B, G])
+    T = Delaunay(V).simplices
+    c = V[T].mean(1)
+    T = T[np.linalg.norm(c, axis=1) < radius(np.arctan2(c[:, 1], c[:, 0]))]
+    used = np.unique(T)
+    remap = np.full(len(V), -1)
+    order = used[np.lexsort((V[used, 1], np.floor(V[used, 0] / (8 * h))))]
+    remap[order] = np.arange(len(order))
+    V, T = V[order], remap[T]
+    e1, e2 = V[T[:, 1]] - V[T[:, 0]], V[T[:, 2]] - V[T[:, 0]]
+    flip = e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0] < 0
+    T[flip] = T[flip][:, [0, 2, 1]]
+    T = T[np.argsort(T.min(1), kind="stable")]
+    return V, T, h
+
+
+def star_meshes(n_env=4096, seed=0, R=10e-3, tri_range=(2000, 20000)):
+    m = Model()
+    m.add_ground(0.0, mu=0.6)
+    hs, extra = [], []
+    for i in range(n_env):
+        rng = np.random.default_rng([seed, i])
+        V, T, h = _star_mesh(rng, R, int(rng.integers(tri_range[0], tri_range[1] + 1)))
+        V = V + np.array([0.0, -V[:, 1].min() + 2.0 * h])
+        hs.append(m.add_soft_mesh(V, T, E=float(np.exp(rng.uniform(np.log(2e5), np.log(2e6)))),
+                                  nu=float(rng.uniform(0.40, 0.45)), density=1100.0, env_id=i))
+        extra.append((rng, V, T, h))
+    m.finalize()
+    m.gravity = (0.0, -9.81)
+    tendons = []
+    for hd, (rng, V, T, h) in zip(hs, extra):
+        c = V[T].mean(1)
+        ctr = np.array([0.0, V[:, 1].min() + R])
+        taken = np.zeros(len(T), bool)
+        for _ in range(int(rng.integers(1, 4))):
+            for _try in range(50):
+                p0 = ctr + rng.uniform(-0.5, 0.5, 2) * R
+                rad = rng.uniform(0.12, 0.22) * R
+                sel = np.linalg.norm(c - p0, axis=1) < rad
+                near = np.linalg.norm(c - p0, axis=1) < rad + 3.0 * h
+                if sel.sum() >= 4 and not (near & taken).any():
+                    taken |= sel
+                    m.add_affine_region(hd, sel)
+                    break
+        tl = []
+        for _ in range(2):
+            way = ctr + np.stack([np.linspace(-0.6, 0.6, 4), rng.uniform(-0.4, 0.4, 4)], 1) * R
+            t = m.add_tendon(hd, through=way)
+            m.set_tendon_tension(t, float(rng.uniform(0.0, 50.0)))
+            tl.append(t)
+        tendons.append(tl)
+    return dict(model=m, dt=2e-3, command=lambda step: None, bodies=hs, tendons=tendons, size=2 * R,
+                name="star_meshes")
+
+
+CONFIGS = {"c1": strip_on_ground, "c2": random_blobs, "c3": pads_lift_block, "c4": pneumatic_fingers,
+           "c5": star_meshes}
