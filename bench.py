@@ -16,7 +16,9 @@ node-position difference between one step of the measured precision and one floa
 from the identical state, over each scene's block size; `roofline` is the dominant kernel
 (the scene path's one step kernel, or the plain path's element kernel) with its algorithmic
 bytes over its event-timed duration against HBM peak; `element_roofline` is the float64
-elastic gradient+diagonal element kernel alone; `cpu_baseline` times the float64 CPU oracle
+elastic gradient+diagonal element kernel alone on this workload's mesh, and
+`element_roofline_large` the same kernel at the size of the star-mesh configuration (45 M
+triangles, working set far beyond the Infinity Cache); `cpu_baseline` times the float64 CPU oracle
 on a bounded sample of the same workload.
 """
 from __future__ import annotations
@@ -44,6 +46,7 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--no-large", action="store_true", help="skip the 45 M-triangle element-kernel leg")
     return ap.parse_args()
 
 
@@ -187,8 +190,9 @@ def elastic_roofline(res):
     g = torch.zeros_like(s.x)
     d = torch.zeros_like(s.x)
     P = lambda t: C.c_void_p(t.data_ptr())
-    args = (P(s.x), P(s.tri), P(s.Bm), P(s.area), P(s.mu), P(s.lam), s.n_tri, s.h2, 1, P(s.el_owner),
-            P(s.el_range), s.el_window, P(g), P(d), h)
+    tiles = s.elem_tiles_struct()
+    args = (P(s.x), P(s.tri), P(s.Bm), P(s.area), P(s.mu), P(s.lam), s.n_tri, s.h2, 1, C.byref(tiles),
+            P(g), P(d), h)
     for _ in range(3):
         L.d2f_elastic_grad_diag_f64(*args)
     n = 20
@@ -205,6 +209,71 @@ def elastic_roofline(res):
                 peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
                 traffic=measured_traffic("d2f::k_elastic_grad_diag", res),
                 avg_us=round(dur * 1e6, 2), bytes_per_launch=int(bytes_))
+
+
+def element_roofline_large(n_meshes=32, copies=128, seed=0):
+    """The float64 element kernel at the size of the star-mesh configuration (about 45 M
+    triangles, 23 M nodes, a ~3 GB working set far beyond the 256 MB Infinity Cache): 32
+    seeded Delaunay star meshes of 2k-20k triangles (diff_2d_fem_amd.workloads) tiled 128
+    times, randomly deformed, with the element kernel's node tiles. HIP events on the
+    launch stream; algorithmic bytes = 68 T + 48 N per launch."""
+    import ctypes as C
+    import torch
+    from diff_2d_fem_amd import _lib
+    from diff_2d_fem_amd.layout import element_tiles
+    from diff_2d_fem_amd.mesh import lame
+    from diff_2d_fem_amd.workloads import _star_mesh
+    Vs, Ts, Bs, As, MUs, LAs = [], [], [], [], [], []
+    for i in range(n_meshes):
+        rng = np.random.default_rng([seed, i])
+        V, T, _ = _star_mesh(rng, 10e-3, int(rng.integers(2000, 20001)))
+        D = np.stack([V[T[:, 1]] - V[T[:, 0]], V[T[:, 2]] - V[T[:, 0]]], 2)     # columns = edges
+        mu, la = lame(float(np.exp(rng.uniform(np.log(2e5), np.log(2e6)))), float(rng.uniform(0.40, 0.45)))
+        Vs.append(V + rng.normal(scale=2e-5, size=V.shape))
+        Ts.append(T)
+        Bs.append(np.linalg.inv(D).reshape(-1, 4))
+        As.append(0.5 * np.abs(np.linalg.det(D)))
+        MUs.append(np.full(len(T), mu))
+        LAs.append(np.full(len(T), la))
+    nv = np.array([len(v) for v in Vs])
+    off = np.concatenate([[0], np.cumsum(np.tile(nv, copies))[:-1]])
+    tri = np.concatenate([Ts[k % n_meshes] + off[k] for k in range(n_meshes * copies)]).astype(np.int32)
+    x = np.concatenate([Vs[k % n_meshes] for k in range(n_meshes * copies)])
+    cat = lambda L: np.concatenate([L[k % n_meshes] for k in range(n_meshes * copies)])
+    Bm, area, mu, lam = cat(Bs), cat(As), cat(MUs), cat(LAs)
+    T, N = len(tri), len(x)
+    dev = torch.device("cuda")
+    et = element_tiles(tri, N, device=dev)
+    tiles = _lib.ElemTiles()
+    for k in ("node_off", "nodes", "owned", "tri_local"):
+        setattr(tiles, k, et[k].data_ptr())
+    tiles.max_nodes = et["max_nodes"]
+    tt = lambda a, dt=torch.float64: torch.as_tensor(a, dtype=dt, device=dev)
+    xd, trid, Bd, Ad, mud, lad = tt(x), tt(tri, torch.int32), tt(Bm), tt(area), tt(mu), tt(lam)
+    g = torch.zeros_like(xd)
+    d = torch.zeros_like(xd)
+    L = _lib.lib()
+    st = torch.cuda.current_stream()
+    h = C.c_void_p(st.cuda_stream)
+    P = lambda t: C.c_void_p(t.data_ptr())
+    args = (P(xd), P(trid), P(Bd), P(Ad), P(mud), P(lad), T, 1e-6, 1, C.byref(tiles), P(g), P(d), h)
+    for _ in range(3):
+        _lib.check(L.d2f_elastic_grad_diag_f64(*args), "elastic grad/diag")
+    n = 10
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(n):
+        L.d2f_elastic_grad_diag_f64(*args)
+    e1.record(st)
+    torch.cuda.synchronize()
+    dur = e0.elapsed_time(e1) / n * 1e-3
+    bytes_ = 68.0 * T + 48.0 * N
+    ach = bytes_ / dur / 1e9
+    return dict(kernel="k_elastic_grad_diag (float64)", workload=f"{n_meshes} star meshes x {copies}",
+                triangles=int(T), nodes=int(N), bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS,
+                unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4), avg_us=round(dur * 1e6, 1),
+                bytes_per_launch=int(bytes_), max_chunk_nodes=et["max_nodes"],
+                owned_frac=round(float(et["owned"].float().mean()), 4))
 
 
 def cpu_baseline(args, steps):
@@ -258,6 +327,8 @@ def main():
     if path == "scene":
         out["roofline"] = scene_roofline(main_run, precision)
     out["element_roofline"] = elastic_roofline(main_run)
+    if not args.no_large:
+        out["element_roofline_large"] = element_roofline_large()
     if path == "plain":
         out["roofline"] = dict(out["element_roofline"])
     if not (path == "scene" and precision == "double"):

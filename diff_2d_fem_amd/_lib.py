@@ -17,6 +17,11 @@ MAX_COLORS = 24
 P = C.c_void_p
 
 
+class ElemTiles(C.Structure):
+    _fields_ = [("node_off", C.c_void_p), ("nodes", C.c_void_p), ("owned", C.c_void_p),
+                ("tri_local", C.c_void_p), ("max_nodes", C.c_int32), ("pad_", C.c_int32)]
+
+
 class Collider(C.Structure):
     _fields_ = [("kind", C.c_int32), ("pad_", C.c_int32)] + [
         (k, C.c_double) for k in ("cx", "cy", "rx", "ry", "rref", "r", "mu")]
@@ -50,7 +55,7 @@ _SCENE_FIELDS = (
     + [("max_nb", C.c_int32), ("max_cells", C.c_int32)]
     + [("excl_bits", P), ("excl_bits_off", P), ("excl_bits_lds", C.c_int32), ("max_pairs", C.c_int32)]
     + [("max_nn", C.c_int32), ("nacc_lds", C.c_int32)]
-    + [("el_owner", P), ("el_range", P), ("el_window", C.c_int32), ("pad2_", C.c_int32)]
+    + [("el", ElemTiles)]
 )
 
 
@@ -93,7 +98,7 @@ def lib():
     if hasattr(L, "d2f_step_scene"):
         L.d2f_step_scene.argtypes = [C.POINTER(Scene), C.POINTER(Params), P]
     i32, f64 = C.c_int32, C.c_double
-    L.d2f_elastic_grad_diag_f64.argtypes = [P, P, P, P, P, P, i32, f64, i32, P, P, i32, P, P, P]
+    L.d2f_elastic_grad_diag_f64.argtypes = [P, P, P, P, P, P, i32, f64, i32, C.POINTER(ElemTiles), P, P, P]
     L.d2f_elastic_pHp_f64.argtypes = [P, P, P, P, P, P, P, P, i32, f64, i32, P, P]
     L.d2f_inversion_alpha_f64.argtypes = [P, P, P, P, i32, f64, P, P]
     L.d2f_elastic_energy_f64.argtypes = [P, P, P, P, P, P, P, i32, f64, P, P]

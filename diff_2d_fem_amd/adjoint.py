@@ -55,6 +55,7 @@ class AdjointPNCG2D:
             for k in ("cx", "cy", "rx", "ry", "rref", "r", "mu"):
                 setattr(self._cols[i], k, float(c[k]))
         self._n_col = len(cols)
+        self._tiles = solver.elem_tiles_struct()
 
     # ---- plumbing
     def _t(self, a):
@@ -125,8 +126,8 @@ class AdjointPNCG2D:
         d = self._mass2.expand(self.N, 2).clone()
         _lib.check(L.d2f_elastic_grad_diag_f64(self._p(xw), self._p(s.tri), self._p(s.Bm), self._p(s.area),
                                                self._p(s.mu), self._p(s.lam), self.n_tri, self.h2, 1,
-                                               self._p(s.el_owner), self._p(s.el_range), s.el_window,
-                                               self._p(g), self._p(d), self._stream()), "elastic grad/diag")
+                                               C.byref(self._tiles), self._p(g), self._p(d), self._stream()),
+                   "elastic grad/diag")
         if (use_disc or use_sdf) and self._n_col:
             which = 2 if (use_disc and use_sdf) else (0 if use_disc else 1)
             _lib.check(L.d2f_collider_grad_diag_f64(self._p(xw), self._p(s.bnd), s.n_b, self._cols,
@@ -268,8 +269,8 @@ class AdjointPNCG2D:
         d = torch.zeros((self.N, 2), dtype=F64, device=self.dev)
         _lib.check(L.d2f_elastic_grad_diag_f64(self._p(xw), self._p(s.tri), self._p(s.Bm), self._p(s.area),
                                                self._p(s.mu), self._p(s.lam), self.n_tri, 1.0, 1,
-                                               self._p(s.el_owner), self._p(s.el_range), s.el_window,
-                                               self._p(g), self._p(d), self._stream()), "elastic grad/diag")
+                                               C.byref(self._tiles), self._p(g), self._p(d), self._stream()),
+                   "elastic grad/diag")
         return -g.cpu().numpy()
 
     def force_objective(self, x, nodes, direction):

@@ -91,38 +91,40 @@ __device__ __forceinline__ M2<R> load_B(const double *Bm, int t) {
 }
 
 // Elastic gradient and diagonal (reference implicit/kernels.py:86-128), tiled: workgroup c
-// takes triangles [c * D2F_ELEM_CHUNK, (c + 1) * D2F_ELEM_CHUNK), ELEM_PER_THREAD per thread,
-// sums their node contributions in LDS over the chunk's node range
-// [chunk_range[2c], chunk_range[2c+1]], then adds each node once: plain read-modify-write where
-// owner[n] == c (every triangle at n is in this chunk, so no other workgroup writes n), an
-// atomic otherwise. Without the tiling arrays, or when a chunk's range is wider than `window`,
-// every contribution is an atomic.
+// takes triangles [c * D2F_ELEM_CHUNK, (c + 1) * D2F_ELEM_CHUNK), ELEM_PER_THREAD per thread.
+// It gathers the positions of the chunk's nodes (the tile's node list) into LDS, sums every
+// triangle's contributions per node in LDS (the triangles address the list through their
+// 16-bit local indices), then adds each node once: plain read-modify-write where the chunk
+// owns the node (no other workgroup writes it), an atomic otherwise.
 constexpr int ELEM_TPB = 512;
 constexpr int ELEM_PER_THREAD = D2F_ELEM_CHUNK / ELEM_TPB;
 
-__global__ __launch_bounds__(ELEM_TPB) void k_elastic_grad_diag(
-    const double *x, const int32_t *tri, const double *Bm, const double *area, const double *mu,
-    const double *lam, int T, double h2, int clamp, const int32_t *owner, const int32_t *chunk_range,
-    int window, double *grad, double *diag) {
-  extern __shared__ double acc[];  // 4 * window: grad x, grad y, diag x, diag y
+__global__ __launch_bounds__(ELEM_TPB) void k_elastic_grad_diag_tiled(
+    const double *x, const double *Bm, const double *area, const double *mu, const double *lam, int T,
+    double h2, int clamp, d2f_elem_tiles tl, double *grad, double *diag) {
+  extern __shared__ double sh[];  // nodes of the chunk: x, y, then grad x, grad y, diag x, diag y
   // Workgroups are dealt to the 8 XCDs round-robin; give each XCD a contiguous run of chunks
-  // so that the halo nodes neighbouring chunks share stay in one L2.
+  // so that the nodes neighbouring chunks share stay in one L2.
   const int nbk = gridDim.x, q8 = nbk / 8, r8 = nbk % 8, xcd = blockIdx.x % 8;
   const int c = xcd * q8 + min(xcd, r8) + blockIdx.x / 8;
-  const int base = c * D2F_ELEM_CHUNK + threadIdx.x;
-  const bool tiling = owner != nullptr && chunk_range != nullptr;
-  const int lo = tiling ? chunk_range[2 * c] : 0, hi = tiling ? chunk_range[2 * c + 1] : -1;
-  const bool tiled = tiling && hi >= lo && hi - lo < window;
-  if (tiled) {
-    for (int j = threadIdx.x; j < 4 * (hi - lo + 1); j += ELEM_TPB) acc[j] = 0.0;
-    __syncthreads();
+  const int n0 = tl.node_off[c], nn = tl.node_off[c + 1] - n0;
+  double *xs = sh, *acc = sh + 2 * tl.max_nodes;
+  for (int j = threadIdx.x; j < nn; j += ELEM_TPB) {
+    const V2<double> p = ld2(x, tl.nodes[n0 + j]);
+    xs[2 * j] = p.x;
+    xs[2 * j + 1] = p.y;
+    acc[4 * j] = acc[4 * j + 1] = acc[4 * j + 2] = acc[4 * j + 3] = 0.0;
   }
+  __syncthreads();
+  const int base = c * D2F_ELEM_CHUNK + threadIdx.x;
 #pragma unroll
   for (int k = 0; k < ELEM_PER_THREAD; ++k) {
     const int t = base + k * ELEM_TPB;
     if (t >= T) continue;
-    const int ii[3] = {tri[3 * t], tri[3 * t + 1], tri[3 * t + 2]};
-    const V2<double> p0 = ld2(x, ii[0]), p1 = ld2(x, ii[1]), p2 = ld2(x, ii[2]);
+    const ushort4 l = reinterpret_cast<const ushort4 *>(tl.tri_local)[t];
+    const int li[3] = {l.x, l.y, l.z};
+    const V2<double> p0 = {xs[2 * li[0]], xs[2 * li[0] + 1]}, p1 = {xs[2 * li[1]], xs[2 * li[1] + 1]},
+                     p2 = {xs[2 * li[2]], xs[2 * li[2] + 1]};
     const M2<double> B = load_B<double>(Bm, t);
     const double m_ = mu[t], l_ = lam[t], A = area[t];
     const M2<double> Ds = {p1.x - p0.x, p2.x - p0.x, p1.y - p0.y, p2.y - p0.y};
@@ -133,32 +135,21 @@ __global__ __launch_bounds__(ELEM_TPB) void k_elastic_grad_diag(
     if (clamp)
 #pragma unroll
       for (int v = 0; v < 3; ++v) d[v] = {fmax(d[v].x, 0.0), fmax(d[v].y, 0.0)};
-    if (tiled) {
 #pragma unroll
-      for (int v = 0; v < 3; ++v) {
-        double *a = acc + 4 * (ii[v] - lo);
-        atomicAdd(a, g[v].x);
-        atomicAdd(a + 1, g[v].y);
-        atomicAdd(a + 2, d[v].x);
-        atomicAdd(a + 3, d[v].y);
-      }
-    } else {
-#pragma unroll
-      for (int v = 0; v < 3; ++v) {
-        atomic_add2(grad, ii[v], h2 * g[v]);
-        atomic_add2(diag, ii[v], h2 * d[v]);
-      }
+    for (int v = 0; v < 3; ++v) {
+      double *a = acc + 4 * li[v];
+      atomicAdd(a, g[v].x);
+      atomicAdd(a + 1, g[v].y);
+      atomicAdd(a + 2, d[v].x);
+      atomicAdd(a + 3, d[v].y);
     }
   }
-  if (!tiled) return;
   __syncthreads();
-  for (int f = 0; lo + (int)threadIdx.x + f * ELEM_TPB <= hi; ++f) {
-    const int n = lo + threadIdx.x + f * ELEM_TPB;
-    const double *a = acc + 4 * (n - lo);
-    const double gx = a[0], gy = a[1], dx = a[2], dy = a[3];
-    if (gx == 0.0 && gy == 0.0 && dx == 0.0 && dy == 0.0) continue;  // not a node of this chunk
-    const V2<double> gg = {h2 * gx, h2 * gy}, dd = {h2 * dx, h2 * dy};
-    if (owner[n] == c) {
+  for (int j = threadIdx.x; j < nn; j += ELEM_TPB) {
+    const int n = tl.nodes[n0 + j];
+    const double *a = acc + 4 * j;
+    const V2<double> gg = {h2 * a[0], h2 * a[1]}, dd = {h2 * a[2], h2 * a[3]};
+    if (tl.owned[n0 + j]) {
       st2(grad, n, ld2(grad, n) + gg);
       st2(diag, n, ld2(diag, n) + dd);
     } else {
@@ -168,17 +159,49 @@ __global__ __launch_bounds__(ELEM_TPB) void k_elastic_grad_diag(
   }
 }
 
+// The same with one atomic per contribution (no tiles, or a chunk too large for LDS).
+__global__ void k_elastic_grad_diag(const double *x, const int32_t *tri, const double *Bm,
+                                    const double *area, const double *mu, const double *lam, int T,
+                                    double h2, int clamp, double *grad, double *diag) {
+  GRID_LOOP(t, T) {
+    if (t < T) {
+      const int ii[3] = {tri[3 * t], tri[3 * t + 1], tri[3 * t + 2]};
+      const V2<double> p0 = ld2(x, ii[0]), p1 = ld2(x, ii[1]), p2 = ld2(x, ii[2]);
+      const M2<double> B = load_B<double>(Bm, t);
+      const double m_ = mu[t], l_ = lam[t], A = area[t];
+      const M2<double> Ds = {p1.x - p0.x, p2.x - p0.x, p1.y - p0.y, p2.y - p0.y};
+      const NH<double> h = nh_eval(mat_mul(Ds, B), m_, l_);
+      V2<double> g[3], d[3];
+      nh_grad(h, B, m_, A, g[0], g[1], g[2]);
+      nh_diag(h, B, m_, l_, A, d[0], d[1], d[2]);
+      if (clamp)
+#pragma unroll
+        for (int v = 0; v < 3; ++v) d[v] = {fmax(d[v].x, 0.0), fmax(d[v].y, 0.0)};
+#pragma unroll
+      for (int v = 0; v < 3; ++v) {
+        atomic_add2(grad, ii[v], h2 * g[v]);
+        atomic_add2(diag, ii[v], h2 * d[v]);
+      }
+    }
+  }
+}
+
 static int launch_elastic_grad_diag(const double *x, const int32_t *tri, const double *Bm,
                                     const double *area, const double *mu, const double *lam, int T,
-                                    double h2, int clamp, const int32_t *owner,
-                                    const int32_t *chunk_range, int window, double *grad,
+                                    double h2, int clamp, const d2f_elem_tiles *tiles, double *grad,
                                     double *diag, hipStream_t st) {
   if (T == 0) return 0;
-  const int w = owner && chunk_range ? window : 0;
-  D2F_CHECK(w >= 0 && w <= D2F_ELEM_WINDOW_MAX, "elastic grad/diag: window out of range");
-  const int blocks = (T + D2F_ELEM_CHUNK - 1) / D2F_ELEM_CHUNK;
-  hipLaunchKernelGGL(k_elastic_grad_diag, dim3(blocks), dim3(ELEM_TPB), sizeof(double) * 4 * (size_t)w, st,
-                     x, tri, Bm, area, mu, lam, T, h2, clamp, owner, chunk_range, w, grad, diag);
+  if (tiles && tiles->node_off && tiles->max_nodes > 0 && tiles->max_nodes <= D2F_ELEM_NODES_MAX) {
+    D2F_CHECK(tiles->nodes && tiles->owned && tiles->tri_local, "elastic grad/diag: incomplete tiles");
+    const int blocks = (T + D2F_ELEM_CHUNK - 1) / D2F_ELEM_CHUNK;
+    hipLaunchKernelGGL(k_elastic_grad_diag_tiled, dim3(blocks), dim3(ELEM_TPB),
+                       sizeof(double) * 6 * (size_t)tiles->max_nodes, st, x, Bm, area, mu, lam, T, h2,
+                       clamp, *tiles, grad, diag);
+  } else {
+    const int blocks = (int)std::min<long>((T + 255) / 256, 65536L);
+    hipLaunchKernelGGL(k_elastic_grad_diag, dim3(blocks), dim3(256), 0, st, x, tri, Bm, area, mu, lam, T, h2,
+                       clamp, grad, diag);
+  }
   D2F_HIP(hipGetLastError());
   return 0;
 }
@@ -1156,8 +1179,8 @@ struct Plain {
   int grad_diag() {
     LAUNCH(k_grad_diag_mass, s.N, s, p.h2);
     {
-      const int rc = launch_elastic_grad_diag(s.x, s.tri, s.Bm, s.area, s.mu, s.lam, s.T, p.h2, 1,
-                                              s.el_owner, s.el_range, s.el_window, s.grad, s.diag, st);
+      const int rc = launch_elastic_grad_diag(s.x, s.tri, s.Bm, s.area, s.mu, s.lam, s.T, p.h2, 1, &s.el,
+                                              s.grad, s.diag, st);
       if (rc) return rc;
     }
     if (s.n_rg > 0)
@@ -1383,11 +1406,10 @@ void d2f_struct_sizes(int64_t *out) {
 
 int d2f_elastic_grad_diag_f64(const double *x, const int32_t *tri, const double *Bm, const double *area,
                               const double *mu, const double *lam, int32_t T, double h2, int32_t clamp,
-                              const int32_t *owner, const int32_t *chunk_range, int32_t window,
-                              double *grad, double *diag, void *stream) {
+                              const d2f_elem_tiles *tiles, double *grad, double *diag, void *stream) {
   D2F_CHECK(T >= 0 && x && tri && Bm && area && mu && lam && grad && diag, "d2f_elastic_grad_diag_f64: bad argument");
-  return launch_elastic_grad_diag(x, tri, Bm, area, mu, lam, T, h2, clamp, owner, chunk_range, window,
-                                  grad, diag, (hipStream_t)stream);
+  return launch_elastic_grad_diag(x, tri, Bm, area, mu, lam, T, h2, clamp, tiles, grad, diag,
+                                  (hipStream_t)stream);
 }
 
 int d2f_elastic_pHp_f64(const double *x, const double *p, const int32_t *tri, const double *Bm,

@@ -68,30 +68,35 @@ def starts_of(keys, n):
 
 
 ELEM_CHUNK = 1024          # triangles per workgroup of the tiled element kernel (D2F_ELEM_CHUNK)
-ELEM_WINDOW_MAX = 4096     # largest node window a chunk accumulates in LDS
+ELEM_NODES_MAX = 2048      # longest node list one chunk sums in LDS (D2F_ELEM_NODES_MAX)
 
 
-def element_owners(tri, n_nodes, chunk=ELEM_CHUNK):
-    """Node ownership for the tiled element kernel. Triangle t belongs to chunk t // chunk;
-    owner[n] is that chunk when every triangle at node n belongs to it (the chunk may then
-    add into the node with a plain read-modify-write), else -1 (atomics). Also returns each
-    chunk's node range (smallest and largest node index of its triangles, interleaved) and
-    the window: the largest range size, capped at ELEM_WINDOW_MAX (a chunk with a wider
-    range falls back to atomics)."""
-    tri = np.asarray(tri, np.int64).reshape(-1, 3)
-    T = len(tri)
-    if T == 0:
-        return np.full(n_nodes, -1, np.int32), np.zeros(0, np.int32), 1
-    c = np.repeat(np.arange(T, dtype=np.int64) // chunk, 3)
-    flat = tri.reshape(-1)
-    lo = np.full(n_nodes, np.iinfo(np.int64).max)
-    hi = np.full(n_nodes, -1, np.int64)
-    np.minimum.at(lo, flat, c)
-    np.maximum.at(hi, flat, c)
-    owner = np.where(lo == hi, hi, -1).astype(np.int32)
-    starts = np.arange(0, T, chunk)
-    wmin = np.minimum.reduceat(tri.min(1), starts)
-    wmax = np.maximum.reduceat(tri.max(1), starts)
-    rng = np.stack([wmin, wmax], 1).reshape(-1).astype(np.int32)
-    window = int(min((wmax - wmin + 1).max(), ELEM_WINDOW_MAX))
-    return owner, rng, window
+def element_tiles(tri, n_nodes, chunk=ELEM_CHUNK, device="cpu"):
+    """Node tiles of the tiled element kernel. Triangle t belongs to chunk t // chunk. Returns
+    torch tensors on `device`:
+      node_off  (n_chunks + 1,) int32   chunk c's nodes are nodes[node_off[c]:node_off[c + 1]]
+      nodes     int32                   ascending within a chunk
+      owned     uint8                   1 when every triangle at that node is in that chunk
+      tri_local (T, 4) int16 (read as uint16)  positions of each triangle's nodes in its
+                                        chunk's list (column 3 is padding)
+    and max_nodes, the longest list."""
+    import torch
+    t = torch.as_tensor(np.asarray(tri, np.int64).reshape(-1, 3), device=device)
+    T = t.shape[0]
+    nch = (T + chunk - 1) // chunk
+    cid = torch.arange(T, device=device).div(chunk, rounding_mode="floor")
+    key = (cid[:, None] * n_nodes + t).reshape(-1)
+    uniq, inv = torch.unique(key, sorted=True, return_inverse=True)
+    ucid = uniq.div(n_nodes, rounding_mode="floor")
+    node = uniq - ucid * n_nodes
+    off = torch.searchsorted(ucid, torch.arange(nch + 1, device=device))
+    local = inv.reshape(T, 3) - off[cid][:, None]
+    count = torch.bincount(node, minlength=n_nodes)
+    owned = (count[node] == 1).to(torch.uint8)
+    tl = torch.zeros((T, 4), dtype=torch.int32, device=device)
+    tl[:, :3] = local.to(torch.int32)
+    max_nodes = int((off[1:] - off[:-1]).max()) if nch else 0
+    if max_nodes > 65535:
+        raise ValueError("a chunk touches more than 65535 nodes")
+    return dict(node_off=off.to(torch.int32), nodes=node.to(torch.int32), owned=owned,
+                tri_local=tl.to(torch.int16), max_nodes=max_nodes)

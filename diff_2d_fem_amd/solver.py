@@ -72,9 +72,8 @@ class DeviceSolver:
         self.block_size = 512              # scene path: 2 waves per SIMD, no register spills
         self._layout = None
         self.tri2env = t(np.asarray(me.tri2env), I32)
-        from .layout import element_owners
-        own, rng, self.el_window = element_owners(me.tri, self.N)
-        self.el_owner, self.el_range = t(own, I32), t(rng, I32)
+        from .layout import element_tiles
+        self.el_tiles = element_tiles(me.tri, self.N, device=self.dev)
         self.node_start = t(np.asarray(me.node_starts), I32)
         self.bnd = t(np.asarray(me.boundary_nodes), I32)
         self.b2e = t(np.asarray(me.boundary2env), I32)
@@ -445,6 +444,15 @@ class DeviceSolver:
     def _ptr(self, tensor):
         return None if tensor is None else C.c_void_p(tensor.data_ptr())
 
+    def elem_tiles_struct(self):
+        """The element kernel's node tiles as the C struct (pointers into el_tiles)."""
+        et = self.el_tiles
+        st = _lib.ElemTiles()
+        for k in ("node_off", "nodes", "owned", "tri_local"):
+            setattr(st, k, et[k].data_ptr())
+        st.max_nodes = et["max_nodes"]
+        return st
+
     def scene_struct(self):
         s = _lib.Scene()
         s.B, s.N, s.T, s.n_b, s.n_bc = self.B, self.N, self.n_tri, self.n_b, self.n_bc
@@ -457,7 +465,7 @@ class DeviceSolver:
                   "q_inf", "sweep_beta", "tc", "active", "iters"):
             setattr(s, k, p(k))
         s.free_ = p("free")
-        s.el_owner, s.el_range, s.el_window = p("el_owner"), p("el_range"), self.el_window
+        s.el = self.elem_tiles_struct()
         if self.ipc is not None:
             for k in ("ea", "eb", "ela", "elb", "e2env", "ca", "cb", "excl", "excl_off"):
                 setattr(s, k, p(k))

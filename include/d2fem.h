@@ -36,7 +36,20 @@ extern "C" {
 #define D2F_MAX_COLLIDERS 8
 #define D2F_MAX_COLORS 24
 #define D2F_ELEM_CHUNK 1024       /* triangles per workgroup of the tiled element kernel */
-#define D2F_ELEM_WINDOW_MAX 4096  /* largest node window one chunk sums in LDS */
+#define D2F_ELEM_NODES_MAX 2048   /* largest node list one chunk sums in LDS */
+
+/* Node tiles of the element kernel. Triangle t belongs to chunk t / D2F_ELEM_CHUNK; chunk c
+ * touches the nodes nodes[node_off[c] .. node_off[c+1]) (ascending); owned[k] = 1 when every
+ * triangle at nodes[k] belongs to chunk c (the chunk then adds into it without an atomic);
+ * tri_local[4t .. 4t+2] are the positions of triangle t's nodes in its chunk's list (the
+ * fourth entry is padding). max_nodes = the longest list. */
+typedef struct d2f_elem_tiles {
+  const int32_t *node_off;            /* n_chunks + 1 */
+  const int32_t *nodes;               /* node_off[n_chunks] */
+  const uint8_t *owned;               /* node_off[n_chunks] */
+  const uint16_t *tri_local;          /* 4 T */
+  int32_t max_nodes, pad_;
+} d2f_elem_tiles;
 
 /* A rigid collider (reference sim_model.py:167-174; kinds core/objects.py:22-24):
  * 0 disc, 1 ellipse, 2 rounded box, 3 rounded triangle, 4 half-space (rx, ry = unit normal). */
@@ -129,10 +142,8 @@ typedef struct d2f_scene {
   /* largest per-scene node count; nacc_lds = 1: the element pass accumulates every node's
    * gradient and diagonal in LDS (32 bytes per node), 0: colour-by-colour read-modify-write */
   int32_t max_nn, nacc_lds;
-  /* per-kernel path: node ownership of the tiled element kernel (see d2f_elastic_grad_diag_f64;
-   * NULL: atomics only) and its node window */
-  const int32_t *el_owner, *el_range;
-  int32_t el_window, pad2_;
+  /* per-kernel path: node tiles of the element kernel (node_off NULL: atomics only) */
+  d2f_elem_tiles el;
 } d2f_scene;
 
 /* Solver options and step constants (reference PNCG2DSolver.__init__, implicit/solver.py:97-226,
@@ -159,17 +170,15 @@ void d2f_struct_sizes(int64_t *out);
 
 /* ---- per-kernel entry points (float64), one per reference kernel family ------------------ */
 /* grad += h2 dPsi/dx, diag += h2 max(diag d2Psi/dx2, 0) over all triangles.
- * Replaces grad_diag_elastic_kernel (implicit/kernels.py:86-128). Tiled: triangle t belongs to
- * chunk t / D2F_ELEM_CHUNK; owner[n] = c when every triangle at node n is in chunk c (else -1);
- * chunk_range[2c], chunk_range[2c+1] = smallest and largest node index of chunk c's triangles;
- * window = the largest range size (<= D2F_ELEM_WINDOW_MAX). Chunk c sums its contributions in
- * LDS and adds them once per node: plain read-modify-write for the nodes it owns, atomics for
- * the rest. owner or chunk_range NULL: one atomic per contribution. */
+ * Replaces grad_diag_elastic_kernel (implicit/kernels.py:86-128). With tiles (see
+ * d2f_elem_tiles; max_nodes <= D2F_ELEM_NODES_MAX) chunk c gathers its nodes' positions into
+ * LDS, sums its triangles' contributions there and adds them once per node: plain
+ * read-modify-write for the nodes it owns, atomics for the rest. tiles NULL: one atomic per
+ * contribution. */
 int d2f_elastic_grad_diag_f64(const double *x, const int32_t *tri, const double *Bm,
                               const double *area, const double *mu, const double *lam, int32_t T,
-                              double h2, int32_t clamp, const int32_t *owner,
-                              const int32_t *chunk_range, int32_t window, double *grad,
-                              double *diag, void *stream);
+                              double h2, int32_t clamp, const d2f_elem_tiles *tiles,
+                              double *grad, double *diag, void *stream);
 /* pHp[env] += h2 max(p^T d2Psi p, 0) per triangle. Replaces pHp_elastic_kernel
  * (implicit/kernels.py:259-273). */
 int d2f_elastic_pHp_f64(const double *x, const double *p, const int32_t *tri, const double *Bm,

@@ -13,7 +13,7 @@ import pytest
 import torch
 
 from diff_2d_fem_amd import ImplicitIntegrator, _lib
-from diff_2d_fem_amd.layout import element_owners
+from diff_2d_fem_amd.layout import element_tiles
 from oracle import terms as OT
 from oracle.pncg import OracleIntegrator
 from tests.scenes import strip, grip, close_jaws
@@ -53,18 +53,20 @@ def test_elastic_grad_diag(deformed):
     tri = torch.tensor(me.tri, dtype=torch.int32, device=d)
     Bm = torch.tensor(me.D_m_inv.reshape(-1, 4), device=d)
     A, mu, lam = (torch.tensor(v, device=d) for v in (me.rest_area, me.mu, me.lam))
-    own, rng, window = element_owners(me.tri, len(x))
-    assert window > 1 and (own >= 0).any() and (own < 0).any()
-    ot, rt = torch.tensor(own, device=d), torch.tensor(rng, device=d)
-    # atomics only; tiled with ownership; tiled with a window too small for any chunk (fallback)
-    for owner, win in ((None, 0), (ot, window), (ot, 8)):
+    et = element_tiles(me.tri, len(x), device=d)
+    assert et["max_nodes"] > 1 and et["owned"].bool().any() and not et["owned"].bool().all()
+    tiles = _lib.ElemTiles()
+    for k in ("node_off", "nodes", "owned", "tri_local"):
+        setattr(tiles, k, et[k].data_ptr())
+    tiles.max_nodes = et["max_nodes"]
+    # atomics only (no tiles) and tiled
+    for tl in (None, tiles):
         for clamp in (1, 0):
             # the kernel accumulates: start from values at the scale of the elastic terms
             g0 = torch.tensor(1e-7 * np.random.default_rng(5).normal(size=x.shape), device=d)
             g, dg = g0.clone(), g0.clone()
             _lib.check(L.d2f_elastic_grad_diag_f64(_p(xt), _p(tri), _p(Bm), _p(A), _p(mu), _p(lam), T,
-                                                   4e-6, clamp, None if owner is None else _p(owner),
-                                                   None if owner is None else _p(rt), win,
+                                                   4e-6, clamp, None if tl is None else C.byref(tl),
                                                    _p(g), _p(dg), _stream()), "grad_diag")
             go, do = g0.cpu().numpy().copy(), g0.cpu().numpy().copy()
             OT.elastic_grad_diag(x, me.tri, me.D_m_inv, me.rest_area, me.mu, me.lam, 4e-6, clamp, go, do)

@@ -93,24 +93,29 @@ def test_model_refuses_cpu_device():
         Model(device="cpu")
 
 
-def test_element_ownership_layout():
-    """Tiled element kernel layout: owner[n] is the one chunk holding every triangle at n (else
-    -1); the ranges bound each chunk's node indices; checked by brute force."""
-    from diff_2d_fem_amd.layout import element_owners
+def test_element_tiles_layout():
+    """Node tiles of the element kernel, checked by brute force: each chunk's list is exactly
+    the sorted set of nodes its triangles touch, a node is owned by a chunk iff no other
+    chunk touches it, and the local indices point back at the triangle's nodes."""
+    from diff_2d_fem_amd.layout import element_tiles
     me = build_scene([dict(b) for b in scenes()["multi"]])
     tri = np.asarray(me.tri)
     N = me.rest_pos.shape[0]
-    own, rng, window = element_owners(tri, N, chunk=37)
+    ch = 37
+    et = element_tiles(tri, N, chunk=ch)
+    off, nodes = et["node_off"].numpy(), et["nodes"].numpy()
+    owned, loc = et["owned"].numpy(), et["tri_local"].numpy().astype(np.int64) & 0xFFFF
+    nch = (len(tri) + ch - 1) // ch
+    assert len(off) == nch + 1
     chunks_at = [set() for _ in range(N)]
     for t, tr in enumerate(tri):
         for n in tr:
-            chunks_at[n].add(t // 37)
-    for n in range(N):
-        want = next(iter(chunks_at[n])) if len(chunks_at[n]) == 1 else -1
-        assert own[n] == want
-    nch = (len(tri) + 36) // 37
-    assert len(rng) == 2 * nch
+            chunks_at[n].add(t // ch)
     for c in range(nch):
-        sub = tri[c * 37:(c + 1) * 37]
-        assert rng[2 * c] == sub.min() and rng[2 * c + 1] == sub.max()
-    assert window == int((rng[1::2] - rng[0::2] + 1).max())
+        sub = tri[c * ch:(c + 1) * ch]
+        lst = nodes[off[c]:off[c + 1]]
+        assert np.array_equal(lst, np.unique(sub))
+        for k, n in enumerate(lst):
+            assert owned[off[c] + k] == (chunks_at[n] == {c})
+        assert np.array_equal(lst[loc[c * ch:(c + 1) * ch, :3]], sub)
+    assert et["max_nodes"] == int(np.diff(off).max())
