@@ -207,7 +207,7 @@ def elastic_roofline(res):
     ach = bytes_ / dur / 1e9
     return dict(kernel="k_elastic_grad_diag (float64)", bound="hbm", achieved=round(ach, 1),
                 peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
-                traffic=measured_traffic("d2f::k_elastic_grad_diag", res),
+                traffic=measured_traffic("d2f::k_elastic_grad_diag_tiled", res),
                 avg_us=round(dur * 1e6, 2), bytes_per_launch=int(bytes_))
 
 
