@@ -386,6 +386,7 @@ class DeviceSolver:
                 raise ValueError("a pressure edge has an end that is not a boundary node")
         self.max_cells = int(max_cells)
         self._plan_lds()
+        self._apply_lds_overrides()
         self._layout = True
 
     LDS_BUDGET = 150 * 1024          # bytes of LDS one scene's workgroup may take
@@ -426,6 +427,16 @@ class DeviceSolver:
         e_per = np.bincount(self.e2env.cpu().numpy(), minlength=self.B)
         if self.max_nb < 65536 and e_per.max(initial=0) < 32768:
             self.max_pairs = int(max(0, min(left, 64 * self.max_nb)))
+
+    def _apply_lds_overrides(self):
+        """Caps on the LDS plan, set through `lds_overrides` (a dict with any of nacc_lds,
+        excl_bits_lds, max_pairs): each planned value is replaced by the smaller of the two.
+        They select the kernel's fallbacks (colour-by-colour element pass, exclusion from the
+        byte matrix, grid traversal or a pair list that overflows)."""
+        for k, v in (getattr(self, "lds_overrides", None) or {}).items():
+            if k not in ("nacc_lds", "excl_bits_lds", "max_pairs"):
+                raise KeyError(k)
+            setattr(self, k, min(int(getattr(self, k)), int(v)))
 
     def refresh_moduli(self):
         """Re-upload element moduli after mu_np / lam_np changed (both layouts)."""

@@ -187,3 +187,35 @@ def test_scene_mixed_grip_close_to_float64():
     err = np.abs(mg.get_x() - oi.solver.x).max() / 8e-3
     print(f"grip mixed: max |dx| / size = {err:.3e}")
     assert err < 1e-5
+
+
+@pytest.mark.parametrize("override", [dict(nacc_lds=0), dict(excl_bits_lds=0), dict(max_pairs=0),
+                                      dict(max_pairs=8), dict(nacc_lds=0, excl_bits_lds=0, max_pairs=0)])
+def test_scene_fallback_paths(override):
+    """The scene kernel's fallbacks for scenes too large for LDS, forced on the grip scene:
+    colour-by-colour element pass (nacc_lds=0), exclusion from the byte matrix
+    (excl_bits_lds=0), grid traversal (max_pairs=0) and a pair list that overflows
+    (max_pairs=8). Same bar as the default layout: 1e-9 of the body size."""
+    kw = dict(max_iter=60, eps=0.0, conv_check=0)
+
+    def each(m, pads, k):
+        close_jaws(m, pads, k, n_close=8)
+    mo, po = grip(n_env=2)
+    mg, pg = grip(n_env=2)
+    oi = OracleIntegrator(**kw)
+    gi = ImplicitIntegrator(path="scene", precision="double", **kw)
+    for k in range(1, 13):
+        each(mo, po, k)
+        each(mg, pg, k)
+        oi.simulate(mo, 1e-3)
+        if k == 1:
+            gi.attach(mg, 1e-3)
+            s = mg._solver
+            s.lds_overrides = override
+            s.build_layout()
+            for key, v in override.items():
+                assert getattr(s, key) <= v
+        gi.simulate(mg, 1e-3)
+    err = np.abs(mg.get_x() - oi.solver.x).max() / 8e-3
+    print(f"{override}: max |dx| / size = {err:.3e}")
+    assert err < 1e-9
