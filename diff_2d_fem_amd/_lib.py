@@ -54,7 +54,7 @@ _SCENE_FIELDS = (
                         "rg_start", "rn_start", "pe_start", "td_start", "ey_start")]
     + [("max_nb", C.c_int32), ("max_cells", C.c_int32)]
     + [("excl_bits", P), ("excl_bits_off", P), ("excl_bits_lds", C.c_int32), ("max_pairs", C.c_int32)]
-    + [("max_nn", C.c_int32), ("nacc_lds", C.c_int32)]
+    + [("max_nn", C.c_int32), ("nacc_lds", C.c_int32), ("xc_lds", C.c_int32), ("pad3_", C.c_int32)]
     + [("el", ElemTiles)]
 )
 

@@ -38,6 +38,23 @@ __device__ __forceinline__ void atomic_add2(double *a, int i, V2<double> v) {
   atomicAdd(a + 2 * i, v.x);
   atomicAdd(a + 2 * i + 1, v.y);
 }
+// The same on pointers known to be in global memory (address space 1).
+#ifndef D2F_GLOBAL
+#define D2F_GLOBAL __attribute__((address_space(1)))
+#endif
+typedef double d2f_dvec2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ V2<double> ld2(const D2F_GLOBAL double *a, int i) {
+  const d2f_dvec2 t = reinterpret_cast<const D2F_GLOBAL d2f_dvec2 *>(a)[i];
+  return {t.x, t.y};
+}
+__device__ __forceinline__ void st2(D2F_GLOBAL double *a, int i, V2<double> v) {
+  d2f_dvec2 t;
+  t.x = v.x;
+  t.y = v.y;
+  reinterpret_cast<D2F_GLOBAL d2f_dvec2 *>(a)[i] = t;
+}
+// float64 atomic add on a global pointer
+__device__ __forceinline__ void atomic_add_g(D2F_GLOBAL double *a, double v) { atomicAdd((double *)a, v); }
 // min / max of non-negative-or-any doubles by compare-and-swap (deterministic result).
 __device__ __forceinline__ void atomic_min_d(double *a, double v) {
   unsigned long long *p = reinterpret_cast<unsigned long long *>(a);

@@ -10,10 +10,25 @@
 // Template R is the element arithmetic: double (float64 path) or float (mixed: float32 element
 // stress and curvature from float64 positions; float64 state, dot products and energies).
 #include "common.h"
+#include "scene_dev.h"
 
 namespace d2f {
 
-using S = d2f_scene;
+using SH = d2f_scene;  // host view (kernel argument, launch)
+// device views, read in place from the kernel-argument segment (constant address space):
+// uniform fields become scalar loads and array pointers are known to be global
+using S = const __attribute__((address_space(4))) SceneG;
+using KPm = const __attribute__((address_space(4))) d2f_params;
+using LdsD = __attribute__((address_space(3))) double;
+using LdsI = __attribute__((address_space(3))) int;
+using LdsU = __attribute__((address_space(3))) uint32_t;
+using GD = D2F_GLOBAL double;
+using GCD = const D2F_GLOBAL double;
+typedef double dvec4 __attribute__((ext_vector_type(4)));
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+// atomics on LDS: the explicit cast to a generic pointer is folded back to an LDS operation
+__device__ __forceinline__ void lds_add(LdsD *a, double v) { atomicAdd((double *)a, v); }
+__device__ __forceinline__ int lds_add(LdsI *a, int v) { return atomicAdd((int *)a, v); }
 using Pm = d2f_params;
 constexpr int MAXC = D2F_MAX_COLORS;
 constexpr int RED_K = 20;  // widest block reduction
@@ -26,11 +41,17 @@ struct SceneScalars {
   int gnx, gny, active, iters, pending, stop;
   int npairs, plist, excl_lds;  // candidate pairs of the last build, list valid, bitset in LDS
 };
+using LdsScalars = __attribute__((address_space(3))) SceneScalars;
+
+// A collider read from the kernel arguments.
+__device__ __forceinline__ Collider to_collider_k(const __attribute__((address_space(4))) d2f_collider &c) {
+  return Collider{c.kind, c.cx, c.cy, c.rx, c.ry, c.rref, c.r, c.mu};
+}
 
 template <int TPB>
 struct Block {
   static constexpr int NW = TPB / WAVE;
-  double *red;  // [NW * RED_K]
+  LdsD *red;  // [NW * RED_K]
 
   template <int K>
   __device__ void sum(double (&v)[K]) {
@@ -86,7 +107,7 @@ struct Block {
       const int t = __shfl_up(incl, o, WAVE);
       if ((int)(threadIdx.x % WAVE) >= o) incl += t;
     }
-    int *ired = reinterpret_cast<int *>(red);
+    LdsI *ired = reinterpret_cast<LdsI *>(red);
     const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
     __syncthreads();
     if (l == WAVE - 1) ired[w] = incl;
@@ -109,28 +130,58 @@ struct Elem {
   R Amu, Alam;
 };
 
+// The element arrays of one precision, read once from the scene struct into registers: the
+// struct lives in LDS, and a loop with LDS atomics in it would otherwise re-read every pointer
+// from there on each trip.
 template <typename R>
-__device__ __forceinline__ Elem<R> load_elem(const S &s, int t);
+struct ElemPtrs {
+  const D2F_GLOBAL int32_t *tri;
+  const D2F_GLOBAL R *Bm, *Amu, *Alam;
+};
+template <typename R>
+__device__ __forceinline__ ElemPtrs<R> elem_ptrs(S &s);
+template <>
+__device__ __forceinline__ ElemPtrs<double> elem_ptrs<double>(S &s) {
+  return {s.sc_tri, s.sc_Bm, s.sc_Amu, s.sc_Alam};
+}
+template <>
+__device__ __forceinline__ ElemPtrs<float> elem_ptrs<float>(S &s) {
+  return {s.sc_tri, s.sc_Bm32, s.sc_Amu32, s.sc_Alam32};
+}
+template <typename R>
+__device__ __forceinline__ Elem<R> load_elem(const ElemPtrs<R> &q, int t) {
+  Elem<R> el;
+  el.i0 = q.tri[3 * t];
+  el.i1 = q.tri[3 * t + 1];
+  el.i2 = q.tri[3 * t + 2];
+  el.B = {q.Bm[4 * t], q.Bm[4 * t + 1], q.Bm[4 * t + 2], q.Bm[4 * t + 3]};
+  el.Amu = q.Amu[t];
+  el.Alam = q.Alam[t];
+  return el;
+}
+
+template <typename R>
+__device__ __forceinline__ Elem<R> load_elem(S &s, int t);
 
 template <>
-__device__ __forceinline__ Elem<double> load_elem<double>(const S &s, int t) {
+__device__ __forceinline__ Elem<double> load_elem<double>(S &s, int t) {
   Elem<double> el;
   el.i0 = s.sc_tri[3 * t];
   el.i1 = s.sc_tri[3 * t + 1];
   el.i2 = s.sc_tri[3 * t + 2];
-  const double4 b = reinterpret_cast<const double4 *>(s.sc_Bm)[t];
+  const dvec4 b = reinterpret_cast<const D2F_GLOBAL dvec4 *>(s.sc_Bm)[t];
   el.B = {b.x, b.y, b.z, b.w};
   el.Amu = s.sc_Amu[t];
   el.Alam = s.sc_Alam[t];
   return el;
 }
 template <>
-__device__ __forceinline__ Elem<float> load_elem<float>(const S &s, int t) {
+__device__ __forceinline__ Elem<float> load_elem<float>(S &s, int t) {
   Elem<float> el;
   el.i0 = s.sc_tri[3 * t];
   el.i1 = s.sc_tri[3 * t + 1];
   el.i2 = s.sc_tri[3 * t + 2];
-  const float4 b = reinterpret_cast<const float4 *>(s.sc_Bm32)[t];
+  const fvec4 b = reinterpret_cast<const D2F_GLOBAL fvec4 *>(s.sc_Bm32)[t];
   el.B = {b.x, b.y, b.z, b.w};
   el.Amu = s.sc_Amu32[t];
   el.Alam = s.sc_Alam32[t];
@@ -152,24 +203,26 @@ __device__ __forceinline__ V2<double> up(V2<R> v) {
 // ----------------------------------------------------------------------------- the kernel
 template <typename R, int TPB>
 struct Step {
-  const S &s;
-  const Pm &p;
-  SceneScalars &sc;
+  S &s;
+  KPm &p;
+  LdsScalars &sc;
   Block<TPB> blk;
-  double *bpos;   // 2 nb: boundary positions
-  double *bacc;   // 4 nb: boundary accumulators (grad x, grad y, diag x, diag y)
-  double *bref;   // 2 nb: boundary positions the grid and pair list were built from
-  double *nacc;   // 4 nn: element gradient and diagonal per node (when nacc_lds)
-  int *cstart;    // max_cells + 1
-  int *citem;     // nb
-  int *ccur;      // max_cells
-  uint32_t *xbits;  // exclusion bitset (excl_bits_lds words)
-  int *plist;     // max_pairs packed (edge - k0) << 16 | jp
+  LdsD *bpos;     // 2 nb: boundary positions
+  LdsD *bacc;     // 4 nb: boundary accumulators (grad x, grad y, diag x, diag y)
+  LdsD *bref;     // 2 nb: boundary positions the grid and pair list were built from
+  LdsD *nacc;     // 4 nn: element gradient and diagonal per node (when nacc_lds)
+  LdsD *xc;       // 2 nn: the scene's positions (when xc_lds), written through with s.x
+  bool xcon;
+  LdsI *cstart;   // max_cells + 1
+  LdsI *citem;    // nb
+  LdsI *ccur;     // max_cells
+  LdsU *xbits;    // exclusion bitset (excl_bits_lds words)
+  LdsI *plist;    // max_pairs packed (edge - k0) << 16 | jp
   int e, tid;
   int n0, n1, b0, nb, k0, k1, t0, t1, c0, bc0, bc1, g0, g1, r0, r1, pe0, pe1, td0, td1;
   int64_t eoff;
 
-  __device__ Step(const S &s_, const Pm &p_, SceneScalars &sc_, double *lds)
+  __device__ Step(S &s_, KPm &p_, LdsScalars &sc_, LdsD *lds)
       : s(s_), p(p_), sc(sc_) {
     e = blockIdx.x;
     tid = threadIdx.x;
@@ -178,11 +231,12 @@ struct Step {
     bacc = bpos + 2 * s.max_nb;
     bref = bacc + 4 * s.max_nb;
     nacc = bref + 2 * s.max_nb;
-    cstart = reinterpret_cast<int *>(nacc + (s.nacc_lds ? 4 * s.max_nn : 0));
+    xc = nacc + (s.nacc_lds ? 4 * s.max_nn : 0);
+    cstart = reinterpret_cast<LdsI *>(xc + (s.xc_lds ? 2 * s.max_nn : 0));
     citem = cstart + s.max_cells + 1;
     ccur = citem + s.max_nb;
-    xbits = reinterpret_cast<uint32_t *>(ccur + s.max_cells);
-    plist = reinterpret_cast<int *>(xbits + s.excl_bits_lds);
+    xbits = reinterpret_cast<LdsU *>(ccur + s.max_cells);
+    plist = reinterpret_cast<LdsI *>(xbits + s.excl_bits_lds);
     n0 = s.node_start[e];
     n1 = s.node_start[e + 1];
     b0 = s.bnd_start[e];
@@ -203,6 +257,25 @@ struct Step {
     td0 = s.n_td > 0 ? s.td_start[e] : 0;
     td1 = s.n_td > 0 ? s.td_start[e + 1] : 0;
     eoff = s.n_e > 0 ? s.excl_off[e] : 0;
+    xcon = s.xc_lds && n1 - n0 <= s.max_nn;
+  }
+
+  // Position of node i (of this scene): from the LDS copy when it is kept, else from s.x.
+  __device__ __forceinline__ V2<double> X(int i) const {
+    if (xcon) {
+      const int l = 2 * (i - n0);
+      return {xc[l], xc[l + 1]};
+    }
+    return ld2(s.x, i);
+  }
+  // Write node i's position to s.x and, when it is kept, to the LDS copy.
+  __device__ __forceinline__ void xput(int i, V2<double> v) {
+    st2(s.x, i, v);
+    if (xcon) {
+      const int l = 2 * (i - n0);
+      xc[l] = v.x;
+      xc[l + 1] = v.y;
+    }
   }
 
   __device__ double fr(int i) const { return s.free_[i] ? 1.0 : 0.0; }
@@ -216,7 +289,7 @@ struct Step {
     const bool have = sc.plist;
     double dmax = 0.0;
     for (int j = tid; j < nb; j += TPB) {
-      const V2<double> x = ld2(s.x, s.bnd[b0 + j]);
+      const V2<double> x = X(s.bnd[b0 + j]);
       bpos[2 * j] = x.x;
       bpos[2 * j + 1] = x.y;
       if (have) dmax = fmax(dmax, len(V2<double>{x.x - bref[2 * j], x.y - bref[2 * j + 1]}));
@@ -265,7 +338,7 @@ struct Step {
     const int nc = sc.gnx * sc.gny;
     for (int c = tid; c <= nc; c += TPB) cstart[c] = 0;
     __syncthreads();
-    for (int j = tid; j < nb; j += TPB) atomicAdd(&cstart[cell_of(bpos[2 * j], bpos[2 * j + 1])], 1);
+    for (int j = tid; j < nb; j += TPB) lds_add(&cstart[cell_of(bpos[2 * j], bpos[2 * j + 1])], 1);
     __syncthreads();
     // exclusive scan of the counts, in chunks per thread
     const int chunk = (nc + TPB - 1) / TPB;
@@ -288,7 +361,7 @@ struct Step {
     __syncthreads();
     for (int j = tid; j < nb; j += TPB) {
       const int c = cell_of(bpos[2 * j], bpos[2 * j + 1]);
-      citem[atomicAdd(&ccur[c], 1)] = j;
+      citem[lds_add(&ccur[c], 1)] = j;
     }
     __syncthreads();
     for (int c = tid; c < nc; c += TPB)
@@ -318,13 +391,13 @@ struct Step {
   // p^T H p share, 2 the energy share.
   __device__ double term(int mode, double coef, int jp, int ja, int jb, int ip, int ia, int ib,
                          double wa, double wb, double d, V2<double> n, double ddd, double mu,
-                         const double *pdir, double h2) {
+                         GCD *pdir, double h2) {
     const double dhat = p.dhat, kappa = p.ipc_kappa;
     const double bp = barrier_bp(d, dhat);
     const double f_n = -kappa * bp;
     const V2<double> tn = {-n.y, n.x};
-    const V2<double> u = (ld2(s.x, ip) - ld2(s.xn, ip)) - wa * (ld2(s.x, ia) - ld2(s.xn, ia)) -
-                         wb * (ld2(s.x, ib) - ld2(s.xn, ib));
+    const V2<double> u = (X(ip) - ld2(s.xn, ip)) - wa * (X(ia) - ld2(s.xn, ia)) -
+                         wb * (X(ib) - ld2(s.xn, ib));
     const double ut = dot(u, tn);
     const double vt = fabs(ut) * p.inv_dt;
     const double kt = mu * f_n * p.inv_dt / fmax(vt, p.epsv);
@@ -334,24 +407,24 @@ struct Step {
       const V2<double> hd = {h2 * (kappa * bpp * n.x * n.x + kt * tn.x * tn.x),
                              h2 * (kappa * bpp * n.y * n.y + kt * tn.y * tn.y)};
       const double fp = fr(ip), fa = fr(ia), fb = fr(ib);
-      double *A = bacc + 4 * jp;
-      atomicAdd(A, coef * fp * g.x);
-      atomicAdd(A + 1, coef * fp * g.y);
-      atomicAdd(A + 2, coef * hd.x);
-      atomicAdd(A + 3, coef * hd.y);
+      LdsD *A = bacc + 4 * jp;
+      lds_add(A, coef * fp * g.x);
+      lds_add(A + 1, coef * fp * g.y);
+      lds_add(A + 2, coef * hd.x);
+      lds_add(A + 3, coef * hd.y);
       if (wa != 0.0) {
-        double *Ba = bacc + 4 * ja;
-        atomicAdd(Ba, -coef * wa * fa * g.x);
-        atomicAdd(Ba + 1, -coef * wa * fa * g.y);
-        atomicAdd(Ba + 2, coef * wa * wa * hd.x);
-        atomicAdd(Ba + 3, coef * wa * wa * hd.y);
+        LdsD *Ba = bacc + 4 * ja;
+        lds_add(Ba, -coef * wa * fa * g.x);
+        lds_add(Ba + 1, -coef * wa * fa * g.y);
+        lds_add(Ba + 2, coef * wa * wa * hd.x);
+        lds_add(Ba + 3, coef * wa * wa * hd.y);
       }
       if (wb != 0.0) {
-        double *Bb = bacc + 4 * jb;
-        atomicAdd(Bb, -coef * wb * fb * g.x);
-        atomicAdd(Bb + 1, -coef * wb * fb * g.y);
-        atomicAdd(Bb + 2, coef * wb * wb * hd.x);
-        atomicAdd(Bb + 3, coef * wb * wb * hd.y);
+        LdsD *Bb = bacc + 4 * jb;
+        lds_add(Bb, -coef * wb * fb * g.x);
+        lds_add(Bb + 1, -coef * wb * fb * g.y);
+        lds_add(Bb + 2, coef * wb * wb * hd.x);
+        lds_add(Bb + 3, coef * wb * wb * hd.y);
       }
       return 0.0;
     }
@@ -380,7 +453,7 @@ struct Step {
 
   // One node-edge pair (reference implicit/ipc_contact.py:175-296, the body of the query loop):
   // edge k (global), scene-local boundary node jp that is neither end nor excluded.
-  __device__ void pair_eval(int mode, int k, int jp, const double *pdir, double t_c, double h2,
+  __device__ void pair_eval(int mode, int k, int jp, GCD *pdir, double t_c, double h2,
                             double &psum, double &cmin, double &amin) {
     const double dhat = p.dhat, reach = p.ipc_reach;
     const double dlo = 1.0e-9 * dhat, onset = 0.02 * dhat;
@@ -500,7 +573,7 @@ struct Step {
         candidates(k, [&](int jp) {
           const V2<double> xp = {bpos[2 * jp], bpos[2 * jp + 1]};
           if (seg_dist(xp, xa, xb) < reach) {
-            const int q = atomicAdd(&sc.npairs, 1);
+            const int q = lds_add(&sc.npairs, 1);
             if (q < s.max_pairs) plist[q] = ((k - k0) << 16) | jp;
           }
         });
@@ -513,7 +586,7 @@ struct Step {
 
   // One pass over the pairs (reference implicit/ipc_contact.py:175-296): mode 0 gradient +
   // diagonal into bacc, 1 p^T H p + clearance + step limit, 2 energy, 3 clearance + step limit.
-  __device__ void ipc_pass(int mode, const double *pdir, double t_c, double h2, double &psum,
+  __device__ void ipc_pass(int mode, GCD *pdir, double t_c, double h2, double &psum,
                            double &cmin, double &amin) {
     psum = 0.0;
     cmin = BIG;
@@ -536,7 +609,7 @@ struct Step {
     double q = 0.0;
     for (int b = bc0 + tid; b < bc1; b += TPB) {
       const int i = s.bc_idx[b];
-      const V2<double> d = ld2(s.bc_target, b) - ld2(s.x, i);
+      const V2<double> d = ld2(s.bc_target, b) - X(i);
       st2(s.bc_q, i, d);
       q = fmax(q, len(d));
     }
@@ -558,8 +631,8 @@ struct Step {
     for (int b = bc0 + tid; b < bc1; b += TPB) {
       const int i = s.bc_idx[b];
       const V2<double> tg = ld2(s.bc_target, b);
-      const V2<double> xi = t >= 1.0 ? tg : ld2(s.x, i) + t * (tg - ld2(s.x, i));
-      st2(s.x, i, xi);
+      const V2<double> xi = t >= 1.0 ? tg : X(i) + t * (tg - X(i));
+      xput(i, xi);
       st2(s.xhat, i, xi);
     }
     __syncthreads();
@@ -586,29 +659,23 @@ struct Step {
     for (int j = tid; j < 4 * nb; j += TPB) bacc[j] = 0.0;
     __syncthreads();
     const int ta = s.sc_cs[c0], tb = s.sc_cs[c0 + MAXC];
-    for (int t = ta + tid; t < tb; t += TPB) {
-      const Elem<R> el = load_elem<R>(s, t);
-      const NH<R> h = elem_F(el, ld2(s.x, el.i0), ld2(s.x, el.i1), ld2(s.x, el.i2));
-      V2<R> g[3], d[3];
-      nh_grad(h, el.B, el.Amu, R(1), g[0], g[1], g[2]);
-      nh_diag(h, el.B, el.Amu, el.Alam, R(1), d[0], d[1], d[2]);
-      const int li[3] = {el.i0 - n0, el.i1 - n0, el.i2 - n0};
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        double *a = nacc + 4 * li[q];
-        atomicAdd(a, double(g[q].x));
-        atomicAdd(a + 1, double(g[q].y));
-        atomicAdd(a + 2, fmax(double(d[q].x), 0.0));
-        atomicAdd(a + 3, fmax(double(d[q].y), 0.0));
-      }
+    // two triangles per trip, both loaded before either is computed, so that their global
+    // loads are in flight together
+    const ElemPtrs<R> ep = elem_ptrs<R>(s);
+    int t = ta + tid;
+    for (; t + TPB < tb; t += 2 * TPB) {
+      const Elem<R> e0 = load_elem<R>(ep, t), e1 = load_elem<R>(ep, t + TPB);
+      elem_to_lds(e0);
+      elem_to_lds(e1);
     }
+    if (t < tb) elem_to_lds(load_elem<R>(ep, t));
     __syncthreads();
     for (int i = n0 + tid; i < n1; i += TPB) {
       const V2<double> g = ld2(s.grad, i);
       st2(s.grad_prev, i, g);
       const double m = s.mass[i], f = fr(i);
-      const V2<double> d = ld2(s.x, i) - ld2(s.xhat, i);
-      const double *a = nacc + 4 * (i - n0);
+      const V2<double> d = X(i) - ld2(s.xhat, i);
+      const LdsD *a = nacc + 4 * (i - n0);
       V2<double> gn = {m * d.x * f, m * d.y * f};
       if (s.fext) gn = gn - h2 * ld2(s.fext, i);
       gn = gn + h2 * V2<double>{a[0], a[1]};
@@ -616,6 +683,23 @@ struct Step {
       st2(s.diag, i, V2<double>{m + h2 * a[2], m + h2 * a[3]});
     }
     __syncthreads();
+  }
+
+  // One triangle's gradient and clamped diagonal into the LDS node accumulators.
+  __device__ __forceinline__ void elem_to_lds(const Elem<R> &el) {
+    const NH<R> h = elem_F(el, X(el.i0), X(el.i1), X(el.i2));
+    V2<R> g[3], d[3];
+    nh_grad(h, el.B, el.Amu, R(1), g[0], g[1], g[2]);
+    nh_diag(h, el.B, el.Amu, el.Alam, R(1), d[0], d[1], d[2]);
+    const int li[3] = {el.i0 - n0, el.i1 - n0, el.i2 - n0};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      LdsD *a = nacc + 4 * li[q];
+      lds_add(a, double(g[q].x));
+      lds_add(a + 1, double(g[q].y));
+      lds_add(a + 2, fmax(double(d[q].x), 0.0));
+      lds_add(a + 3, fmax(double(d[q].y), 0.0));
+    }
   }
 
   // The same, colour by colour with plain read-modify-write of the node arrays (no two
@@ -626,7 +710,7 @@ struct Step {
       const V2<double> g = ld2(s.grad, i);
       st2(s.grad_prev, i, g);
       const double m = s.mass[i], f = fr(i);
-      const V2<double> d = ld2(s.x, i) - ld2(s.xhat, i);
+      const V2<double> d = X(i) - ld2(s.xhat, i);
       V2<double> gn = {m * d.x * f, m * d.y * f};
       if (s.fext) gn = gn - h2 * ld2(s.fext, i);
       st2(s.grad, i, gn);
@@ -640,7 +724,7 @@ struct Step {
       if (ta == tb) continue;
       for (int t = ta + tid; t < tb; t += TPB) {
         const Elem<R> el = load_elem<R>(s, t);
-        const NH<R> h = elem_F(el, ld2(s.x, el.i0), ld2(s.x, el.i1), ld2(s.x, el.i2));
+        const NH<R> h = elem_F(el, X(el.i0), X(el.i1), X(el.i2));
         V2<R> g0, g1, g2, d0, d1, d2;
         nh_grad(h, el.B, el.Amu, R(1), g0, g1, g2);
         nh_diag(h, el.B, el.Amu, el.Alam, R(1), d0, d1, d2);
@@ -676,23 +760,23 @@ struct Step {
     for (int k = pe0 + tid; k < pe1; k += TPB) {
       const int a = s.pe_a[k], b = s.pe_b[k];
       const double pr = s.ch_p[s.pe_ch[k]];
-      const V2<double> tv = ld2(s.x, b) - ld2(s.x, a);
+      const V2<double> tv = X(b) - X(a);
       const V2<double> g = (-0.5 * h2 * pr) * V2<double>{tv.y, -tv.x};
       const int ja = s.bloc[a], jb = s.bloc[b];
-      atomicAdd(bacc + 4 * ja, g.x);
-      atomicAdd(bacc + 4 * ja + 1, g.y);
-      atomicAdd(bacc + 4 * jb, g.x);
-      atomicAdd(bacc + 4 * jb + 1, g.y);
+      lds_add(bacc + 4 * ja, g.x);
+      lds_add(bacc + 4 * ja + 1, g.y);
+      lds_add(bacc + 4 * jb, g.x);
+      lds_add(bacc + 4 * jb + 1, g.y);
     }
     __syncthreads();
     // colliders, friction and the accumulated contact terms, one thread per boundary node
     for (int j = tid; j < nb; j += TPB) {
       const int n = s.bnd[b0 + j];
-      const V2<double> x = ld2(s.x, n);
+      const V2<double> x = X(n);
       V2<double> g = ld2(s.grad, n), dg = ld2(s.diag, n);
       for (int k = 0; k < s.n_col; ++k) {
         V2<double> gg, dd;
-        if (collider_grad_diag(to_collider(s.col[k]), x, p.dhat, p.kappa, h2, 1, gg, dd)) {
+        if (collider_grad_diag(to_collider_k(s.col[k]), x, p.dhat, p.kappa, h2, 1, gg, dd)) {
           g = g + gg;
           dg = dg + dd;
         }
@@ -700,7 +784,7 @@ struct Step {
       if (p.friction && s.free_[n]) {
         const V2<double> xn = ld2(s.xn, n);
         for (int k = 0; k < s.n_col; ++k) {
-          const Collider c = to_collider(s.col[k]);
+          const Collider c = to_collider_k(s.col[k]);
           FricFrame f;
           if (!friction_frame(c, x, xn, p.dhat, p.kappa, p.inv_dt, f)) continue;
           g = g + (h2 * c.mu * f.f_n / fmax(f.vtm, p.epsv)) * f.vt;
@@ -719,7 +803,7 @@ struct Step {
       for (int k = td0; k < td1; ++k) {
         const int ja = s.ey_start[k], jz = s.ey_start[k + 1];
         double L = 0.0;
-        for (int j = ja; j + 1 < jz; ++j) L += len(ld2(s.x, s.ey_node[j + 1]) - ld2(s.x, s.ey_node[j]));
+        for (int j = ja; j + 1 < jz; ++j) L += len(X(s.ey_node[j + 1]) - X(s.ey_node[j]));
         s.td_L[k] = L;
         double tau, ke;
         if (s.td_stiff[k] > 0.0) {
@@ -735,16 +819,16 @@ struct Step {
         for (int j = ja; j < jz; ++j) {
           const int n = s.ey_node[j];
           V2<double> gL = {0.0, 0.0}, geo = {0.0, 0.0};
-          const V2<double> xj = ld2(s.x, n);
+          const V2<double> xj = X(n);
           if (j > ja) {
-            const V2<double> t = xj - ld2(s.x, s.ey_node[j - 1]);
+            const V2<double> t = xj - X(s.ey_node[j - 1]);
             const double l = fmax(len(t), 1.0e-12), il = 1.0 / l;
             const V2<double> d = {t.x / l, t.y / l};
             gL = gL + d;
             geo = geo + V2<double>{(1.0 - d.x * d.x) * il, (1.0 - d.y * d.y) * il};
           }
           if (j + 1 < jz) {
-            const V2<double> t = ld2(s.x, s.ey_node[j + 1]) - xj;
+            const V2<double> t = X(s.ey_node[j + 1]) - xj;
             const double l = fmax(len(t), 1.0e-12), il = 1.0 / l;
             const V2<double> d = {t.x / l, t.y / l};
             gL = gL - d;
@@ -801,7 +885,7 @@ struct Step {
       for (int k = s.rn_start[g] + tid; k < s.rn_start[g + 1]; k += TPB) {
         const int i = s.rn_node[k];
         const double m = s.mass[i];
-        const V2<double> x = ld2(s.x, i);
+        const V2<double> x = X(i);
         a[0] += m * x.x;
         a[1] += m * x.y;
         a[2] += m;
@@ -813,7 +897,7 @@ struct Step {
       for (int k = 0; k < 12; ++k) v[k] = 0.0;
       for (int k = s.rn_start[g] + tid; k < s.rn_start[g + 1]; k += TPB) {
         const int i = s.rn_node[k];
-        const V2<double> x = ld2(s.x, i);
+        const V2<double> x = X(i);
         const V2<double> rr = {x.x - cx, x.y - cy};
         const V2<double> gg = ld2(s.grad, i), gp = ld2(s.grad_prev, i);
         const double m = s.mass[i];
@@ -840,7 +924,7 @@ struct Step {
           s.rg_gq[3 * g + r] = v[r];
           s.rg_gq_prev[3 * g + r] = v[3 + r];
         }
-        double *H = s.rg_H + 9 * g;
+        GD *H = s.rg_H + 9 * g;
         H[0] = v[6]; H[1] = 0.0; H[2] = v[7];
         H[3] = 0.0; H[4] = v[8]; H[5] = v[9];
         H[6] = v[7]; H[7] = v[9]; H[8] = v[10];
@@ -911,11 +995,11 @@ struct Step {
     __syncthreads();
     for (int g = g0; g < g1; ++g) expand(g, s.ag_pq, s.p);
     for (int g = r0; g < r1; ++g) {
-      const double *a = s.rg_acc + 3 * g, *z = s.rg_zq + 3 * g;
+      GCD *a = s.rg_acc + 3 * g, *z = s.rg_zq + 3 * g;
       const double cx = a[0] / a[2], cy = a[1] / a[2];
       for (int k = s.rn_start[g] + tid; k < s.rn_start[g + 1]; k += TPB) {
         const int i = s.rn_node[k];
-        const V2<double> x = ld2(s.x, i);
+        const V2<double> x = X(i);
         const V2<double> rr = {x.x - cx, x.y - cy};
         st2(s.p, i, ld2(s.p, i) - V2<double>{z[0] - z[2] * rr.y, z[1] + z[2] * rr.x});
       }
@@ -923,18 +1007,21 @@ struct Step {
     __syncthreads();
   }
 
-  __device__ void expand(int g, const double *v6, double *out) {
-    const double *v = v6 + 6 * g;
+  __device__ void expand(int g, GCD *v6, GD *out) {
+    GCD *v = v6 + 6 * g;
     for (int k = s.an_start[g] + tid; k < s.an_start[g + 1]; k += TPB) {
-      const double *w = s.an_w + 3 * k;
-      st2(out, s.an_node[k], V2<double>{w[0] * v[0] + w[1] * v[1] + w[2] * v[2],
-                                        w[0] * v[3] + w[1] * v[4] + w[2] * v[5]});
+      GCD *w = s.an_w + 3 * k;
+      const V2<double> val = {w[0] * v[0] + w[1] * v[1] + w[2] * v[2], w[0] * v[3] + w[1] * v[4] + w[2] * v[5]};
+      if (out == s.x)
+        xput(s.an_node[k], val);
+      else
+        st2(out, s.an_node[k], val);
     }
   }
 
   // 6x6 block preconditioner of one rigid region (reference implicit/affine.py:179-205)
   __device__ void affine_reduce(int b, double (&v)[5]) {
-    const double *q = s.ag_q + 6 * b;
+    GCD *q = s.ag_q + 6 * b;
     const double Mu = s.ag_Mu[b], La = s.ag_La[b];
     const NH<double> h = nh_eval(M2<double>{q[1], q[2], q[4], q[5]}, Mu, La);
     const double P[4] = {Mu * h.F.a + h.c * h.G.a, Mu * h.F.b + h.c * h.G.b, Mu * h.F.c + h.c * h.G.c,
@@ -972,8 +1059,10 @@ struct Step {
   // reference implicit/affine.py:457-473
   __device__ void rigid_reduce(int b, double (&v)[5]) {
     double Hi[9];
-    invert33(s.rg_H + 9 * b, Hi);
-    const double *g = s.rg_gq + 3 * b, *g0_ = s.rg_gq_prev + 3 * b;
+    double Hl[9];
+    for (int k = 0; k < 9; ++k) Hl[k] = s.rg_H[9 * b + k];
+    invert33(Hl, Hi);
+    GCD *g = s.rg_gq + 3 * b, *g0_ = s.rg_gq_prev + 3 * b;
     const double y[3] = {g[0] - g0_[0], g[1] - g0_[1], g[2] - g0_[2]};
     double z[3], zy[3];
     for (int r = 0; r < 3; ++r) {
@@ -1000,7 +1089,7 @@ struct Step {
     const int tsoft = s.sc_cs[c0 + MAXC];
     for (int t = t0 + tid; t < t1; t += TPB) {
       const int i0 = s.sc_tri[3 * t], i1 = s.sc_tri[3 * t + 1], i2 = s.sc_tri[3 * t + 2];
-      const V2<double> x0 = ld2(s.x, i0), x1 = ld2(s.x, i1), x2 = ld2(s.x, i2);
+      const V2<double> x0 = X(i0), x1 = X(i1), x2 = X(i2);
       const V2<double> q0 = ld2(s.p, i0), q1 = ld2(s.p, i1), q2 = ld2(s.p, i2);
       if (t < tsoft) {
         const Elem<R> el = load_elem<R>(s, t);
@@ -1014,9 +1103,9 @@ struct Step {
     // colliders: curvature, clearance, friction curvature, step cap
     for (int j = tid; j < nb; j += TPB) {
       const int n = s.bnd[b0 + j];
-      const V2<double> x = ld2(s.x, n), pp = ld2(s.p, n);
+      const V2<double> x = X(n), pp = ld2(s.p, n);
       for (int k = 0; k < s.n_col; ++k) {
-        const Collider c = to_collider(s.col[k]);
+        const Collider c = to_collider_k(s.col[k]);
         double dist;
         ph += collider_pHp(c, x, pp, p.dhat, p.kappa, h2, 1, dist);
         cmin = fmin(cmin, dist);
@@ -1024,7 +1113,7 @@ struct Step {
       if (p.friction && s.free_[n]) {
         const V2<double> xn = ld2(s.xn, n);
         for (int k = 0; k < s.n_col; ++k) {
-          const Collider c = to_collider(s.col[k]);
+          const Collider c = to_collider_k(s.col[k]);
           FricFrame f;
           if (!friction_frame(c, x, xn, p.dhat, p.kappa, p.inv_dt, f)) continue;
           const double kt = h2 * c.mu * f.f_n * p.inv_dt / fmax(f.vtm, p.epsv);
@@ -1032,7 +1121,7 @@ struct Step {
           ph += kt * pt * pt;
         }
       }
-      for (int k = 0; k < s.n_col; ++k) amin = fmin(amin, collider_ccd(to_collider(s.col[k]), x, pp, p.ccd_eta));
+      for (int k = 0; k < s.n_col; ++k) amin = fmin(amin, collider_ccd(to_collider_k(s.col[k]), x, pp, p.ccd_eta));
     }
     // pressure (reference implicit/actuation.py:102-110)
     for (int k = pe0 + tid; k < pe1; k += TPB) {
@@ -1057,7 +1146,7 @@ struct Step {
       const double p_inf = -vm[0];
       // rigid regions' own curvature and the tendons' (single thread, in order)
       for (int g = g0; g < g1; ++g) {
-        const double *q = s.ag_q + 6 * g, *pq = s.ag_pq + 6 * g;
+        GCD *q = s.ag_q + 6 * g, *pq = s.ag_pq + 6 * g;
         const double Mu = s.ag_Mu[g], La = s.ag_La[g];
         const NH<double> h = nh_eval(M2<double>{q[1], q[2], q[4], q[5]}, Mu, La);
         pH += h2 * fmax(nh_quad(h, M2<double>{pq[1], pq[2], pq[4], pq[5]}, Mu, La), 0.0);
@@ -1066,7 +1155,7 @@ struct Step {
         double dL = 0.0;
         for (int j = s.ey_start[k]; j + 1 < s.ey_start[k + 1]; ++j) {
           const int na = s.ey_node[j], nbd = s.ey_node[j + 1];
-          const V2<double> t = ld2(s.x, nbd) - ld2(s.x, na);
+          const V2<double> t = X(nbd) - X(na);
           const double l = fmax(len(t), 1.0e-12), il = 1.0 / l;
           const V2<double> d = {t.x / l, t.y / l};
           const V2<double> dp = ld2(s.p, nbd) - ld2(s.p, na);
@@ -1106,8 +1195,8 @@ struct Step {
   }
 
   // ------------------------------------------------------------------ update and line search
-  __device__ void set_x(const double *from, double a) {
-    for (int i = n0 + tid; i < n1; i += TPB) st2(s.x, i, ld2(from, i) + a * ld2(s.p, i));
+  __device__ void set_x(GCD *from, double a) {
+    for (int i = n0 + tid; i < n1; i += TPB) xput(i, ld2(from, i) + a * ld2(s.p, i));
   }
 
   __device__ void affine_to(int which, double a) {  // 1: q += a pq ; 2: q = q0 + a pq
@@ -1127,24 +1216,24 @@ struct Step {
     const double h2 = p.h2;
     double E = 0.0;
     for (int i = n0 + tid; i < n1; i += TPB) {
-      const V2<double> d = ld2(s.x, i) - ld2(s.xhat, i);
+      const V2<double> d = X(i) - ld2(s.xhat, i);
       E += 0.5 * s.mass[i] * dot(d, d);
-      if (s.fext) E += -h2 * dot(ld2(s.fext, i), ld2(s.x, i) - ld2(s.xn, i));
+      if (s.fext) E += -h2 * dot(ld2(s.fext, i), X(i) - ld2(s.xn, i));
     }
     const int tsoft = s.sc_cs[c0 + MAXC];
     for (int t = t0 + tid; t < tsoft; t += TPB) {
       const Elem<R> el = load_elem<R>(s, t);
-      const NH<R> h = elem_F(el, ld2(s.x, el.i0), ld2(s.x, el.i1), ld2(s.x, el.i2));
+      const NH<R> h = elem_F(el, X(el.i0), X(el.i1), X(el.i2));
       E += h2 * double(nh_psi(h, el.Amu, el.Alam));
     }
     for (int j = tid; j < nb; j += TPB) {
       const int n = s.bnd[b0 + j];
-      const V2<double> x = ld2(s.x, n);
-      for (int k = 0; k < s.n_col; ++k) E += collider_energy(to_collider(s.col[k]), x, p.dhat, p.kappa, h2);
+      const V2<double> x = X(n);
+      for (int k = 0; k < s.n_col; ++k) E += collider_energy(to_collider_k(s.col[k]), x, p.dhat, p.kappa, h2);
       if (p.friction && s.free_[n]) {
         const V2<double> xn = ld2(s.xn, n);
         for (int k = 0; k < s.n_col; ++k) {
-          const Collider c = to_collider(s.col[k]);
+          const Collider c = to_collider_k(s.col[k]);
           FricFrame f;
           if (!friction_frame(c, x, xn, p.dhat, p.kappa, p.inv_dt, f)) continue;
           const double ut = f.vtm / p.inv_dt, a = p.epsv / p.inv_dt;
@@ -1155,7 +1244,7 @@ struct Step {
     for (int k = pe0 + tid; k < pe1; k += TPB) {
       const int ch = s.pe_ch[k];
       const V2<double> ref = ld2(s.ch_ref, ch);
-      E += -h2 * s.ch_p[ch] * 0.5 * cross(ld2(s.x, s.pe_a[k]) - ref, ld2(s.x, s.pe_b[k]) - ref);
+      E += -h2 * s.ch_p[ch] * 0.5 * cross(X(s.pe_a[k]) - ref, X(s.pe_b[k]) - ref);
     }
     if (p.ipc_on && k1 > k0) {
       contact_prepare();
@@ -1166,14 +1255,14 @@ struct Step {
     E = blk.sum1(E);
     if (tid == 0) {
       for (int g = g0; g < g1; ++g) {
-        const double *q = s.ag_q + 6 * g;
+        GCD *q = s.ag_q + 6 * g;
         const NH<double> h = nh_eval(M2<double>{q[1], q[2], q[4], q[5]}, s.ag_Mu[g], s.ag_La[g]);
         E += h2 * nh_psi(h, s.ag_Mu[g], s.ag_La[g]);
       }
       for (int k = td0; k < td1; ++k) {
         double L = 0.0;
         for (int j = s.ey_start[k]; j + 1 < s.ey_start[k + 1]; ++j)
-          L += len(ld2(s.x, s.ey_node[j + 1]) - ld2(s.x, s.ey_node[j]));
+          L += len(X(s.ey_node[j + 1]) - X(s.ey_node[j]));
         if (s.td_stiff[k] > 0.0) {
           const double ext = L - s.td_L0[k];
           if (ext > 0.0) E += 0.5 * h2 * s.td_stiff[k] * ext * ext;
@@ -1198,7 +1287,7 @@ struct Step {
     }
     // energy backtracking gate (reference implicit/solver.py:403-425)
     const double E0 = energy();
-    for (int i = n0 + tid; i < n1; i += TPB) st2(s.x0, i, ld2(s.x, i));
+    for (int i = n0 + tid; i < n1; i += TPB) st2(s.x0, i, X(i));
     if (tid == 0)
       for (int k = 6 * g0; k < 6 * g1; ++k) s.ag_q0[k] = s.ag_q[k];
     __syncthreads();
@@ -1233,6 +1322,10 @@ struct Step {
     __syncthreads();
     for (int i = n0 + tid; i < n1; i += TPB) {
       const V2<double> x = ld2(s.x, i);
+      if (xcon) {
+        xc[2 * (i - n0)] = x.x;
+        xc[2 * (i - n0) + 1] = x.y;
+      }
       st2(s.xn, i, x);
       st2(s.xhat, i, x + (p.dt * p.drag) * ld2(s.v, i));
     }
@@ -1248,7 +1341,7 @@ struct Step {
       for (int b = bc0 + tid; b < bc1; b += TPB) {
         const int n = s.bc_idx[b];
         const V2<double> t = ld2(s.bc_target, b);
-        st2(s.x, n, t);
+        xput(n, t);
         st2(s.xhat, n, t);
       }
     } else if (bc1 > bc0) {
@@ -1281,7 +1374,7 @@ struct Step {
       update();
       __syncthreads();
     }
-    for (int i = n0 + tid; i < n1; i += TPB) st2(s.v, i, p.inv_dt * (ld2(s.x, i) - ld2(s.xn, i)));
+    for (int i = n0 + tid; i < n1; i += TPB) st2(s.v, i, p.inv_dt * (X(i) - ld2(s.xn, i)));
     if (tid == 0) {
       s.active[e] = sc.active;
       s.iters[e] = sc.iters;
@@ -1305,28 +1398,28 @@ struct Step {
   }
 };
 
+// offset of the second kernel argument (the params) in the kernel-argument segment
+constexpr size_t PM_OFFSET = (sizeof(SH) + alignof(Pm) - 1) / alignof(Pm) * alignof(Pm);
+static_assert(alignof(SH) == 8 && alignof(Pm) == 8, "kernel-argument layout");
+
 template <typename R, int TPB>
-__global__ __launch_bounds__(TPB) void k_scene(S s, Pm p) {
+__global__ __launch_bounds__(TPB) void k_scene(SH s, Pm p) {
   extern __shared__ double lds[];
   __shared__ SceneScalars sc;
-  // The arguments are read through references everywhere below; a reference to a by-value
-  // kernel argument makes the compiler copy it into every lane's private memory, so they are
-  // copied once into LDS instead.
-  __shared__ S s_lds;
-  __shared__ Pm p_lds;
-  if (threadIdx.x == 0) {
-    s_lds = s;
-    p_lds = p;
-  }
-  __syncthreads();
-  Step<R, TPB> st(s_lds, p_lds, sc, lds);
+  // The arguments are read in place from the kernel-argument segment (s at offset 0, p after
+  // it): a reference to a by-value argument would make the compiler copy it into every lane's
+  // private memory, and a copy in LDS would hide that its pointers are global.
+  S *ks = (S *)__builtin_amdgcn_kernarg_segment_ptr();
+  KPm *kp = (KPm *)((const __attribute__((address_space(4))) char *)ks + PM_OFFSET);
+  Step<R, TPB> st(*ks, *kp, *(LdsScalars *)&sc, (LdsD *)lds);
   st.run();
 }
 
 template <typename R, int TPB>
-static int launch(const S &s, const Pm &p, hipStream_t stream) {
+static int launch(const SH &s, const Pm &p, hipStream_t stream) {
   const size_t lds = sizeof(double) * (Block<TPB>::NW * RED_K + 8 * (size_t)s.max_nb +
-                                       (s.nacc_lds ? 4 * (size_t)s.max_nn : 0)) +
+                                       (s.nacc_lds ? 4 * (size_t)s.max_nn : 0) +
+                                       (s.xc_lds ? 2 * (size_t)s.max_nn : 0)) +
                      sizeof(int) * (2 * (size_t)s.max_cells + 1 + s.max_nb + (size_t)s.excl_bits_lds +
                                     (size_t)s.max_pairs);
   if (lds > 150 * 1024) {

@@ -398,7 +398,7 @@ class DeviceSolver:
         positions and accumulators, grid), then per-node element accumulators, then the
         bit-packed exclusion matrix, then the candidate-pair list (bit jp * n_b_e + j of a
         scene's bitset is its exclusion of boundary nodes jp and j)."""
-        self.excl_bits_lds, self.max_pairs, self.nacc_lds = 0, 0, 0
+        self.excl_bits_lds, self.max_pairs, self.nacc_lds, self.xc_lds = 0, 0, 0, 0
         self.excl_bits = self.excl_bits_off = None
         nn = np.diff(np.asarray(self.me.node_starts, np.int64))
         self.max_nn = int(nn.max(initial=0))
@@ -406,6 +406,9 @@ class DeviceSolver:
         if base + 32 * self.max_nn <= self.LDS_BUDGET:
             self.nacc_lds = 1
             base += 32 * self.max_nn
+        if base + 16 * self.max_nn + 4 * 4 * self.max_nb <= self.LDS_BUDGET:
+            self.xc_lds = 1                  # positions, if a modest pair list still fits
+            base += 16 * self.max_nn
         if self.ipc is None or self.n_e == 0:
             return
         me = self.me
@@ -426,15 +429,15 @@ class DeviceSolver:
         left = (self.LDS_BUDGET - base - 4 * self.excl_bits_lds) // 4
         e_per = np.bincount(self.e2env.cpu().numpy(), minlength=self.B)
         if self.max_nb < 65536 and e_per.max(initial=0) < 32768:
-            self.max_pairs = int(max(0, min(left, 64 * self.max_nb)))
+            self.max_pairs = int(max(0, min(left, 16 * self.max_nb)))
 
     def _apply_lds_overrides(self):
         """Caps on the LDS plan, set through `lds_overrides` (a dict with any of nacc_lds,
-        excl_bits_lds, max_pairs): each planned value is replaced by the smaller of the two.
+        xc_lds, excl_bits_lds, max_pairs): each planned value is replaced by the smaller of the two.
         They select the kernel's fallbacks (colour-by-colour element pass, exclusion from the
         byte matrix, grid traversal or a pair list that overflows)."""
         for k, v in (getattr(self, "lds_overrides", None) or {}).items():
-            if k not in ("nacc_lds", "excl_bits_lds", "max_pairs"):
+            if k not in ("nacc_lds", "xc_lds", "excl_bits_lds", "max_pairs"):
                 raise KeyError(k)
             setattr(self, k, min(int(getattr(self, k)), int(v)))
 
@@ -519,7 +522,7 @@ class DeviceSolver:
             if self.excl_bits is not None:
                 s.excl_bits, s.excl_bits_off = p("excl_bits"), p("excl_bits_off")
             s.excl_bits_lds, s.max_pairs = self.excl_bits_lds, self.max_pairs
-            s.max_nn, s.nacc_lds = self.max_nn, self.nacc_lds
+            s.max_nn, s.nacc_lds, s.xc_lds = self.max_nn, self.nacc_lds, self.xc_lds
         s.n_col = len(self.colliders)
         for i, c in enumerate(self.colliders):
             s.col[i].kind = int(c["kind"])

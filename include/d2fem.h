@@ -142,6 +142,9 @@ typedef struct d2f_scene {
   /* largest per-scene node count; nacc_lds = 1: the element pass accumulates every node's
    * gradient and diagonal in LDS (32 bytes per node), 0: colour-by-colour read-modify-write */
   int32_t max_nn, nacc_lds;
+  /* xc_lds = 1: the scene's positions are also kept in LDS (16 bytes per node), written
+   * through on every update and read from there */
+  int32_t xc_lds, pad3_;
   /* per-kernel path: node tiles of the element kernel (node_off NULL: atomics only) */
   d2f_elem_tiles el;
 } d2f_scene;

@@ -189,12 +189,14 @@ def test_scene_mixed_grip_close_to_float64():
     assert err < 1e-5
 
 
-@pytest.mark.parametrize("override", [dict(nacc_lds=0), dict(excl_bits_lds=0), dict(max_pairs=0),
-                                      dict(max_pairs=8), dict(nacc_lds=0, excl_bits_lds=0, max_pairs=0)])
+@pytest.mark.parametrize("override", [dict(nacc_lds=0), dict(xc_lds=0), dict(excl_bits_lds=0),
+                                      dict(max_pairs=0), dict(max_pairs=8),
+                                      dict(nacc_lds=0, xc_lds=0, excl_bits_lds=0, max_pairs=0)])
 def test_scene_fallback_paths(override):
     """The scene kernel's fallbacks for scenes too large for LDS, forced on the grip scene:
-    colour-by-colour element pass (nacc_lds=0), exclusion from the byte matrix
-    (excl_bits_lds=0), grid traversal (max_pairs=0) and a pair list that overflows
+    colour-by-colour element pass (nacc_lds=0), positions read from global memory
+    (xc_lds=0), exclusion from the byte matrix (excl_bits_lds=0), grid traversal
+    (max_pairs=0) and a pair list that overflows
     (max_pairs=8). Same bar as the default layout: 1e-9 of the body size."""
     kw = dict(max_iter=60, eps=0.0, conv_check=0)
 
