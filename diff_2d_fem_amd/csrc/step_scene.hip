@@ -222,7 +222,7 @@ struct Step {
   int n0, n1, b0, nb, k0, k1, t0, t1, c0, bc0, bc1, g0, g1, r0, r1, pe0, pe1, td0, td1;
   int64_t eoff;
 
-  __device__ Step(S &s_, KPm &p_, LdsScalars &sc_, LdsD *lds)
+  __device__ __forceinline__ Step(S &s_, KPm &p_, LdsScalars &sc_, LdsD *lds)
       : s(s_), p(p_), sc(sc_) {
     e = blockIdx.x;
     tid = threadIdx.x;
@@ -278,14 +278,14 @@ struct Step {
     }
   }
 
-  __device__ double fr(int i) const { return s.free_[i] ? 1.0 : 0.0; }
+  __device__ __forceinline__ double fr(int i) const { return s.free_[i] ? 1.0 : 0.0; }
 
   // ------------------------------------------------------------------ IPC broadphase grid
   // The boundary positions in LDS (current) are compared with those the pair list was built
   // from; when some node has moved more than half the skin, or there is no valid list, the
   // grid and the list are rebuilt. A pair within reach now was within reach + skin at the
   // build (each end moved at most skin / 2), so the list holds every pair a pass needs.
-  __device__ void contact_prepare() {
+  __device__ __forceinline__ void contact_prepare() {
     const bool have = sc.plist;
     double dmax = 0.0;
     for (int j = tid; j < nb; j += TPB) {
@@ -303,7 +303,7 @@ struct Step {
 
   // A uniform grid over the boundary positions in LDS (cell >= reach + skin, so a pair within
   // that distance is found among the cells the edge's box +- reach + skin overlaps).
-  __device__ void build_grid() {
+  __device__ __forceinline__ void build_grid() {
     double bb[4] = {BIG, BIG, BIG, BIG};  // min x, min y, -max x, -max y
     for (int j = tid; j < nb; j += TPB) {
       const V2<double> x = {bpos[2 * j], bpos[2 * j + 1]};
@@ -389,7 +389,7 @@ struct Step {
   // One barrier-and-friction term (reference implicit/ipc_contact.py:131-171); jp, ja, jb are
   // scene-local boundary indices. mode 0 accumulates into bacc (LDS atomics), 1 returns the
   // p^T H p share, 2 the energy share.
-  __device__ double term(int mode, double coef, int jp, int ja, int jb, int ip, int ia, int ib,
+  __device__ __forceinline__ double term(int mode, double coef, int jp, int ja, int jb, int ip, int ia, int ib,
                          double wa, double wb, double d, V2<double> n, double ddd, double mu,
                          GCD *pdir, double h2) {
     const double dhat = p.dhat, kappa = p.ipc_kappa;
@@ -453,7 +453,7 @@ struct Step {
 
   // One node-edge pair (reference implicit/ipc_contact.py:175-296, the body of the query loop):
   // edge k (global), scene-local boundary node jp that is neither end nor excluded.
-  __device__ void pair_eval(int mode, int k, int jp, GCD *pdir, double t_c, double h2,
+  __device__ __forceinline__ void pair_eval(int mode, int k, int jp, GCD *pdir, double t_c, double h2,
                             double &psum, double &cmin, double &amin) {
     const double dhat = p.dhat, reach = p.ipc_reach;
     const double dlo = 1.0e-9 * dhat, onset = 0.02 * dhat;
@@ -536,7 +536,7 @@ struct Step {
   // Candidates of edge k from the grid: calls f(jp) for every boundary node in the cells the
   // edge's box +- reach overlaps that is neither end of the edge nor excluded against one.
   template <typename F>
-  __device__ void candidates(int k, F &&f) {
+  __device__ __forceinline__ void candidates(int k, F &&f) {
     const double reach = p.ipc_reach + p.ipc_skin;
     const int la = s.ela[k], lb = s.elb[k];
     const double ax = bpos[2 * la], ay = bpos[2 * la + 1], bx = bpos[2 * lb], by = bpos[2 * lb + 1];
@@ -561,7 +561,7 @@ struct Step {
 
   // The pairs within reach at the positions the grid was built from, into the LDS list; the
   // list is used by every pass at these positions unless it overflowed.
-  __device__ void build_pairs() {
+  __device__ __forceinline__ void build_pairs() {
     if (tid == 0) sc.npairs = 0;
     __syncthreads();
     const bool fits = s.max_pairs > 0 && k1 - k0 < 32768;  // edge index packed in 15 bits
@@ -586,7 +586,7 @@ struct Step {
 
   // One pass over the pairs (reference implicit/ipc_contact.py:175-296): mode 0 gradient +
   // diagonal into bacc, 1 p^T H p + clearance + step limit, 2 energy, 3 clearance + step limit.
-  __device__ void ipc_pass(int mode, GCD *pdir, double t_c, double h2, double &psum,
+  __device__ __forceinline__ void ipc_pass(int mode, GCD *pdir, double t_c, double h2, double &psum,
                            double &cmin, double &amin) {
     psum = 0.0;
     cmin = BIG;
@@ -605,7 +605,7 @@ struct Step {
 
   // ------------------------------------------------------------------ kinematic sweep
   // reference implicit/ipc_contact.py:308-331, 456-483
-  __device__ void sweep() {
+  __device__ __forceinline__ void sweep() {
     double q = 0.0;
     for (int b = bc0 + tid; b < bc1; b += TPB) {
       const int i = s.bc_idx[b];
@@ -639,7 +639,7 @@ struct Step {
   }
 
   // ------------------------------------------------------------------ gradient + diagonal
-  __device__ void grad_diag() {
+  __device__ __forceinline__ void grad_diag() {
     if (s.nacc_lds && n1 - n0 <= s.max_nn) {
       grad_diag_lds();
     } else {
@@ -652,7 +652,7 @@ struct Step {
   // LDS with LDS atomics, all soft triangles in one pass; then one pass over the nodes writes
   // grad_prev, grad = inertia - h^2 f_ext + h^2 elastic, diag = mass + h^2 elastic
   // (reference grad_diag_mass_kernel, implicit/kernels.py:73-82).
-  __device__ void grad_diag_lds() {
+  __device__ __forceinline__ void grad_diag_lds() {
     const double h2 = p.h2;
     const int nn = n1 - n0;
     for (int j = tid; j < 4 * nn; j += TPB) nacc[j] = 0.0;
@@ -704,7 +704,7 @@ struct Step {
 
   // The same, colour by colour with plain read-modify-write of the node arrays (no two
   // triangles of a colour share a node), for scenes whose nodes do not fit in LDS.
-  __device__ void grad_diag_colored() {
+  __device__ __forceinline__ void grad_diag_colored() {
     const double h2 = p.h2;
     for (int i = n0 + tid; i < n1; i += TPB) {
       const V2<double> g = ld2(s.grad, i);
@@ -741,7 +741,7 @@ struct Step {
 
   // Everything after elasticity: rigid-mode diagonal snapshot, contact, pressure, colliders,
   // friction, tendons, affine and rigid-mode gathers.
-  __device__ void grad_diag_rest() {
+  __device__ __forceinline__ void grad_diag_rest() {
     const double h2 = p.h2;
     // rigid modes: the diagonal as it stands after elasticity (reference affine.py:541-542)
     if (r1 > r0) {
@@ -934,7 +934,7 @@ struct Step {
   }
 
   // ------------------------------------------------------------------ Dai-Kou direction
-  __device__ void direction(int it) {
+  __device__ __forceinline__ void direction(int it) {
     double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (int i = n0 + tid; i < n1; i += TPB) {
       const double f = fr(i);
@@ -1007,7 +1007,7 @@ struct Step {
     __syncthreads();
   }
 
-  __device__ void expand(int g, GCD *v6, GD *out) {
+  __device__ __forceinline__ void expand(int g, GCD *v6, GD *out) {
     GCD *v = v6 + 6 * g;
     for (int k = s.an_start[g] + tid; k < s.an_start[g + 1]; k += TPB) {
       GCD *w = s.an_w + 3 * k;
@@ -1020,7 +1020,7 @@ struct Step {
   }
 
   // 6x6 block preconditioner of one rigid region (reference implicit/affine.py:179-205)
-  __device__ void affine_reduce(int b, double (&v)[5]) {
+  __device__ __forceinline__ void affine_reduce(int b, double (&v)[5]) {
     GCD *q = s.ag_q + 6 * b;
     const double Mu = s.ag_Mu[b], La = s.ag_La[b];
     const NH<double> h = nh_eval(M2<double>{q[1], q[2], q[4], q[5]}, Mu, La);
@@ -1057,7 +1057,7 @@ struct Step {
   }
 
   // reference implicit/affine.py:457-473
-  __device__ void rigid_reduce(int b, double (&v)[5]) {
+  __device__ __forceinline__ void rigid_reduce(int b, double (&v)[5]) {
     double Hi[9];
     double Hl[9];
     for (int k = 0; k < 9; ++k) Hl[k] = s.rg_H[9 * b + k];
@@ -1076,7 +1076,7 @@ struct Step {
   }
 
   // ------------------------------------------------------------------ curvature and caps
-  __device__ void curvature_caps(int it) {
+  __device__ __forceinline__ void curvature_caps(int it) {
     const double h2 = p.h2;
     // [0] pHp, [1] p_inf (as -max), [2] alpha min, [3] clr_min
     double ph = 0.0, pinf = 0.0, amin = BIG, cmin = BIG;
@@ -1195,11 +1195,11 @@ struct Step {
   }
 
   // ------------------------------------------------------------------ update and line search
-  __device__ void set_x(GCD *from, double a) {
+  __device__ __forceinline__ void set_x(GCD *from, double a) {
     for (int i = n0 + tid; i < n1; i += TPB) xput(i, ld2(from, i) + a * ld2(s.p, i));
   }
 
-  __device__ void affine_to(int which, double a) {  // 1: q += a pq ; 2: q = q0 + a pq
+  __device__ __forceinline__ void affine_to(int which, double a) {  // 1: q += a pq ; 2: q = q0 + a pq
     if (tid == 0)
       for (int g = g0; g < g1; ++g)
         for (int r = 0; r < 6; ++r) {
@@ -1212,7 +1212,7 @@ struct Step {
   }
 
   // total incremental potential of the scene (reference implicit/solver.py:453-498)
-  __device__ double energy() {
+  __device__ __forceinline__ double energy() {
     const double h2 = p.h2;
     double E = 0.0;
     for (int i = n0 + tid; i < n1; i += TPB) {
@@ -1276,7 +1276,7 @@ struct Step {
     return sc.Ener;
   }
 
-  __device__ void update() {
+  __device__ __forceinline__ void update() {
     if (!sc.active) return;  // uniform: sc is shared
     const double a = sc.alpha;
     if (!p.line_search) {
@@ -1305,7 +1305,7 @@ struct Step {
   }
 
   // ------------------------------------------------------------------ the step
-  __device__ void run() {
+  __device__ __forceinline__ void run() {
     // exclusion bitset of this scene into LDS when it fits
     if (tid == 0) {
       sc.excl_lds = 0;
