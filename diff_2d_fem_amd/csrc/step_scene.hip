@@ -210,7 +210,7 @@ struct Step {
   LdsD *bpos;     // 2 nb: boundary positions
   LdsD *bacc;     // 4 nb: boundary accumulators (grad x, grad y, diag x, diag y)
   LdsD *bref;     // 2 nb: boundary positions the grid and pair list were built from
-  LdsD *nacc;     // 4 nn: element gradient and diagonal per node (when nacc_lds)
+  LdsD *nacc;     // 4 x max_nn, component-major: grad x, grad y, diag x, diag y of each node
   LdsD *xc;       // 2 nn: the scene's positions (when xc_lds), written through with s.x
   bool xcon;
   LdsI *cstart;   // max_cells + 1
@@ -655,7 +655,8 @@ struct Step {
   __device__ __forceinline__ void grad_diag_lds() {
     const double h2 = p.h2;
     const int nn = n1 - n0;
-    for (int j = tid; j < 4 * nn; j += TPB) nacc[j] = 0.0;
+    const int ns = s.max_nn;
+    for (int j = tid; j < 4 * ns; j += TPB) nacc[j] = 0.0;
     for (int j = tid; j < 4 * nb; j += TPB) bacc[j] = 0.0;
     __syncthreads();
     const int ta = s.sc_cs[c0], tb = s.sc_cs[c0 + MAXC];
@@ -675,18 +676,20 @@ struct Step {
       st2(s.grad_prev, i, g);
       const double m = s.mass[i], f = fr(i);
       const V2<double> d = X(i) - ld2(s.xhat, i);
-      const LdsD *a = nacc + 4 * (i - n0);
+      const LdsD *a = nacc + (i - n0);
       V2<double> gn = {m * d.x * f, m * d.y * f};
       if (s.fext) gn = gn - h2 * ld2(s.fext, i);
-      gn = gn + h2 * V2<double>{a[0], a[1]};
+      gn = gn + h2 * V2<double>{a[0], a[ns]};
       st2(s.grad, i, gn);
-      st2(s.diag, i, V2<double>{m + h2 * a[2], m + h2 * a[3]});
+      st2(s.diag, i, V2<double>{m + h2 * a[2 * ns], m + h2 * a[3 * ns]});
     }
     __syncthreads();
   }
 
-  // One triangle's gradient and clamped diagonal into the LDS node accumulators.
+  // One triangle's gradient and clamped diagonal into the LDS node accumulators (component
+  // k of node l at nacc[k * max_nn + l]: consecutive lanes hit spread-out banks).
   __device__ __forceinline__ void elem_to_lds(const Elem<R> &el) {
+    const int ns = s.max_nn;
     const NH<R> h = elem_F(el, X(el.i0), X(el.i1), X(el.i2));
     V2<R> g[3], d[3];
     nh_grad(h, el.B, el.Amu, R(1), g[0], g[1], g[2]);
@@ -694,11 +697,11 @@ struct Step {
     const int li[3] = {el.i0 - n0, el.i1 - n0, el.i2 - n0};
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      LdsD *a = nacc + 4 * li[q];
+      LdsD *a = nacc + li[q];
       lds_add(a, double(g[q].x));
-      lds_add(a + 1, double(g[q].y));
-      lds_add(a + 2, fmax(double(d[q].x), 0.0));
-      lds_add(a + 3, fmax(double(d[q].y), 0.0));
+      lds_add(a + ns, double(g[q].y));
+      lds_add(a + 2 * ns, fmax(double(d[q].x), 0.0));
+      lds_add(a + 3 * ns, fmax(double(d[q].y), 0.0));
     }
   }
 
