@@ -31,8 +31,19 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
+def gen_headers():
+    """Regenerate csrc/scene_dev.h (device mirror of d2f_scene) from include/d2fem.h."""
+    sys.path.insert(0, str(CSRC))
+    try:
+        import gen_scene_dev
+    finally:
+        sys.path.pop(0)
+    return gen_scene_dev.generate()
+
+
 def build(verbose=False):
     """Compile and link; returns the path of the library. Raises on a compiler error."""
+    gen_headers()
     srcs = sorted(CSRC.glob("*.hip"))
     objs = []
     for src in srcs:

@@ -32,3 +32,19 @@ def test_library_loads_and_exports_every_declared_symbol():
     assert not missing, f"declared in d2fem.h but not exported: {missing}"
     assert L.d2f_abi_version() == 1
     assert L.d2f_last_error() is not None
+
+
+def test_scene_device_mirror_is_current():
+    """csrc/scene_dev.h (the scene kernel's view of d2f_scene) matches what its generator writes
+    from include/d2fem.h today, and names every field of the header's struct."""
+    import importlib.util
+    csrc = Path(__file__).resolve().parents[1] / "diff_2d_fem_amd" / "csrc"
+    spec = importlib.util.spec_from_file_location("gen_scene_dev", csrc / "gen_scene_dev.py")
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    flds = gen.fields(HEADER.read_text())
+    assert (csrc / "scene_dev.h").read_text() == gen.render(flds)
+    names = [n for _, _, n in flds]
+    assert len(names) == len(set(names))
+    assert {"x", "v", "tri", "el", "excl_bits", "max_pairs", "xc_lds"} <= set(names)
+    assert sum(1 for f in _lib.Scene._fields_) == len(names)
