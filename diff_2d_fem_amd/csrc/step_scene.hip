@@ -48,15 +48,57 @@ __device__ __forceinline__ Collider to_collider_k(const __attribute__((address_s
   return Collider{c.kind, c.cx, c.cy, c.rx, c.ry, c.rref, c.r, c.mu};
 }
 
+// Cross-lane steps of one double in DPP form (two 32-bit halves moved by the VALU, no LDS):
+// CTRL 0xB1 swaps neighbouring lanes, 0x4E lanes two apart, 0x124 / 0x128 rotate each row of
+// 16 lanes by 4 / 8.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, int(b), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(int(b), l), hi = __builtin_amdgcn_readlane(int(b >> 32), l);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+// Reduction over each row of 16 lanes, the row's result in all of its lanes (all lanes active).
+__device__ __forceinline__ double row_sum(double v) {
+  v += dpp_f64<0xB1>(v);
+  v += dpp_f64<0x4E>(v);
+  v += dpp_f64<0x124>(v);
+  return v + dpp_f64<0x128>(v);
+}
+__device__ __forceinline__ double row_min(double v) {
+  v = fmin(v, dpp_f64<0xB1>(v));
+  v = fmin(v, dpp_f64<0x4E>(v));
+  v = fmin(v, dpp_f64<0x124>(v));
+  return fmin(v, dpp_f64<0x128>(v));
+}
+// Whole-wave results, uniform over the wave: the four row results read back as scalars.
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+  v = row_sum(v);
+  return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+}
+__device__ __forceinline__ double wave_min_dpp(double v) {
+  v = row_min(v);
+  return fmin(fmin(readlane_f64(v, 0), readlane_f64(v, 16)), fmin(readlane_f64(v, 32), readlane_f64(v, 48)));
+}
+
+// Workgroup reductions, called by every thread of the workgroup with every lane active; each
+// thread gets the result. Lane 0 of each wave parks the wave's values in red[]; then lane w of
+// every wave picks up wave w's value and one row reduction combines them (NW <= 16).
 template <int TPB>
 struct Block {
   static constexpr int NW = TPB / WAVE;
+  static_assert(NW <= 16, "one row of lanes combines the waves");
   LdsD *red;  // [NW * RED_K]
 
   template <int K>
   __device__ void sum(double (&v)[K]) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+    for (int k = 0; k < K; ++k) v[k] = wave_sum_dpp(v[k]);
     const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
     __syncthreads();
     if (l == 0)
@@ -64,17 +106,13 @@ struct Block {
       for (int k = 0; k < K; ++k) red[w * K + k] = v[k];
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      double r = 0.0;
-      for (int i = 0; i < NW; ++i) r += red[i * K + k];
-      v[k] = r;
-    }
+    for (int k = 0; k < K; ++k) v[k] = readlane_f64(row_sum(l < NW ? double(red[l * K + k]) : 0.0), 0);
   }
   // component-wise minimum
   template <int K>
   __device__ void min(double (&v)[K]) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) v[k] = wave_min(v[k]);
+    for (int k = 0; k < K; ++k) v[k] = wave_min_dpp(v[k]);
     const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
     __syncthreads();
     if (l == 0)
@@ -82,11 +120,8 @@ struct Block {
       for (int k = 0; k < K; ++k) red[w * K + k] = v[k];
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      double r = red[k];
-      for (int i = 1; i < NW; ++i) r = fmin(r, red[i * K + k]);
-      v[k] = r;
-    }
+    for (int k = 0; k < K; ++k)
+      v[k] = readlane_f64(row_min(l < NW ? double(red[l * K + k]) : __builtin_huge_val()), 0);
   }
   __device__ double sum1(double a) {
     double v[1] = {a};
