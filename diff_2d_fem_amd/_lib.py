@@ -70,7 +70,8 @@ class Params(C.Structure):
                                            "ipc_reach", "ipc_gap", "ipc_skin")]
                 + [(k, C.c_int32) for k in ("iter_max", "conv_check", "friction", "line_search",
                                             "n_backtrack", "ipc_on", "ipc_per_body", "rigid_modes_on",
-                                            "precision", "block_size")])
+                                            "precision", "block_size", "node_on", "node_fric")]
+                + [(k, C.c_double) for k in ("node_k", "node_r", "node_mu")])
 
 
 _lib = None
@@ -107,6 +108,8 @@ def lib():
     L.d2f_contact_force.argtypes = [C.POINTER(Scene), C.POINTER(Params), i32, i32, i32, P, P, P, P]
     L.d2f_collider_hvp_f64.argtypes = [P, P, P, i32, P, i32, i32, f64, f64, f64, P, P]
     L.d2f_collider_grad_diag_f64.argtypes = [P, P, i32, P, i32, i32, f64, f64, f64, i32, P, P, P]
+    L.d2f_node_grad_diag_f64.argtypes = [P, P, P, i32, P, P, f64, f64, f64, i32, P, P, P]
+    L.d2f_node_hvp_f64.argtypes = [P, P, P, P, i32, P, P, f64, f64, f64, P, P]
     L.d2f_elastic_drho_f64.argtypes = [P, P, P, P, P, P, P, i32, f64, f64, P, P]
     sizes = (C.c_int64 * 3)()
     L.d2f_struct_sizes(sizes)

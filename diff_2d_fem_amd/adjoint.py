@@ -10,12 +10,13 @@ trajectory x_0 .. x_T is differentiated by the damped backward-Euler recursion
 
 with H_k = M + h^2 (K_elastic + K_contact) at x_k, restricted to the free degrees of freedom,
 the contact active set frozen at x_k. The operator holds inertia, elasticity and the
-colliders' barriers; friction, contact between bodies, pressure, tendons and rigid regions
-have no tangent in it (transient_adjoint refuses a model with any of the last four, as the
-reference does).
+colliders' barriers, and with use_self the "node" contact's penalty between bodies. Friction,
+IPC contact between bodies, pressure, tendons and rigid regions have no tangent in it
+(transient_adjoint refuses a model with IPC contact, friction between bodies, pressure, tendons
+or rigid regions, as the reference does).
 
 Everything runs on the GPU: the Hessian-vector products and dJ/drho are HIP kernels
-(d2f_elastic_hvp_f64, d2f_collider_hvp_f64, d2f_elastic_drho_f64), and the preconditioned
+(d2f_elastic_hvp_f64, d2f_collider_hvp_f64, d2f_node_hvp_f64, d2f_elastic_drho_f64), and the preconditioned
 MINRES iteration keeps its vectors in device memory; only its scalars come to the host.
 """
 from __future__ import annotations
@@ -87,7 +88,10 @@ class AdjointPNCG2D:
         self.s.refresh_moduli()
 
     # ---- operator H v = M v + h^2 (K_elastic + K_contact) v (reference :231-256)
-    def _hvp(self, xw, vw, use_disc=False, use_sdf=False, include_mass=True):
+    def _node_on(self, use_self):
+        return bool(use_self) and self.s.node is not None
+
+    def _hvp(self, xw, vw, use_disc=False, use_sdf=False, include_mass=True, use_self=False):
         L = _lib.lib()
         s = self.s
         y = self._y
@@ -101,11 +105,17 @@ class AdjointPNCG2D:
                                               self._n_col, which, float(s.cfg.ipc_d_hat),
                                               float(s.cfg.obstacle_stiffness), self.h2, self._p(y),
                                               self._stream()), "d2f_collider_hvp_f64")
+        if self._node_on(use_self):           # reference :239-245, the pairs frozen at x
+            nd = s.node
+            _lib.check(L.d2f_node_hvp_f64(self._p(xw), self._p(vw), self._p(s.bnd), self._p(s.bnd_start), s.B,
+                                          self._p(s.excl), self._p(s.excl_off), nd["r"], nd["k"], self.h2,
+                                          self._p(y), self._stream()), "d2f_node_hvp_f64")
         return self._mass2 * vw + y if include_mass else y.clone()
 
     def apply_H(self, xstar, v, use_self=False, use_disc=False, use_sdf=False, include_mass=True):
         """H v at x* as an (N, 2) array (the reference's matrix-free product)."""
-        return self._hvp(self._t(xstar), self._t(v), use_disc, use_sdf, include_mass).cpu().numpy()
+        return self._hvp(self._t(xstar), self._t(v), use_disc, use_sdf, include_mass,
+                         use_self=use_self).cpu().numpy()
 
     def spd_probe(self, xstar, use_self=False, include_mass=True, n=16, seed=0):
         """min and max Rayleigh quotient p^T H p / p^T p over random free directions."""
@@ -114,12 +124,12 @@ class AdjointPNCG2D:
         vals = []
         for _ in range(n):
             p = self._t(rng.standard_normal((self.N, 2))) * self._free
-            Hp = self._hvp(xw, p, include_mass=include_mass) * self._free
+            Hp = self._hvp(xw, p, include_mass=include_mass, use_self=use_self) * self._free
             vals.append(float((p * Hp).sum() / (p * p).sum()))
         return min(vals), max(vals)
 
     # ---- Jacobi preconditioner diag(M + h^2 (K_elastic + contact)) (reference :356-372)
-    def _diag_precond(self, xw, use_disc=False, use_sdf=False):
+    def _diag_precond(self, xw, use_disc=False, use_sdf=False, use_self=False):
         L = _lib.lib()
         s = self.s
         g = torch.zeros((self.N, 2), dtype=F64, device=self.dev)
@@ -135,6 +145,12 @@ class AdjointPNCG2D:
                                                     float(s.cfg.obstacle_stiffness), self.h2, 1,
                                                     self._p(g), self._p(d), self._stream()),
                        "collider grad/diag")
+        if self._node_on(use_self):           # reference :332-335: clamped, as in the solve
+            nd = s.node
+            _lib.check(L.d2f_node_grad_diag_f64(self._p(xw), self._p(s.bnd), self._p(s.bnd_start), s.B,
+                                                self._p(s.excl), self._p(s.excl_off), nd["r"], nd["k"],
+                                                self.h2, 1, self._p(g), self._p(d), self._stream()),
+                       "d2f_node_grad_diag_f64")
         return torch.clamp(d, min=1e-30)
 
     # ---- Krylov solves on the free degrees of freedom
@@ -198,10 +214,10 @@ class AdjointPNCG2D:
         minres_fail, minres_last_info (and minres_resid when minres_track_resid is set)."""
         xw = self._t(xk)
         b = self._t(rhs) * self._free
-        dinv = self._free / self._diag_precond(xw, use_disc, use_sdf)
+        dinv = self._free / self._diag_precond(xw, use_disc, use_sdf, use_self=use_self)
 
         def op(v):
-            return self._hvp(xw, v, use_disc, use_sdf) * self._free
+            return self._hvp(xw, v, use_disc, use_sdf, use_self=use_self) * self._free
         lam, info, its = self._minres(op, b, dinv, float(rtol), int(maxiter))
         self.minres_calls = getattr(self, "minres_calls", 0) + 1
         self.minres_iters = its
@@ -228,7 +244,7 @@ class AdjointPNCG2D:
         r0 = float(torch.linalg.vector_norm(r))
         neg, it = 0, 0
         for it in range(1, iters + 1):
-            Ap = self._hvp(xw, p * free, include_mass=include_mass) * free
+            Ap = self._hvp(xw, p * free, include_mass=include_mass, use_self=use_self) * free
             pAp = float((p * Ap).sum())
             if pAp <= 0.0:
                 neg += 1
@@ -297,6 +313,8 @@ class AdjointPNCG2D:
             missing.append("Tendon")
         if s.affine is not None:
             missing.append("AffineBodies")
+        if s.node is not None and s.node["fric"]:
+            missing.append("BodyFriction")
         if missing:
             raise NotImplementedError(
                 f"the adjoint operator has no tangent for {', '.join(sorted(missing))}: the gradient "

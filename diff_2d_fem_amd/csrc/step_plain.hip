@@ -10,6 +10,7 @@
 #include <climits>
 
 #include "common.h"
+#include "node_contact.h"
 
 namespace d2f {
 
@@ -429,6 +430,110 @@ __global__ __launch_bounds__(256) void k_ipc_env(S s, int mode, PairCtx c, const
   const double t_c = (mode == 1 || mode == 3) ? tc[e] : 0.0;
   for (int k = s.e_start[e] + threadIdx.x; k < s.e_start[e + 1]; k += blockDim.x)
     ipc_edge(s, k, mode, c, pdir, grad, diag, freev, s.pHp, E, clr, alpha, t_c, s.mu_node, bp, bn);
+}
+
+// ------------------------------------------------------------------------- "node" contact
+// One workgroup per scene. Its boundary positions (and, as the mode needs, the slips over the
+// step and the direction) are staged in LDS when they fit. Each thread visits the partners of its
+// boundary nodes by a scan of the scene's boundary list in order; the reference finds the same
+// partners through a hash grid (implicit/kernels.py:772-869, implicit/body_contact.py:37-167).
+template <bool STAGED>
+__global__ __launch_bounds__(256) void k_node_env(int mode, NodeArgs a) {
+  extern __shared__ double sh[];
+  const int e = blockIdx.x;
+  const int start = a.bnd_start[e], n_be = a.bnd_start[e + 1] - start;
+  const bool fric = mode <= 2 && a.fric;
+  const bool dir = mode == 1 || mode == 3;
+  double *sx = sh, *su = sh + 2 * a.max_nb, *sp = sh + 4 * a.max_nb;
+  if (STAGED) {
+    for (int j = threadIdx.x; j < n_be; j += blockDim.x) {
+      const int n = a.bnd[start + j];
+      const V2<double> x = ld2(a.x, n);
+      sx[2 * j] = x.x;
+      sx[2 * j + 1] = x.y;
+      if (fric) {
+        const V2<double> u = x - ld2(a.xn, n);
+        su[2 * j] = u.x;
+        su[2 * j + 1] = u.y;
+      }
+      if (dir) {
+        const V2<double> q = ld2(a.pdir, n);
+        sp[2 * j] = q.x;
+        sp[2 * j + 1] = q.y;
+      }
+    }
+    __syncthreads();
+  }
+  auto X = [&](int j) -> V2<double> {
+    if (STAGED) return {sx[2 * j], sx[2 * j + 1]};
+    return ld2(a.x, a.bnd[start + j]);
+  };
+  auto U = [&](int j) -> V2<double> {
+    if (STAGED) return {su[2 * j], su[2 * j + 1]};
+    const int n = a.bnd[start + j];
+    return ld2(a.x, n) - ld2(a.xn, n);
+  };
+  auto P = [&](int j) -> V2<double> {
+    if (STAGED) return {sp[2 * j], sp[2 * j + 1]};
+    return ld2(a.pdir, a.bnd[start + j]);
+  };
+  const int64_t eoff = a.excl_off[e];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n_be; i += blockDim.x) {
+    const int n = a.bnd[start + i];
+    const V2<double> xa = X(i);
+    const V2<double> ua = fric ? U(i) : V2<double>{0.0, 0.0};
+    const V2<double> pa = dir ? P(i) : V2<double>{0.0, 0.0};
+    V2<double> g = {0.0, 0.0}, gf = {0.0, 0.0};
+    double hx = 0.0, hy = 0.0, fx = 0.0, fy = 0.0, val = 0.0, valf = 0.0;
+    const uint8_t *row = a.excl + eoff + (int64_t)i * n_be;
+    for (int jl = 0; jl < n_be; ++jl) {
+      if (jl == i || row[jl]) continue;
+      NodeVisit v;
+      if (!node_visit(xa, X(jl), a.r, v)) continue;
+      if (mode == 0) node_grad_diag(v, a.k, a.clamp, g, hx, hy);
+      else if (mode == 1) val += node_pHp(v, a.k, a.clamp, pa - P(jl));
+      else if (mode == 2) val += node_energy(v, a.k);
+      else g = g + node_hvp(v, a.k, pa - P(jl));
+      if (fric) {
+        const double mu = node_pair_mu(a.mu, a.per_body, a.mu_node, n, a.bnd[start + jl]);
+        const V2<double> du = ua - U(jl);
+        if (mode == 0) node_fric_grad_diag(v, a.k, mu, a.epsv, a.inv_dt, du, gf, fx, fy);
+        else if (mode == 1) valf += node_fric_pHp(v, a.k, mu, a.epsv, a.inv_dt, du, pa - P(jl));
+        else valf += node_fric_energy(v, a.k, mu, a.epsv, a.inv_dt, du);
+      }
+    }
+    if (mode == 0) {
+      atomic_add2(a.g, n, a.h2 * g);
+      atomic_add2(a.dg, n, V2<double>{a.h2 * hx, a.h2 * hy});
+      if (fric) {
+        const double f = a.freev ? (a.freev[n] ? 1.0 : 0.0) : 1.0;
+        atomic_add2(a.g, n, f * (a.h2 * gf));
+        atomic_add2(a.dg, n, V2<double>{f * (a.h2 * fx), f * (a.h2 * fy)});
+      }
+    } else if (mode == 3) {
+      atomic_add2(a.g, n, a.h2 * g);
+    } else {
+      acc += a.h2 * val + a.h2 * valf;
+    }
+  }
+  if (mode == 1 || mode == 2) {
+    const double w = wave_sum(acc);
+    if (threadIdx.x % WAVE == 0 && w != 0.0) atomicAdd(a.acc + e, w);
+  }
+}
+
+int launch_node_env(int mode, const NodeArgs &a, hipStream_t st) {
+  if (a.B <= 0) return 0;
+  // staged layout: positions, slips at 2 max_nb, direction at 4 max_nb (max_nb 0: unknown)
+  const size_t lds = sizeof(double) * 6 * (size_t)a.max_nb;
+  if (a.max_nb > 0 && lds <= 48 * 1024) {
+    hipLaunchKernelGGL(k_node_env<true>, dim3(a.B), dim3(256), lds, st, mode, a);
+  } else {
+    hipLaunchKernelGGL(k_node_env<false>, dim3(a.B), dim3(256), 0, st, mode, a);
+  }
+  D2F_HIP(hipGetLastError());
+  return 0;
 }
 
 // alpha <= 0.45 reach / p_inf (reference implicit/ipc_contact.py:300-305)
@@ -1148,6 +1253,35 @@ struct Plain {
     return 0;
   }
 
+  // "node" contact pass (reference implicit/solver.py:279-285, 370-375, 475-481)
+  int node_pass(int mode, double h2, double *acc) {
+    NodeArgs a{};
+    a.x = s.x;
+    a.xn = s.xn;
+    a.pdir = s.p;
+    a.bnd = s.bnd;
+    a.bnd_start = s.bnd_start;
+    a.B = s.B;
+    a.max_nb = s.max_nb;
+    a.excl = s.excl;
+    a.excl_off = s.excl_off;
+    a.freev = s.free_;
+    a.mu_node = s.mu_node;
+    a.k = p.node_k;
+    a.r = p.node_r;
+    a.mu = p.node_mu;
+    a.epsv = p.epsv;
+    a.inv_dt = p.inv_dt;
+    a.h2 = h2;
+    a.clamp = 1;
+    a.fric = p.node_fric;
+    a.per_body = p.ipc_per_body;
+    a.g = s.grad;
+    a.dg = s.diag;
+    a.acc = acc;
+    return launch_node_env(mode, a, st);
+  }
+
   int ccd_sweep() {  // IPCContact.ccd along bc_q into sweep_beta (reference ipc_contact.py:432-439)
     LAUNCH(k_fill, s.B, s.sweep_beta, s.B, 1.0);
     LAUNCH(k_fill, s.B, s.clr_ipc, s.B, BIG);
@@ -1186,6 +1320,10 @@ struct Plain {
     if (s.n_rg > 0)
       D2F_HIP(hipMemcpyAsync(s.diag_el, s.diag, sizeof(double) * 2 * s.N, hipMemcpyDeviceToDevice, st));
     if (s.n_col > 0) LAUNCH(k_colliders_grad_diag, s.n_b, s, p);
+    if (p.node_on) {
+      const int r = node_pass(0, p.h2, nullptr);
+      if (r) return r;
+    }
     // terms, in the reference's order: external force, affine (none), IPC, pressure, tendon
     if (s.fext) LAUNCH(k_extforce_grad, s.N, s, p.h2);
     if (p.ipc_on) {
@@ -1239,6 +1377,10 @@ struct Plain {
     LAUNCH(k_pHp_mass, s.N, s);
     LAUNCH(k_elastic_pHp, s.T, s.x, s.p, s.tri, s.Bm, s.area, s.mu, s.lam, s.tri2env, s.T, p.h2, 1, s.pHp);
     if (s.n_col > 0) LAUNCH(k_colliders_pHp, s.n_b, s, p);
+    if (p.node_on) {
+      const int r = node_pass(1, p.h2, s.pHp);
+      if (r) return r;
+    }
     if (s.n_ag > 0) LAUNCH(k_affine_pHp_energy, s.n_ag, s, 1, p.h2, s.Ener);
     if (p.ipc_on) {
       LAUNCH(k_fill, s.B, s.clr_ipc, s.B, BIG);
@@ -1263,6 +1405,10 @@ struct Plain {
     LAUNCH(k_node_energy, s.N, s, p.h2, E);
     LAUNCH(k_elastic_energy, s.T, s.x, s.tri, s.Bm, s.area, s.mu, s.lam, s.tri2env, s.T, p.h2, E);
     if (s.n_col > 0) LAUNCH(k_colliders_energy, s.n_b, s, p, E);
+    if (p.node_on) {
+      const int r = node_pass(2, p.h2, E);
+      if (r) return r;
+    }
     if (s.n_ag > 0) LAUNCH(k_affine_pHp_energy, s.n_ag, s, 2, p.h2, E);
     if (p.ipc_on) {
       int r = ipc_pass(2, s.p, s.grad, s.diag, s.free_, E, s.clr_ipc, s.alpha, p.h2, p.ipc_mu);
@@ -1469,6 +1615,8 @@ int d2f_ipc_min_distance(const d2f_scene *s, const d2f_params *p, void *stream) 
 
 int d2f_step_plain(const d2f_scene *s, const d2f_params *p, void *stream) {
   D2F_CHECK(s && p, "d2f_step_plain: null scene or params");
+  D2F_CHECK(!p->node_on || (!p->ipc_on && s->excl && s->excl_off && p->node_r > 0.0),
+            "d2f_step_plain: node contact needs excl, excl_off, node_r > 0 and no IPC contact");
   D2F_CHECK(s->B > 0 && s->N > 0 && s->T >= 0 && s->n_col >= 0 && s->n_col <= D2F_MAX_COLLIDERS,
             "d2f_step_plain: bad sizes (B=%d N=%d T=%d n_col=%d)", s->B, s->N, s->T, s->n_col);
   D2F_CHECK(p->iter_max >= 0 && p->dt > 0.0, "d2f_step_plain: bad params");
@@ -1510,6 +1658,13 @@ int d2f_contact_force(const d2f_scene *s, const d2f_params *p, int32_t colliders
     }
     Plain run(s1, p2, st);
     const int r = run.ipc_pass(0, s->p, g, dg, all_free, s->Ener, s->clr_min, s->alpha, 1.0, p2.ipc_mu);
+    if (r) return r;
+  }
+  if (bodies && p->node_on) {  // reference implicit/solver.py:591-601: penalty, then pair friction
+    Pm p2 = p1;
+    p2.node_fric = friction ? p->node_fric : 0;
+    Plain run(s1, p2, st);
+    const int r = run.node_pass(0, 1.0, nullptr);
     if (r) return r;
   }
   D2F_HIP(hipGetLastError());

@@ -10,6 +10,7 @@
 // Template R is the element arithmetic: double (float64 path) or float (mixed: float32 element
 // stress and curvature from float64 positions; float64 state, dot products and energies).
 #include "common.h"
+#include "node_contact.h"
 #include "scene_dev.h"
 
 namespace d2f {
@@ -291,7 +292,7 @@ struct Step {
     pe1 = s.n_pe > 0 ? s.pe_start[e + 1] : 0;
     td0 = s.n_td > 0 ? s.td_start[e] : 0;
     td1 = s.n_td > 0 ? s.td_start[e + 1] : 0;
-    eoff = s.n_e > 0 ? s.excl_off[e] : 0;
+    eoff = s.excl_off ? s.excl_off[e] : 0;
     xcon = s.xc_lds && n1 - n0 <= s.max_nn;
   }
 
@@ -320,6 +321,9 @@ struct Step {
   // from; when some node has moved more than half the skin, or there is no valid list, the
   // grid and the list are rebuilt. A pair within reach now was within reach + skin at the
   // build (each end moved at most skin / 2), so the list holds every pair a pass needs.
+  // Contact distance the broadphase serves: the IPC reach or the "node" contact radius.
+  __device__ __forceinline__ double creach() const { return p.node_on ? p.node_r : p.ipc_reach; }
+
   __device__ __forceinline__ void contact_prepare() {
     const bool have = sc.plist;
     double dmax = 0.0;
@@ -332,7 +336,12 @@ struct Step {
     dmax = blk.max1(dmax);
     if (!have || !(2.0 * dmax <= p.ipc_skin)) {
       build_grid();
-      build_pairs();
+      if (p.node_on) {  // node pairs are found in the grid itself; plist marks it current
+        if (tid == 0) sc.plist = 1;
+        __syncthreads();
+      } else {
+        build_pairs();
+      }
     }
   }
 
@@ -351,7 +360,7 @@ struct Step {
     }
     blk.template min<4>(bb);
     if (tid == 0) {
-      double h = p.ipc_reach + p.ipc_skin;
+      double h = creach() + p.ipc_skin;
       const double wx = -bb[2] - bb[0], wy = -bb[3] - bb[1];
       int nx = 1, ny = 1;
       if (wx >= 0.0 && wy >= 0.0 && wx < 1.0e12 && wy < 1.0e12) {  // finite positions
@@ -638,6 +647,79 @@ struct Step {
       candidates(k, [&](int jp) { pair_eval(mode, k, jp, pdir, t_c, h2, psum, cmin, amin); });
   }
 
+  // ------------------------------------------------------------------ "node" contact
+  // Partners of scene-local boundary node j: every other boundary node in the grid cells within
+  // node_r + skin of it that is not excluded against j, in cell order (reference
+  // implicit/kernels.py:772-869 finds the same set through a hash-grid query). f(jb, xb).
+  template <typename F>
+  __device__ __forceinline__ void node_partners(int j, V2<double> xa, F &&f) {
+    const double reach = p.node_r + p.ipc_skin;
+    int ix0 = (int)((xa.x - reach - sc.gx0) / sc.gh), ix1 = (int)((xa.x + reach - sc.gx0) / sc.gh);
+    int iy0 = (int)((xa.y - reach - sc.gy0) / sc.gh), iy1 = (int)((xa.y + reach - sc.gy0) / sc.gh);
+    ix0 = max(ix0, 0);
+    iy0 = max(iy0, 0);
+    ix1 = min(ix1, sc.gnx - 1);
+    iy1 = min(iy1, sc.gny - 1);
+    for (int iy = iy0; iy <= iy1; ++iy)
+      for (int ix = ix0; ix <= ix1; ++ix) {
+        const int c = iy * sc.gnx + ix;
+        for (int q = cstart[c]; q < cstart[c + 1]; ++q) {
+          const int jb = citem[q];
+          if (jb == j || excluded(j, jb)) continue;
+          f(jb, V2<double>{bpos[2 * jb], bpos[2 * jb + 1]});
+        }
+      }
+  }
+
+  // Penalty and pair-friction gradient and diagonal of boundary node j (node n at x), added to
+  // g, dg (reference implicit/kernels.py:772-808, implicit/body_contact.py:37-82).
+  __device__ __forceinline__ void node_grad(int j, int n, V2<double> x, double h2, V2<double> &g,
+                                            V2<double> &dg) {
+    V2<double> gp = {0.0, 0.0}, gf = {0.0, 0.0};
+    double hx = 0.0, hy = 0.0, fx = 0.0, fy = 0.0;
+    const bool fric = p.node_fric;
+    const V2<double> ua = fric ? x - ld2(s.xn, n) : V2<double>{0.0, 0.0};
+    node_partners(j, x, [&](int jb, V2<double> xb) {
+      NodeVisit v;
+      if (!node_visit(x, xb, p.node_r, v)) return;
+      node_grad_diag(v, p.node_k, 1, gp, hx, hy);
+      if (fric) {
+        const int nj = s.bnd[b0 + jb];
+        const double mu = node_pair_mu(p.node_mu, p.ipc_per_body, s.mu_node, n, nj);
+        node_fric_grad_diag(v, p.node_k, mu, p.epsv, p.inv_dt, ua - (xb - ld2(s.xn, nj)), gf, fx, fy);
+      }
+    });
+    g = g + h2 * gp;
+    dg = dg + V2<double>{h2 * hx, h2 * hy};
+    if (fric && s.free_[n]) {
+      g = g + h2 * gf;
+      dg = dg + V2<double>{h2 * fx, h2 * fy};
+    }
+  }
+
+  // Half of p^T H p (mode 1) or of the energy (mode 2) of boundary node j's pairs, times h2
+  // (reference implicit/kernels.py:811-869, implicit/body_contact.py:86-167).
+  __device__ __forceinline__ double node_sum(int mode, int j, int n, V2<double> x, double h2) {
+    double val = 0.0, valf = 0.0;
+    const bool fric = p.node_fric;
+    const V2<double> ua = fric ? x - ld2(s.xn, n) : V2<double>{0.0, 0.0};
+    const V2<double> pa = mode == 1 ? ld2(s.p, n) : V2<double>{0.0, 0.0};
+    node_partners(j, x, [&](int jb, V2<double> xb) {
+      NodeVisit v;
+      if (!node_visit(x, xb, p.node_r, v)) return;
+      const int nj = s.bnd[b0 + jb];
+      const V2<double> dp = mode == 1 ? pa - ld2(s.p, nj) : V2<double>{0.0, 0.0};
+      val += mode == 1 ? node_pHp(v, p.node_k, 1, dp) : node_energy(v, p.node_k);
+      if (fric) {
+        const double mu = node_pair_mu(p.node_mu, p.ipc_per_body, s.mu_node, n, nj);
+        const V2<double> du = ua - (xb - ld2(s.xn, nj));
+        valf += mode == 1 ? node_fric_pHp(v, p.node_k, mu, p.epsv, p.inv_dt, du, dp)
+                          : node_fric_energy(v, p.node_k, mu, p.epsv, p.inv_dt, du);
+      }
+    });
+    return h2 * val + h2 * valf;
+  }
+
   // ------------------------------------------------------------------ kinematic sweep
   // reference implicit/ipc_contact.py:308-331, 456-483
   __device__ __forceinline__ void sweep() {
@@ -807,6 +889,7 @@ struct Step {
       lds_add(bacc + 4 * jb + 1, g.y);
     }
     __syncthreads();
+    if (p.node_on && nb > 0) contact_prepare();
     // colliders, friction and the accumulated contact terms, one thread per boundary node
     for (int j = tid; j < nb; j += TPB) {
       const int n = s.bnd[b0 + j];
@@ -830,6 +913,7 @@ struct Step {
           dg = dg + V2<double>{kt * f.n.y * f.n.y, kt * f.n.x * f.n.x};
         }
       }
+      if (p.node_on) node_grad(j, n, x, h2, g, dg);
       g = g + V2<double>{bacc[4 * j], bacc[4 * j + 1]};
       dg = dg + V2<double>{bacc[4 * j + 2], bacc[4 * j + 3]};
       st2(s.grad, n, g);
@@ -1160,6 +1244,7 @@ struct Step {
         }
       }
       for (int k = 0; k < s.n_col; ++k) amin = fmin(amin, collider_ccd(to_collider_k(s.col[k]), x, pp, p.ccd_eta));
+      if (p.node_on) ph += node_sum(1, j, n, x, h2);
     }
     // pressure (reference implicit/actuation.py:102-110)
     for (int k = pe0 + tid; k < pe1; k += TPB) {
@@ -1264,10 +1349,12 @@ struct Step {
       const NH<R> h = elem_F(el, X(el.i0), X(el.i1), X(el.i2));
       E += h2 * double(nh_psi(h, el.Amu, el.Alam));
     }
+    if (p.node_on && nb > 0) contact_prepare();
     for (int j = tid; j < nb; j += TPB) {
       const int n = s.bnd[b0 + j];
       const V2<double> x = X(n);
       for (int k = 0; k < s.n_col; ++k) E += collider_energy(to_collider_k(s.col[k]), x, p.dhat, p.kappa, h2);
+      if (p.node_on) E += node_sum(2, j, n, x, h2);
       if (p.friction && s.free_[n]) {
         const V2<double> xn = ld2(s.xn, n);
         for (int k = 0; k < s.n_col; ++k) {
@@ -1350,7 +1437,7 @@ struct Step {
       sc.plist = 0;
     }
     __syncthreads();
-    if (p.ipc_on && k1 > k0 && s.excl_bits) {
+    if (((p.ipc_on && k1 > k0) || p.node_on) && s.excl_bits) {
       const int64_t w0 = s.excl_bits_off[e], nw = s.excl_bits_off[e + 1] - w0;
       if (nw <= s.excl_bits_lds) {
         for (int64_t i = tid; i < nw; i += TPB) xbits[i] = s.excl_bits[w0 + i];
@@ -1488,6 +1575,8 @@ extern "C" int d2f_step_scene(const d2f_scene *s, const d2f_params *p, void *str
   D2F_CHECK(s->max_pairs == 0 || s->max_nb < 65536,
             "d2f_step_scene: the pair list packs boundary indices in 16 bits (max_nb < 65536)");
   D2F_CHECK(!p->ipc_on || (s->bloc && s->e_start), "d2f_step_scene: IPC needs bloc and e_start");
+  D2F_CHECK(!p->node_on || (!p->ipc_on && s->excl && s->excl_off && p->node_r > 0.0 && p->ipc_skin >= 0.0),
+            "d2f_step_scene: node contact needs excl, excl_off, node_r > 0 and no IPC contact");
   hipStream_t st = (hipStream_t)stream;
   const int tpb = p->block_size > 0 ? p->block_size : 256;
   if (p->precision == 1) {

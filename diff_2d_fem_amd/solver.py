@@ -45,6 +45,7 @@ class DeviceSolver:
         self.line_search, self.n_bt, self.ls_tol = bool(line_search), int(n_backtrack), float(ls_tol)
         self.cg_guard = 0.0
         self.ipc = None
+        self.node = None                   # "node" contact between bodies, when set up
         self.affine = None
         self.rigid = None
         self.B = int(me.n_envs)
@@ -139,8 +140,21 @@ class DeviceSolver:
         self.e_start = t(np.searchsorted(env, np.arange(self.B + 1)), I32)
         self.max_nb = int(np.max(np.asarray(me.boundary_per_env))) if self.B else 0
 
+    def setup_node(self, stiffness, radius, mu, mu_node=None, friction=False):
+        """The "node" penalty contact between boundary nodes of one environment, with Coulomb
+        friction on its pairs when `friction` (reference implicit/solver.py:215-226,
+        implicit/body_contact.py:170-210)."""
+        me = self.me
+        t = self._t
+        self.node = dict(k=float(stiffness), r=float(radius), mu=float(mu), fric=bool(friction),
+                         per_body=mu_node is not None)
+        self.excl = t(np.asarray(me.excl_flat, np.uint8), U8)
+        self.excl_off = t(np.asarray(me.excl_offset, np.int64), I64)
+        self.mu_node = None if mu_node is None else t(mu_node, F64)
+        self.max_nb = int(np.max(np.asarray(me.boundary_per_env))) if self.B else 0
+
     def refresh_exclusion(self):
-        if self.ipc is not None:
+        if self.ipc is not None or self.node is not None:
             self.excl = self._t(np.asarray(self.me.excl_flat, np.uint8), U8)
             self._layout = None
 
@@ -409,7 +423,7 @@ class DeviceSolver:
         if base + 16 * self.max_nn + 4 * 4 * self.max_nb <= self.LDS_BUDGET:
             self.xc_lds = 1                  # positions, if a modest pair list still fits
             base += 16 * self.max_nn
-        if self.ipc is None or self.n_e == 0:
+        if (self.ipc is None or self.n_e == 0) and self.node is None:
             return
         me = self.me
         flat = np.asarray(me.excl_flat, np.uint8)
@@ -426,6 +440,8 @@ class DeviceSolver:
         wmax = int(np.max(np.diff(woff)))
         if wmax <= self.EXCL_LDS_MAX and base + 4 * wmax <= self.LDS_BUDGET:
             self.excl_bits_lds = wmax
+        if self.ipc is None:             # node contact: partners come from the grid itself
+            return
         left = (self.LDS_BUDGET - base - 4 * self.excl_bits_lds) // 4
         e_per = np.bincount(self.e2env.cpu().numpy(), minlength=self.B)
         if self.max_nb < 65536 and e_per.max(initial=0) < 32768:
@@ -485,6 +501,10 @@ class DeviceSolver:
                 setattr(s, k, p(k))
             s.mu_node = self._ptr(self.mu_node)
             s.e_start, s.max_nb = p("e_start"), self.max_nb
+        if self.node is not None:
+            s.excl, s.excl_off = p("excl"), p("excl_off")
+            s.mu_node = self._ptr(self.mu_node)
+            s.max_nb = self.max_nb
         s.fext = self._ptr(self.fext)
         if self.pressure is not None:
             s.n_pe, s.n_ch = self.pressure["n_pe"], self.pressure["n_ch"]
@@ -545,6 +565,12 @@ class DeviceSolver:
             q.ipc_reach, q.ipc_gap = self.ipc["reach"], self.ipc["gap"]
             q.ipc_skin = self.SKIN * self.ipc["reach"]
             q.ipc_per_body = int(self.ipc["per_body"])
+        if self.node is not None:
+            nd = self.node
+            q.node_on, q.node_fric = 1, int(nd["fric"])
+            q.node_k, q.node_r, q.node_mu = nd["k"], nd["r"], nd["mu"]
+            q.ipc_skin = self.SKIN * nd["r"]
+            q.ipc_per_body = int(nd["per_body"])
         q.rigid_modes_on = int(self.rigid is not None)
         q.precision = 1 if self.precision == "mixed" else 0
         q.block_size = int(self.block_size)

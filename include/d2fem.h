@@ -161,6 +161,14 @@ typedef struct d2f_params {
   int32_t iter_max, conv_check, friction, line_search, n_backtrack;
   int32_t ipc_on, ipc_per_body, rigid_modes_on, precision; /* precision: 0 float64, 1 mixed */
   int32_t block_size;                        /* scene path: threads per workgroup (256/512/1024) */
+  /* "node" contact between bodies (reference implicit/kernels.py:755-869,
+   * implicit/body_contact.py): penalty node_k (node_r - d) between boundary nodes of one scene
+   * closer than node_r, not excluded against each other; node_fric adds Coulomb friction on the
+   * same pairs with coefficient node_mu (per body through scene mu_node when ipc_per_body). Uses
+   * the scene's excl / excl_off; exclusive with ipc_on. ipc_skin is the scene path's grid
+   * margin for either contact. */
+  int32_t node_on, node_fric;
+  double node_k, node_r, node_mu;
 } d2f_params;
 
 /* ---- library ---------------------------------------------------------------------------- */
@@ -247,6 +255,20 @@ int d2f_collider_grad_diag_f64(const double *x, const int32_t *bnd, int32_t n_b,
                                const d2f_collider *cols, int32_t n_col, int32_t which, double dhat,
                                double kappa, double h2, int32_t clamp, double *g, double *dg,
                                void *stream);
+/* Penalty part of the "node" contact on its own, one pass over every scene's boundary list
+ * (bnd_start: B + 1 offsets into bnd; excl / excl_off: the scenes' n_be x n_be exclusion bytes).
+ * grad += h2 g, diag += h2 diag (each clamped at 0 when clamp) of every boundary node's own
+ * visits. Replaces grad_diag_self_bp_kernel (implicit/kernels.py:772-808) as the adjoint's
+ * preconditioner calls it (implicit/adjoint.py:319-336). */
+int d2f_node_grad_diag_f64(const double *x, const int32_t *bnd, const int32_t *bnd_start, int32_t B,
+                           const uint8_t *excl, const int64_t *excl_off, double radius,
+                           double stiffness, double h2, int32_t clamp, double *g, double *dg,
+                           void *stream);
+/* y += h2 H_node v at x, own rows, unclamped (the contact set is the pairs closer than radius at
+ * x). Replaces self_bp_hvp_kernel (implicit/adjoint.py:115-143). */
+int d2f_node_hvp_f64(const double *x, const double *v, const int32_t *bnd, const int32_t *bnd_start,
+                     int32_t B, const uint8_t *excl, const int64_t *excl_off, double radius,
+                     double stiffness, double h2, double *y, void *stream);
 /* out[t] += -(1 - Emin) h2 lam . f_t(x) with f_t the element's full-density elastic force (mu0,
  * lam0). Replaces drho_kernel (implicit/adjoint.py:165-178). */
 int d2f_elastic_drho_f64(const double *x, const double *lam, const int32_t *tri, const double *Bm,

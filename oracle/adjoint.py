@@ -1,8 +1,8 @@
 """The oracle's adjoint of the implicit step — TEST INFRASTRUCTURE ONLY.
 
 A float64 numpy restatement of the reference's AdjointPNCG2D (implicit/adjoint.py:181-399)
-over an OracleSolver: the operator H = M + h^2 (K_elastic + K_colliders) restricted to the
-free degrees of freedom, its Jacobi preconditioner, scipy's MINRES, dJ/drho per element and
+over an OracleSolver: the operator H = M + h^2 (K_elastic + K_colliders [+ K_node with
+use_self]) restricted to the free degrees of freedom, its Jacobi preconditioner, scipy's MINRES, dJ/drho per element and
 the damped backward-Euler recursion over a trajectory. The product computes the same on the
 GPU (diff_2d_fem_amd.adjoint); this is its checker.
 """
@@ -40,33 +40,38 @@ class OracleAdjoint:
         return [c for c in self.s.colliders if (use_disc and c["kind"] == 0) or (use_sdf and c["kind"] != 0)]
 
     # reference implicit/adjoint.py:231-256
-    def apply_H(self, x, v, use_disc=False, use_sdf=False, include_mass=True):
+    def apply_H(self, x, v, use_disc=False, use_sdf=False, include_mass=True, use_self=False):
         y = np.zeros((self.N, 2))
         K.elastic_hvp(x, v, self.tri, self.B, self.area, self.s.mu, self.s.lam, self.h2, y)
         for c in self._cols(use_disc, use_sdf):
             K.collider_hvp(c, x, v, self.s.bnd, self.s.dhat, self.s.kappa, self.h2, y)
+        if use_self and self.s.node is not None:
+            self.s.node.hvp(x, v, self.h2, y)
         return self.mass[:, None] * v + y if include_mass else y
 
     # reference implicit/adjoint.py:356-372
-    def diag_precond(self, x, use_disc=False, use_sdf=False):
+    def diag_precond(self, x, use_disc=False, use_sdf=False, use_self=False):
         g = np.zeros((self.N, 2))
         d = np.repeat(self.mass[:, None], 2, axis=1).copy()
         K.elastic_grad_diag(x, self.tri, self.B, self.area, self.s.mu, self.s.lam, self.h2, 1, g, d)
         for c in self._cols(use_disc, use_sdf):
             K.collider_grad_diag(c, x, self.s.bnd, self.s.dhat, self.s.kappa, self.h2, 1, g, d)
+        if use_self and self.s.node is not None:
+            self.s.node.grad_diag(x, self.h2, 1, g, d)
         return np.maximum(d.reshape(-1), 1e-30)
 
     # reference implicit/adjoint.py:375-404
-    def adjoint_solve(self, x, rhs, use_disc=False, use_sdf=False, rtol=1e-12, maxiter=20000):
+    def adjoint_solve(self, x, rhs, use_disc=False, use_sdf=False, rtol=1e-12, maxiter=20000,
+                      use_self=False):
         from scipy.sparse.linalg import LinearOperator, minres
         ff = self.free.reshape(-1) > 0.5
         nf = int(ff.sum())
-        dinv = 1.0 / self.diag_precond(x, use_disc, use_sdf)[ff]
+        dinv = 1.0 / self.diag_precond(x, use_disc, use_sdf, use_self)[ff]
 
         def mv(vf):
             full = np.zeros((self.N, 2))
             full.reshape(-1)[ff] = vf
-            return (self.apply_H(x, full, use_disc, use_sdf) * self.free).reshape(-1)[ff]
+            return (self.apply_H(x, full, use_disc, use_sdf, use_self=use_self) * self.free).reshape(-1)[ff]
         A = LinearOperator((nf, nf), matvec=mv)
         M = LinearOperator((nf, nf), matvec=lambda v: dinv * v)
         lf, info = minres(A, np.asarray(rhs).reshape(-1)[ff], M=M, rtol=rtol, maxiter=maxiter)
@@ -85,7 +90,7 @@ class OracleAdjoint:
         return -(1.0 - self.Emin) * self.h2 * s
 
     # reference implicit/adjoint.py:373-399
-    def transient_adjoint(self, xs, seed, drag, use_disc=False, use_sdf=False, **kw):
+    def transient_adjoint(self, xs, seed, drag, use_disc=False, use_sdf=False, use_self=False, **kw):
         T = len(xs) - 1
         m2 = self.mass[:, None]
         lam = {T + 1: np.zeros((self.N, 2)), T + 2: np.zeros((self.N, 2))}
@@ -94,6 +99,6 @@ class OracleAdjoint:
             rhs = (1.0 + drag) * m2 * lam[k + 1] - drag * m2 * lam[k + 2]
             if k in seed:
                 rhs = rhs + seed[k]
-            lam[k] = self.adjoint_solve(xs[k], rhs, use_disc, use_sdf, **kw)
+            lam[k] = self.adjoint_solve(xs[k], rhs, use_disc, use_sdf, use_self=use_self, **kw)
             g += self.drho(xs[k], lam[k])
         return g

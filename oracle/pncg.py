@@ -16,6 +16,7 @@ import numpy as np
 from . import terms as K
 from .bodies import AffineBodies, ExternalForce, Pressure, RigidModes, Tendon
 from .ipc import IPCContact
+from .node import BodyFriction, NodeContact
 
 
 class OracleSolver:
@@ -34,6 +35,7 @@ class OracleSolver:
         self.line_search, self.n_bt, self.ls_tol = bool(line_search), int(n_backtrack), float(ls_tol)
         self.terms, self.modes = [], []
         self.ipc = None
+        self.node = None                 # NodeContact when body_contact is "node"
         self.affine = None
         self.cg_guard = 0.0
         self.B = int(me.n_envs)
@@ -94,6 +96,8 @@ class OracleSolver:
         for col in self.colliders:
             K.collider_grad_diag(col, self.x, self.bnd, self.dhat, self.kappa, self.h2, 1,
                                  self.grad, self.diagH)
+        if self.node is not None:
+            self.node.grad_diag(self.x, self.h2, 1, self.grad, self.diagH)
         if self.friction:
             for col in self.colliders:
                 K.friction_grad_diag(col, self.x, self.xn, self.bnd, self.dhat, self.kappa, col["mu"],
@@ -164,6 +168,8 @@ class OracleSolver:
             for col in self.colliders:
                 K.friction_pHp(col, self.x, self.xn, self.p, self.bnd, self.b2e, self.dhat, self.kappa,
                                col["mu"], self.epsv, self.inv_dt, self.h2, self.free, self.pHp)
+        if self.node is not None:
+            self.node.pHp(self.x, self.p, self.h2, 1, self.pHp)
         for term in self.terms:
             term.pHp(self)
 
@@ -214,6 +220,8 @@ class OracleSolver:
                          self.tri2env, buf)
         for col in self.colliders:
             K.collider_energy(col, self.x, self.bnd, self.b2e, self.dhat, self.kappa, self.h2, buf)
+        if self.node is not None:
+            self.node.energy(self.x, self.h2, buf)
         if self.friction:
             for col in self.colliders:
                 K.friction_energy(col, self.x, self.xn, self.bnd, self.b2e, self.dhat, self.kappa,
@@ -285,6 +293,7 @@ class OracleIntegrator:
                  near_factor=4.0, alpha_max=3.0, area_shrink=0.1, ccd_eta=0.9, self_radius=None,
                  tol_x=0.0, conv_check=8, line_search=False, n_backtrack=4, ls_tol=1e-7,
                  rigid_modes=True, iters_per_cfl=2.0, max_iter_cap=2000, cg_restart=None):
+        self.self_radius = self_radius
         self.max_iter = None if max_iter is None else int(max_iter)
         self.iters_per_cfl, self.max_iter_cap = float(iters_per_cfl), int(max_iter_cap)
         self.eps = float(1e-10 if eps is None else eps)
@@ -339,7 +348,10 @@ class OracleIntegrator:
         if model.body_contact not in ("ipc", "node"):
             raise ValueError(f"model.body_contact must be 'ipc' or 'node', not {model.body_contact!r}")
         if self.self_collision and model.body_contact == "node":
-            raise NotImplementedError("the oracle restates the 'ipc' body contact only")
+            r = self.self_radius if self.self_radius is not None else 1.5 * model.dx
+            s.node = NodeContact(s, model.cfg.contact_stiffness, r)
+            if mu_bb > 0.0 or (mu_node is not None and np.max(mu_node) > 0.0):
+                s.terms.append(BodyFriction(s, s.node, mu_bb, mu_node))
         if self.self_collision and model.body_contact == "ipc":
             s.ipc = IPCContact(s, mu_bb, dhat=model.dhat, kappa=model.body_kappa,
                                reach=max(2.0 * model.dx, 2.0 * model.dhat), mu_node=mu_node)

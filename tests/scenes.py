@@ -93,3 +93,59 @@ def press_targets(x0, top, t, T):
     x[top, 1] = x0[top, 1] - 0.003 * t / T
     x[top, 0] = x0[top, 0] + 0.002 * t / T
     return x
+
+
+def node_stack(n_env=2, px=0.5e-3, mu=0.0, block_mu=None, gap=1.55, tilt=2.0, rigid=False):
+    """A block resting on a held slab with the "node" contact between bodies (a small version of
+    the sliding and slope set-ups of reference tools/measure_contact.py:23-49): the block's lower
+    face starts `gap` px above the slab's upper face, just outside the contact radius of 1.5 px,
+    falls into contact within a few steps and settles on the penalty, and gravity is tilted by
+    `tilt` m/s^2 along the slab so that the pairs slide. Scene i shifts the block by i/4 px so
+    that the scenes differ. mu: model.body_friction; block_mu: the block's
+    own coefficient (per-body friction); rigid: the block is an affine body."""
+    m = Model()
+    m.body_contact = "node"
+    m.body_friction = mu
+    slabs = []
+    for i in range(n_env):
+        slabs.append(m.add_soft_body(np.ones((4, 20), np.uint8), physical_size=(20 * px, 4 * px),
+                                     center=(0.0, -2.0 * px), E=1.0e6, nu=0.40, density=1100.0, env_id=i))
+        add = m.add_affine_body if rigid else m.add_soft_body
+        kw = dict(physical_size=(6 * px, 6 * px), center=(0.25 * i * px, (3.0 + gap) * px),
+                  density=1100.0, env_id=i, mu=block_mu)
+        if not rigid:
+            kw.update(E=0.5e6, nu=0.40)
+        add(np.ones((6, 6), np.uint8), **kw)
+    m.finalize()
+    m.gravity = (tilt, -9.81)
+    for h in slabs:
+        m.set_kinematic_constraint(h, np.arange(len(m.rest_x(h))))
+    return m
+
+
+def node_press(px=0.5e-3, gap=1.4):
+    """A 6 x 10 px soft block standing on a held slab through the "node" contact, its lower face
+    `gap` px above the slab's (inside the 1.5 px radius, so the same pairs act throughout), top
+    row held and driven: pressed by 0.04 px and sheared by 0.1 px over the steps (see
+    node_press_targets). The adjoint's scene for contact between bodies. Returns (model, block
+    handle, the block's rest positions, its top nodes)."""
+    m = Model()
+    m.body_contact = "node"
+    m.body_friction = 0.0
+    slab = m.add_soft_body(np.ones((4, 20), np.uint8), physical_size=(20 * px, 4 * px),
+                           center=(0.0, -2.0 * px), E=1.0e6, nu=0.40, density=1100.0)
+    h = m.add_soft_body(np.ones((10, 6), np.uint8), physical_size=(6 * px, 10 * px),
+                        center=(0.0, (5.0 + gap) * px), E=1.0e6, nu=0.40, density=1100.0)
+    m.finalize()
+    m.set_kinematic_constraint(slab, np.arange(len(m.rest_x(slab))))
+    x0 = m.get_body_x(h)
+    top = np.where(x0[:, 1] > x0[:, 1].max() - 1e-9)[0]
+    m.set_kinematic_constraint(h, top)
+    return m, h, x0, top
+
+
+def node_press_targets(x0, top, t, T, px=0.5e-3):
+    x = x0.copy()
+    x[top, 1] = x0[top, 1] - 0.04 * px * t / T
+    x[top, 0] = x0[top, 0] + 0.1 * px * t / T
+    return x
