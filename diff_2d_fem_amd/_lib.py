@@ -56,7 +56,14 @@ _SCENE_FIELDS = (
     + [("excl_bits", P), ("excl_bits_off", P), ("excl_bits_lds", C.c_int32), ("max_pairs", C.c_int32)]
     + [("max_nn", C.c_int32), ("nacc_lds", C.c_int32), ("xc_lds", C.c_int32), ("pad3_", C.c_int32)]
     + [("el", ElemTiles)]
+    + [("n_sp", C.c_int32), ("pad4_", C.c_int32)]
+    + [(k, P) for k in ("sp_a", "sp_b", "sp2env", "sp_anchor", "sp_k", "sp_L0", "sp_start")]
 )
+
+
+class ElemConsts(C.Structure):
+    _fields_ = [(k, P) for k in ("Bm", "area", "mu", "lam", "sc_pos", "sc_Bm", "sc_Amu", "sc_Alam",
+                                 "sc_Bm32", "sc_Amu32", "sc_Alam32")]
 
 
 class Scene(C.Structure):
@@ -108,6 +115,7 @@ def lib():
     L.d2f_contact_force.argtypes = [C.POINTER(Scene), C.POINTER(Params), i32, i32, i32, P, P, P, P]
     L.d2f_collider_hvp_f64.argtypes = [P, P, P, i32, P, i32, i32, f64, f64, f64, P, P]
     L.d2f_collider_grad_diag_f64.argtypes = [P, P, i32, P, i32, i32, f64, f64, f64, i32, P, P, P]
+    L.d2f_apply_activation_f64.argtypes = [P, P, i32, P, P, P, P, C.POINTER(ElemConsts), P]
     L.d2f_node_grad_diag_f64.argtypes = [P, P, P, i32, P, P, f64, f64, f64, i32, P, P, P]
     L.d2f_node_hvp_f64.argtypes = [P, P, P, P, i32, P, P, f64, f64, f64, P, P]
     L.d2f_elastic_drho_f64.argtypes = [P, P, P, P, P, P, P, i32, f64, f64, P, P]

@@ -11,9 +11,9 @@ trajectory x_0 .. x_T is differentiated by the damped backward-Euler recursion
 with H_k = M + h^2 (K_elastic + K_contact) at x_k, restricted to the free degrees of freedom,
 the contact active set frozen at x_k. The operator holds inertia, elasticity and the
 colliders' barriers, and with use_self the "node" contact's penalty between bodies. Friction,
-IPC contact between bodies, pressure, tendons and rigid regions have no tangent in it
-(transient_adjoint refuses a model with IPC contact, friction between bodies, pressure, tendons
-or rigid regions, as the reference does).
+IPC contact between bodies, pressure, tendons, springs, active strain and rigid regions have no
+tangent in it (transient_adjoint refuses a model with any of them except collider friction, as
+the reference does).
 
 Everything runs on the GPU: the Hessian-vector products and dJ/drho are HIP kernels
 (d2f_elastic_hvp_f64, d2f_collider_hvp_f64, d2f_node_hvp_f64, d2f_elastic_drho_f64), and the preconditioned
@@ -315,11 +315,16 @@ class AdjointPNCG2D:
             missing.append("AffineBodies")
         if s.node is not None and s.node["fric"]:
             missing.append("BodyFriction")
+        if s.springs is not None:
+            missing.append("Springs")
+        if s.activation is not None:
+            missing.append("ActiveStrain")
         if missing:
             raise NotImplementedError(
                 f"the adjoint operator has no tangent for {', '.join(sorted(missing))}: the gradient "
                 f"would be of a different system. Differentiate without these actuators."
-                + (" For contact between bodies, use ImplicitIntegrator(self_collision=False)."
+                + (" For contact between bodies, use ImplicitIntegrator(self_collision=False) or "
+                   "model.body_contact = 'node', which the adjoint does have a tangent for."
                    if "IPCContact" in missing else ""))
 
     def transient_adjoint(self, xs, seed, drag=None, use_disc=False, use_self=False, use_sdf=False,

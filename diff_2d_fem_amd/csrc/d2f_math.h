@@ -563,4 +563,32 @@ __device__ inline void invert33(const double *H, double *Hi) {
   Hi[7] = -(a * h - b * g) * id;
   Hi[8] = (a * e - b * d) * id;
 }
+// ------------------------------------------------------------------ springs and pins
+// E = k/2 (|d| - L0)^2, d = x_a - x_b (reference implicit/actuation.py:368-392): the tension
+// k (|d| - L0), the tangential stiffness k (1 - L0/|d|) clamped at 0, the length and the unit
+// direction of d.
+struct SpringGeo {
+  V2<double> n;
+  double tension, tang, len;
+};
+__device__ __forceinline__ SpringGeo spring_geo(V2<double> xa, V2<double> xb, double k, double L0) {
+  const V2<double> d = xa - xb;
+  SpringGeo g;
+  g.len = sqrt(dot(d, d) + 1.0e-30);
+  g.tension = k * (g.len - L0);
+  g.tang = fmax(k * (1.0 - L0 / g.len), 0.0);
+  g.n = {d.x / g.len, d.y / g.len};
+  return g;
+}
+// Diagonal of the spring's Hessian at either end, h2 (k n n^T + tang (I - n n^T)).
+__device__ __forceinline__ V2<double> spring_diag(const SpringGeo &g, double k, double h2) {
+  return {h2 * (k * g.n.x * g.n.x + g.tang * (1.0 - g.n.x * g.n.x)),
+          h2 * (k * g.n.y * g.n.y + g.tang * (1.0 - g.n.y * g.n.y))};
+}
+// p^T H p of the spring along the relative direction dp = p_a - p_b (p_a for a world spring).
+__device__ __forceinline__ double spring_pHp(const SpringGeo &g, double k, double h2, V2<double> dp) {
+  const double along = dot(dp, g.n);
+  return h2 * (k * along * along + g.tang * (dot(dp, dp) - along * along));
+}
+
 }  // namespace d2f

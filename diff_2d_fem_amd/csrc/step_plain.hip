@@ -606,6 +606,67 @@ __global__ void k_pressure(S s, int mode, double h2, double *E) {
   }
 }
 
+// reference implicit/actuation.py:395-446: mode 0 gradient and diagonal at both ends, 1 p^T H p
+// per environment, 2 energy per environment
+__global__ void k_springs(S s, int mode, double h2, double *E) {
+  GRID_LOOP(j, s.n_sp) {
+    if (j < s.n_sp) {
+      const int a = s.sp_a[j], b = s.sp_b[j];
+      const double k = s.sp_k[j];
+      const V2<double> xb = b >= 0 ? ld2(s.x, b) : ld2(s.sp_anchor, j);
+      const SpringGeo q = spring_geo(ld2(s.x, a), xb, k, s.sp_L0[j]);
+      if (mode == 0) {
+        const V2<double> g = (h2 * q.tension) * q.n;
+        const V2<double> hd = spring_diag(q, k, h2);
+        atomic_add2(s.grad, a, g);
+        atomic_add2(s.diag, a, hd);
+        if (b >= 0) {
+          atomic_add2(s.grad, b, -1.0 * g);
+          atomic_add2(s.diag, b, hd);
+        }
+      } else if (mode == 1) {
+        V2<double> dp = ld2(s.p, a);
+        if (b >= 0) dp = dp - ld2(s.p, b);
+        atomicAdd(s.pHp + s.sp2env[j], spring_pHp(q, k, h2, dp));
+      } else {
+        const double ext = q.len - s.sp_L0[j];
+        atomicAdd(E + s.sp2env[j], 0.5 * h2 * k * ext * ext);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------- active strain
+// Bm = Bm0 A^-1, area = area0 det A per element of each region, in both layouts (reference
+// implicit/actuation.py:328-337)
+__global__ void k_apply_activation(const int32_t *elem, const int32_t *ereg, int n_el, const double *Bm0,
+                                   const double *area0, const double *Ainv, const double *detA,
+                                   d2f_elem_consts o) {
+  GRID_LOOP(j, n_el) {
+    if (j < n_el) {
+      const int t = elem[j], r = ereg[j];
+      const double *B0 = Bm0 + 4 * t, *Ai = Ainv + 4 * r;
+      const double B[4] = {B0[0] * Ai[0] + B0[1] * Ai[2], B0[0] * Ai[1] + B0[1] * Ai[3],
+                           B0[2] * Ai[0] + B0[3] * Ai[2], B0[2] * Ai[1] + B0[3] * Ai[3]};
+      const double A = area0[t] * detA[r];
+      for (int c = 0; c < 4; ++c) o.Bm[4 * t + c] = B[c];
+      o.area[t] = A;
+      if (o.sc_pos) {
+        const int q = o.sc_pos[t];
+        for (int c = 0; c < 4; ++c) {
+          o.sc_Bm[4 * q + c] = B[c];
+          o.sc_Bm32[4 * q + c] = float(B[c]);
+        }
+        const double Amu = A * o.mu[t], Alam = A * o.lam[t];
+        o.sc_Amu[q] = Amu;
+        o.sc_Alam[q] = Alam;
+        o.sc_Amu32[q] = float(Amu);
+        o.sc_Alam32[q] = float(Alam);
+      }
+    }
+  }
+}
+
 __device__ __forceinline__ V2<double> seg_dir(V2<double> a, V2<double> b, double &il) {
   const V2<double> t = b - a;
   const double l = fmax(len(t), 1.0e-12);
@@ -1330,6 +1391,7 @@ struct Plain {
       int r = ipc_pass(0, s.p, s.grad, s.diag, s.free_, s.Ener, s.clr_min, s.alpha, p.h2, p.ipc_mu);
       if (r) return r;
     }
+    if (s.n_sp > 0) LAUNCH(k_springs, s.n_sp, s, 0, p.h2, s.Ener);
     if (s.n_pe > 0) LAUNCH(k_pressure, s.n_pe, s, 0, p.h2, s.Ener);
     if (s.n_td > 0) {
       D2F_HIP(hipMemsetAsync(s.td_L, 0, sizeof(double) * s.n_td, st));
@@ -1388,6 +1450,7 @@ struct Plain {
       if (r) return r;
       LAUNCH(k_far_cap, s.B, s.B, s.p_inf, p.ipc_reach, s.alpha);
     }
+    if (s.n_sp > 0) LAUNCH(k_springs, s.n_sp, s, 1, p.h2, s.Ener);
     if (s.n_pe > 0) LAUNCH(k_pressure, s.n_pe, s, 1, p.h2, s.Ener);
     if (s.n_td > 0) {
       D2F_HIP(hipMemsetAsync(s.td_dL, 0, sizeof(double) * s.n_td, st));
@@ -1414,6 +1477,7 @@ struct Plain {
       int r = ipc_pass(2, s.p, s.grad, s.diag, s.free_, E, s.clr_ipc, s.alpha, p.h2, p.ipc_mu);
       if (r) return r;
     }
+    if (s.n_sp > 0) LAUNCH(k_springs, s.n_sp, s, 2, p.h2, E);
     if (s.n_pe > 0) LAUNCH(k_pressure, s.n_pe, s, 2, p.h2, E);
     if (s.n_td > 0) {
       D2F_HIP(hipMemsetAsync(s.td_L, 0, sizeof(double) * s.n_td, st));
@@ -1622,6 +1686,22 @@ int d2f_step_plain(const d2f_scene *s, const d2f_params *p, void *stream) {
   D2F_CHECK(p->iter_max >= 0 && p->dt > 0.0, "d2f_step_plain: bad params");
   Plain run(*s, *p, (hipStream_t)stream);
   return run.run();
+}
+
+int d2f_apply_activation_f64(const int32_t *elem, const int32_t *ereg, int32_t n_el, const double *Bm0,
+                             const double *area0, const double *Ainv, const double *detA,
+                             const d2f_elem_consts *out, void *stream) {
+  D2F_CHECK(n_el >= 0 && out, "d2f_apply_activation_f64: bad argument");
+  if (n_el == 0) return 0;
+  D2F_CHECK(elem && ereg && Bm0 && area0 && Ainv && detA && out->Bm && out->area,
+            "d2f_apply_activation_f64: null pointer");
+  D2F_CHECK(!out->sc_pos || (out->mu && out->lam && out->sc_Bm && out->sc_Amu && out->sc_Alam && out->sc_Bm32 &&
+                             out->sc_Amu32 && out->sc_Alam32),
+            "d2f_apply_activation_f64: scene layout incomplete");
+  hipStream_t st = (hipStream_t)stream;
+  LAUNCH(k_apply_activation, n_el, elem, ereg, n_el, Bm0, area0, Ainv, detA, *out);
+  D2F_HIP(hipGetLastError());
+  return 0;
 }
 
 int d2f_collider_force(const d2f_scene *s, const d2f_params *p, int32_t index, int32_t friction, double *F,

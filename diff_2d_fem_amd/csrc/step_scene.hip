@@ -255,7 +255,7 @@ struct Step {
   LdsU *xbits;    // exclusion bitset (excl_bits_lds words)
   LdsI *plist;    // max_pairs packed (edge - k0) << 16 | jp
   int e, tid;
-  int n0, n1, b0, nb, k0, k1, t0, t1, c0, bc0, bc1, g0, g1, r0, r1, pe0, pe1, td0, td1;
+  int n0, n1, b0, nb, k0, k1, t0, t1, c0, bc0, bc1, g0, g1, r0, r1, pe0, pe1, td0, td1, sp0, sp1;
   int64_t eoff;
 
   __device__ __forceinline__ Step(S &s_, KPm &p_, LdsScalars &sc_, LdsD *lds)
@@ -292,6 +292,8 @@ struct Step {
     pe1 = s.n_pe > 0 ? s.pe_start[e + 1] : 0;
     td0 = s.n_td > 0 ? s.td_start[e] : 0;
     td1 = s.n_td > 0 ? s.td_start[e + 1] : 0;
+    sp0 = s.n_sp > 0 ? s.sp_start[e] : 0;
+    sp1 = s.n_sp > 0 ? s.sp_start[e + 1] : 0;
     eoff = s.excl_off ? s.excl_off[e] : 0;
     xcon = s.xc_lds && n1 - n0 <= s.max_nn;
   }
@@ -962,6 +964,22 @@ struct Step {
         }
       }
     }
+    // springs and pins: one thread, in order (reference implicit/actuation.py:395-413)
+    if (sp1 > sp0 && tid == 0) {
+      for (int j = sp0; j < sp1; ++j) {
+        const int a = s.sp_a[j], b = s.sp_b[j];
+        const double k = s.sp_k[j];
+        const SpringGeo q = spring_geo(X(a), b >= 0 ? X(b) : ld2(s.sp_anchor, j), k, s.sp_L0[j]);
+        const V2<double> g = (h2 * q.tension) * q.n;
+        const V2<double> hd = spring_diag(q, k, h2);
+        st2(s.grad, a, ld2(s.grad, a) + g);
+        st2(s.diag, a, ld2(s.diag, a) + hd);
+        if (b >= 0) {
+          st2(s.grad, b, ld2(s.grad, b) - g);
+          st2(s.diag, b, ld2(s.diag, b) + hd);
+        }
+      }
+    }
     __syncthreads();
     // affine bodies: J^T g, J^T D J per group, then the slaved nodes leave the nodal iteration
     // (reference implicit/affine.py:157-175)
@@ -1288,6 +1306,13 @@ struct Step {
         }
         pH += h2 * s.td_keff[k] * dL * dL;
       }
+      for (int j = sp0; j < sp1; ++j) {
+        const int a = s.sp_a[j], b = s.sp_b[j];
+        const SpringGeo q = spring_geo(X(a), b >= 0 ? X(b) : ld2(s.sp_anchor, j), s.sp_k[j], s.sp_L0[j]);
+        V2<double> dp = ld2(s.p, a);
+        if (b >= 0) dp = dp - ld2(s.p, b);
+        pH += spring_pHp(q, s.sp_k[j], h2, dp);
+      }
       double a = fmin(p.alpha_max, vm[1]);
       if (p.ipc_on && p_inf > 0.0) a = fmin(a, 0.45 * p.ipc_reach / p_inf);
       sc.pHp = pH;
@@ -1394,6 +1419,12 @@ struct Step {
         } else {
           E += h2 * fmax(s.td_T[k], 0.0) * L;
         }
+      }
+      for (int j = sp0; j < sp1; ++j) {
+        const int b = s.sp_b[j];
+        const SpringGeo q = spring_geo(X(s.sp_a[j]), b >= 0 ? X(b) : ld2(s.sp_anchor, j), s.sp_k[j], s.sp_L0[j]);
+        const double ext = q.len - s.sp_L0[j];
+        E += 0.5 * h2 * s.sp_k[j] * ext * ext;
       }
       sc.Ener = E;
     }

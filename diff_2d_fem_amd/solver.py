@@ -101,6 +101,8 @@ class DeviceSolver:
         self.fext = None
         self.pressure = None
         self.tendon = None
+        self.springs = None
+        self.activation = None
         self._empty_i = torch.zeros(1, dtype=I32, device=self.dev)
         self._empty_f = torch.zeros(8, dtype=F64, device=self.dev)
 
@@ -339,9 +341,60 @@ class DeviceSolver:
         self.td_T.copy_(torch.as_tensor(np.asarray(tension, np.float64)[self.td_order]))
         self.td_L0.copy_(torch.as_tensor(np.asarray(length, np.float64)[self.td_order]))
 
+    def setup_springs(self, a, b, k, s2env):
+        """Springs and pins (b = -1: to a world point), grouped by environment, the model's
+        order kept within one (reference implicit/actuation.py:448-476)."""
+        t = self._t
+        s2env = np.asarray(s2env, np.int64)
+        o = np.argsort(s2env, kind="stable")
+        self.sp_order = o
+        self.springs = dict(n=len(o))
+        self.sp_a = t(np.asarray(a, np.int64)[o], I32)
+        self.sp_b = t(np.asarray(b, np.int64)[o], I32)
+        self.sp_k = t(np.asarray(k, np.float64)[o], F64)
+        self.sp2env = t(s2env[o], I32)
+        self.sp_start = t(np.searchsorted(s2env[o], np.arange(self.B + 1)), I32)
+        self.sp_L0 = torch.zeros(len(o), dtype=F64, device=self.dev)
+        self.sp_anchor = torch.zeros((len(o), 2), dtype=F64, device=self.dev)
+
+    def set_springs(self, anchor, length):
+        o = self.sp_order
+        self.sp_anchor.copy_(torch.as_tensor(np.asarray(anchor, np.float64).reshape(-1, 2)[o]))
+        self.sp_L0.copy_(torch.as_tensor(np.asarray(length, np.float64)[o]))
+
+    def setup_activation(self, elem, ereg, n_regions):
+        """Active regions: their elements' rest shape follows an activation A per region
+        (reference implicit/actuation.py:340-366). The undeformed rest shape is kept on the device
+        the first time."""
+        if getattr(self, "Bm0", None) is None:
+            self.Bm0, self.area0 = self.Bm.clone(), self.area.clone()
+        self.act_elem = self._t(np.asarray(elem, np.int64), I32)
+        self.act_ereg = self._t(np.asarray(ereg, np.int64), I32)
+        self.activation = dict(n_el=len(elem), n_reg=int(n_regions))
+
+    def set_activation(self, A):
+        """A: (n_regions, 2, 2) activation of each region's rest shape: Bm = Bm0 A^-1 and
+        area = area0 det A, in both element layouts."""
+        A = np.asarray(A, np.float64).reshape(-1, 2, 2)
+        Ainv = self._t(np.linalg.inv(A).reshape(-1, 4), F64)
+        detA = self._t(np.linalg.det(A), F64)
+        o = _lib.ElemConsts()
+        o.Bm, o.area, o.mu, o.lam = (self._ptr(v) for v in (self.Bm, self.area, self.mu, self.lam))
+        if self._layout is not None:
+            for k in ("sc_pos", "sc_Bm", "sc_Amu", "sc_Alam", "sc_Bm32", "sc_Amu32", "sc_Alam32"):
+                setattr(o, k, self._ptr(getattr(self, k)))
+        L = _lib.lib()
+        _lib.check(L.d2f_apply_activation_f64(self._ptr(self.act_elem), self._ptr(self.act_ereg),
+                                              self.activation["n_el"], self._ptr(self.Bm0),
+                                              self._ptr(self.area0), self._ptr(Ainv), self._ptr(detA),
+                                              C.byref(o), self._stream()), "d2f_apply_activation_f64")
+        self._act_keep = (Ainv, detA)       # alive until the launch has run
+
     def clear_actuators(self):
         self.pressure = None
         self.tendon = None
+        self.springs = None
+        self.activation = None
 
     # ------------------------------------------------------------------ scene-persistent layout
     def build_layout(self, max_cells=1024):
@@ -381,8 +434,11 @@ class DeviceSolver:
         self.sc_tri_start = t(np.searchsorted(ks, np.arange(self.B + 1) * M), I32)
         self.sc_cs = t(np.searchsorted(ks, (np.arange(self.B)[:, None] * M + np.arange(M)[None, :]).reshape(-1)), I32)
         self.sc_tri = t(tri[perm].astype(np.int32), I32)
-        Bm = np.asarray(me.D_m_inv, np.float64).reshape(-1, 4)[perm]
-        area = np.asarray(me.rest_area, np.float64)[perm]
+        Bm = self.Bm.cpu().numpy().reshape(-1, 4)[perm]       # current rest shape (activation)
+        area = self.area.cpu().numpy()[perm]
+        pos = np.empty(T, np.int64)
+        pos[perm] = np.arange(T)
+        self.sc_pos = t(pos, I32)
         Amu, Alam = area * self.mu_np[perm], area * self.lam_np[perm]
         self.sc_Bm, self.sc_Amu, self.sc_Alam = t(Bm, F64), t(Amu, F64), t(Alam, F64)
         self.sc_Bm32, self.sc_Amu32 = t(Bm.astype(np.float32), torch.float32), t(Amu.astype(np.float32), torch.float32)
@@ -463,7 +519,7 @@ class DeviceSolver:
         self.lam = self._t(self.lam_np, F64)
         if self._layout is not None:
             perm = self.sc_perm
-            area = np.asarray(self.me.rest_area, np.float64)[perm]
+            area = self.area.cpu().numpy()[perm]
             Amu, Alam = area * self.mu_np[perm], area * self.lam_np[perm]
             self.sc_Amu.copy_(torch.as_tensor(Amu))
             self.sc_Alam.copy_(torch.as_tensor(Alam))
@@ -543,6 +599,10 @@ class DeviceSolver:
                 s.excl_bits, s.excl_bits_off = p("excl_bits"), p("excl_bits_off")
             s.excl_bits_lds, s.max_pairs = self.excl_bits_lds, self.max_pairs
             s.max_nn, s.nacc_lds, s.xc_lds = self.max_nn, self.nacc_lds, self.xc_lds
+        if self.springs is not None:
+            s.n_sp = self.springs["n"]
+            for k in ("sp_a", "sp_b", "sp2env", "sp_anchor", "sp_k", "sp_L0", "sp_start"):
+                setattr(s, k, p(k))
         s.n_col = len(self.colliders)
         for i, c in enumerate(self.colliders):
             s.col[i].kind = int(c["kind"])

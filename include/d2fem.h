@@ -147,6 +147,13 @@ typedef struct d2f_scene {
   int32_t xc_lds, pad3_;
   /* per-kernel path: node tiles of the element kernel (node_off NULL: atomics only) */
   d2f_elem_tiles el;
+  /* springs and pins [n_sp], grouped by environment (reference implicit/actuation.py:368-476):
+   * E = k/2 (|x_a - x_b| - L0)^2 with x_b = sp_anchor when sp_b < 0 (a spring to the world) */
+  int32_t n_sp, pad4_;
+  const int32_t *sp_a, *sp_b, *sp2env;  /* n_sp */
+  const double *sp_anchor;              /* 2 n_sp */
+  const double *sp_k, *sp_L0;           /* n_sp */
+  const int32_t *sp_start;              /* B+1 springs per scene (scene path) */
 } d2f_scene;
 
 /* Solver options and step constants (reference PNCG2DSolver.__init__, implicit/solver.py:97-226,
@@ -239,6 +246,24 @@ int d2f_collider_force(const d2f_scene *s, const d2f_params *prm, int32_t index,
  * PNCG2DSolver.contact_force (implicit/solver.py:554-602). */
 int d2f_contact_force(const d2f_scene *s, const d2f_params *prm, int32_t colliders, int32_t bodies,
                       int32_t friction, const uint8_t *all_free, double *g, double *dg, void *stream);
+
+/* ---- active strain ------------------------------------------------------------------------ */
+/* Element constants an activation rewrites: the reference-order arrays and, when sc_pos is not
+ * NULL, the scene layout (element t sits at sc_pos[t]; constants premultiplied by the area, in
+ * float64 and float32). */
+typedef struct d2f_elem_consts {
+  double *Bm, *area;                   /* 4T, T */
+  const double *mu, *lam;              /* T, current moduli */
+  const int32_t *sc_pos;               /* T, or NULL */
+  double *sc_Bm, *sc_Amu, *sc_Alam;    /* 4T, T, T */
+  float *sc_Bm32, *sc_Amu32, *sc_Alam32;
+} d2f_elem_consts;
+/* Bm[t] = Bm0[t] Ainv[r], area[t] = area0[t] detA[r] for each element t = elem[j] of region
+ * r = ereg[j] (and the scene layout's copies). Replaces apply_activation_kernel
+ * (implicit/actuation.py:328-337). */
+int d2f_apply_activation_f64(const int32_t *elem, const int32_t *ereg, int32_t n_el, const double *Bm0,
+                             const double *area0, const double *Ainv, const double *detA,
+                             const d2f_elem_consts *out, void *stream);
 
 /* ---- adjoint (float64) ----------------------------------------------------------------------- */
 /* y += h2 H_contact v at x for the selected colliders (which: 0 discs, 1 the other shapes through

@@ -432,3 +432,90 @@ class Tendon:
         e = np.where(self.stiff > 0.0, np.where(ext > 0.0, 0.5 * s.h2 * self.stiff * ext * ext, 0.0),
                      s.h2 * np.maximum(self.Tcmd, 0.0) * L)
         np.add.at(buf, self.t2env, e)
+
+
+class Springs:
+    """Linear springs and pins, E = k/2 (|d| - L0)^2 with d = x_a - x_b, x_b a world anchor
+    when b = -1 (reference implicit/actuation.py:368-476). The tangential stiffness
+    k (1 - L0/|d|) is clamped at 0 in the diagonal and in p^T H p."""
+    in_adjoint = False
+
+    def __init__(self, s, node_a, node_b, stiffness, s2env):
+        self.a = np.asarray(node_a, np.int64)
+        self.b = np.asarray(node_b, np.int64)
+        self.k = np.asarray(stiffness, np.float64)
+        self.env = np.asarray(s2env, np.int64)
+        self.L0 = np.zeros(len(self.a))
+        self.anchor = np.zeros((len(self.a), 2))
+
+    def set(self, anchor, length):
+        self.anchor = np.asarray(anchor, np.float64).reshape(-1, 2).copy()
+        self.L0 = np.asarray(length, np.float64).copy()
+
+    def _geom(self, x):
+        """(n, tension, tangential stiffness, length) of every spring (reference :386-392)."""
+        world = self.b < 0
+        xb = np.where(world[:, None], self.anchor, x[np.maximum(self.b, 0)])
+        d = x[self.a] - xb
+        ln = np.sqrt((d * d).sum(1) + 1.0e-30)
+        return d / ln[:, None], self.k * (ln - self.L0), np.maximum(self.k * (1.0 - self.L0 / ln), 0.0), ln
+
+    def _rel(self, v):
+        """v_a - v_b per spring (v_a for a spring to the world)."""
+        out = v[self.a].copy()
+        both = self.b >= 0
+        out[both] -= v[self.b[both]]
+        return out
+
+    # reference implicit/actuation.py:395-413
+    def grad_diag(self, s):
+        n, ten, tang, _ = self._geom(s.x)
+        g = s.h2 * ten[:, None] * n
+        hd = s.h2 * (self.k[:, None] * n * n + tang[:, None] * (1.0 - n * n))
+        np.add.at(s.grad, self.a, g)
+        np.add.at(s.diagH, self.a, hd)
+        both = self.b >= 0
+        np.add.at(s.grad, self.b[both], -g[both])
+        np.add.at(s.diagH, self.b[both], hd[both])
+
+    # reference implicit/actuation.py:416-434
+    def pHp(self, s):
+        n, _, tang, _ = self._geom(s.x)
+        dp = self._rel(s.p)
+        along = (dp * n).sum(1)
+        np.add.at(s.pHp, self.env, s.h2 * (self.k * along * along + tang * ((dp * dp).sum(1) - along * along)))
+
+    # reference implicit/actuation.py:437-446
+    def energy(self, s, buf):
+        _, _, _, ln = self._geom(s.x)
+        ext = ln - self.L0
+        np.add.at(buf, self.env, 0.5 * s.h2 * self.k * ext * ext)
+
+
+class ActiveStrain:
+    """Active material: the rest shape of each region's elements becomes A Dm, so the elastic
+    term sees Bm = Bm0 A^-1 and area = area0 det A (reference implicit/actuation.py:319-366). No
+    energy term of its own."""
+    in_adjoint = False
+
+    def __init__(self, s, elem, ereg, n_regions):
+        self.elem = np.asarray(elem, np.int64)
+        self.ereg = np.asarray(ereg, np.int64)
+        self.n_reg = int(n_regions)
+        if getattr(s, "Bm0", None) is None:
+            s.Bm0, s.area0 = s.Bm.copy(), s.area.copy()
+
+    def set(self, s, A):
+        A = np.asarray(A, np.float64).reshape(-1, 2, 2)
+        Ainv, detA = np.linalg.inv(A), np.linalg.det(A)
+        s.Bm[self.elem] = s.Bm0[self.elem] @ Ainv[self.ereg]
+        s.area[self.elem] = s.area0[self.elem] * detA[self.ereg]
+
+    def grad_diag(self, s):
+        pass
+
+    def pHp(self, s):
+        pass
+
+    def energy(self, s, buf):
+        pass

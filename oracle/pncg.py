@@ -14,7 +14,7 @@ import math
 import numpy as np
 
 from . import terms as K
-from .bodies import AffineBodies, ExternalForce, Pressure, RigidModes, Tendon
+from .bodies import ActiveStrain, AffineBodies, ExternalForce, Pressure, RigidModes, Springs, Tendon
 from .ipc import IPCContact
 from .node import BodyFriction, NodeContact
 
@@ -375,9 +375,12 @@ class OracleIntegrator:
             keep = [t for t in s.terms if isinstance(t, ExternalForce) or getattr(t, "persistent", False)]
             s.terms = list(keep)
             self._force = next((t for t in keep if isinstance(t, ExternalForce)), None)
-            self._pressure = self._tendon = None
-            if model._springs or model._regions:
-                raise NotImplementedError("the oracle restates gravity, pressure and tendons")
+            self._pressure = self._tendon = self._spring = self._strain = None
+            if model._springs:
+                S = model._springs
+                self._spring = Springs(s, [k["a"] for k in S], [k["b"] for k in S],
+                                       [k["stiffness"] for k in S], [k["env"] for k in S])
+                s.terms.append(self._spring)
             if model._chambers:
                 C = model._chambers
                 self._pressure = Pressure(s, np.concatenate([c["ea"] for c in C]),
@@ -397,6 +400,11 @@ class OracleIntegrator:
                 self._tendon = Tendon(s, node, prv, nxt, tid, [model.body_env(t["body"]) for t in T],
                                       [t["stiffness"] for t in T], [t["length"] for t in T])
                 s.terms.append(self._tendon)
+            if model._regions:
+                R = model._regions
+                self._strain = ActiveStrain(s, np.concatenate([r["elems"] for r in R]),
+                                            np.concatenate([np.full(len(r["elems"]), i) for i, r in enumerate(R)]),
+                                            len(R))
             self._act_version = model._act_version
             self._ctrl_version = None
         if self._ctrl_version != model._ctrl_version:
@@ -404,6 +412,10 @@ class OracleIntegrator:
                 self._pressure.set([c["pressure"] for c in model._chambers])
             if self._tendon is not None:
                 self._tendon.set([t["tension"] for t in model._tendons], [t["length"] for t in model._tendons])
+            if self._strain is not None:
+                self._strain.set(s, np.stack([r["A"] for r in model._regions]))
+            if self._spring is not None:
+                self._spring.set([k["anchor"] for k in model._springs], [k["length"] for k in model._springs])
             g = model._gravity
             if model._fext is not None or g != (0.0, 0.0):
                 if self._force is None:

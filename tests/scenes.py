@@ -149,3 +149,57 @@ def node_press_targets(x0, top, t, T, px=0.5e-3):
     x[top, 1] = x0[top, 1] - 0.04 * px * t / T
     x[top, 0] = x0[top, 0] + 0.1 * px * t / T
     return x
+
+
+def spring_rig(n_env=2, px=0.5e-3):
+    """Springs and pins (reference implicit/actuation.py:368-476, sim_model.py:988-1060): per
+    scene a soft block pinned to the world at its top-left corner, a spring with a rest length
+    from its right face to a second block, and a world spring from the second block to an
+    anchor; gravity on. Scene i shifts everything by i px. Returns (model, spring handles,
+    rigging function(model, step) that drives the spring lengths and the anchor)."""
+    m = Model()
+    pairs = []
+    for i in range(n_env):
+        dx = i * px
+        a = m.add_soft_body(np.ones((6, 8), np.uint8), physical_size=(8 * px, 6 * px), center=(dx, 0.0),
+                            E=1.0e6, nu=0.40, density=1100.0, env_id=i)
+        b = m.add_soft_body(np.ones((6, 6), np.uint8), physical_size=(6 * px, 6 * px),
+                            center=(dx + 12 * px, 0.0), E=0.5e6, nu=0.40, density=1100.0, env_id=i)
+        pairs.append((a, b, dx))
+    m.finalize()
+    m.gravity = (0.0, -9.81)
+    handles = []
+    for a, b, dx in pairs:
+        pin = m.add_pin(a, (dx - 4 * px, 3 * px), stiffness=5.0e5)
+        sp = m.add_spring(a, (dx + 4 * px, 0.0), b, (dx + 9 * px, 0.0), stiffness=2.0e4)
+        ws = m.add_spring(b, (dx + 15 * px, 3 * px), None, (dx + 16 * px, 5 * px), stiffness=1.0e4,
+                          rest_length=1.5 * px)
+        handles.append((pin, sp, ws))
+    L0 = [m._springs[sp]["length"] for _, sp, _ in handles]
+
+    def rig(model, k):
+        for (pin, sp, ws), l0 in zip(handles, L0):
+            model.set_spring_length(sp, l0 * (1.0 - 0.03 * min(k, 8)))      # the strut pulls in
+            model.set_spring_anchor(ws, np.asarray(model._springs[ws]["anchor"]) + np.array([0.0, 0.05 * px]))
+    return m, handles, rig
+
+
+def bilayer(n_env=2, px=0.5e-3, fiber=None):
+    """A 6 x 24 px cantilever, left column held, whose lower half is active material (the
+    bimetal strip of reference tests/test_actuation.py:305-327). Scene i activates by
+    (1 - 0.01 (i + 1)); `fiber` None: isotropic, else the fibre angle. Returns (model, region
+    handles)."""
+    m = Model()
+    hs = []
+    for i in range(n_env):
+        hs.append(m.add_soft_body(np.ones((6, 24), np.uint8), physical_size=(24 * px, 6 * px),
+                                  center=(0.0, 0.0), E=1.0e6, nu=0.40, density=1100.0, env_id=i))
+    m.finalize()
+    regions = []
+    for i, h in enumerate(hs):
+        X = m.rest_x(h)
+        m.set_kinematic_constraint(h, np.where(X[:, 0] < X[:, 0].min() + 1e-9)[0])
+        lower = np.zeros((6, 24), bool)
+        lower[:3] = True
+        regions.append(m.add_active_region(h, lower, fiber=fiber))
+    return m, regions
