@@ -103,18 +103,21 @@ constexpr int ELEM_PER_THREAD = D2F_ELEM_CHUNK / ELEM_TPB;
 __global__ __launch_bounds__(ELEM_TPB) void k_elastic_grad_diag_tiled(
     const double *x, const double *Bm, const double *area, const double *mu, const double *lam, int T,
     double h2, int clamp, d2f_elem_tiles tl, double *grad, double *diag) {
-  extern __shared__ double sh[];  // nodes of the chunk: x, y, then grad x, grad y, diag x, diag y
+  // nodes of the chunk, one plane per component (max_nodes each): x, y, grad x, grad y, diag x,
+  // diag y; neighbouring lanes then touch neighbouring banks
+  extern __shared__ double sh[];
   // Workgroups are dealt to the 8 XCDs round-robin; give each XCD a contiguous run of chunks
   // so that the nodes neighbouring chunks share stay in one L2.
   const int nbk = gridDim.x, q8 = nbk / 8, r8 = nbk % 8, xcd = blockIdx.x % 8;
   const int c = xcd * q8 + min(xcd, r8) + blockIdx.x / 8;
   const int n0 = tl.node_off[c], nn = tl.node_off[c + 1] - n0;
-  double *xs = sh, *acc = sh + 2 * tl.max_nodes;
+  const int M = tl.max_nodes;
+  double *xs = sh, *ys = sh + M, *acc = sh + 2 * M;
   for (int j = threadIdx.x; j < nn; j += ELEM_TPB) {
     const V2<double> p = ld2(x, tl.nodes[n0 + j]);
-    xs[2 * j] = p.x;
-    xs[2 * j + 1] = p.y;
-    acc[4 * j] = acc[4 * j + 1] = acc[4 * j + 2] = acc[4 * j + 3] = 0.0;
+    xs[j] = p.x;
+    ys[j] = p.y;
+    acc[j] = acc[M + j] = acc[2 * M + j] = acc[3 * M + j] = 0.0;
   }
   __syncthreads();
   const int base = c * D2F_ELEM_CHUNK + threadIdx.x;
@@ -124,8 +127,7 @@ __global__ __launch_bounds__(ELEM_TPB) void k_elastic_grad_diag_tiled(
     if (t >= T) continue;
     const ushort4 l = reinterpret_cast<const ushort4 *>(tl.tri_local)[t];
     const int li[3] = {l.x, l.y, l.z};
-    const V2<double> p0 = {xs[2 * li[0]], xs[2 * li[0] + 1]}, p1 = {xs[2 * li[1]], xs[2 * li[1] + 1]},
-                     p2 = {xs[2 * li[2]], xs[2 * li[2] + 1]};
+    const V2<double> p0 = {xs[li[0]], ys[li[0]]}, p1 = {xs[li[1]], ys[li[1]]}, p2 = {xs[li[2]], ys[li[2]]};
     const M2<double> B = load_B<double>(Bm, t);
     const double m_ = mu[t], l_ = lam[t], A = area[t];
     const M2<double> Ds = {p1.x - p0.x, p2.x - p0.x, p1.y - p0.y, p2.y - p0.y};
@@ -138,18 +140,18 @@ __global__ __launch_bounds__(ELEM_TPB) void k_elastic_grad_diag_tiled(
       for (int v = 0; v < 3; ++v) d[v] = {fmax(d[v].x, 0.0), fmax(d[v].y, 0.0)};
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
-      double *a = acc + 4 * li[v];
+      double *a = acc + li[v];
       atomicAdd(a, g[v].x);
-      atomicAdd(a + 1, g[v].y);
-      atomicAdd(a + 2, d[v].x);
-      atomicAdd(a + 3, d[v].y);
+      atomicAdd(a + M, g[v].y);
+      atomicAdd(a + 2 * M, d[v].x);
+      atomicAdd(a + 3 * M, d[v].y);
     }
   }
   __syncthreads();
   for (int j = threadIdx.x; j < nn; j += ELEM_TPB) {
     const int n = tl.nodes[n0 + j];
-    const double *a = acc + 4 * j;
-    const V2<double> gg = {h2 * a[0], h2 * a[1]}, dd = {h2 * a[2], h2 * a[3]};
+    const double *a = acc + j;
+    const V2<double> gg = {h2 * a[0], h2 * a[M]}, dd = {h2 * a[2 * M], h2 * a[3 * M]};
     if (tl.owned[n0 + j]) {
       st2(grad, n, ld2(grad, n) + gg);
       st2(diag, n, ld2(diag, n) + dd);
