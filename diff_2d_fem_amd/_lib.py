@@ -115,6 +115,13 @@ def lib():
     L.d2f_contact_force.argtypes = [C.POINTER(Scene), C.POINTER(Params), i32, i32, i32, P, P, P, P]
     L.d2f_collider_hvp_f64.argtypes = [P, P, P, i32, P, i32, i32, f64, f64, f64, P, P]
     L.d2f_collider_grad_diag_f64.argtypes = [P, P, i32, P, i32, i32, f64, f64, f64, i32, P, P, P]
+    i64 = C.c_int64
+    for name in ("d2f_unique_pairs", "d2f_unique_triples"):
+        getattr(L, name).restype = i64
+    L.d2f_unique_pairs.argtypes = [P, i64, P, P, P, P]
+    L.d2f_unique_triples.argtypes = [P, i64, P, P, P]
+    L.d2f_hop_exclusion.argtypes = [P, i64, i64, P, i64, i32, P]
+    L.d2f_material_exclusion.argtypes = [P, i64, P, i64, P, i64, f64, P]
     L.d2f_apply_activation_f64.argtypes = [P, P, i32, P, P, P, P, C.POINTER(ElemConsts), P]
     L.d2f_node_grad_diag_f64.argtypes = [P, P, P, i32, P, P, f64, f64, f64, i32, P, P, P]
     L.d2f_node_hvp_f64.argtypes = [P, P, P, P, i32, P, P, f64, f64, f64, P, P]

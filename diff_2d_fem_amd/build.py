@@ -1,5 +1,5 @@
-"""Build libd2fem_hip.so in-tree: every csrc/*.hip compiled by hipcc for gfx950 and linked
-into one shared library next to this file.
+"""Build libd2fem_hip.so in-tree: every csrc/*.hip compiled by hipcc for gfx950 (and the
+host-only csrc/*.cpp by the host compiler) and linked into one shared library next to this file.
 
     python -m diff_2d_fem_amd.build
 """
@@ -41,14 +41,21 @@ def gen_headers():
     return gen_scene_dev.generate()
 
 
+HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall"]
+
+
 def build(verbose=False):
-    """Compile and link; returns the path of the library. Raises on a compiler error."""
+    """Compile and link; returns the path of the library. Raises on a compiler error. csrc/*.hip
+    go through hipcc for gfx950, csrc/*.cpp (host-only code) through the host compiler."""
     gen_headers()
-    srcs = sorted(CSRC.glob("*.hip"))
+    srcs = sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
     objs = []
     for src in srcs:
         obj = CSRC / (src.stem + ".o")
-        cmd = [hipcc(), *FLAGS, "-c", str(src), "-o", str(obj)]
+        if src.suffix == ".cpp":
+            cmd = [os.environ.get("CXX", "g++"), *HOST_FLAGS, "-c", str(src), "-o", str(obj)]
+        else:
+            cmd = [hipcc(), *FLAGS, "-c", str(src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
         r = subprocess.run(cmd, capture_output=True, text=True)

@@ -13,15 +13,51 @@ exactly (core/mesh.py:74-109, 112-160, 380-479):
   * boundary nodes are the sorted set of boundary-edge ends;
   * scenes (environments) are laid out in order, bodies of one scene in handle order.
 
-Everything is numpy; nothing here touches the GPU.
+The edge tables and the self-contact exclusion run in the library's host C++ (csrc/assemble.cpp,
+d2f_unique_pairs / d2f_unique_triples / d2f_hop_exclusion / d2f_material_exclusion); the rest
+is numpy. Nothing here touches the GPU.
 """
 from __future__ import annotations
 
+import ctypes as C
 import warnings
 from dataclasses import dataclass, field
 
 import numpy as np
-import scipy.sparse as sp
+
+
+def _native():
+    from . import _lib
+    return _lib.lib()
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+def unique_pairs(E):
+    """The distinct unordered pairs among the rows of E (m, 2): (key (u, 2) sorted pairs in
+    lexicographic order, first (u,) row each first occurs at, count (u,), inverse (m,)), as
+    numpy.unique(numpy.sort(E, 1), axis=0, return_index, return_inverse, return_counts)."""
+    E = np.ascontiguousarray(np.asarray(E, np.int64).reshape(-1, 2))
+    m = len(E)
+    key, inv = np.empty((m, 2), np.int64), np.empty(m, np.int64)
+    first, cnt = np.empty(m, np.int64), np.empty(m, np.int64)
+    u = _native().d2f_unique_pairs(_ptr(E), m, _ptr(key), _ptr(first), _ptr(cnt), _ptr(inv))
+    if u < 0:
+        raise ValueError("unique_pairs: bad argument")
+    return key[:u], first[:u], cnt[:u], inv
+
+
+def unique_triples(R):
+    """The same for rows of three: (key (u, 3), first (u,), count (u,))."""
+    R = np.ascontiguousarray(np.asarray(R, np.int64).reshape(-1, 3))
+    m = len(R)
+    key, first, cnt = np.empty((m, 3), np.int64), np.empty(m, np.int64), np.empty(m, np.int64)
+    u = _native().d2f_unique_triples(_ptr(R), m, _ptr(key), _ptr(first), _ptr(cnt))
+    if u < 0:
+        raise ValueError("unique_triples: bad argument")
+    return key[:u], first[:u], cnt[:u]
 
 
 def lame(E, nu):
@@ -119,45 +155,42 @@ def grid_mesh(nx, ny, Lx, Ly, center, mask):
 
 def _boundary_edges(tri):
     """Edges used by exactly one triangle, (min, max), in first-met order."""
-    e = tri[:, [0, 1, 1, 2, 2, 0]].reshape(-1, 2)
-    key = np.sort(e, 1)
-    uniq, first, cnt = np.unique(key, axis=0, return_index=True, return_counts=True)
+    uniq, first, cnt, _ = unique_pairs(tri[:, [0, 1, 1, 2, 2, 0]].reshape(-1, 2))
     single = cnt == 1
     order = np.argsort(first[single], kind="stable")
-    return uniq[single][order].astype(np.int64)
+    return uniq[single][order]
 
 
 def _hop_exclusion(tri, n_nodes, boundary, k):
-    """Boundary-node pairs within k mesh-edge hops of each other (incl. each node itself)."""
+    """Boundary-node pairs within k mesh-edge hops of each other (incl. each node itself),
+    (reference core/mesh.py:133-153), by breadth-first search in d2f_hop_exclusion."""
     nb = boundary.size
     if nb == 0:
         return np.zeros((0, 0), bool)
-    a = np.concatenate([tri[:, 0], tri[:, 1], tri[:, 2], tri[:, 1], tri[:, 2], tri[:, 0]])
-    b = np.concatenate([tri[:, 1], tri[:, 2], tri[:, 0], tri[:, 0], tri[:, 1], tri[:, 2]])
-    adj = sp.coo_matrix((np.ones(a.size, np.int32), (a, b)), shape=(n_nodes, n_nodes)).tocsr()
-    step = ((adj + sp.identity(n_nodes, np.int32, format="csr")) > 0).astype(np.int32)
-    reach = step[boundary]
-    for _ in range(k - 1):
-        reach = ((reach @ step) > 0).astype(np.int32)
-    return (reach[:, boundary].toarray() > 0) if k > 0 else np.eye(nb, dtype=bool)
+    tri = np.ascontiguousarray(tri, np.int64)
+    boundary = np.ascontiguousarray(boundary, np.int64)
+    out = np.empty((nb, nb), np.uint8)
+    rc = _native().d2f_hop_exclusion(_ptr(tri), len(tri), int(n_nodes), _ptr(boundary), nb, int(k), _ptr(out))
+    if rc != 0:
+        raise ValueError("hop exclusion: bad mesh indices")
+    return out.astype(bool)
 
 
 def material_exclusion(rest_pos, tri, boundary, radius):
     """Boundary-node pairs closer than `radius` along mesh edges at rest
-    (reference core/mesh.py:163-186, same graph and same shortest-path routine)."""
-    from scipy.sparse.csgraph import dijkstra
+    (reference core/mesh.py:163-186: the same edge graph, rest edge lengths as weights, shortest
+    paths stopped at the radius), in d2f_material_exclusion."""
     n = rest_pos.shape[0]
-    e = np.concatenate([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]])
-    e = np.unique(np.sort(e, 1), axis=0)
-    w = np.linalg.norm(rest_pos[e[:, 0]] - rest_pos[e[:, 1]], axis=1)
-    G = sp.coo_matrix((w, (e[:, 0], e[:, 1])), shape=(n, n)).tocsr()
+    e, _, _, _ = unique_pairs(np.concatenate([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]]))
+    pos = np.ascontiguousarray(rest_pos, np.float64)
+    boundary = np.ascontiguousarray(boundary, np.int64)
     nb = len(boundary)
-    out = np.zeros((nb, nb), bool)
-    chunk = max(1, int(2e7 // max(n, 1)))
-    for s in range(0, nb, chunk):
-        d = dijkstra(G, directed=False, indices=boundary[s:s + chunk], limit=radius)
-        out[s:s + chunk] = d[:, boundary] < radius
-    return out
+    out = np.zeros((nb, nb), np.uint8)
+    rc = _native().d2f_material_exclusion(_ptr(pos), n, _ptr(e), len(e), _ptr(boundary), nb, float(radius),
+                                          _ptr(out))
+    if rc != 0:
+        raise ValueError("material exclusion: bad mesh indices or radius")
+    return out.astype(bool)
 
 
 def finish_body(rest_pos, tri, k_excl, excl_radius=0.0):
@@ -184,12 +217,6 @@ def finish_body(rest_pos, tri, k_excl, excl_radius=0.0):
 
 
 # ---------------------------------------------------------------------------- mesh bodies
-def _point_segment_distance(P, a, b):
-    t = b - a
-    s = np.clip(((P - a) @ t) / max(float(t @ t), 1e-300), 0.0, 1.0)
-    return np.linalg.norm(P - (a + s[:, None] * t), axis=1)
-
-
 def _crossing_segments(Va, Ea, Vb, Eb):
     """Index pairs of segments that cross properly (each strictly separates the other's
     ends); with Va is Vb, pairs sharing a vertex are skipped."""
@@ -236,7 +263,7 @@ def check_mesh(vertices, triangles):
     rep = (T[:, 0] == T[:, 1]) | (T[:, 1] == T[:, 2]) | (T[:, 2] == T[:, 0])
     if rep.any():
         raise ValueError(f"triangles {np.nonzero(rep)[0][:10].tolist()} repeat a vertex")
-    _, first, cnt = np.unique(np.sort(T, 1), axis=0, return_index=True, return_counts=True)
+    _, first, cnt = unique_triples(T)
     if (cnt > 1).any():
         raise ValueError(f"duplicate triangles (same three vertices), e.g. triangle {first[cnt > 1][0]}")
     used = np.zeros(len(V), bool)
@@ -255,8 +282,7 @@ def check_mesh(vertices, triangles):
                       f"{qual.min():.3g}): they stiffen the solve and limit its step.", stacklevel=3)
     E = np.concatenate([T[:, [0, 1]], T[:, [1, 2]], T[:, [2, 0]]])
     opp = np.concatenate([T[:, 2], T[:, 0], T[:, 1]])
-    key, inv, cnt = np.unique(np.sort(E, 1), axis=0, return_inverse=True, return_counts=True)
-    inv = inv.reshape(-1)
+    key, _, cnt, inv = unique_pairs(E)
     if (cnt > 2).any():
         a, b = key[np.argmax(cnt > 2)]
         raise ValueError(f"edge ({a}, {b}) is shared by {cnt.max()} triangles; at most two may share one")
@@ -282,12 +308,21 @@ def check_mesh(vertices, triangles):
         raise ValueError(f"vertices {pairs[0, 0]} and {pairs[0, 1]} are at the same place; merge them")
     bnodes = np.unique(bed)
     mid = 0.5 * (V[bed[:, 0]] + V[bed[:, 1]])
-    for k, cand in enumerate(cKDTree(V[bnodes]).query_ball_point(mid, 0.5 * L + tol)):
-        cand = bnodes[np.asarray(cand, np.int64)]
-        cand = cand[(cand != bed[k, 0]) & (cand != bed[k, 1])]
-        if len(cand) and (_point_segment_distance(V[cand], V[bed[k, 0]], V[bed[k, 1]]) < tol).any():
-            raise ValueError(f"vertex {cand[0]} lies on boundary edge ({bed[k, 0]}, {bed[k, 1]}) "
-                             f"without being part of it (a crack or a T-junction)")
+    near = cKDTree(V[bnodes]).query_ball_point(mid, 0.5 * L + tol)
+    ek = np.repeat(np.arange(len(bed)), [len(c) for c in near])
+    if len(ek):
+        cand = bnodes[np.concatenate([np.asarray(c, np.int64) for c in near])]
+        keep = (cand != bed[ek, 0]) & (cand != bed[ek, 1])
+        ek, cand = ek[keep], cand[keep]
+        a, b = V[bed[ek, 0]], V[bed[ek, 1]]
+        t = b - a
+        sv = np.clip(((V[cand] - a) * t).sum(1) / np.maximum((t * t).sum(1), 1e-300), 0.0, 1.0)
+        on = np.linalg.norm(V[cand] - (a + sv[:, None] * t), axis=1) < tol
+        if on.any():                  # named as the reference does: the edge's first candidate
+            k = ek[int(np.argmax(on))]
+            c0 = cand[int(np.argmax(ek == k))]
+            raise ValueError(f"vertex {c0} lies on boundary edge ({bed[k, 0]}, {bed[k, 1]}) without being "
+                             f"part of it (a crack or a T-junction); merge the two sides")
     i, j = _crossing_segments(V, bed, V, bed)
     if len(i):
         raise ValueError(f"boundary edges ({bed[i[0], 0]}, {bed[i[0], 1]}) and "

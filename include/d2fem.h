@@ -247,6 +247,26 @@ int d2f_collider_force(const d2f_scene *s, const d2f_params *prm, int32_t index,
 int d2f_contact_force(const d2f_scene *s, const d2f_params *prm, int32_t colliders, int32_t bodies,
                       int32_t friction, const uint8_t *all_free, double *g, double *dg, void *stream);
 
+/* ---- scene assembly (host memory, host code) -------------------------------------------- */
+/* The distinct unordered pairs among the rows of E[m][2]: each row sorted, the distinct rows in
+ * lexicographic order into key[u][2], with the row each first occurs at (first[u]), how many rows
+ * equal it (count[u]) and, when inverse is not NULL, each row's pair (inverse[row]). Returns the
+ * number of distinct pairs (-1: bad argument). Replaces the edge tables of core/mesh.py:126-131
+ * (_finish's boundary edges), 177 (_material_excl) and 247-248 (check_mesh). */
+int64_t d2f_unique_pairs(const int64_t *E, int64_t m, int64_t *key, int64_t *first, int64_t *count,
+                         int64_t *inverse);
+/* The same for rows of three (the duplicate-triangle check of core/mesh.py:224). */
+int64_t d2f_unique_triples(const int64_t *R, int64_t m, int64_t *key, int64_t *first, int64_t *count);
+/* out[nb][nb] = 1 where boundary nodes i and j are within k mesh-edge hops (each with itself),
+ * 0 elsewhere. Replaces the breadth-first search of core/mesh.py:133-153. */
+int d2f_hop_exclusion(const int64_t *tri, int64_t T, int64_t n_nodes, const int64_t *boundary, int64_t nb,
+                      int32_t k, uint8_t *out);
+/* out[i][j] |= 1 where boundary nodes i and j are closer than radius along the mesh's edges at
+ * rest (edges: the distinct edges, pos: rest positions). Replaces _material_excl
+ * (core/mesh.py:163-186): shortest paths over the same graph, stopped at the radius. */
+int d2f_material_exclusion(const double *pos, int64_t n_nodes, const int64_t *edges, int64_t m,
+                           const int64_t *boundary, int64_t nb, double radius, uint8_t *out);
+
 /* ---- active strain ------------------------------------------------------------------------ */
 /* Element constants an activation rewrites: the reference-order arrays and, when sc_pos is not
  * NULL, the scene layout (element t sits at sc_pos[t]; constants premultiplied by the area, in

@@ -1,0 +1,91 @@
+"""The native scene assembly (csrc/assemble.cpp) against numpy / scipy restatements of the
+reference's steps (core/mesh.py:112-186, 224, 247-248): the same edge tables, boundary edges
+and self-contact exclusion sets, bit for bit, on pixel bodies with holes and on Delaunay star
+meshes; bad indices raise. The pixel-scene fixtures written by the reference's own
+core/mesh.py (test_mesh.py) cover the assembled scenes end to end."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.sparse.csgraph import dijkstra
+
+from diff_2d_fem_amd import mesh as M
+from diff_2d_fem_amd.workloads import _star_mesh
+
+
+def _np_unique_pairs(E):
+    key, first, inv, cnt = np.unique(np.sort(E, 1), axis=0, return_index=True, return_inverse=True,
+                                     return_counts=True)
+    return key, first, cnt, inv.reshape(-1)
+
+
+def _np_boundary_edges(tri):
+    e = tri[:, [0, 1, 1, 2, 2, 0]].reshape(-1, 2)
+    uniq, first, cnt, _ = _np_unique_pairs(e)
+    single = cnt == 1
+    return uniq[single][np.argsort(first[single], kind="stable")]
+
+
+def _np_hop_exclusion(tri, n, boundary, k):
+    a = np.concatenate([tri[:, 0], tri[:, 1], tri[:, 2], tri[:, 1], tri[:, 2], tri[:, 0]])
+    b = np.concatenate([tri[:, 1], tri[:, 2], tri[:, 0], tri[:, 0], tri[:, 1], tri[:, 2]])
+    adj = sp.coo_matrix((np.ones(a.size, np.int32), (a, b)), shape=(n, n)).tocsr()
+    step = ((adj + sp.identity(n, np.int32, format="csr")) > 0).astype(np.int32)
+    reach = step[boundary]
+    for _ in range(k - 1):
+        reach = ((reach @ step) > 0).astype(np.int32)
+    return (reach[:, boundary].toarray() > 0) if k > 0 else np.eye(len(boundary), dtype=bool)
+
+
+def _np_material_exclusion(pos, tri, boundary, radius):
+    n = len(pos)
+    e = np.unique(np.sort(np.concatenate([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]]), 1), axis=0)
+    w = np.linalg.norm(pos[e[:, 0]] - pos[e[:, 1]], axis=1)
+    G = sp.coo_matrix((w, (e[:, 0], e[:, 1])), shape=(n, n)).tocsr()
+    d = dijkstra(G, directed=False, indices=boundary, limit=radius)
+    return d[:, boundary] < radius
+
+
+def _meshes():
+    out = []
+    for seed in range(3):
+        rng = np.random.default_rng(seed)
+        V, T, h = _star_mesh(rng, 10e-3, int(rng.integers(600, 3000)))
+        out.append((f"star{seed}", np.asarray(V, np.float64), np.asarray(T, np.int64), h))
+    m = np.ones((12, 16), bool)
+    m[3:6, 4:9] = False
+    m[8:10, 2:4] = False
+    V, T, _, _ = M.grid_mesh(16, 12, 16e-3, 12e-3, (0.0, 0.0), m.T)
+    out.append(("pixels", np.asarray(V, np.float64), np.asarray(T, np.int64), 1e-3))
+    return out
+
+
+@pytest.mark.parametrize("case", _meshes(), ids=lambda c: c[0])
+def test_native_assembly_is_bit_identical(case):
+    name, V, T, h = case
+    E = np.concatenate([T[:, [0, 1]], T[:, [1, 2]], T[:, [2, 0]]])
+    for got, want in zip(M.unique_pairs(E), _np_unique_pairs(E)):
+        assert np.array_equal(got, want)
+    k3, f3, c3 = M.unique_triples(T)
+    kw, fw, cw = np.unique(np.sort(T, 1), axis=0, return_index=True, return_counts=True)
+    assert np.array_equal(k3, kw) and np.array_equal(f3, fw) and np.array_equal(c3, cw)
+    bed = M._boundary_edges(T)
+    assert np.array_equal(bed, _np_boundary_edges(T))
+    boundary = np.unique(bed.reshape(-1))
+    for k in (0, 1, 2, 3):
+        assert np.array_equal(M._hop_exclusion(T, len(V), boundary, k), _np_hop_exclusion(T, len(V), boundary, k))
+    for r in (1.5 * h, 3.0 * h):
+        got = M.material_exclusion(V, T, boundary, r)
+        assert np.array_equal(got, _np_material_exclusion(V, T, boundary, r))
+        assert got.sum() > len(boundary)                       # some pairs beyond the node itself
+
+
+def test_duplicate_rows_and_bad_indices():
+    E = np.array([[3, 1], [1, 3], [0, 2], [2, 0], [1, 3]])
+    key, first, cnt, inv = M.unique_pairs(E)
+    assert key.tolist() == [[0, 2], [1, 3]] and first.tolist() == [2, 0] and cnt.tolist() == [2, 3]
+    assert inv.tolist() == [1, 1, 0, 0, 1]
+    T = np.array([[0, 1, 2], [1, 2, 3]], np.int64)
+    with pytest.raises(ValueError):
+        M._hop_exclusion(T, 3, np.array([0, 1]), 2)             # node 3 is out of range
+    with pytest.raises(ValueError):
+        M.material_exclusion(np.zeros((3, 2)), T, np.array([0, 1]), 1.0)
