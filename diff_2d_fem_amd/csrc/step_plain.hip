@@ -97,7 +97,7 @@ __device__ __forceinline__ M2<R> load_B(const double *Bm, int t) {
 // triangle's contributions per node in LDS (the triangles address the list through their
 // 16-bit local indices), then adds each node once: plain read-modify-write where the chunk
 // owns the node (no other workgroup writes it), an atomic otherwise.
-constexpr int ELEM_TPB = 512;
+constexpr int ELEM_TPB = 256;   // 4 triangles per thread: 4 waves per workgroup, up to 5 workgroups per CU
 constexpr int ELEM_PER_THREAD = D2F_ELEM_CHUNK / ELEM_TPB;
 
 __global__ __launch_bounds__(ELEM_TPB) void k_elastic_grad_diag_tiled(
