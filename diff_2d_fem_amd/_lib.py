@@ -138,7 +138,12 @@ def lib():
     return L
 
 
+D2F_ECAP = -3          # a scene exceeds a compiled capacity of the scene-persistent path
+
+
 def check(rc, what):
     if rc != 0:
         msg = lib().d2f_last_error().decode(errors="replace")
-        raise NativeError(f"{what} failed ({rc}): {msg}")
+        err = NativeError(f"{what} failed ({rc}): {msg}")
+        err.code = int(rc)
+        raise err

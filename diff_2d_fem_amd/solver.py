@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import warnings
 
 import numpy as np
 import torch
@@ -654,7 +655,15 @@ class DeviceSolver:
         else:
             if not hasattr(L, "d2f_step_scene"):
                 raise RuntimeError("this build of libd2fem_hip.so has no scene-persistent path")
-            _lib.check(L.d2f_step_scene(C.byref(s), C.byref(q), stream), "d2f_step_scene")
+            rc = L.d2f_step_scene(C.byref(s), C.byref(q), stream)
+            if rc == _lib.D2F_ECAP:
+                # a scene too large for one workgroup's LDS: the per-kernel path takes any size
+                msg = _lib.lib().d2f_last_error().decode(errors="replace")
+                warnings.warn(f"{msg}. Stepping with path='plain' (float64) from here on.", stacklevel=3)
+                self.path, self.precision = "plain", "double"
+                _lib.check(L.d2f_step_plain(C.byref(s), C.byref(self.params_struct()), stream), "d2f_step_plain")
+                return
+            _lib.check(rc, "d2f_step_scene")
 
     def min_distance(self):
         """Smallest node-edge distance per environment (1e30 where none is within reach)."""

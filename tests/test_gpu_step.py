@@ -221,3 +221,29 @@ def test_scene_fallback_paths(override):
     err = np.abs(mg.get_x() - oi.solver.x).max() / 8e-3
     print(f"{override}: max |dx| / size = {err:.3e}")
     assert err < 1e-9
+
+
+def test_scene_too_large_for_lds_falls_back_to_plain():
+    """A scene whose boundary overflows one workgroup's LDS (a 2 x 3000 px strip: about 6,000
+    boundary nodes) makes d2f_step_scene return D2F_ECAP; the solver warns and steps it with the
+    per-kernel path: the plain path's result, up to the order of its float64 atomics."""
+    def make():
+        from diff_2d_fem_amd import Model
+        m = Model()
+        m.add_ground(0.0, mu=0.5)
+        m.add_soft_body(np.ones((2, 3000), np.uint8), physical_size=(1.5, 0.001), center=(0.0, 0.0008),
+                        E=0.5e6, nu=0.40, density=1100.0)
+        m.finalize()
+        m.gravity = (0.0, -9.81)
+        return m
+    kw = dict(max_iter=20, eps=0.0, conv_check=0, self_collision=False)
+    ms, mp = make(), make()
+    gs = ImplicitIntegrator(path="scene", precision="mixed", **kw)
+    gp = ImplicitIntegrator(path="plain", precision="double", **kw)
+    with pytest.warns(UserWarning, match="path='plain'"):
+        gs.simulate(ms, 1e-3)
+    gp.simulate(mp, 1e-3)
+    assert ms._solver.path == "plain" and ms._solver.precision == "double"
+    gs.simulate(ms, 1e-3)
+    gp.simulate(mp, 1e-3)
+    assert np.abs(ms.get_x() - mp.get_x()).max() < 1e-12
