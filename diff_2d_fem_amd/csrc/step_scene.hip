@@ -318,6 +318,28 @@ struct Step {
 
   __device__ __forceinline__ double fr(int i) const { return s.free_[i] ? 1.0 : 0.0; }
 
+  // A loop over the scene's nodes in batches of NB per thread: every input of a batch is loaded
+  // before any is used, so the batch's memory latencies overlap. Each node's inputs and
+  // arithmetic, and the order in which a thread visits its nodes, are those of a loop with one
+  // node per trip.
+  template <int NB, typename Load, typename Use>
+  __device__ __forceinline__ void node_batches(Load &&load, Use &&use) {
+    using In = decltype(load(0));
+    for (int b = n0 + tid; b < n1; b += NB * TPB) {
+      In in[NB];
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        const int i = b + u * TPB;
+        if (i < n1) in[u] = load(i);
+      }
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        const int i = b + u * TPB;
+        if (i < n1) use(i, in[u]);
+      }
+    }
+  }
+
   // ------------------------------------------------------------------ IPC broadphase grid
   // The boundary positions in LDS (current) are compared with those the pair list was built
   // from; when some node has moved more than half the skin, or there is no valid list, the
@@ -790,18 +812,31 @@ struct Step {
     }
     if (t < tb) elem_to_lds(load_elem<R>(ep, t));
     __syncthreads();
-    for (int i = n0 + tid; i < n1; i += TPB) {
-      const V2<double> g = ld2(s.grad, i);
-      st2(s.grad_prev, i, g);
-      const double m = s.mass[i], f = fr(i);
-      const V2<double> d = X(i) - ld2(s.xhat, i);
-      const LdsD *a = nacc + (i - n0);
-      V2<double> gn = {m * d.x * f, m * d.y * f};
-      if (s.fext) gn = gn - h2 * ld2(s.fext, i);
-      gn = gn + h2 * V2<double>{a[0], a[ns]};
-      st2(s.grad, i, gn);
-      st2(s.diag, i, V2<double>{m + h2 * a[2 * ns], m + h2 * a[3 * ns]});
-    }
+    struct NodeIn {
+      V2<double> g, x, xh, fe;
+      double m, f;
+    };
+    node_batches<2>(
+        [&](int i) {
+          NodeIn q;
+          q.g = ld2(s.grad, i);
+          q.m = s.mass[i];
+          q.f = fr(i);
+          q.x = X(i);
+          q.xh = ld2(s.xhat, i);
+          q.fe = s.fext ? ld2(s.fext, i) : V2<double>{0.0, 0.0};
+          return q;
+        },
+        [&](int i, const NodeIn &q) {
+          st2(s.grad_prev, i, q.g);
+          const V2<double> d = q.x - q.xh;
+          const LdsD *a = nacc + (i - n0);
+          V2<double> gn = {q.m * d.x * q.f, q.m * d.y * q.f};
+          if (s.fext) gn = gn - h2 * q.fe;
+          gn = gn + h2 * V2<double>{a[0], a[ns]};
+          st2(s.grad, i, gn);
+          st2(s.diag, i, V2<double>{q.m + h2 * a[2 * ns], q.m + h2 * a[3 * ns]});
+        });
     __syncthreads();
   }
 
@@ -1076,22 +1111,34 @@ struct Step {
   // ------------------------------------------------------------------ Dai-Kou direction
   __device__ __forceinline__ void direction(int it) {
     double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int i = n0 + tid; i < n1; i += TPB) {
-      const double f = fr(i);
-      V2<double> g = ld2(s.grad, i);
-      g = {g.x * f, g.y * f};
-      st2(s.grad, i, g);
-      const V2<double> dh = ld2(s.diag, i);
-      const double Px = f / fmax(dh.x, 1.0e-12), Py = f / fmax(dh.y, 1.0e-12);
-      const V2<double> pp = ld2(s.p, i);
-      const V2<double> yy = g - ld2(s.grad_prev, i);
-      const double Pg0 = Px * g.x, Pg1 = Py * g.y, Py0 = Px * yy.x, Py1 = Py * yy.y;
-      v[0] += g.x * Pg0 + g.y * Pg1;
-      v[1] += g.x * pp.x + g.y * pp.y;
-      v[2] += g.x * Py0 + g.y * Py1;
-      v[3] += yy.x * pp.x + yy.y * pp.y;
-      v[4] += yy.x * Py0 + yy.y * Py1;
-    }
+    struct DirIn {
+      V2<double> g, dh, pp, gp;
+      double f;
+    };
+    node_batches<4>(
+        [&](int i) {
+          DirIn q;
+          q.f = fr(i);
+          q.g = ld2(s.grad, i);
+          q.dh = ld2(s.diag, i);
+          q.pp = ld2(s.p, i);
+          q.gp = ld2(s.grad_prev, i);
+          return q;
+        },
+        [&](int i, const DirIn &q) {
+          const double f = q.f;
+          const V2<double> g = {q.g.x * f, q.g.y * f};
+          st2(s.grad, i, g);
+          const double Px = f / fmax(q.dh.x, 1.0e-12), Py = f / fmax(q.dh.y, 1.0e-12);
+          const V2<double> pp = q.pp;
+          const V2<double> yy = g - q.gp;
+          const double Pg0 = Px * g.x, Pg1 = Py * g.y, Py0 = Px * yy.x, Py1 = Py * yy.y;
+          v[0] += g.x * Pg0 + g.y * Pg1;
+          v[1] += g.x * pp.x + g.y * pp.y;
+          v[2] += g.x * Py0 + g.y * Py1;
+          v[3] += yy.x * pp.x + yy.y * pp.y;
+          v[4] += yy.x * Py0 + yy.y * Py1;
+        });
     blk.template sum<5>(v);
     if (tid == 0) {
       for (int g = g0; g < g1; ++g) affine_reduce(g, v);
@@ -1125,13 +1172,25 @@ struct Step {
     }
     __syncthreads();
     const double bb = sc.beta;
-    for (int i = n0 + tid; i < n1; i += TPB) {
-      const double f = fr(i);
-      const V2<double> dh = ld2(s.diag, i), g = ld2(s.grad, i), pp = ld2(s.p, i);
-      const double px = -f / fmax(dh.x, 1.0e-12) * g.x + bb * pp.x;
-      const double py = -f / fmax(dh.y, 1.0e-12) * g.y + bb * pp.y;
-      st2(s.p, i, V2<double>{px * f, py * f});
-    }
+    struct UpdIn {
+      V2<double> dh, g, pp;
+      double f;
+    };
+    node_batches<4>(
+        [&](int i) {
+          UpdIn q;
+          q.f = fr(i);
+          q.dh = ld2(s.diag, i);
+          q.g = ld2(s.grad, i);
+          q.pp = ld2(s.p, i);
+          return q;
+        },
+        [&](int i, const UpdIn &q) {
+          const double f = q.f;
+          const double px = -f / fmax(q.dh.x, 1.0e-12) * q.g.x + bb * q.pp.x;
+          const double py = -f / fmax(q.dh.y, 1.0e-12) * q.g.y + bb * q.pp.y;
+          st2(s.p, i, V2<double>{px * f, py * f});
+        });
     __syncthreads();
     for (int g = g0; g < g1; ++g) expand(g, s.ag_pq, s.p);
     for (int g = r0; g < r1; ++g) {
@@ -1220,11 +1279,21 @@ struct Step {
     const double h2 = p.h2;
     // [0] pHp, [1] p_inf (as -max), [2] alpha min, [3] clr_min
     double ph = 0.0, pinf = 0.0, amin = BIG, cmin = BIG;
-    for (int i = n0 + tid; i < n1; i += TPB) {
-      const V2<double> pp = ld2(s.p, i);
-      ph += s.mass[i] * dot(pp, pp);
-      pinf = fmax(pinf, len(pp));
-    }
+    struct MassIn {
+      V2<double> pp;
+      double m;
+    };
+    node_batches<4>(
+        [&](int i) {
+          MassIn q;
+          q.pp = ld2(s.p, i);
+          q.m = s.mass[i];
+          return q;
+        },
+        [&](int i, const MassIn &q) {
+          ph += q.m * dot(q.pp, q.pp);
+          pinf = fmax(pinf, len(q.pp));
+        });
     // elastic curvature of the soft elements, inversion cap over all elements
     const int tsoft = s.sc_cs[c0 + MAXC];
     for (int t = t0 + tid; t < t1; t += TPB) {
@@ -1344,7 +1413,8 @@ struct Step {
 
   // ------------------------------------------------------------------ update and line search
   __device__ __forceinline__ void set_x(GCD *from, double a) {
-    for (int i = n0 + tid; i < n1; i += TPB) xput(i, ld2(from, i) + a * ld2(s.p, i));
+    node_batches<4>([&](int i) { return ld2(from, i) + a * ld2(s.p, i); },
+                    [&](int i, const V2<double> &x) { xput(i, x); });
   }
 
   __device__ __forceinline__ void affine_to(int which, double a) {  // 1: q += a pq ; 2: q = q0 + a pq
