@@ -323,21 +323,25 @@ struct Step {
   // arithmetic, and the order in which a thread visits its nodes, are those of a loop with one
   // node per trip.
   template <int NB, typename Load, typename Use>
-  __device__ __forceinline__ void node_batches(Load &&load, Use &&use) {
+  __device__ __forceinline__ void range_batches(int lo, int hi, Load &&load, Use &&use) {
     using In = decltype(load(0));
-    for (int b = n0 + tid; b < n1; b += NB * TPB) {
+    for (int b = lo + tid; b < hi; b += NB * TPB) {
       In in[NB];
 #pragma unroll
       for (int u = 0; u < NB; ++u) {
         const int i = b + u * TPB;
-        if (i < n1) in[u] = load(i);
+        if (i < hi) in[u] = load(i);
       }
 #pragma unroll
       for (int u = 0; u < NB; ++u) {
         const int i = b + u * TPB;
-        if (i < n1) use(i, in[u]);
+        if (i < hi) use(i, in[u]);
       }
     }
+  }
+  template <int NB, typename Load, typename Use>
+  __device__ __forceinline__ void node_batches(Load &&load, Use &&use) {
+    range_batches<NB>(n0, n1, load, use);
   }
 
   // ------------------------------------------------------------------ IPC broadphase grid
