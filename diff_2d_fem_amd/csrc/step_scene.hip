@@ -322,22 +322,27 @@ struct Step {
   // before any is used, so the batch's memory latencies overlap. Each node's inputs and
   // arithmetic, and the order in which a thread visits its nodes, are those of a loop with one
   // node per trip.
+  // `t` is the thread's slot among the `nt` threads that share the range.
   template <int NB, typename Load, typename Use>
-  __device__ __forceinline__ void range_batches(int lo, int hi, Load &&load, Use &&use) {
+  __device__ __forceinline__ void range_batches(int lo, int hi, int t, int nt, Load &&load, Use &&use) {
     using In = decltype(load(0));
-    for (int b = lo + tid; b < hi; b += NB * TPB) {
+    for (int b = lo + t; b < hi; b += NB * nt) {
       In in[NB];
 #pragma unroll
       for (int u = 0; u < NB; ++u) {
-        const int i = b + u * TPB;
+        const int i = b + u * nt;
         if (i < hi) in[u] = load(i);
       }
 #pragma unroll
       for (int u = 0; u < NB; ++u) {
-        const int i = b + u * TPB;
+        const int i = b + u * nt;
         if (i < hi) use(i, in[u]);
       }
     }
+  }
+  template <int NB, typename Load, typename Use>
+  __device__ __forceinline__ void range_batches(int lo, int hi, Load &&load, Use &&use) {
+    range_batches<NB>(lo, hi, tid, TPB, load, use);
   }
   template <int NB, typename Load, typename Use>
   __device__ __forceinline__ void node_batches(Load &&load, Use &&use) {
@@ -1119,7 +1124,20 @@ struct Step {
       V2<double> g, dh, pp, gp;
       double f;
     };
-    node_batches<4>(
+    // The reductions of the affine and rigid-mode groups (a 6x6 Cholesky or a 3x3 inverse
+    // each: serial chains of float64 divisions and square roots) read only the groups' own
+    // gathered state, not the node arrays this loop writes. With such groups, the last wave
+    // does them, one lane per group, while the other waves run the node loop; their
+    // contributions to the five dots enter the block sum with the nodes'.
+    constexpr int LW = TPB - WAVE;
+    static_assert(LW > 0, "the side wave needs at least two waves");
+    const bool side = g1 > g0 || r1 > r0;
+    if (side && tid >= LW) {
+      for (int g = g0 + (tid - LW); g < g1; g += WAVE) affine_reduce(g, v);
+      for (int g = r0 + (tid - LW); g < r1; g += WAVE) rigid_reduce(g, v);
+    } else {
+      range_batches<4>(
+        n0, n1, tid, side ? LW : TPB,
         [&](int i) {
           DirIn q;
           q.f = fr(i);
@@ -1143,10 +1161,9 @@ struct Step {
           v[3] += yy.x * pp.x + yy.y * pp.y;
           v[4] += yy.x * Py0 + yy.y * Py1;
         });
+    }
     blk.template sum<5>(v);
     if (tid == 0) {
-      for (int g = g0; g < g1; ++g) affine_reduce(g, v);
-      for (int g = r0; g < r1; ++g) rigid_reduce(g, v);
       sc.gPg = v[0];
       sc.g_p = v[1];
       sc.g_Py = v[2];
