@@ -88,42 +88,68 @@ __device__ __forceinline__ double wave_min_dpp(double v) {
 }
 
 // Workgroup reductions, called by every thread of the workgroup with every lane active; each
-// thread gets the result. Lane 0 of each wave parks the wave's values in red[]; then lane w of
-// every wave picks up wave w's value and one row reduction combines them (NW <= 16).
+// thread gets the result.
+// * K <= 2: each wave reduces its values completely; lane 0 of each wave parks them in red[];
+//   then lane w of every wave picks up wave w's value and one row reduction combines them.
+// * K >= 3: each wave only reduces its rows of 16 lanes, and lane 0 of each row parks the row
+//   result (P = 4 NW partials per value). Wave w then combines the partials of values
+//   w, w + NW, ... and parks the totals, which every thread reads back. A wave issues about
+//   K row reductions plus K / NW combinations instead of K full reductions plus K
+//   combinations, for one more barrier.
 template <int TPB>
 struct Block {
   static constexpr int NW = TPB / WAVE;
-  static_assert(NW <= 16, "one row of lanes combines the waves");
-  LdsD *red;  // [NW * RED_K]
+  static constexpr int P = 4 * NW;                   // row partials per value
+  static constexpr int RED_SIZE = RED_K * P + RED_K;  // doubles of LDS: partials, then totals
+  static_assert(NW <= 16, "one wave combines the row partials");
+  LdsD *red;  // [RED_SIZE]
 
-  template <int K>
-  __device__ void sum(double (&v)[K]) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) v[k] = wave_sum_dpp(v[k]);
+  template <bool MIN>
+  static __device__ __forceinline__ double row_op(double v) { return MIN ? row_min(v) : row_sum(v); }
+  template <bool MIN>
+  static __device__ __forceinline__ double op(double a, double b) { return MIN ? fmin(a, b) : a + b; }
+
+  template <bool MIN, int K>
+  __device__ __forceinline__ void reduce(double (&v)[K]) {
+    static_assert(K <= RED_K, "reduction wider than the LDS scratch");
+    const double idle = MIN ? __builtin_huge_val() : 0.0;
     const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
-    __syncthreads();
-    if (l == 0)
+    if constexpr (K <= 2) {
 #pragma unroll
-      for (int k = 0; k < K; ++k) red[w * K + k] = v[k];
-    __syncthreads();
+      for (int k = 0; k < K; ++k) v[k] = MIN ? wave_min_dpp(v[k]) : wave_sum_dpp(v[k]);
+      __syncthreads();
+      if (l == 0)
 #pragma unroll
-    for (int k = 0; k < K; ++k) v[k] = readlane_f64(row_sum(l < NW ? double(red[l * K + k]) : 0.0), 0);
+        for (int k = 0; k < K; ++k) red[w * K + k] = v[k];
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < K; ++k) v[k] = readlane_f64(row_op<MIN>(l < NW ? double(red[l * K + k]) : idle), 0);
+    } else {
+#pragma unroll
+      for (int k = 0; k < K; ++k) v[k] = row_op<MIN>(v[k]);
+      __syncthreads();
+      if ((l & 15) == 0)
+#pragma unroll
+        for (int k = 0; k < K; ++k) red[k * P + 4 * w + (l >> 4)] = v[k];
+      __syncthreads();
+      LdsD *tot = red + RED_K * P;
+      for (int k = w; k < K; k += NW) {
+        const double t = row_op<MIN>(l < P ? double(red[k * P + l]) : idle);
+        double r = readlane_f64(t, 0);
+        if constexpr (P > 16) r = op<MIN>(r, readlane_f64(t, 16));
+        if constexpr (P > 32) r = op<MIN>(r, op<MIN>(readlane_f64(t, 32), readlane_f64(t, 48)));
+        if (l == 0) tot[k] = r;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < K; ++k) v[k] = tot[k];
+    }
   }
+  template <int K>
+  __device__ void sum(double (&v)[K]) { reduce<false>(v); }
   // component-wise minimum
   template <int K>
-  __device__ void min(double (&v)[K]) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) v[k] = wave_min_dpp(v[k]);
-    const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
-    __syncthreads();
-    if (l == 0)
-#pragma unroll
-      for (int k = 0; k < K; ++k) red[w * K + k] = v[k];
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-      v[k] = readlane_f64(row_min(l < NW ? double(red[l * K + k]) : __builtin_huge_val()), 0);
-  }
+  __device__ void min(double (&v)[K]) { reduce<true>(v); }
   __device__ double sum1(double a) {
     double v[1] = {a};
     sum<1>(v);
@@ -263,7 +289,7 @@ struct Step {
     e = blockIdx.x;
     tid = threadIdx.x;
     blk.red = lds;
-    bpos = lds + Block<TPB>::NW * RED_K;
+    bpos = lds + Block<TPB>::RED_SIZE;
     bacc = bpos + 2 * s.max_nb;
     bref = bacc + 4 * s.max_nb;
     nacc = bref + 2 * s.max_nb;
@@ -1035,6 +1061,7 @@ struct Step {
         const int i = s.an_node[k];
         const double w0 = s.an_w[3 * k], w1 = s.an_w[3 * k + 1], w2 = s.an_w[3 * k + 2];
         const V2<double> gg = ld2(s.grad, i), dd = ld2(s.diag, i);
+        st2(s.grad, i, V2<double>{0.0, 0.0});   // each slaved node is gathered by one thread
         const double w[3] = {w0, w1, w2};
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
@@ -1060,8 +1087,6 @@ struct Step {
           s.ag_Hy[9 * g + r] = v[12 + map[r]];
         }
       }
-      for (int k = s.an_start[g] + tid; k < s.an_start[g + 1]; k += TPB)
-        st2(s.grad, s.an_node[k], V2<double>{0.0, 0.0});
     }
     // rigid-mode coarse space (reference implicit/affine.py:405-453)
     for (int g = r0; g < r1; ++g) {
@@ -1664,7 +1689,7 @@ __global__ __launch_bounds__(TPB) void k_scene(SH s, Pm p) {
 
 template <typename R, int TPB>
 static int launch(const SH &s, const Pm &p, hipStream_t stream) {
-  const size_t lds = sizeof(double) * (Block<TPB>::NW * RED_K + 8 * (size_t)s.max_nb +
+  const size_t lds = sizeof(double) * (Block<TPB>::RED_SIZE + 8 * (size_t)s.max_nb +
                                        (s.nacc_lds ? 4 * (size_t)s.max_nn : 0) +
                                        (s.xc_lds ? 2 * (size_t)s.max_nn : 0)) +
                      sizeof(int) * (2 * (size_t)s.max_cells + 1 + s.max_nb + (size_t)s.excl_bits_lds +

@@ -465,15 +465,17 @@ class DeviceSolver:
     SKIN = 0.5                       # pair-list margin, in units of the contact reach
 
     def _plan_lds(self):
-        """Split each scene workgroup's LDS budget: fixed scratch (reductions, boundary
-        positions and accumulators, grid), then per-node element accumulators, then the
-        bit-packed exclusion matrix, then the candidate-pair list (bit jp * n_b_e + j of a
-        scene's bitset is its exclusion of boundary nodes jp and j)."""
+        """Split each scene workgroup's LDS budget (for the current block_size): fixed scratch
+        (reductions, boundary positions and accumulators, grid), then per-node element
+        accumulators, then the bit-packed exclusion matrix, then the candidate-pair list (bit
+        jp * n_b_e + j of a scene's bitset is its exclusion of boundary nodes jp and j)."""
         self.excl_bits_lds, self.max_pairs, self.nacc_lds, self.xc_lds = 0, 0, 0, 0
         self.excl_bits = self.excl_bits_off = None
         nn = np.diff(np.asarray(self.me.node_starts, np.int64))
         self.max_nn = int(nn.max(initial=0))
-        base = 8 * (16 * 20 + 8 * self.max_nb) + 4 * (2 * self.max_cells + 1 + self.max_nb)
+        nw = int(self.block_size) // 64            # waves per workgroup
+        red = 20 * 4 * nw + 20                     # block reductions: 20 values x 4 rows per wave, totals
+        base = 8 * (red + 8 * self.max_nb) + 4 * (2 * self.max_cells + 1 + self.max_nb)
         if base + 32 * self.max_nn <= self.LDS_BUDGET:
             self.nacc_lds = 1
             base += 32 * self.max_nn
