@@ -1075,10 +1075,13 @@ struct Step {
           v[12 + r] += dd.y * ww[r];
         }
       }
+      double prev[6];                 // last iteration's J^T g, loaded while the sum runs
+      if (tid == 0)
+        for (int r = 0; r < 6; ++r) prev[r] = s.ag_gq[6 * g + r];
       blk.template sum<18>(v);
       if (tid == 0) {
         for (int r = 0; r < 6; ++r) {
-          s.ag_gq_prev[6 * g + r] = s.ag_gq[6 * g + r];
+          s.ag_gq_prev[6 * g + r] = prev[r];
           s.ag_gq[6 * g + r] = v[r];
         }
         const int map[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
