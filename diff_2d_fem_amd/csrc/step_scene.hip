@@ -556,11 +556,24 @@ struct Step {
 
   // One node-edge pair (reference implicit/ipc_contact.py:175-296, the body of the query loop):
   // edge k (global), scene-local boundary node jp that is neither end nor excluded.
+  // What a pair reads from global memory before any arithmetic: the edge's nodes (global and
+  // scene-local boundary numbering), its vertex-correction coefficients, node jp's number.
+  struct PairIn {
+    int k, jp, a, b, la, lb, pn;
+    double ca, cb;
+  };
+  __device__ __forceinline__ PairIn pair_load(int k, int jp) {
+    return {k, jp, s.ea[k], s.eb[k], s.ela[k], s.elb[k], s.bnd[b0 + jp], s.ca[k], s.cb[k]};
+  }
   __device__ __forceinline__ void pair_eval(int mode, int k, int jp, GCD *pdir, double t_c, double h2,
-                            double &psum, double &cmin, double &amin) {
+                                            double &psum, double &cmin, double &amin) {
+    pair_eval(mode, pair_load(k, jp), pdir, t_c, h2, psum, cmin, amin);
+  }
+  __device__ __forceinline__ void pair_eval(int mode, const PairIn &in, GCD *pdir, double t_c, double h2,
+                                            double &psum, double &cmin, double &amin) {
     const double dhat = p.dhat, reach = p.ipc_reach;
     const double dlo = 1.0e-9 * dhat, onset = 0.02 * dhat;
-    const int a = s.ea[k], b = s.eb[k], la = s.ela[k], lb = s.elb[k];
+    const int jp = in.jp, a = in.a, b = in.b, la = in.la, lb = in.lb;
     const V2<double> xa = {bpos[2 * la], bpos[2 * la + 1]}, xb = {bpos[2 * lb], bpos[2 * lb + 1]};
     const V2<double> ab = xb - xa;
     const double L2 = fmax(dot(ab, ab), 1.0e-30);
@@ -573,7 +586,7 @@ struct Step {
     else if (tt >= 1.0) dvec = xp - xb;
     double d = len(dvec);
     if (!(d < reach)) return;
-    const int pn = s.bnd[b0 + jp];
+    const int pn = in.pn;
     if (mode == 1 || mode == 3) {
       cmin = fmin(cmin, d);
       double keep = fmax(p.ipc_gap * d, dhat - onset);
@@ -605,7 +618,7 @@ struct Step {
       }
       psum += term(mode, 1.0, jp, la, lb, pn, a, b, 1.0 - tt, tt, d, (1.0 / d) * dvec, ddd, mu_p, pdir, h2);
     }
-    double coef_a = -s.ca[k];
+    double coef_a = -in.ca;
     if (tt <= 0.0) coef_a += 1.0;
     double d_a = len(w);
     if (coef_a != 0.0 && d_a < dhat) {
@@ -619,7 +632,7 @@ struct Step {
       }
       psum += term(mode, coef_a, jp, la, lb, pn, a, b, 1.0, 0.0, d_a, n_a, ddd, mu_p, pdir, h2);
     }
-    double coef_b = -s.cb[k];
+    double coef_b = -in.cb;
     if (tt >= 1.0) coef_b += 1.0;
     const V2<double> wb_ = xp - xb;
     double d_b = len(wb_);
@@ -695,11 +708,15 @@ struct Step {
     cmin = BIG;
     amin = BIG;
     if (sc.plist) {
-      const int np = sc.npairs;
-      for (int q = tid; q < np; q += TPB) {
-        const int v = plist[q];
-        pair_eval(mode, k0 + (v >> 16), v & 0xffff, pdir, t_c, h2, psum, cmin, amin);
-      }
+      // two pairs per thread loaded before either is evaluated (each thread's pairs in the
+      // same order as one per trip)
+      range_batches<2>(
+          0, sc.npairs,
+          [&](int q) {
+            const int v = plist[q];
+            return pair_load(k0 + (v >> 16), v & 0xffff);
+          },
+          [&](int, const PairIn &in) { pair_eval(mode, in, pdir, t_c, h2, psum, cmin, amin); });
       return;
     }
     for (int k = k0 + tid; k < k1; k += TPB)
