@@ -56,7 +56,7 @@ _SCENE_FIELDS = (
     + [("excl_bits", P), ("excl_bits_off", P), ("excl_bits_lds", C.c_int32), ("max_pairs", C.c_int32)]
     + [("max_nn", C.c_int32), ("nacc_lds", C.c_int32), ("xc_lds", C.c_int32), ("pad3_", C.c_int32)]
     + [("el", ElemTiles)]
-    + [("n_sp", C.c_int32), ("pad4_", C.c_int32)]
+    + [("n_sp", C.c_int32), ("max_ag", C.c_int32)]
     + [(k, P) for k in ("sp_a", "sp_b", "sp2env", "sp_anchor", "sp_k", "sp_L0", "sp_start")]
 )
 

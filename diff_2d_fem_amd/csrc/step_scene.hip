@@ -273,6 +273,17 @@ struct Step {
   LdsD *bacc;     // 4 nb: boundary accumulators (grad x, grad y, diag x, diag y)
   LdsD *bref;     // 2 nb: boundary positions the grid and pair list were built from
   LdsD *nacc;     // 4 x max_nn, component-major: grad x, grad y, diag x, diag y of each node
+  // The scene's affine groups in LDS for the whole step, GST doubles each: coordinates q (6),
+  // their line-search start q0 (6), direction pq (6), preconditioned gradient zq (6), Mu, La.
+  // Loaded from ag_q, ag_pq, ag_zq, ag_Mu, ag_La at the start, written back at the end.
+  static constexpr int GST = 26;
+  LdsD *gst;
+  __device__ __forceinline__ LdsD *gq_(int g) const { return gst + GST * (g - g0); }
+  __device__ __forceinline__ LdsD *gq0_(int g) const { return gq_(g) + 6; }
+  __device__ __forceinline__ LdsD *gpq_(int g) const { return gq_(g) + 12; }
+  __device__ __forceinline__ LdsD *gzq_(int g) const { return gq_(g) + 18; }
+  __device__ __forceinline__ double gMu_(int g) const { return gq_(g)[24]; }
+  __device__ __forceinline__ double gLa_(int g) const { return gq_(g)[25]; }
   LdsD *xc;       // 2 nn: the scene's positions (when xc_lds), written through with s.x
   bool xcon;
   LdsI *cstart;   // max_cells + 1
@@ -292,7 +303,8 @@ struct Step {
     bpos = lds + Block<TPB>::RED_SIZE;
     bacc = bpos + 2 * s.max_nb;
     bref = bacc + 4 * s.max_nb;
-    nacc = bref + 2 * s.max_nb;
+    gst = bref + 2 * s.max_nb;
+    nacc = gst + GST * s.max_ag;
     xc = nacc + (s.nacc_lds ? 4 * s.max_nn : 0);
     cstart = reinterpret_cast<LdsI *>(xc + (s.xc_lds ? 2 * s.max_nn : 0));
     citem = cstart + s.max_cells + 1;
@@ -1234,7 +1246,7 @@ struct Step {
       sc.beta = b;
       sc.gTp = t;
       for (int g = g0; g < g1; ++g)
-        for (int r = 0; r < 6; ++r) s.ag_pq[6 * g + r] = -s.ag_zq[6 * g + r] + b * s.ag_pq[6 * g + r];
+        for (int r = 0; r < 6; ++r) gpq_(g)[r] = -gzq_(g)[r] + b * gpq_(g)[r];
     }
     __syncthreads();
     const double bb = sc.beta;
@@ -1258,7 +1270,7 @@ struct Step {
           st2(s.p, i, V2<double>{px * f, py * f});
         });
     __syncthreads();
-    for (int g = g0; g < g1; ++g) expand(g, s.ag_pq, s.p);
+    for (int g = g0; g < g1; ++g) expand(g, gpq_(g), s.p);
     for (int g = r0; g < r1; ++g) {
       GCD *a = s.rg_acc + 3 * g, *z = s.rg_zq + 3 * g;
       const double cx = a[0] / a[2], cy = a[1] / a[2];
@@ -1272,8 +1284,7 @@ struct Step {
     __syncthreads();
   }
 
-  __device__ __forceinline__ void expand(int g, GCD *v6, GD *out) {
-    GCD *v = v6 + 6 * g;
+  __device__ __forceinline__ void expand(int g, const LdsD *v, GD *out) {
     for (int k = s.an_start[g] + tid; k < s.an_start[g + 1]; k += TPB) {
       GCD *w = s.an_w + 3 * k;
       const V2<double> val = {w[0] * v[0] + w[1] * v[1] + w[2] * v[2], w[0] * v[3] + w[1] * v[4] + w[2] * v[5]};
@@ -1286,8 +1297,8 @@ struct Step {
 
   // 6x6 block preconditioner of one rigid region (reference implicit/affine.py:179-205)
   __device__ __forceinline__ void affine_reduce(int b, double (&v)[5]) {
-    GCD *q = s.ag_q + 6 * b;
-    const double Mu = s.ag_Mu[b], La = s.ag_La[b];
+    const LdsD *q = gq_(b);
+    const double Mu = gMu_(b), La = gLa_(b);
     const NH<double> h = nh_eval(M2<double>{q[1], q[2], q[4], q[5]}, Mu, La);
     const double P[4] = {Mu * h.F.a + h.c * h.G.a, Mu * h.F.b + h.c * h.G.b, Mu * h.F.c + h.c * h.G.c,
                          Mu * h.F.d + h.c * h.G.d};
@@ -1308,7 +1319,7 @@ struct Step {
     cholesky6_solve(L, g, z);
     cholesky6_solve(L, y, zy);
     for (int r = 0; r < 6; ++r) {
-      const double pp = s.ag_pq[6 * b + r];
+      const double pp = gpq_(b)[r];
       v[0] += g[r] * z[r];
       v[1] += g[r] * pp;
       v[2] += g[r] * zy[r];
@@ -1317,7 +1328,7 @@ struct Step {
     }
     for (int r = 0; r < 6; ++r) {
       s.ag_gq[6 * b + r] = g[r];
-      s.ag_zq[6 * b + r] = z[r];
+      gzq_(b)[r] = z[r];
     }
   }
 
@@ -1422,8 +1433,8 @@ struct Step {
       const double p_inf = -vm[0];
       // rigid regions' own curvature and the tendons' (single thread, in order)
       for (int g = g0; g < g1; ++g) {
-        GCD *q = s.ag_q + 6 * g, *pq = s.ag_pq + 6 * g;
-        const double Mu = s.ag_Mu[g], La = s.ag_La[g];
+        const LdsD *q = gq_(g), *pq = gpq_(g);
+        const double Mu = gMu_(g), La = gLa_(g);
         const NH<double> h = nh_eval(M2<double>{q[1], q[2], q[4], q[5]}, Mu, La);
         pH += h2 * fmax(nh_quad(h, M2<double>{pq[1], pq[2], pq[4], pq[5]}, Mu, La), 0.0);
       }
@@ -1487,11 +1498,10 @@ struct Step {
     if (tid == 0)
       for (int g = g0; g < g1; ++g)
         for (int r = 0; r < 6; ++r) {
-          const int k = 6 * g + r;
-          s.ag_q[k] = (which == 1 ? s.ag_q[k] : s.ag_q0[k]) + a * s.ag_pq[k];
+          gq_(g)[r] = (which == 1 ? gq_(g)[r] : gq0_(g)[r]) + a * gpq_(g)[r];
         }
     __syncthreads();
-    for (int g = g0; g < g1; ++g) expand(g, s.ag_q, s.x);
+    for (int g = g0; g < g1; ++g) expand(g, gq_(g), s.x);
     __syncthreads();
   }
 
@@ -1541,9 +1551,9 @@ struct Step {
     E = blk.sum1(E);
     if (tid == 0) {
       for (int g = g0; g < g1; ++g) {
-        GCD *q = s.ag_q + 6 * g;
-        const NH<double> h = nh_eval(M2<double>{q[1], q[2], q[4], q[5]}, s.ag_Mu[g], s.ag_La[g]);
-        E += h2 * nh_psi(h, s.ag_Mu[g], s.ag_La[g]);
+        const LdsD *q = gq_(g);
+        const NH<double> h = nh_eval(M2<double>{q[1], q[2], q[4], q[5]}, gMu_(g), gLa_(g));
+        E += h2 * nh_psi(h, gMu_(g), gLa_(g));
       }
       for (int k = td0; k < td1; ++k) {
         double L = 0.0;
@@ -1581,7 +1591,8 @@ struct Step {
     const double E0 = energy();
     for (int i = n0 + tid; i < n1; i += TPB) st2(s.x0, i, X(i));
     if (tid == 0)
-      for (int k = 6 * g0; k < 6 * g1; ++k) s.ag_q0[k] = s.ag_q[k];
+      for (int g = g0; g < g1; ++g)
+        for (int r = 0; r < 6; ++r) gq0_(g)[r] = gq_(g)[r];
     __syncthreads();
     for (int bt = 0; bt < p.n_backtrack; ++bt) {
       set_x(s.x0, sc.alpha);
@@ -1620,6 +1631,16 @@ struct Step {
       }
       st2(s.xn, i, x);
       st2(s.xhat, i, x + (p.dt * p.drag) * ld2(s.v, i));
+    }
+    for (int j = tid; j < GST * (g1 - g0); j += TPB) {
+      const int g = g0 + j / GST, r = j % GST;
+      double v = 0.0;
+      if (r < 6) v = s.ag_q[6 * g + r];
+      else if (r >= 12 && r < 18) v = s.ag_pq[6 * g + r - 12];
+      else if (r >= 18 && r < 24) v = s.ag_zq[6 * g + r - 18];
+      else if (r == 24) v = s.ag_Mu[g];
+      else if (r == 25) v = s.ag_La[g];
+      gst[j] = v;
     }
     if (tid == 0) {
       sc.pending = 0;
@@ -1667,6 +1688,12 @@ struct Step {
       __syncthreads();
     }
     for (int i = n0 + tid; i < n1; i += TPB) st2(s.v, i, p.inv_dt * (X(i) - ld2(s.xn, i)));
+    for (int j = tid; j < 18 * (g1 - g0); j += TPB) {
+      const int g = g0 + j / 18, r = j % 18;
+      if (r < 6) s.ag_q[6 * g + r] = gq_(g)[r];
+      else if (r < 12) s.ag_pq[6 * g + r - 6] = gpq_(g)[r - 6];
+      else s.ag_zq[6 * g + r - 12] = gzq_(g)[r - 12];
+    }
     if (tid == 0) {
       s.active[e] = sc.active;
       s.iters[e] = sc.iters;
@@ -1710,6 +1737,7 @@ __global__ __launch_bounds__(TPB) void k_scene(SH s, Pm p) {
 template <typename R, int TPB>
 static int launch(const SH &s, const Pm &p, hipStream_t stream) {
   const size_t lds = sizeof(double) * (Block<TPB>::RED_SIZE + 8 * (size_t)s.max_nb +
+                                       Step<R, TPB>::GST * (size_t)s.max_ag +
                                        (s.nacc_lds ? 4 * (size_t)s.max_nn : 0) +
                                        (s.xc_lds ? 2 * (size_t)s.max_nn : 0)) +
                      sizeof(int) * (2 * (size_t)s.max_cells + 1 + s.max_nb + (size_t)s.excl_bits_lds +

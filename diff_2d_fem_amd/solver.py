@@ -475,7 +475,11 @@ class DeviceSolver:
         self.max_nn = int(nn.max(initial=0))
         nw = int(self.block_size) // 64            # waves per workgroup
         red = 20 * 4 * nw + 20                     # block reductions: 20 values x 4 rows per wave, totals
-        base = 8 * (red + 8 * self.max_nb) + 4 * (2 * self.max_cells + 1 + self.max_nb)
+        self.max_ag = 0                            # affine groups of the largest scene: 26 doubles each
+        if self.affine is not None:
+            self.max_ag = int(np.diff(self.ag_start.cpu().numpy()).max(initial=0))
+        base = (8 * (red + 8 * self.max_nb + 26 * self.max_ag)
+                + 4 * (2 * self.max_cells + 1 + self.max_nb))
         if base + 32 * self.max_nn <= self.LDS_BUDGET:
             self.nacc_lds = 1
             base += 32 * self.max_nn
@@ -602,6 +606,7 @@ class DeviceSolver:
                 s.excl_bits, s.excl_bits_off = p("excl_bits"), p("excl_bits_off")
             s.excl_bits_lds, s.max_pairs = self.excl_bits_lds, self.max_pairs
             s.max_nn, s.nacc_lds, s.xc_lds = self.max_nn, self.nacc_lds, self.xc_lds
+            s.max_ag = self.max_ag
         if self.springs is not None:
             s.n_sp = self.springs["n"]
             for k in ("sp_a", "sp_b", "sp2env", "sp_anchor", "sp_k", "sp_L0", "sp_start"):

@@ -148,8 +148,10 @@ typedef struct d2f_scene {
   /* per-kernel path: node tiles of the element kernel (node_off NULL: atomics only) */
   d2f_elem_tiles el;
   /* springs and pins [n_sp], grouped by environment (reference implicit/actuation.py:368-476):
-   * E = k/2 (|x_a - x_b| - L0)^2 with x_b = sp_anchor when sp_b < 0 (a spring to the world) */
-  int32_t n_sp, pad4_;
+   * E = k/2 (|x_a - x_b| - L0)^2 with x_b = sp_anchor when sp_b < 0 (a spring to the world);
+   * max_ag: largest per-scene affine group count (the scene kernel keeps their coordinates,
+   * direction and moduli in LDS, 26 doubles per group) */
+  int32_t n_sp, max_ag;
   const int32_t *sp_a, *sp_b, *sp2env;  /* n_sp */
   const double *sp_anchor;              /* 2 n_sp */
   const double *sp_k, *sp_L0;           /* n_sp */
