@@ -289,6 +289,7 @@ struct Step {
   LdsI *cstart;   // max_cells + 1
   LdsI *citem;    // nb
   LdsI *ccur;     // max_cells
+  LdsI *lbnd;     // the scene's boundary nodes (bnd[b0 ..]), copied once per step
   LdsU *xbits;    // exclusion bitset (excl_bits_lds words)
   LdsI *plist;    // max_pairs packed (edge - k0) << 16 | jp
   int e, tid;
@@ -309,7 +310,8 @@ struct Step {
     cstart = reinterpret_cast<LdsI *>(xc + (s.xc_lds ? 2 * s.max_nn : 0));
     citem = cstart + s.max_cells + 1;
     ccur = citem + s.max_nb;
-    xbits = reinterpret_cast<LdsU *>(ccur + s.max_cells);
+    lbnd = ccur + s.max_cells;
+    xbits = reinterpret_cast<LdsU *>(lbnd + s.max_nb);
     plist = reinterpret_cast<LdsI *>(xbits + s.excl_bits_lds);
     n0 = s.node_start[e];
     n1 = s.node_start[e + 1];
@@ -399,7 +401,7 @@ struct Step {
     const bool have = sc.plist;
     double dmax = 0.0;
     for (int j = tid; j < nb; j += TPB) {
-      const V2<double> x = X(s.bnd[b0 + j]);
+      const V2<double> x = X(lbnd[j]);
       bpos[2 * j] = x.x;
       bpos[2 * j + 1] = x.y;
       if (have) dmax = fmax(dmax, len(V2<double>{x.x - bref[2 * j], x.y - bref[2 * j + 1]}));
@@ -575,7 +577,7 @@ struct Step {
     double ca, cb;
   };
   __device__ __forceinline__ PairIn pair_load(int k, int jp) {
-    return {k, jp, s.ea[k], s.eb[k], s.ela[k], s.elb[k], s.bnd[b0 + jp], s.ca[k], s.cb[k]};
+    return {k, jp, s.ea[k], s.eb[k], s.ela[k], s.elb[k], lbnd[jp], s.ca[k], s.cb[k]};
   }
   __device__ __forceinline__ void pair_eval(int mode, int k, int jp, GCD *pdir, double t_c, double h2,
                                             double &psum, double &cmin, double &amin) {
@@ -772,7 +774,7 @@ struct Step {
       if (!node_visit(x, xb, p.node_r, v)) return;
       node_grad_diag(v, p.node_k, 1, gp, hx, hy);
       if (fric) {
-        const int nj = s.bnd[b0 + jb];
+        const int nj = lbnd[jb];
         const double mu = node_pair_mu(p.node_mu, p.ipc_per_body, s.mu_node, n, nj);
         node_fric_grad_diag(v, p.node_k, mu, p.epsv, p.inv_dt, ua - (xb - ld2(s.xn, nj)), gf, fx, fy);
       }
@@ -795,7 +797,7 @@ struct Step {
     node_partners(j, x, [&](int jb, V2<double> xb) {
       NodeVisit v;
       if (!node_visit(x, xb, p.node_r, v)) return;
-      const int nj = s.bnd[b0 + jb];
+      const int nj = lbnd[jb];
       const V2<double> dp = mode == 1 ? pa - ld2(s.p, nj) : V2<double>{0.0, 0.0};
       val += mode == 1 ? node_pHp(v, p.node_k, 1, dp) : node_energy(v, p.node_k);
       if (fric) {
@@ -993,7 +995,7 @@ struct Step {
     if (p.node_on && nb > 0) contact_prepare();
     // colliders, friction and the accumulated contact terms, one thread per boundary node
     for (int j = tid; j < nb; j += TPB) {
-      const int n = s.bnd[b0 + j];
+      const int n = lbnd[j];
       const V2<double> x = X(n);
       V2<double> g = ld2(s.grad, n), dg = ld2(s.diag, n);
       for (int k = 0; k < s.n_col; ++k) {
@@ -1388,7 +1390,7 @@ struct Step {
     }
     // colliders: curvature, clearance, friction curvature, step cap
     for (int j = tid; j < nb; j += TPB) {
-      const int n = s.bnd[b0 + j];
+      const int n = lbnd[j];
       const V2<double> x = X(n), pp = ld2(s.p, n);
       for (int k = 0; k < s.n_col; ++k) {
         const Collider c = to_collider_k(s.col[k]);
@@ -1522,7 +1524,7 @@ struct Step {
     }
     if (p.node_on && nb > 0) contact_prepare();
     for (int j = tid; j < nb; j += TPB) {
-      const int n = s.bnd[b0 + j];
+      const int n = lbnd[j];
       const V2<double> x = X(n);
       for (int k = 0; k < s.n_col; ++k) E += collider_energy(to_collider_k(s.col[k]), x, p.dhat, p.kappa, h2);
       if (p.node_on) E += node_sum(2, j, n, x, h2);
@@ -1632,6 +1634,7 @@ struct Step {
       st2(s.xn, i, x);
       st2(s.xhat, i, x + (p.dt * p.drag) * ld2(s.v, i));
     }
+    for (int j = tid; j < nb; j += TPB) lbnd[j] = s.bnd[b0 + j];
     for (int j = tid; j < GST * (g1 - g0); j += TPB) {
       const int g = g0 + j / GST, r = j % GST;
       double v = 0.0;
@@ -1740,7 +1743,7 @@ static int launch(const SH &s, const Pm &p, hipStream_t stream) {
                                        Step<R, TPB>::GST * (size_t)s.max_ag +
                                        (s.nacc_lds ? 4 * (size_t)s.max_nn : 0) +
                                        (s.xc_lds ? 2 * (size_t)s.max_nn : 0)) +
-                     sizeof(int) * (2 * (size_t)s.max_cells + 1 + s.max_nb + (size_t)s.excl_bits_lds +
+                     sizeof(int) * (2 * (size_t)s.max_cells + 1 + 2 * (size_t)s.max_nb + (size_t)s.excl_bits_lds +
                                     (size_t)s.max_pairs);
   if (lds > 150 * 1024) {
     set_error("d2f_step_scene: %zu bytes of LDS per scene (max_nb=%d, max_cells=%d) exceed the 150 KiB "

@@ -479,7 +479,7 @@ class DeviceSolver:
         if self.affine is not None:
             self.max_ag = int(np.diff(self.ag_start.cpu().numpy()).max(initial=0))
         base = (8 * (red + 8 * self.max_nb + 26 * self.max_ag)
-                + 4 * (2 * self.max_cells + 1 + self.max_nb))
+                + 4 * (2 * self.max_cells + 1 + 2 * self.max_nb))
         if base + 32 * self.max_nn <= self.LDS_BUDGET:
             self.nacc_lds = 1
             base += 32 * self.max_nn
