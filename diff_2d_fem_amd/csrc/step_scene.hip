@@ -1375,18 +1375,48 @@ struct Step {
         });
     // elastic curvature of the soft elements, inversion cap over all elements
     const int tsoft = s.sc_cs[c0 + MAXC];
-    for (int t = t0 + tid; t < t1; t += TPB) {
-      const int i0 = s.sc_tri[3 * t], i1 = s.sc_tri[3 * t + 1], i2 = s.sc_tri[3 * t + 2];
-      const V2<double> x0 = X(i0), x1 = X(i1), x2 = X(i2);
-      const V2<double> q0 = ld2(s.p, i0), q1 = ld2(s.p, i1), q2 = ld2(s.p, i2);
-      if (t < tsoft) {
-        const Elem<R> el = load_elem<R>(s, t);
-        const NH<R> h = elem_F(el, x0, x1, x2);
-        const M2<R> dDs = {R(q1.x - q0.x), R(q2.x - q0.x), R(q1.y - q0.y), R(q2.y - q0.y)};
-        const M2<R> dF = mat_mul(dDs, el.B);
-        ph += h2 * fmax(double(nh_quad(h, dF, el.Amu, el.Alam)), 0.0);
+    if constexpr (sizeof(R) == 4) {
+      // Two triangles per trip: both triangles' indices and constants (the constant arrays
+      // cover the rigid triangles too) are loaded first, then both direction gathers, then both
+      // are evaluated; each thread visits its triangles in the order of one per trip. (The
+      // float64 kernel keeps one triangle per trip: there the extra registers spill.)
+      const ElemPtrs<R> ep = elem_ptrs<R>(s);
+      const auto curv = [&](int t, const Elem<R> &el, V2<double> q0, V2<double> q1, V2<double> q2) {
+        const V2<double> x0 = X(el.i0), x1 = X(el.i1), x2 = X(el.i2);
+        if (t < tsoft) {
+          const NH<R> h = elem_F(el, x0, x1, x2);
+          const M2<R> dDs = {R(q1.x - q0.x), R(q2.x - q0.x), R(q1.y - q0.y), R(q2.y - q0.y)};
+          const M2<R> dF = mat_mul(dDs, el.B);
+          ph += h2 * fmax(double(nh_quad(h, dF, el.Amu, el.Alam)), 0.0);
+        }
+        amin = fmin(amin, inversion_root(x1 - x0, x2 - x0, q1 - q0, q2 - q0, p.area_shrink));
+      };
+      int t = t0 + tid;
+      for (; t + TPB < t1; t += 2 * TPB) {
+        const Elem<R> e0 = load_elem<R>(ep, t), e1 = load_elem<R>(ep, t + TPB);
+        const V2<double> a0 = ld2(s.p, e0.i0), a1 = ld2(s.p, e0.i1), a2 = ld2(s.p, e0.i2);
+        const V2<double> b0 = ld2(s.p, e1.i0), b1 = ld2(s.p, e1.i1), b2 = ld2(s.p, e1.i2);
+        curv(t, e0, a0, a1, a2);
+        curv(t + TPB, e1, b0, b1, b2);
       }
-      amin = fmin(amin, inversion_root(x1 - x0, x2 - x0, q1 - q0, q2 - q0, p.area_shrink));
+      if (t < t1) {
+        const Elem<R> e0 = load_elem<R>(ep, t);
+        curv(t, e0, ld2(s.p, e0.i0), ld2(s.p, e0.i1), ld2(s.p, e0.i2));
+      }
+    } else {
+      for (int t = t0 + tid; t < t1; t += TPB) {
+        const int i0 = s.sc_tri[3 * t], i1 = s.sc_tri[3 * t + 1], i2 = s.sc_tri[3 * t + 2];
+        const V2<double> x0 = X(i0), x1 = X(i1), x2 = X(i2);
+        const V2<double> q0 = ld2(s.p, i0), q1 = ld2(s.p, i1), q2 = ld2(s.p, i2);
+        if (t < tsoft) {
+          const Elem<R> el = load_elem<R>(s, t);
+          const NH<R> h = elem_F(el, x0, x1, x2);
+          const M2<R> dDs = {R(q1.x - q0.x), R(q2.x - q0.x), R(q1.y - q0.y), R(q2.y - q0.y)};
+          const M2<R> dF = mat_mul(dDs, el.B);
+          ph += h2 * fmax(double(nh_quad(h, dF, el.Amu, el.Alam)), 0.0);
+        }
+        amin = fmin(amin, inversion_root(x1 - x0, x2 - x0, q1 - q0, q2 - q0, p.area_shrink));
+      }
     }
     // colliders: curvature, clearance, friction curvature, step cap
     for (int j = tid; j < nb; j += TPB) {
