@@ -1394,10 +1394,10 @@ struct Step {
       int t = t0 + tid;
       for (; t + TPB < t1; t += 2 * TPB) {
         const Elem<R> e0 = load_elem<R>(ep, t), e1 = load_elem<R>(ep, t + TPB);
-        const V2<double> a0 = ld2(s.p, e0.i0), a1 = ld2(s.p, e0.i1), a2 = ld2(s.p, e0.i2);
-        const V2<double> b0 = ld2(s.p, e1.i0), b1 = ld2(s.p, e1.i1), b2 = ld2(s.p, e1.i2);
-        curv(t, e0, a0, a1, a2);
-        curv(t + TPB, e1, b0, b1, b2);
+        const V2<double> u0 = ld2(s.p, e0.i0), u1 = ld2(s.p, e0.i1), u2 = ld2(s.p, e0.i2);
+        const V2<double> w0 = ld2(s.p, e1.i0), w1 = ld2(s.p, e1.i1), w2 = ld2(s.p, e1.i2);
+        curv(t, e0, u0, u1, u2);
+        curv(t + TPB, e1, w0, w1, w2);
       }
       if (t < t1) {
         const Elem<R> e0 = load_elem<R>(ep, t);
