@@ -110,7 +110,8 @@ def test_node_steps_fixed_iterations(path, fric):
     assert (gi.stats(mg)["iters"] == oi.stats()["iters"]).all()
 
 
-@pytest.mark.parametrize("case", ["byte_exclusion", "rigid_block", "line_search", "converged"])
+@pytest.mark.parametrize("case", ["byte_exclusion", "rigid_block", "line_search", "rigid_line_search",
+                                  "converged"])
 def test_node_scene_variants(case):
     kw = dict(max_iter=60, eps=0.0, conv_check=0)
     make = lambda: node_stack(n_env=2, mu=0.1, tilt=3.0)
@@ -121,6 +122,9 @@ def test_node_scene_variants(case):
         make = lambda: node_stack(n_env=2, mu=0.05, tilt=3.0, rigid=True)
     elif case == "line_search":
         kw["line_search"] = True
+    elif case == "rigid_line_search":       # backtracking restarts the affine block from q0
+        make = lambda: node_stack(n_env=2, mu=0.05, tilt=3.0, rigid=True)
+        kw.update(line_search=True, n_backtrack=4)
     else:
         kw = {}
     oi, gi, mo, mg = _run_pair(make, 10, 1e-3, "scene", kw, override=override)

@@ -168,6 +168,20 @@ def test_grip_steps_fixed_iterations(path):
     assert err < 1e-9
 
 
+@pytest.mark.parametrize("path", ["plain", "scene"])
+def test_grip_line_search(path):
+    """The grip with the energy line search on: backtracking restarts the rigid block from its
+    coordinates at the start of the iteration."""
+    kw = dict(max_iter=60, eps=0.0, conv_check=0, line_search=True, n_backtrack=4)
+
+    def each(m, pads, k):
+        close_jaws(m, pads, k, n_close=8)
+    oi, gi, mo, mg = _run_pair(lambda: grip(n_env=2), 12, 1e-3, path, kw, each)
+    err = np.abs(mg.get_x() - oi.solver.x).max() / 8e-3
+    print(f"grip line search {path}: max |dx| / size = {err:.3e}")
+    assert err < 1e-9
+
+
 def test_scene_mixed_strip_close_to_float64():
     """Mixed precision (float32 element arithmetic) on the strip: positions within 1e-5 of the
     body size of the float64 oracle after the same fixed iterations."""
