@@ -1798,6 +1798,8 @@ extern "C" int d2f_step_scene(const d2f_scene *s, const d2f_params *p, void *str
             "d2f_step_scene: the scene has no per-scene layout");
   D2F_CHECK(s->max_cells >= 1 && s->max_nb >= 0, "d2f_step_scene: bad grid capacity");
   D2F_CHECK(s->excl_bits_lds >= 0 && s->max_pairs >= 0, "d2f_step_scene: negative LDS capacity");
+  D2F_CHECK(s->n_ag == 0 || (s->max_ag > 0 && s->ag_start),
+            "d2f_step_scene: affine groups need ag_start and max_ag >= every scene's group count");
   D2F_CHECK(s->excl_bits_lds == 0 || (s->excl_bits && s->excl_bits_off),
             "d2f_step_scene: excl_bits_lds set without the packed exclusion bitset");
   D2F_CHECK(s->max_pairs == 0 || s->max_nb < 65536,

@@ -149,8 +149,8 @@ typedef struct d2f_scene {
   d2f_elem_tiles el;
   /* springs and pins [n_sp], grouped by environment (reference implicit/actuation.py:368-476):
    * E = k/2 (|x_a - x_b| - L0)^2 with x_b = sp_anchor when sp_b < 0 (a spring to the world);
-   * max_ag: largest per-scene affine group count (the scene kernel keeps their coordinates,
-   * direction and moduli in LDS, 26 doubles per group) */
+   * max_ag: largest per-scene affine group count, required (> 0) when n_ag > 0 (the scene
+   * kernel keeps their coordinates, direction and moduli in LDS, 26 doubles per group) */
   int32_t n_sp, max_ag;
   const int32_t *sp_a, *sp_b, *sp2env;  /* n_sp */
   const double *sp_anchor;              /* 2 n_sp */

@@ -48,3 +48,18 @@ def test_scene_device_mirror_is_current():
     assert len(names) == len(set(names))
     assert {"x", "v", "tri", "el", "excl_bits", "max_pairs", "xc_lds"} <= set(names)
     assert sum(1 for f in _lib.Scene._fields_) == len(names)
+
+
+def test_step_scene_rejects_affine_groups_without_lds_capacity():
+    """d2f_step_scene checks its arguments before any GPU call: a scene with affine groups but
+    no max_ag (the LDS the scene kernel keeps their state in) is refused with a message."""
+    L = _lib.lib()
+    s, p = _lib.Scene(), _lib.Params()
+    dummy = ctypes.create_string_buffer(64)             # never dereferenced by the host checks
+    addr = ctypes.cast(dummy, ctypes.c_void_p).value
+    s.B, s.N, s.max_cells = 1, 3, 1
+    s.sc_tri = s.sc_cs = s.sc_tri_start = s.ag_start = addr
+    s.n_ag, s.max_ag = 1, 0
+    rc = L.d2f_step_scene(ctypes.byref(s), ctypes.byref(p), None)
+    assert rc != 0
+    assert "max_ag" in L.d2f_last_error().decode()
