@@ -1526,12 +1526,11 @@ struct Step {
                     [&](int i, const V2<double> &x) { xput(i, x); });
   }
 
-  __device__ __forceinline__ void affine_to(int which, double a) {  // 1: q += a pq ; 2: q = q0 + a pq
+  // line search: q = q0 + a pq, then the slaved nodes from q
+  __device__ __forceinline__ void affine_from_q0(double a) {
     if (tid == 0)
       for (int g = g0; g < g1; ++g)
-        for (int r = 0; r < 6; ++r) {
-          gq_(g)[r] = (which == 1 ? gq_(g)[r] : gq0_(g)[r]) + a * gpq_(g)[r];
-        }
+        for (int r = 0; r < 6; ++r) gq_(g)[r] = gq0_(g)[r] + a * gpq_(g)[r];
     __syncthreads();
     for (int g = g0; g < g1; ++g) expand(g, gq_(g), s.x);
     __syncthreads();
@@ -1614,9 +1613,17 @@ struct Step {
     if (!sc.active) return;  // uniform: sc is shared
     const double a = sc.alpha;
     if (!p.line_search) {
+      // q += a pq touches only the groups' LDS state, which set_x does not read: thread 0 does
+      // it first, and the slaved nodes are expanded from the new q after set_x's barrier.
+      if (tid == 0)
+        for (int g = g0; g < g1; ++g)
+          for (int r = 0; r < 6; ++r) gq_(g)[r] = gq_(g)[r] + a * gpq_(g)[r];
       set_x(s.x, a);
       __syncthreads();
-      if (g1 > g0) affine_to(1, a);
+      if (g1 > g0) {
+        for (int g = g0; g < g1; ++g) expand(g, gq_(g), s.x);
+        __syncthreads();
+      }
       return;
     }
     // energy backtracking gate (reference implicit/solver.py:403-425)
@@ -1629,14 +1636,14 @@ struct Step {
     for (int bt = 0; bt < p.n_backtrack; ++bt) {
       set_x(s.x0, sc.alpha);
       __syncthreads();
-      if (g1 > g0) affine_to(2, sc.alpha);
+      if (g1 > g0) affine_from_q0(sc.alpha);
       const double E1 = energy();
       if (tid == 0 && E1 > E0 + p.ls_tol * (fabs(E0) + 1.0)) sc.alpha *= 0.5;
       __syncthreads();
     }
     set_x(s.x0, sc.alpha);
     __syncthreads();
-    if (g1 > g0) affine_to(2, sc.alpha);
+    if (g1 > g0) affine_from_q0(sc.alpha);
   }
 
   // ------------------------------------------------------------------ the step
