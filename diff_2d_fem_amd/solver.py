@@ -142,6 +142,7 @@ class DeviceSolver:
             raise RuntimeError("boundary edges are not grouped by environment")
         self.e_start = t(np.searchsorted(env, np.arange(self.B + 1)), I32)
         self.max_nb = int(np.max(np.asarray(me.boundary_per_env))) if self.B else 0
+        self._layout = None                 # the scene layout plans LDS for the contact lists
 
     def setup_node(self, stiffness, radius, mu, mu_node=None, friction=False):
         """The "node" penalty contact between boundary nodes of one environment, with Coulomb
@@ -211,6 +212,7 @@ class DeviceSolver:
         self.ag_q, self.ag_q0, self.ag_pq, self.ag_gq, self.ag_gq_prev, self.ag_zq = (z(6) for _ in range(6))
         self.ag_Hx, self.ag_Hy = z(9), z(9)
         self.sync_affine(None)
+        self._layout = None                 # the scene layout sorts rigid triangles and sizes their LDS
 
     def sync_affine(self, regions=None):
         """q from the current node positions by least squares (reference implicit/affine.py:331-344).
@@ -282,6 +284,7 @@ class DeviceSolver:
         self.rn_start = t(np.searchsorted(grp, np.arange(n + 1)), I32)
         z = lambda k: torch.zeros((n, k), dtype=F64, device=self.dev)
         self.rg_acc, self.rg_gq, self.rg_gq_prev, self.rg_zq, self.rg_H = z(3), z(3), z(3), z(3), z(9)
+        self._layout = None
 
     # ------------------------------------------------------------------ actuation
     def set_fext(self, f):
