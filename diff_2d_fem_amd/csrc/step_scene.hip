@@ -109,47 +109,56 @@ struct Block {
   template <bool MIN>
   static __device__ __forceinline__ double op(double a, double b) { return MIN ? fmin(a, b) : a + b; }
 
-  template <bool MIN, int K>
+  // v[k] summed for k < NS, minimum for k >= NS
+  template <int NS, int K>
   __device__ __forceinline__ void reduce(double (&v)[K]) {
     static_assert(K <= RED_K, "reduction wider than the LDS scratch");
-    const double idle = MIN ? __builtin_huge_val() : 0.0;
     const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
     if constexpr (K <= 2) {
 #pragma unroll
-      for (int k = 0; k < K; ++k) v[k] = MIN ? wave_min_dpp(v[k]) : wave_sum_dpp(v[k]);
+      for (int k = 0; k < K; ++k) v[k] = k >= NS ? wave_min_dpp(v[k]) : wave_sum_dpp(v[k]);
       __syncthreads();
       if (l == 0)
 #pragma unroll
         for (int k = 0; k < K; ++k) red[w * K + k] = v[k];
       __syncthreads();
 #pragma unroll
-      for (int k = 0; k < K; ++k) v[k] = readlane_f64(row_op<MIN>(l < NW ? double(red[l * K + k]) : idle), 0);
+      for (int k = 0; k < K; ++k)
+        v[k] = k >= NS ? readlane_f64(row_min(l < NW ? double(red[l * K + k]) : __builtin_huge_val()), 0)
+                       : readlane_f64(row_sum(l < NW ? double(red[l * K + k]) : 0.0), 0);
     } else {
 #pragma unroll
-      for (int k = 0; k < K; ++k) v[k] = row_op<MIN>(v[k]);
+      for (int k = 0; k < K; ++k) v[k] = k >= NS ? row_min(v[k]) : row_sum(v[k]);
       __syncthreads();
       if ((l & 15) == 0)
 #pragma unroll
         for (int k = 0; k < K; ++k) red[k * P + 4 * w + (l >> 4)] = v[k];
       __syncthreads();
       LdsD *tot = red + RED_K * P;
-      for (int k = w; k < K; k += NW) {
-        const double t = row_op<MIN>(l < P ? double(red[k * P + l]) : idle);
-        double r = readlane_f64(t, 0);
-        if constexpr (P > 16) r = op<MIN>(r, readlane_f64(t, 16));
-        if constexpr (P > 32) r = op<MIN>(r, op<MIN>(readlane_f64(t, 32), readlane_f64(t, 48)));
-        if (l == 0) tot[k] = r;
+      for (int k = w; k < K; k += NW) {   // uniform over the wave
+        if (k >= NS) tot_row<true>(tot, k, l);
+        else tot_row<false>(tot, k, l);
       }
       __syncthreads();
 #pragma unroll
       for (int k = 0; k < K; ++k) v[k] = tot[k];
     }
   }
+  // the total of value k from its P row partials, parked by lane 0
+  template <bool MIN>
+  __device__ __forceinline__ void tot_row(LdsD *tot, int k, int l) {
+    const double idle = MIN ? __builtin_huge_val() : 0.0;
+    const double t = row_op<MIN>(l < P ? double(red[k * P + l]) : idle);
+    double r = readlane_f64(t, 0);
+    if constexpr (P > 16) r = op<MIN>(r, readlane_f64(t, 16));
+    if constexpr (P > 32) r = op<MIN>(r, op<MIN>(readlane_f64(t, 32), readlane_f64(t, 48)));
+    if (l == 0) tot[k] = r;
+  }
   template <int K>
-  __device__ void sum(double (&v)[K]) { reduce<false>(v); }
+  __device__ void sum(double (&v)[K]) { reduce<K>(v); }
   // component-wise minimum
   template <int K>
-  __device__ void min(double (&v)[K]) { reduce<true>(v); }
+  __device__ void min(double (&v)[K]) { reduce<0>(v); }
   __device__ double sum1(double a) {
     double v[1] = {a};
     sum<1>(v);
@@ -1456,10 +1465,11 @@ struct Step {
       ipc_c = cm;
       amin = fmin(amin, am);
     }
-    double vs[1] = {ph};
-    blk.template sum<1>(vs);
-    double vm[4] = {-pinf, amin, cmin, ipc_c};
-    blk.template min<4>(vm);
+    // one reduction: the sum of pHp, then the minima of -p_inf, alpha, clearance, IPC clearance
+    double vr[5] = {ph, -pinf, amin, cmin, ipc_c};
+    blk.template reduce<1>(vr);
+    const double vs[1] = {vr[0]};
+    const double vm[4] = {vr[1], vr[2], vr[3], vr[4]};
     if (tid == 0) {
       double pH = vs[0];
       const double p_inf = -vm[0];
