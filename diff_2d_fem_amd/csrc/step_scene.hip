@@ -41,6 +41,7 @@ struct SceneScalars {
   double gx0, gy0, gh;          // broadphase grid origin and cell size
   int gnx, gny, active, iters, pending, stop;
   int npairs, plist, excl_lds;  // candidate pairs of the last build, list valid, bitset in LDS
+  int stale;                    // stamp of the last contact_prepare that found a node past skin/2
 };
 using LdsScalars = __attribute__((address_space(3))) SceneScalars;
 
@@ -299,6 +300,7 @@ struct Step {
   LdsI *citem;    // nb
   LdsI *ccur;     // max_cells
   LdsI *lbnd;     // the scene's boundary nodes (bnd[b0 ..]), copied once per step
+  int prep_stamp = 0;   // calls of contact_prepare so far (the same in every thread)
   LdsU *xbits;    // exclusion bitset (excl_bits_lds words)
   LdsI *plist;    // max_pairs packed (edge - k0) << 16 | jp
   int e, tid;
@@ -406,17 +408,19 @@ struct Step {
   // Contact distance the broadphase serves: the IPC reach or the "node" contact radius.
   __device__ __forceinline__ double creach() const { return p.node_on ? p.node_r : p.ipc_reach; }
 
+  // A node that moved more than half the skin marks the list stale with this call's stamp
+  // (no reduction needed: one barrier, then every thread reads the same flag).
   __device__ __forceinline__ void contact_prepare() {
     const bool have = sc.plist;
-    double dmax = 0.0;
+    const int stamp = ++prep_stamp;
     for (int j = tid; j < nb; j += TPB) {
       const V2<double> x = X(lbnd[j]);
       bpos[2 * j] = x.x;
       bpos[2 * j + 1] = x.y;
-      if (have) dmax = fmax(dmax, len(V2<double>{x.x - bref[2 * j], x.y - bref[2 * j + 1]}));
+      if (have && 2.0 * len(V2<double>{x.x - bref[2 * j], x.y - bref[2 * j + 1]}) > p.ipc_skin) sc.stale = stamp;
     }
-    dmax = blk.max1(dmax);
-    if (!have || !(2.0 * dmax <= p.ipc_skin)) {
+    __syncthreads();
+    if (!have || sc.stale == stamp) {
       build_grid();
       if (p.node_on) {  // node pairs are found in the grid itself; plist marks it current
         if (tid == 0) sc.plist = 1;
@@ -1661,6 +1665,7 @@ struct Step {
     // exclusion bitset of this scene into LDS when it fits
     if (tid == 0) {
       sc.excl_lds = 0;
+      sc.stale = 0;
       sc.plist = 0;
     }
     __syncthreads();
