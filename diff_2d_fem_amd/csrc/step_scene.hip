@@ -1094,7 +1094,7 @@ struct Step {
         }
       }
     }
-    __syncthreads();
+    if (td1 > td0 || sp1 > sp0) __syncthreads();   // the scene's counts: uniform over the workgroup
     // affine bodies: J^T g, J^T D J per group, then the slaved nodes leave the nodal iteration
     // (reference implicit/affine.py:157-175)
     for (int g = g0; g < g1; ++g) {
@@ -1296,7 +1296,7 @@ struct Step {
         st2(s.p, i, ld2(s.p, i) - V2<double>{z[0] - z[2] * rr.y, z[1] + z[2] * rr.x});
       }
     }
-    __syncthreads();
+    if (g1 > g0 || r1 > r0) __syncthreads();   // without groups nothing was written since the last
   }
 
   __device__ __forceinline__ void expand(int g, const LdsD *v, GD *out) {
