@@ -1739,8 +1739,7 @@ struct Step {
       grad_diag();
       direction(it);
       curvature_caps(it);
-      update();
-      __syncthreads();
+      update();   // ends on a barrier (or returns right after curvature_caps's last one)
     }
     for (int i = n0 + tid; i < n1; i += TPB) st2(s.v, i, p.inv_dt * (X(i) - ld2(s.xn, i)));
     for (int j = tid; j < 18 * (g1 - g0); j += TPB) {
