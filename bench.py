@@ -297,8 +297,9 @@ def cpu_baseline(args, steps):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
-              f"environments shard across GPUs with no communication, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU: there is no multi-GPU path (asked for --gpus {args.gpus}); "
+              f"environments are independent, so a run on N GPUs would give each GPU its own scenes "
+              f"with no communication", file=sys.stderr)
         sys.exit(2)
     import torch
     from diff_2d_fem_amd import _lib
