@@ -664,7 +664,7 @@ class DeviceSolver:
             _lib.check(L.d2f_step_plain(C.byref(s), C.byref(q), stream), "d2f_step_plain")
         else:
             if not hasattr(L, "d2f_step_scene"):
-                raise RuntimeError("this build of libd2fem_hip.so has no scene-persistent path")
+                raise RuntimeError("libd2fem_hip.so has no d2f_step_scene (the scene-persistent path)")
             rc = L.d2f_step_scene(C.byref(s), C.byref(q), stream)
             if rc == _lib.D2F_ECAP:
                 # a scene too large for one workgroup's LDS: the per-kernel path takes any size

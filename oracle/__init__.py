@@ -2,7 +2,7 @@
 
 A float64 numpy restatement of the reference algorithm (Diff_2D_FEM implicit/solver.py,
 implicit/kernels.py, core/math.py, core/sdf.py, implicit/ipc_contact.py, implicit/affine.py,
-implicit/actuation.py, implicit/integrator.py, implicit/adjoint.py), used as the checker the
+implicit/actuation.py, implicit/integrator.py, implicit/adjoint.py), the reference results the
 GPU path is compared against. Only tests/, __graft_entry__.smoke() and the cpu_baseline leg
 of bench.py may import it; the product package never does.
 

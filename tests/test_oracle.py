@@ -1,4 +1,4 @@
-"""Pinning the float64 CPU oracle (test infrastructure) before it is trusted as the checker.
+"""Pinning the float64 CPU oracle (test infrastructure) before the GPU path is compared with it.
 
 Every function of the oracle is fixed by at least one check that a plausible mistake (a dropped
 term, a wrong sign or index, a transposed operand) would fail:
