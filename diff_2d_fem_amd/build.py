@@ -42,26 +42,38 @@ def gen_headers():
 
 
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall"]
+# sources compiled as several objects, part k with -DD2F_SCENE_PART=k (the scene kernel: its
+# float and float64 instantiations)
+SPLIT = {"step_scene.hip": 2}
 
 
 def build(verbose=False):
     """Compile and link; returns the path of the library. Raises on a compiler error. csrc/*.hip
-    go through hipcc for gfx950, csrc/*.cpp (host-only code) through the host compiler."""
+    go through hipcc for gfx950, csrc/*.cpp (host-only code) through the host compiler; the
+    sources compile in parallel (at most 8 at a time)."""
+    from concurrent.futures import ThreadPoolExecutor
     gen_headers()
-    srcs = sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
-    objs = []
-    for src in srcs:
-        obj = CSRC / (src.stem + ".o")
+    jobs = []
+    for src in sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp")):
+        parts = SPLIT.get(src.name, 1)
+        jobs += [(src, k, parts) for k in range(parts)]
+
+    def compile_one(job):
+        src, k, parts = job
+        obj = CSRC / (src.stem + (f".{k}" if parts > 1 else "") + ".o")
         if src.suffix == ".cpp":
             cmd = [os.environ.get("CXX", "g++"), *HOST_FLAGS, "-c", str(src), "-o", str(obj)]
         else:
-            cmd = [hipcc(), *FLAGS, "-c", str(src), "-o", str(obj)]
+            part = [f"-DD2F_SCENE_PART={k}"] if parts > 1 else []
+            cmd = [hipcc(), *FLAGS, *part, "-c", str(src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
-            raise RuntimeError(f"hipcc failed on {src.name}:\n{r.stderr}")
-        objs.append(obj)
+            raise RuntimeError(f"compiler failed on {src.name}:\n{r.stderr}")
+        return obj
+    with ThreadPoolExecutor(max_workers=min(8, max(1, len(jobs)))) as ex:
+        objs = list(ex.map(compile_one, jobs))
     tmp = OUT.with_suffix(".so.tmp")
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -81,7 +81,9 @@ __device__ __forceinline__ void atomic_max_d(double *a, double v) {
 template <typename R> struct NH {
   M2<R> F;   // deformation gradient
   M2<R> G;   // F^-T
+  M2<R> D;   // F - F^-T (the gradient's mu term)
   R logJ;
+  R lL;      // lam log J
   R c;       // lam log J - mu
 };
 
@@ -92,8 +94,35 @@ __device__ __forceinline__ NH<R> nh_eval(M2<R> F, R mu, R lam) {
   const R J = F.a * F.d - F.b * F.c;
   const R iJ = R(1) / J;
   h.G = {F.d * iJ, -F.c * iJ, -F.b * iJ, F.a * iJ};
+  h.D = {F.a - h.G.a, F.b - h.G.b, F.c - h.G.c, F.d - h.G.d};
   h.logJ = log(J > R(1.0e-6) ? J : R(1.0e-6));
-  h.c = lam * h.logJ - mu;
+  h.lL = lam * h.logJ;
+  h.c = h.lL - mu;
+  return h;
+}
+
+// The same from the displacement gradient H = F - I, for float arithmetic near the rest state.
+// Formed from F directly, J, log J and F - F^-T are differences of O(1) terms, so a float
+// keeps only about 6e-8 / strain of their relative accuracy. From H, with s = J - 1 =
+// tr H + det H:
+//   log J = log1p(s)
+//   (F - F^-T)11 = (2 H11 + det H + s H11) / J,  (F - F^-T)22 = (2 H22 + det H + s H22) / J,
+//   (F - F^-T)12 = H12 + H21 / J,                 (F - F^-T)21 = H21 + H12 / J,
+// every term small with the strain, so the stress keeps the float's relative accuracy.
+template <typename R>
+__device__ __forceinline__ NH<R> nh_eval_strain(M2<R> H, R mu, R lam) {
+  NH<R> h;
+  h.F = {R(1) + H.a, H.b, H.c, R(1) + H.d};
+  const R detH = H.a * H.d - H.b * H.c;
+  const R sJ = (H.a + H.d) + detH;
+  const R J = R(1) + sJ;
+  const R iJ = R(1) / J;
+  h.G = {h.F.d * iJ, -H.c * iJ, -H.b * iJ, h.F.a * iJ};
+  h.D = {(R(2) * H.a + detH + sJ * H.a) * iJ, H.b + H.c * iJ, H.c + H.b * iJ,
+         (R(2) * H.d + detH + sJ * H.d) * iJ};
+  h.logJ = J > R(1.0e-6) ? log1p(sJ) : log(R(1.0e-6));
+  h.lL = lam * h.logJ;
+  h.c = h.lL - mu;
   return h;
 }
 
@@ -122,7 +151,14 @@ template <typename R>
 __device__ __forceinline__ void nh_grad(const NH<R> &h, M2<R> B, R mu, R A, V2<R> &g0, V2<R> &g1,
                                         V2<R> &g2) {
   const M2<R> &F = h.F, &G = h.G;
-  const M2<R> P = {mu * F.a + h.c * G.a, mu * F.b + h.c * G.b, mu * F.c + h.c * G.c, mu * F.d + h.c * G.d};
+  M2<R> P;
+  if constexpr (sizeof(R) == 4) {
+    // float: P = mu (F - F^-T) + lam log J F^-T, from the cancellation-free F - F^-T
+    const R l = h.lL;
+    P = {mu * h.D.a + l * G.a, mu * h.D.b + l * G.b, mu * h.D.c + l * G.c, mu * h.D.d + l * G.d};
+  } else {
+    P = {mu * F.a + h.c * G.a, mu * F.b + h.c * G.b, mu * F.c + h.c * G.c, mu * F.d + h.c * G.d};
+  }
   // (P B^T)_ij = P_i . B_j (rows)
   g1 = {A * (P.a * B.a + P.b * B.b), A * (P.c * B.a + P.d * B.b)};
   g2 = {A * (P.a * B.c + P.b * B.d), A * (P.c * B.c + P.d * B.d)};

@@ -1,14 +1,18 @@
 // step_scene.hip — the scene-persistent path: one workgroup owns one environment and runs the
 // whole implicit step (every PNCG iteration of reference implicit/solver.py:252-451) inside one
-// launch. Per-environment scalars live in LDS; reductions are fixed-order block reductions; the
-// elastic scatter is colour-by-colour read-modify-write (no atomics, deterministic); contact
-// terms accumulate per boundary node in LDS; the node-edge broadphase is a per-scene uniform
-// grid built in LDS. An environment leaves the loop at the first convergence check it is
-// converged at — the state it leaves is the one the reference's global loop would leave it in,
-// because a converged environment's positions no longer change (implicit/kernels.py:375-434).
+// launch. Per-environment scalars live in LDS; reductions are fixed-order block reductions. The
+// elastic scatter adds into per-node LDS accumulators with float64 LDS atomics when the scene's
+// nodes fit (the order of those additions, and with it the last bits of a sum, can differ
+// between runs), else colour by colour with plain read-modify-write of the node arrays (no two
+// triangles of a colour share a node). Contact terms accumulate per boundary node in LDS; the
+// node-edge broadphase is a per-scene uniform grid and Verlet pair list built in LDS. An
+// environment leaves the loop at the first convergence check it is converged at — the state it
+// leaves is the one the reference's global loop would leave it in, because a converged
+// environment's positions no longer change (implicit/kernels.py:375-434).
 //
 // Template R is the element arithmetic: double (float64 path) or float (mixed: float32 element
-// stress and curvature from float64 positions; float64 state, dot products and energies).
+// stress and curvature from a displacement gradient formed in float64; float64 state, dot
+// products and energies).
 #include "common.h"
 #include "node_contact.h"
 #include "scene_dev.h"
@@ -260,11 +264,55 @@ __device__ __forceinline__ Elem<float> load_elem<float>(S &s, int t) {
   return el;
 }
 
-// F of an element from float64 positions: edge vectors formed in float64, then rounded.
-template <typename R>
+// Neo-Hookean state of an element from float64 positions. float64: F = Ds Bm.
+// Mixed: F is formed in float64 and split into a rotation and a symmetric stretch, F = Q S
+// (2-D polar decomposition: Q turns by atan2(F21 - F12, F11 + F22), which makes Q^T F
+// symmetric). Only the stretch's displacement gradient S - I, small with the strain whatever
+// the rotation, is rounded to float; the float stress is formed from it without cancellation
+// (nh_eval_strain) and turned back by Q (the material is isotropic: P(Q S) = Q P(S)). The
+// float stress so keeps its relative accuracy for small strains under any rotation.
+// ROT = false skips the rotation (H = F - I directly): enough for the curvature passes,
+// whose accuracy sets step lengths only.
+template <typename R, bool ROT = true>
 __device__ __forceinline__ NH<R> elem_F(const Elem<R> &el, V2<double> p0, V2<double> p1, V2<double> p2) {
-  const M2<R> Ds = {R(p1.x - p0.x), R(p2.x - p0.x), R(p1.y - p0.y), R(p2.y - p0.y)};
-  return nh_eval(mat_mul(Ds, el.B), el.Amu, el.Alam);
+  const M2<double> Ds = {p1.x - p0.x, p2.x - p0.x, p1.y - p0.y, p2.y - p0.y};
+  if constexpr (sizeof(R) == 4) {
+    const M2<double> Bd = {double(el.B.a), double(el.B.b), double(el.B.c), double(el.B.d)};
+    const M2<double> Fd = mat_mul(Ds, Bd);
+    if constexpr (!ROT) {
+      const M2<float> H = {float(Fd.a - 1.0), float(Fd.b), float(Fd.c), float(Fd.d - 1.0)};
+      return nh_eval_strain(H, el.Amu, el.Alam);
+    } else {
+      const double tc = Fd.a + Fd.d, ts = Fd.c - Fd.b;
+      const double r2 = tc * tc + ts * ts;
+      const double ir = r2 > 1.0e-60 ? rsqrt(r2) : 0.0;
+      const double cq = r2 > 1.0e-60 ? tc * ir : 1.0, sq = ts * ir;
+      // S = Q^T F
+      const M2<float> H = {float(cq * Fd.a + sq * Fd.c - 1.0), float(cq * Fd.b + sq * Fd.d),
+                           float(cq * Fd.c - sq * Fd.a), float(cq * Fd.d - sq * Fd.b - 1.0)};
+      NH<float> h = nh_eval_strain(H, el.Amu, el.Alam);
+      const float c = float(cq), s = float(sq);
+      const auto turn = [c, s](M2<float> A) -> M2<float> {
+        return {c * A.a - s * A.c, c * A.b - s * A.d, s * A.a + c * A.c, s * A.b + c * A.d};
+      };
+      h.F = turn(h.F);
+      h.G = turn(h.G);
+      h.D = turn(h.D);
+      return h;
+    }
+  } else {
+    return nh_eval(mat_mul(Ds, el.B), el.Amu, el.Alam);
+  }
+}
+
+// Elastic energy density times area of an element, in float64 whatever the element arithmetic
+// (the line search compares energies; the float constants are promoted).
+template <typename R>
+__device__ __forceinline__ double elem_psi(const Elem<R> &el, V2<double> p0, V2<double> p1, V2<double> p2) {
+  const M2<double> Ds = {p1.x - p0.x, p2.x - p0.x, p1.y - p0.y, p2.y - p0.y};
+  const M2<double> Bd = {double(el.B.a), double(el.B.b), double(el.B.c), double(el.B.d)};
+  const double mu = double(el.Amu), lam = double(el.Alam);
+  return nh_psi(nh_eval(mat_mul(Ds, Bd), mu, lam), mu, lam);
 }
 
 template <typename R>
@@ -1397,7 +1445,7 @@ struct Step {
       const auto curv = [&](int t, const Elem<R> &el, V2<double> q0, V2<double> q1, V2<double> q2) {
         const V2<double> x0 = X(el.i0), x1 = X(el.i1), x2 = X(el.i2);
         if (t < tsoft) {
-          const NH<R> h = elem_F(el, x0, x1, x2);
+          const NH<R> h = elem_F<R, false>(el, x0, x1, x2);
           const M2<R> dDs = {R(q1.x - q0.x), R(q2.x - q0.x), R(q1.y - q0.y), R(q2.y - q0.y)};
           const M2<R> dF = mat_mul(dDs, el.B);
           ph += h2 * fmax(double(nh_quad(h, dF, el.Amu, el.Alam)), 0.0);
@@ -1423,7 +1471,7 @@ struct Step {
         const V2<double> q0 = ld2(s.p, i0), q1 = ld2(s.p, i1), q2 = ld2(s.p, i2);
         if (t < tsoft) {
           const Elem<R> el = load_elem<R>(s, t);
-          const NH<R> h = elem_F(el, x0, x1, x2);
+          const NH<R> h = elem_F<R, false>(el, x0, x1, x2);
           const M2<R> dDs = {R(q1.x - q0.x), R(q2.x - q0.x), R(q1.y - q0.y), R(q2.y - q0.y)};
           const M2<R> dF = mat_mul(dDs, el.B);
           ph += h2 * fmax(double(nh_quad(h, dF, el.Amu, el.Alam)), 0.0);
@@ -1562,8 +1610,7 @@ struct Step {
     const int tsoft = s.sc_cs[c0 + MAXC];
     for (int t = t0 + tid; t < tsoft; t += TPB) {
       const Elem<R> el = load_elem<R>(s, t);
-      const NH<R> h = elem_F(el, X(el.i0), X(el.i1), X(el.i2));
-      E += h2 * double(nh_psi(h, el.Amu, el.Alam));
+      E += h2 * elem_psi(el, X(el.i0), X(el.i1), X(el.i2));
     }
     if (p.node_on && nb > 0) contact_prepare();
     for (int j = tid; j < nb; j += TPB) {
@@ -1809,8 +1856,30 @@ static int launch(const SH &s, const Pm &p, hipStream_t stream) {
   return 0;
 }
 
+// The float and float64 kernels are compiled as two objects of this file (D2F_SCENE_PART 0 and
+// 1), so that the build compiles them in parallel.
+#ifndef D2F_SCENE_PART
+#define D2F_SCENE_PART 0
+#endif
+int scene_launch_f32(const SH &s, const Pm &p, int tpb, hipStream_t st);
+int scene_launch_f64(const SH &s, const Pm &p, int tpb, hipStream_t st);
+#if D2F_SCENE_PART == 0
+int scene_launch_f32(const SH &s, const Pm &p, int tpb, hipStream_t st) {
+  if (tpb == 512) return launch<float, 512>(s, p, st);
+  if (tpb == 1024) return launch<float, 1024>(s, p, st);
+  return launch<float, 256>(s, p, st);
+}
+#else
+int scene_launch_f64(const SH &s, const Pm &p, int tpb, hipStream_t st) {
+  if (tpb == 512) return launch<double, 512>(s, p, st);
+  if (tpb == 1024) return launch<double, 1024>(s, p, st);
+  return launch<double, 256>(s, p, st);
+}
+#endif
+
 }  // namespace d2f
 
+#if D2F_SCENE_PART == 0
 using namespace d2f;
 
 extern "C" int d2f_step_scene(const d2f_scene *s, const d2f_params *p, void *stream) {
@@ -1832,12 +1901,9 @@ extern "C" int d2f_step_scene(const d2f_scene *s, const d2f_params *p, void *str
   const int tpb = p->block_size > 0 ? p->block_size : 256;
   if (p->precision == 1) {
     D2F_CHECK(s->sc_Bm32 && s->sc_Amu32 && s->sc_Alam32, "d2f_step_scene: mixed precision needs float32 constants");
-    if (tpb == 512) return launch<float, 512>(*s, *p, st);
-    if (tpb == 1024) return launch<float, 1024>(*s, *p, st);
-    return launch<float, 256>(*s, *p, st);
+    return scene_launch_f32(*s, *p, tpb, st);
   }
   D2F_CHECK(s->sc_Bm && s->sc_Amu && s->sc_Alam, "d2f_step_scene: float64 constants missing");
-  if (tpb == 512) return launch<double, 512>(*s, *p, st);
-  if (tpb == 1024) return launch<double, 1024>(*s, *p, st);
-  return launch<double, 256>(*s, *p, st);
+  return scene_launch_f64(*s, *p, tpb, st);
 }
+#endif

@@ -217,164 +217,219 @@ def finish_body(rest_pos, tri, k_excl, excl_radius=0.0):
 
 
 # ---------------------------------------------------------------------------- mesh bodies
-def _crossing_segments(Va, Ea, Vb, Eb):
-    """Index pairs of segments that cross properly (each strictly separates the other's
-    ends); with Va is Vb, pairs sharing a vertex are skipped."""
-    from scipy.spatial import cKDTree
-    A0, A1, B0, B1 = Va[Ea[:, 0]], Va[Ea[:, 1]], Vb[Eb[:, 0]], Vb[Eb[:, 1]]
-    La, Lb = np.linalg.norm(A1 - A0, axis=1), np.linalg.norm(B1 - B0, axis=1)
-    near = cKDTree(0.5 * (B0 + B1)).query_ball_point(0.5 * (A0 + A1),
-                                                     0.5 * (La + Lb.max()) * (1 + 1e-9))
-    i = np.repeat(np.arange(len(Ea)), [len(c) for c in near])
-    j = (np.concatenate([np.asarray(c, np.int64) for c in near]) if len(i)
-         else np.zeros(0, np.int64))
-    if Va is Vb:
-        keep = (i < j) & ~((Ea[i, 0] == Eb[j, 0]) | (Ea[i, 0] == Eb[j, 1]) |
-                           (Ea[i, 1] == Eb[j, 0]) | (Ea[i, 1] == Eb[j, 1]))
-        i, j = i[keep], j[keep]
+def _sweep_pairs(lo_a, hi_a, lo_b, hi_b):
+    """Index pairs (i, j) of boxes [lo_a[i], hi_a[i]] and [lo_b[j], hi_b[j]] (2-D) whose
+    closures overlap, found by sorting the boxes on x and sweeping: box i meets the b boxes
+    whose x-interval starts before hi_a[i] among those starting after lo_a[i] - (widest b)."""
+    if len(lo_a) == 0 or len(lo_b) == 0:
+        return np.zeros(0, np.int64), np.zeros(0, np.int64)
+    ob = np.argsort(lo_b[:, 0], kind="stable")
+    xs = lo_b[ob, 0]
+    wmax = float((hi_b[:, 0] - lo_b[:, 0]).max())
+    first = np.searchsorted(xs, lo_a[:, 0] - wmax, side="left")
+    last = np.searchsorted(xs, hi_a[:, 0], side="right")
+    cnt = np.maximum(last - first, 0)
+    i = np.repeat(np.arange(len(lo_a)), cnt)
+    if len(i) == 0:
+        return i, i
+    off = np.arange(len(i)) - np.repeat(np.cumsum(cnt) - cnt, cnt)
+    j = ob[np.repeat(first, cnt) + off]
+    keep = ((lo_a[i] <= hi_b[j]) & (lo_b[j] <= hi_a[i])).all(1)
+    return i[keep], j[keep]
 
-    def cr(o, p, q):
-        return (p[:, 0] - o[:, 0]) * (q[:, 1] - o[:, 1]) - (p[:, 1] - o[:, 1]) * (q[:, 0] - o[:, 0])
+
+def _orient(o, p, q):
+    """z of (p - o) x (q - o), row-wise."""
+    return (p[:, 0] - o[:, 0]) * (q[:, 1] - o[:, 1]) - (q[:, 0] - o[:, 0]) * (p[:, 1] - o[:, 1])
+
+
+def _crossing_segments(Va, Ea, Vb, Eb):
+    """Index pairs (i, j) of segments Ea[i] of Va and Eb[j] of Vb that cross properly: each
+    strictly separates the other's two ends (touching, collinear overlap and shared ends do
+    not count). With Va is Vb, pairs that share a vertex are skipped. Candidates are pairs of
+    overlapping bounding boxes (_sweep_pairs)."""
+    A0, A1, B0, B1 = Va[Ea[:, 0]], Va[Ea[:, 1]], Vb[Eb[:, 0]], Vb[Eb[:, 1]]
+    i, j = _sweep_pairs(np.minimum(A0, A1), np.maximum(A0, A1), np.minimum(B0, B1), np.maximum(B0, B1))
+    if Va is Vb:
+        share = (Ea[i][:, :, None] == Eb[j][:, None, :]).any((1, 2))
+        keep = (i < j) & ~share
+        i, j = i[keep], j[keep]
     a0, a1, b0, b1 = A0[i], A1[i], B0[j], B1[j]
-    hit = (cr(a0, a1, b0) * cr(a0, a1, b1) < 0) & (cr(b0, b1, a0) * cr(b0, b1, a1) < 0)
+    hit = (np.sign(_orient(a0, a1, b0)) * np.sign(_orient(a0, a1, b1)) < 0) & \
+          (np.sign(_orient(b0, b1, a0)) * np.sign(_orient(b0, b1, a1)) < 0)
     return i[hit], j[hit]
 
 
-def check_mesh(vertices, triangles):
-    """Validate a triangle mesh body and measure it (reference core/mesh.py:190-286).
-    Raises ValueError naming the defect. Returns dict(vertices, triangles, area, h, h_cfl,
-    L_max): h the median boundary-edge length, h_cfl the smallest sqrt(2 area), L_max the
-    longest boundary edge."""
+def _check_arrays(vertices, triangles):
+    """The mesh as (V float64 (n, 2), T int64 (m, 3)); ValueError for a malformed one."""
     V = np.asarray(vertices, np.float64)
     if V.ndim != 2 or V.shape[1] != 2 or len(V) < 3:
         raise ValueError(f"vertices must be an (n, 2) array with n >= 3; got shape {V.shape}")
-    if not np.isfinite(V).all():
-        raise ValueError(f"vertices {np.nonzero(~np.isfinite(V).all(1))[0][:10].tolist()} are not finite")
+    bad = np.flatnonzero(~np.isfinite(V).all(1))
+    if len(bad):
+        raise ValueError(f"vertices {bad[:10].tolist()} are not finite")
     T = np.asarray(triangles)
     if T.ndim != 2 or T.shape[1] != 3 or len(T) == 0:
         raise ValueError(f"triangles must be an (m, 3) array with m >= 1; got shape {T.shape}")
-    if not np.issubdtype(T.dtype, np.integer):
-        if not (np.issubdtype(T.dtype, np.floating) and np.all(T == np.round(T))):
-            raise ValueError(f"triangles must hold integer vertex indices; got dtype {T.dtype}")
+    integral = np.issubdtype(T.dtype, np.integer) or (
+        np.issubdtype(T.dtype, np.floating) and bool(np.all(np.floor(T) == T)))
+    if not integral:
+        raise ValueError(f"triangles must hold integer vertex indices; got dtype {T.dtype}")
     T = T.astype(np.int64)
-    if T.min() < 0 or T.max() >= len(V):
-        raise ValueError(f"triangle indices run from {T.min()} to {T.max()}, but there are "
-                         f"{len(V)} vertices (indices are 0-based)")
-    rep = (T[:, 0] == T[:, 1]) | (T[:, 1] == T[:, 2]) | (T[:, 2] == T[:, 0])
-    if rep.any():
-        raise ValueError(f"triangles {np.nonzero(rep)[0][:10].tolist()} repeat a vertex")
+    lo, hi = int(T.min()), int(T.max())
+    if lo < 0 or hi >= len(V):
+        raise ValueError(f"triangle indices run from {lo} to {hi}, but there are {len(V)} vertices "
+                         f"(indices are 0-based)")
+    return V, T
+
+
+def _check_triangles(V, T):
+    """Each triangle's own defects: a repeated vertex, a duplicate, unused vertices, (almost)
+    zero area; slivers warn. Returns twice each triangle's area."""
+    same = (T == np.roll(T, 1, axis=1)).any(1)
+    if same.any():
+        raise ValueError(f"triangles {np.flatnonzero(same)[:10].tolist()} repeat a vertex")
     _, first, cnt = unique_triples(T)
     if (cnt > 1).any():
         raise ValueError(f"duplicate triangles (same three vertices), e.g. triangle {first[cnt > 1][0]}")
-    used = np.zeros(len(V), bool)
-    used[T.reshape(-1)] = True
-    if not used.all():
-        raise ValueError(f"vertices {np.nonzero(~used)[0][:10].tolist()} belong to no triangle")
-    p = V[T]
-    e1, e2, e3 = p[:, 1] - p[:, 0], p[:, 2] - p[:, 0], p[:, 2] - p[:, 1]
-    twoA = np.abs(e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0])
-    Lmax2 = np.max([(e1 ** 2).sum(1), (e2 ** 2).sum(1), (e3 ** 2).sum(1)], 0)
-    qual = twoA / Lmax2
-    if (qual < 1e-8).any():
-        raise ValueError(f"triangles {np.nonzero(qual < 1e-8)[0][:10].tolist()} have (almost) zero area")
-    if (qual < 0.05).any():
-        warnings.warn(f"{int((qual < 0.05).sum())} triangles are slivers (worst quality "
-                      f"{qual.min():.3g}): they stiffen the solve and limit its step.", stacklevel=3)
-    E = np.concatenate([T[:, [0, 1]], T[:, [1, 2]], T[:, [2, 0]]])
-    opp = np.concatenate([T[:, 2], T[:, 0], T[:, 1]])
-    key, _, cnt, inv = unique_pairs(E)
+    unused = np.setdiff1d(np.arange(len(V)), T)
+    if len(unused):
+        raise ValueError(f"vertices {unused[:10].tolist()} belong to no triangle; remove them "
+                         f"(vertex i is node i of the body, so none may be left over)")
+    P = V[T]
+    sides = P[:, [1, 2, 2]] - P[:, [0, 0, 1]]                        # e1, e2, e3
+    twoA = np.abs(sides[:, 0, 0] * sides[:, 1, 1] - sides[:, 1, 0] * sides[:, 0, 1])
+    quality = twoA / (sides ** 2).sum(2).max(1)                     # 0.87 equilateral, 0.5 a pixel triangle
+    flat = np.flatnonzero(quality < 1e-8)
+    if len(flat):
+        raise ValueError(f"triangles {flat[:10].tolist()} have (almost) zero area")
+    thin = quality < 0.05
+    if thin.any():
+        warnings.warn(f"{int(thin.sum())} triangles are slivers (2 x area / longest edge^2 below "
+                      f"0.05, worst {quality.min():.3g}): they stiffen the solve and limit its "
+                      f"step. Remesh with a quality constraint.", stacklevel=4)
+    return twoA
+
+
+def _check_edges(V, T):
+    """Every edge is shared by at most two triangles, and two triangles sharing one lie on
+    opposite sides of it. A triangle's third vertex lies left of an edge it traverses when the
+    triangle's own vertex order is counter-clockwise, so two triangles are on opposite sides of
+    their shared edge exactly when their orientations times their traversal directions agree.
+    Returns the boundary edges (sorted pairs, in lexicographic order)."""
+    half = np.concatenate([T[:, [0, 1]], T[:, [1, 2]], T[:, [2, 0]]])
+    key, _, cnt, inv = unique_pairs(half)
     if (cnt > 2).any():
         a, b = key[np.argmax(cnt > 2)]
         raise ValueError(f"edge ({a}, {b}) is shared by {cnt.max()} triangles; at most two may share one")
-
-    def side(a, b, c):
-        return ((V[b, 0] - V[a, 0]) * (V[c, 1] - V[a, 1]) - (V[b, 1] - V[a, 1]) * (V[c, 0] - V[a, 0]))
-    order = np.argsort(inv, kind="stable")
-    inner = np.nonzero(cnt == 2)[0]
-    starts = np.concatenate([[0], np.cumsum(cnt)])[inner]
-    o1, o2 = opp[order[starts]], opp[order[starts + 1]]
-    s1, s2 = side(key[inner, 0], key[inner, 1], o1), side(key[inner, 0], key[inner, 1], o2)
-    if (s1 * s2 >= 0).any():
-        a, b = key[inner[np.argmax(s1 * s2 >= 0)]]
-        raise ValueError(f"the mesh folds over at edge ({a}, {b})")
+    P = V[T]
+    ccw = np.sign(_orient(P[:, 0], P[:, 1], P[:, 2]))
+    sense = np.tile(ccw, 3) * np.where(half[:, 0] < half[:, 1], 1.0, -1.0)
+    inner = cnt == 2
+    by_edge = np.argsort(inv, kind="stable")
+    head = np.concatenate([[0], np.cumsum(cnt)[:-1]])[inner]
+    s1, s2 = sense[by_edge[head]], sense[by_edge[head + 1]]
+    fold = np.flatnonzero(s1 == s2)
+    if len(fold):
+        a, b = key[np.flatnonzero(inner)[fold[0]]]
+        raise ValueError(f"the mesh folds over at edge ({a}, {b}): its two triangles lie on the "
+                         f"same side of it")
     bed = key[cnt == 1]
     if len(bed) == 0:
         raise ValueError("the mesh has no boundary")
-    L = np.linalg.norm(V[bed[:, 1]] - V[bed[:, 0]], axis=1)
+    return bed
+
+
+def _check_outline(V, bed):
+    """No two vertices at one place, no vertex on a boundary edge it is not part of (a crack or
+    a T-junction), no two boundary edges crossing."""
     from scipy.spatial import cKDTree
-    tol = 1e-9 * float(np.linalg.norm(np.ptp(V, 0)))
-    pairs = cKDTree(V).query_pairs(tol, output_type="ndarray")
-    if len(pairs):
-        raise ValueError(f"vertices {pairs[0, 0]} and {pairs[0, 1]} are at the same place; merge them")
+    tol = 1e-9 * float(np.linalg.norm(V.max(0) - V.min(0)))
+    dup = cKDTree(V).query_pairs(tol, output_type="ndarray")
+    if len(dup):
+        raise ValueError(f"vertices {dup[0, 0]} and {dup[0, 1]} are at the same place; merge them")
     bnodes = np.unique(bed)
-    mid = 0.5 * (V[bed[:, 0]] + V[bed[:, 1]])
-    near = cKDTree(V[bnodes]).query_ball_point(mid, 0.5 * L + tol)
-    ek = np.repeat(np.arange(len(bed)), [len(c) for c in near])
-    if len(ek):
-        cand = bnodes[np.concatenate([np.asarray(c, np.int64) for c in near])]
-        keep = (cand != bed[ek, 0]) & (cand != bed[ek, 1])
-        ek, cand = ek[keep], cand[keep]
-        a, b = V[bed[ek, 0]], V[bed[ek, 1]]
-        t = b - a
-        sv = np.clip(((V[cand] - a) * t).sum(1) / np.maximum((t * t).sum(1), 1e-300), 0.0, 1.0)
-        on = np.linalg.norm(V[cand] - (a + sv[:, None] * t), axis=1) < tol
-        if on.any():                  # named as the reference does: the edge's first candidate
-            k = ek[int(np.argmax(on))]
-            c0 = cand[int(np.argmax(ek == k))]
-            raise ValueError(f"vertex {c0} lies on boundary edge ({bed[k, 0]}, {bed[k, 1]}) without being "
+    a, b = V[bed[:, 0]], V[bed[:, 1]]
+    k, n = _sweep_pairs(np.minimum(a, b) - tol, np.maximum(a, b) + tol, V[bnodes], V[bnodes])
+    c = bnodes[n]
+    k, c = k[(c != bed[k, 0]) & (c != bed[k, 1])], c[(c != bed[k, 0]) & (c != bed[k, 1])]
+    if len(k):
+        d = b[k] - a[k]
+        w = V[c] - a[k]
+        t = np.clip((w * d).sum(1) / np.maximum((d * d).sum(1), 1e-300), 0.0, 1.0)
+        on = np.hypot(w[:, 0] - t * d[:, 0], w[:, 1] - t * d[:, 1]) < tol
+        if on.any():
+            e = k[on].min()
+            v = c[on][k[on] == e].min()
+            raise ValueError(f"vertex {v} lies on boundary edge ({bed[e, 0]}, {bed[e, 1]}) without being "
                              f"part of it (a crack or a T-junction); merge the two sides")
     i, j = _crossing_segments(V, bed, V, bed)
     if len(i):
-        raise ValueError(f"boundary edges ({bed[i[0], 0]}, {bed[i[0], 1]}) and "
-                         f"({bed[j[0], 0]}, {bed[j[0], 1]}) cross: the outline intersects itself")
+        first = np.lexsort((j, i))[0]
+        i, j = i[first], j[first]
+        raise ValueError(f"boundary edges ({bed[i, 0]}, {bed[i, 1]}) and ({bed[j, 0]}, {bed[j, 1]}) "
+                         f"cross: the outline intersects itself")
+
+
+def check_mesh(vertices, triangles):
+    """Validate a triangle mesh body and measure it (reference core/mesh.py:190-286: the same
+    defects are refused, with ValueError naming them). Either winding is fine, mixed included.
+
+    Returns dict(vertices (n, 2) float64, triangles (m, 3) int64, area (m,), h, h_cfl, L_max):
+    h the median boundary-edge length (the body's contact length scale), h_cfl the smallest
+    sqrt(2 x area) (the size that sets the iterations a step needs), L_max the longest
+    boundary edge."""
+    V, T = _check_arrays(vertices, triangles)
+    twoA = _check_triangles(V, T)
+    bed = _check_edges(V, T)
+    d = V[bed[:, 1]] - V[bed[:, 0]]
+    L = np.sqrt((d * d).sum(1))
+    _check_outline(V, bed)
     return dict(vertices=V, triangles=T, area=0.5 * twoA, h=float(np.median(L)),
-                h_cfl=float(np.sqrt(twoA).min()), L_max=float(L.max()))
+                h_cfl=float(np.sqrt(twoA.min())), L_max=float(L.max()))
 
 
-def _triangle_pairs_overlap(P, Q, eps):
-    """Do triangles P[k] and Q[k] share an area (separating axes on every edge normal)?"""
-    ok = np.ones(len(P), bool)
-    for S in (P, Q):
-        for e in range(3):
-            d = S[:, (e + 1) % 3] - S[:, e]
-            n = np.stack([-d[:, 1], d[:, 0]], 1) / np.linalg.norm(d, axis=1)[:, None]
-            pp, qq = np.einsum("kij,kj->ki", P, n), np.einsum("kij,kj->ki", Q, n)
-            ok &= np.minimum(pp.max(1), qq.max(1)) - np.maximum(pp.min(1), qq.min(1)) > eps
-    return ok
+def _triangles_overlap(P, Q, eps):
+    """Do triangles P[k] and Q[k] ((K, 3, 2) each) share an area? Separating-axis test: two
+    convex polygons overlap in area exactly when, on the normal of every edge of either, their
+    projections overlap by more than eps (touching along an edge or at a point is no overlap)."""
+    both = np.concatenate([P, Q], 1)                                 # (K, 6, 2)
+    edges = np.concatenate([np.roll(P, -1, 1) - P, np.roll(Q, -1, 1) - Q], 1)   # (K, 6, 2)
+    normal = np.stack([-edges[..., 1], edges[..., 0]], -1)
+    normal /= np.linalg.norm(normal, axis=-1, keepdims=True)
+    proj = np.einsum("kvd,kad->kav", both, normal)                   # (K, axis, vertex)
+    gap = np.minimum(proj[..., :3].max(-1), proj[..., 3:].max(-1)) - \
+        np.maximum(proj[..., :3].min(-1), proj[..., 3:].min(-1))
+    return (gap > eps).all(1)
 
 
 def meshes_overlap(Va, Ta, Vb, Tb):
-    """Do two mesh bodies share an area of any size (reference core/mesh.py:355-377)?"""
-    from scipy.spatial import cKDTree
+    """Do two mesh bodies share an area of any size? Exact for any alignment: every pair of
+    triangles whose bounding boxes meet is tested (reference core/mesh.py:355-377)."""
     lo_a, hi_a, lo_b, hi_b = Va.min(0), Va.max(0), Vb.min(0), Vb.max(0)
     if (hi_a <= lo_b).any() or (hi_b <= lo_a).any():
         return False
     pa, pb = Va[Ta], Vb[Tb]
-    ca, cb = pa.mean(1), pb.mean(1)
-    ra = np.linalg.norm(pa - ca[:, None], axis=2).max(1)
-    rb = np.linalg.norm(pb - cb[:, None], axis=2).max(1)
+    i, j = _sweep_pairs(pa.min(1), pa.max(1), pb.min(1), pb.max(1))
     eps = 1e-9 * float(np.linalg.norm(np.maximum(hi_a, hi_b) - np.minimum(lo_a, lo_b)))
-    near = cKDTree(cb).query_ball_point(ca, (ra + rb.max()) * (1 + 1e-9))
-    i = np.repeat(np.arange(len(Ta)), [len(n) for n in near])
-    if not len(i):
-        return False
-    j = np.concatenate([np.asarray(n, np.int64) for n in near])
-    for s in range(0, len(i), 200000):
-        if _triangle_pairs_overlap(pa[i[s:s + 200000]], pb[j[s:s + 200000]], eps).any():
+    for s in range(0, len(i), 1 << 17):                              # bounded memory
+        if _triangles_overlap(pa[i[s:s + (1 << 17)]], pb[j[s:s + (1 << 17)]], eps).any():
             return True
     return False
 
 
 def mask_to_mesh(mask, physical_size, center):
-    """The triangle mesh a pixel mask makes — (vertices, triangles, pixel (row, col) of each
-    triangle) (reference core/mesh.py:298-310)."""
-    m = (np.asarray(mask) > 0.5).T.copy()
-    if m.ndim != 2 or not m.any():
+    """The mesh a pixel body of this mask is made of — the same nodes and triangles in the same
+    order — as (vertices (n, 2), triangles (m, 3), pixel (m, 2)): pixel[j] is the (row, col) of
+    the mask pixel triangle j comes from (reference core/mesh.py:298-310)."""
+    solid = np.asarray(mask) > 0.5
+    if solid.ndim != 2 or not solid.any():
         raise ValueError("the mask must be a 2-D array with at least one solid pixel")
-    nx, ny = m.shape
-    rest_pos, tri, tri_quad, _ = grid_mesh(nx, ny, float(physical_size[0]), float(physical_size[1]),
-                                           (float(center[0]), float(center[1])), m)
-    return rest_pos, tri, np.stack([tri_quad % ny, tri_quad // ny], 1)
+    H, W = solid.shape
+    V, T, quad, _ = grid_mesh(W, H, float(physical_size[0]), float(physical_size[1]),
+                              (float(center[0]), float(center[1])), np.ascontiguousarray(solid.T))
+    col, row = np.divmod(quad, H)
+    return V, T, np.stack([row, col], 1)
 
 
 # ---------------------------------------------------------------------------- the scene

@@ -133,13 +133,14 @@ def test_line_search(path):
     _compare(make, 6, 2e-3, path, kw, 0.03)
 
 
-def on_colliders(n_env=3):
-    """Soft squares falling with a sideways velocity onto a disc, a box and a triangle
-    collider (one environment above each; colliders are shared by every environment)."""
+def on_colliders(n_env=4):
+    """Soft squares falling with a sideways velocity onto a disc, a box, a triangle and an
+    ellipse collider (one environment above each; colliders are shared by every environment)."""
     m = Model()
     m.add_disc((-0.03, 0.0), 0.008, mu=0.4)
     m.add_object("box", (0.0, 0.0), 0.008, 0.006, corner_radius=1e-3, mu=0.6)
     m.add_object("triangle", (0.03, 0.0), 0.008, 0.008, corner_radius=1e-3, mu=0.2)
+    m.add_object("ellipse", (0.06, 0.0), 0.009, 0.005, mu=0.5)
     for i in range(n_env):
         m.add_soft_body(np.ones((6, 6), np.uint8), physical_size=(0.006, 0.006),
                         center=(-0.03 + 0.03 * i + 0.001, 0.0125), E=0.3e6, nu=0.42, density=1000.0,
@@ -152,9 +153,9 @@ def on_colliders(n_env=3):
 
 
 @pytest.mark.parametrize("path", PATHS)
-def test_disc_box_triangle_colliders(path):
+def test_disc_box_triangle_ellipse_colliders(path):
     oi, gi, mo, mg = _compare(on_colliders, 8, 2e-3, path, FIXED, 0.006)
-    assert oi.solver.clr_min.min() < 1e-3   # every body came near its collider
+    assert oi.solver.clr_min.max() < 1e-3   # every body came near its collider
 
 
 def kinematic_pusher(n_env=2):
@@ -264,13 +265,12 @@ def test_reset_and_set_state(path):
 
 
 @pytest.mark.parametrize("path,precision,tol", [("plain", "double", 1e-8), ("scene", "double", 1e-8),
-                                                ("scene", "mixed", 5e-5)])
+                                                ("scene", "mixed", 1e-5)])
 def test_cantilever_reaches_reference_static_equilibrium(path, precision, tol):
     """The held, loaded beam stepped with a one-second step size reaches the static
     equilibrium of the reference's Newton oracle (golden fixture). Mixed precision forms the
-    stress in float32; at this deflection (0.7 of the length: large rotation, small strain)
-    mu (F - F^-T) loses about 6e-8 / strain of relative accuracy, and the equilibrium lands
-    about 2e-5 of the length away, hence its looser bound."""
+    stress in float32 from a float64 displacement gradient; at this deflection (0.7 of the
+    length: large rotation, small strain) its equilibrium lies within 1e-5 of the length."""
     from tests.scenes import cantilever
     m, f = cantilever()
     gi = ImplicitIntegrator(path=path, precision=precision, max_iter=400, eps=0.0, conv_check=0)
