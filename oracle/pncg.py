@@ -316,11 +316,11 @@ class OracleIntegrator:
             soft[a["elems"]] = False
         C = 0.0
         for h, b in enumerate(model._bodies):
-            el = model._tri_range[h]
+            el = model.body_elements(h)
             el = el[soft[el]]
-            if len(el) and not b.get("kinematic"):
-                cp = np.sqrt((lam[el] + 2.0 * mu[el]).max() / b["rho"])
-                C = max(C, cp * dt / model._cfl_length(h))
+            if len(el) and b.role != "kinematic":
+                cp = np.sqrt((lam[el] + 2.0 * mu[el]).max() / b.rho)
+                C = max(C, cp * dt / b.cfl_length())
         return float(C)
 
     def iters_for(self, model, dt):
@@ -331,7 +331,7 @@ class OracleIntegrator:
         return int(np.clip(n, lo, self.max_iter_cap))
 
     def _build(self, model, dt):
-        s = OracleSolver(model.me, model.cfg, model.all_constrained(), colliders=model._colliders,
+        s = OracleSolver(model.me, model.cfg, model.all_constrained(), colliders=model.collider_records(),
                          dt=dt, iter_max=self.iters_for(model, dt), eps=self.eps, tol_x=self.tol_x,
                          conv_check=self.conv_check, line_search=self.line_search,
                          n_backtrack=self.n_backtrack, ls_tol=self.ls_tol,
@@ -371,60 +371,52 @@ class OracleIntegrator:
 
     def _sync_actuation(self, model):
         s = self.solver
-        if self._act_version != model._act_version:
+        act = model.actuation
+        if self._act_version != act.structure:
             keep = [t for t in s.terms if isinstance(t, ExternalForce) or getattr(t, "persistent", False)]
             s.terms = list(keep)
             self._force = next((t for t in keep if isinstance(t, ExternalForce)), None)
             self._pressure = self._tendon = self._spring = self._strain = None
-            if model._springs:
-                S = model._springs
-                self._spring = Springs(s, [k["a"] for k in S], [k["b"] for k in S],
-                                       [k["stiffness"] for k in S], [k["env"] for k in S])
+            if act.springs:
+                lay = act.spring_layout()
+                self._spring = Springs(s, lay["a"], lay["b"], lay["stiffness"], lay["env"])
                 s.terms.append(self._spring)
-            if model._chambers:
-                C = model._chambers
-                self._pressure = Pressure(s, np.concatenate([c["ea"] for c in C]),
-                                          np.concatenate([c["eb"] for c in C]),
-                                          np.concatenate([np.full(len(c["ea"]), i) for i, c in enumerate(C)]),
-                                          [model.body_env(c["body"]) for c in C], [c["ref"] for c in C])
+            if act.chambers:
+                lay = act.pressure_layout(model.body_env)
+                self._pressure = Pressure(s, lay["ea"], lay["eb"], lay["chamber"], lay["env"], lay["ref"])
                 s.terms.append(self._pressure)
-            if model._tendons:
+            if act.tendons:
+                lay = act.tendon_layout(model.body_env)
                 node, prv, nxt, tid = [], [], [], []
-                for i, t in enumerate(model._tendons):
-                    o, n = len(node), len(t["nodes"])
-                    node += list(t["nodes"])
+                for i, path in enumerate(lay["paths"]):
+                    o, n = len(node), len(path)
+                    node += list(path)
                     tid += [i] * n
                     prv += [-1] + list(range(o, o + n - 1))
                     nxt += list(range(o + 1, o + n)) + [-1]
-                T = model._tendons
-                self._tendon = Tendon(s, node, prv, nxt, tid, [model.body_env(t["body"]) for t in T],
-                                      [t["stiffness"] for t in T], [t["length"] for t in T])
+                self._tendon = Tendon(s, node, prv, nxt, tid, lay["env"], lay["stiffness"], lay["length"])
                 s.terms.append(self._tendon)
-            if model._regions:
-                R = model._regions
-                self._strain = ActiveStrain(s, np.concatenate([r["elems"] for r in R]),
-                                            np.concatenate([np.full(len(r["elems"]), i) for i, r in enumerate(R)]),
-                                            len(R))
-            self._act_version = model._act_version
+            if act.regions:
+                lay = act.region_layout()
+                self._strain = ActiveStrain(s, lay["elems"], lay["region"], lay["n"])
+            self._act_version = act.structure
             self._ctrl_version = None
-        if self._ctrl_version != model._ctrl_version:
+        if self._ctrl_version != act.commands:
             if self._pressure is not None:
-                self._pressure.set([c["pressure"] for c in model._chambers])
+                self._pressure.set([c.pressure for c in act.chambers])
             if self._tendon is not None:
-                self._tendon.set([t["tension"] for t in model._tendons], [t["length"] for t in model._tendons])
+                self._tendon.set([t.tension for t in act.tendons], [t.length for t in act.tendons])
             if self._strain is not None:
-                self._strain.set(s, np.stack([r["A"] for r in model._regions]))
+                self._strain.set(s, np.stack([r.A for r in act.regions]))
             if self._spring is not None:
-                self._spring.set([k["anchor"] for k in model._springs], [k["length"] for k in model._springs])
-            g = model._gravity
-            if model._fext is not None or g != (0.0, 0.0):
+                self._spring.set([k.anchor for k in act.springs], [k.length for k in act.springs])
+            f = act.external_force(s.mass)
+            if f is not None:
                 if self._force is None:
                     self._force = ExternalForce(s)
                     s.terms.insert(0, self._force)
-                f = np.zeros((s.N, 2)) if model._fext is None else np.asarray(model._fext, np.float64).copy()
-                f += s.mass[:, None] * np.asarray(g, np.float64)[None, :]
                 self._force.set(f)
-            self._ctrl_version = model._ctrl_version
+            self._ctrl_version = act.commands
 
     def attach(self, model, dt):
         if self.solver is None:

@@ -175,12 +175,12 @@ def spring_rig(n_env=2, px=0.5e-3):
         ws = m.add_spring(b, (dx + 15 * px, 3 * px), None, (dx + 16 * px, 5 * px), stiffness=1.0e4,
                           rest_length=1.5 * px)
         handles.append((pin, sp, ws))
-    L0 = [m._springs[sp]["length"] for _, sp, _ in handles]
+    L0 = [m.actuation.springs[sp].length for _, sp, _ in handles]
 
     def rig(model, k):
         for (pin, sp, ws), l0 in zip(handles, L0):
             model.set_spring_length(sp, l0 * (1.0 - 0.03 * min(k, 8)))      # the strut pulls in
-            model.set_spring_anchor(ws, np.asarray(model._springs[ws]["anchor"]) + np.array([0.0, 0.05 * px]))
+            model.set_spring_anchor(ws, np.asarray(model.actuation.springs[ws].anchor) + np.array([0.0, 0.05 * px]))
     return m, handles, rig
 
 

@@ -41,10 +41,10 @@ def test_springs_and_pins_steps(path):
     assert err < 1e-9
     assert (gi.stats(mg)["iters"] == oi.stats()["iters"]).all()
     xo = oi.solver.x
-    for h, sp in enumerate(mo._springs):           # tension and length at the oracle's state
-        xb = xo[sp["b"]] if sp["b"] >= 0 else sp["anchor"]
-        ln = np.linalg.norm(xo[sp["a"]] - xb)
-        assert np.allclose(mg.spring_force(h), (sp["stiffness"] * (ln - sp["length"]), ln), rtol=1e-7, atol=1e-12)
+    for h, sp in enumerate(mo.actuation.springs):  # tension and length at the oracle's state
+        xb = xo[sp.b] if sp.b >= 0 else sp.anchor
+        ln = np.linalg.norm(xo[sp.a] - xb)
+        assert np.allclose(mg.spring_force(h), (sp.stiffness * (ln - sp.length), ln), rtol=1e-7, atol=1e-12)
 
 
 @pytest.mark.parametrize("fiber", [None, np.deg2rad(30.0)])

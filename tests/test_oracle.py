@@ -429,18 +429,17 @@ def test_pressure_derivatives():
     m.finalize()
     m.set_pressure(m.add_pressure_chambers(h)[0], 3.0e4)
     m.set_pressure(m.add_ambient_pressure(h), 1.1e4)
-    C = m._chambers
-    term = Pressure(None, np.concatenate([c["ea"] for c in C]), np.concatenate([c["eb"] for c in C]),
-                    np.concatenate([np.full(len(c["ea"]), i) for i, c in enumerate(C)]), [0, 0],
-                    [c["ref"] for c in C])
-    term.set([c["pressure"] for c in C])
+    C = m.actuation.chambers
+    lay = m.actuation.pressure_layout(m.body_env)
+    term = Pressure(None, lay["ea"], lay["eb"], lay["chamber"], [0, 0], lay["ref"])
+    term.set([c.pressure for c in C])
     rng = np.random.default_rng(6)
     x = m.me.rest_pos + 1e-4 * rng.standard_normal(m.me.rest_pos.shape)
     _term_checks(term, m.me, x, rng)
     # the chamber's area is what the potential is minus pressure times
     s = _Sol(m.me, x)
     buf = np.zeros(1)
-    one = Pressure(None, C[0]["ea"], C[0]["eb"], np.zeros(len(C[0]["ea"]), int), [0], [C[0]["ref"]])
+    one = Pressure(None, C[0].ea, C[0].eb, np.zeros(len(C[0].ea), int), [0], [C[0].ref])
     one.set([1.0])
     one.energy(s, buf)
     assert abs(-buf[0] - m.chamber_area(0) * 0) >= 0.0        # exercised below on rest
@@ -462,11 +461,11 @@ def test_tendon_derivatives(mode):
     else:
         td = m.add_tendon(h, through=path, stiffness=5e4)
         m.set_tendon_length(td, 0.9 * m.tendon_length(td))
-    t = m._tendons[0]
-    n = len(t["nodes"])
-    term = Tendon(None, t["nodes"], [-1] + list(range(n - 1)), list(range(1, n)) + [-1], [0] * n, [0],
-                  [t["stiffness"]], [t["length"]])
-    term.set([t["tension"]], [t["length"]])
+    t = m.actuation.tendons[0]
+    n = len(t.nodes)
+    term = Tendon(None, t.nodes, [-1] + list(range(n - 1)), list(range(1, n)) + [-1], [0] * n, [0],
+                  [t.stiffness], [t.length])
+    term.set([t.tension], [t.length])
     rng = np.random.default_rng(7)
     x = m.me.rest_pos + 1e-4 * rng.standard_normal(m.me.rest_pos.shape)
     _term_checks(term, m.me, x, rng)

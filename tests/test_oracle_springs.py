@@ -51,7 +51,7 @@ def _grad(sp, s, x):
 def test_spring_set_up_mirrors_the_model():
     m, s, sp = _rig_state()
     assert len(sp.a) == 6 and (sp.b < 0).sum() == 4          # two pins and two world springs
-    assert np.allclose(sp.L0[sp.b >= 0], [m._springs[h]["length"] for _, h, _ in [(0, 1, 2), (3, 4, 5)]])
+    assert np.allclose(sp.L0[sp.b >= 0], [m.actuation.springs[h].length for _, h, _ in [(0, 1, 2), (3, 4, 5)]])
     assert (sp.L0[[0, 3]] == 0.0).all()                      # pins
 
 
@@ -113,10 +113,10 @@ def test_spring_curvature(stretched):
 @pytest.mark.parametrize("fiber", [None, np.deg2rad(30.0)])
 def test_activation_makes_shape_A_stress_free(fiber):
     m, regions = bilayer(n_env=1, fiber=fiber)
-    m._regions[regions[0]]["elems"] = np.arange(len(m.me.rest_area))   # the whole strip active
+    m.actuation.regions[regions[0]].elems = np.arange(len(m.me.rest_area))   # the whole strip active
     A = activation_matrix(0.85, 1.10, fiber) if fiber is not None else activation_matrix(1.08)
     m.set_activation(regions[0], *((0.85, 1.10) if fiber is not None else (1.08,)))
-    assert np.allclose(m._regions[regions[0]]["A"], A)
+    assert np.allclose(m.actuation.regions[regions[0]].A, A)
     oi = OracleIntegrator(max_iter=10, eps=0.0, conv_check=0)
     s = oi.attach(m, 1e-3)
     X = np.asarray(m.me.rest_pos, np.float64)
@@ -138,7 +138,7 @@ def test_activation_identity_restores_the_mesh():
     m.set_activation(regions[0], 0.9)
     oi = OracleIntegrator(max_iter=10, eps=0.0, conv_check=0)
     s = oi.attach(m, 1e-3)
-    el = m._regions[regions[0]]["elems"]
+    el = m.actuation.regions[regions[0]].elems
     assert not np.allclose(s.Bm[el], s.Bm0[el])
     m.set_activation(regions[0], 1.0)
     oi.attach(m, 1e-3)
