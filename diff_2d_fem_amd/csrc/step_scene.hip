@@ -265,28 +265,36 @@ __device__ __forceinline__ Elem<float> load_elem<float>(S &s, int t) {
 }
 
 // Neo-Hookean state of an element from float64 positions. float64: F = Ds Bm.
-// Mixed: F is formed in float64 and split into a rotation and a symmetric stretch, F = Q S
-// (2-D polar decomposition: Q turns by atan2(F21 - F12, F11 + F22), which makes Q^T F
-// symmetric). Only the stretch's displacement gradient S - I, small with the strain whatever
-// the rotation, is rounded to float; the float stress is formed from it without cancellation
-// (nh_eval_strain) and turned back by Q (the material is isotropic: P(Q S) = Q P(S)). The
-// float stress so keeps its relative accuracy for small strains under any rotation.
-// ROT = false skips the rotation (H = F - I directly): enough for the curvature passes,
-// whose accuracy sets step lengths only.
+// Mixed, for the gradient (ROT = true): F is formed in float64 and split into a rotation and
+// a symmetric stretch, F = Q S (2-D polar decomposition: Q turns by atan2(F21 - F12,
+// F11 + F22), which makes Q^T F symmetric). Only the stretch's displacement gradient S - I,
+// small with the strain whatever the rotation, is rounded to float; the float stress is formed
+// from it without cancellation (nh_eval_strain) and turned back by Q (the material is
+// isotropic: P(Q S) = Q P(S)). The float stress so keeps its relative accuracy for small
+// strains under any rotation. Q's normalisation is a float reciprocal square root refined by
+// one float64 Newton step (relative error ~1e-14; Q is applied both ways, so any error in it
+// only rescales F by that much).
+// Mixed, for the curvature passes (ROT = false): edge vectors formed in float64, then F in
+// float; their accuracy sets step lengths only.
 template <typename R, bool ROT = true>
 __device__ __forceinline__ NH<R> elem_F(const Elem<R> &el, V2<double> p0, V2<double> p1, V2<double> p2) {
-  const M2<double> Ds = {p1.x - p0.x, p2.x - p0.x, p1.y - p0.y, p2.y - p0.y};
   if constexpr (sizeof(R) == 4) {
-    const M2<double> Bd = {double(el.B.a), double(el.B.b), double(el.B.c), double(el.B.d)};
-    const M2<double> Fd = mat_mul(Ds, Bd);
     if constexpr (!ROT) {
-      const M2<float> H = {float(Fd.a - 1.0), float(Fd.b), float(Fd.c), float(Fd.d - 1.0)};
-      return nh_eval_strain(H, el.Amu, el.Alam);
+      const M2<float> Ds = {float(p1.x - p0.x), float(p2.x - p0.x), float(p1.y - p0.y), float(p2.y - p0.y)};
+      return nh_eval(mat_mul(Ds, el.B), el.Amu, el.Alam);
     } else {
+      const M2<double> Ds = {p1.x - p0.x, p2.x - p0.x, p1.y - p0.y, p2.y - p0.y};
+      const M2<double> Bd = {double(el.B.a), double(el.B.b), double(el.B.c), double(el.B.d)};
+      const M2<double> Fd = mat_mul(Ds, Bd);
       const double tc = Fd.a + Fd.d, ts = Fd.c - Fd.b;
       const double r2 = tc * tc + ts * ts;
-      const double ir = r2 > 1.0e-60 ? rsqrt(r2) : 0.0;
-      const double cq = r2 > 1.0e-60 ? tc * ir : 1.0, sq = ts * ir;
+      double cq = 1.0, sq = 0.0;
+      if (r2 > 1.0e-30) {
+        const double y0 = double(rsqrtf(float(r2)));
+        const double ir = y0 * (1.5 - 0.5 * r2 * y0 * y0);
+        cq = tc * ir;
+        sq = ts * ir;
+      }
       // S = Q^T F
       const M2<float> H = {float(cq * Fd.a + sq * Fd.c - 1.0), float(cq * Fd.b + sq * Fd.d),
                            float(cq * Fd.c - sq * Fd.a), float(cq * Fd.d - sq * Fd.b - 1.0)};
@@ -301,6 +309,7 @@ __device__ __forceinline__ NH<R> elem_F(const Elem<R> &el, V2<double> p0, V2<dou
       return h;
     }
   } else {
+    const M2<double> Ds = {p1.x - p0.x, p2.x - p0.x, p1.y - p0.y, p2.y - p0.y};
     return nh_eval(mat_mul(Ds, el.B), el.Amu, el.Alam);
   }
 }
