@@ -66,6 +66,13 @@ class ElemConsts(C.Structure):
                                  "sc_Bm32", "sc_Amu32", "sc_Alam32")]
 
 
+class Minres(C.Structure):
+    _fields_ = ([("B", C.c_int32), ("maxiter", C.c_int32)]
+                + [(k, C.c_void_p) for k in ("node_start", "mass", "fmask", "dinv", "b", "x", "v", "r1", "r2",
+                                             "w1", "w2", "Kv", "scal", "status")]
+                + [("rtol", C.c_double)])
+
+
 class Scene(C.Structure):
     _fields_ = _SCENE_FIELDS
 
@@ -126,9 +133,13 @@ def lib():
     L.d2f_node_grad_diag_f64.argtypes = [P, P, P, i32, P, P, f64, f64, f64, i32, P, P, P]
     L.d2f_node_hvp_f64.argtypes = [P, P, P, P, i32, P, P, f64, f64, f64, P, P]
     L.d2f_elastic_drho_f64.argtypes = [P, P, P, P, P, P, P, i32, f64, f64, P, P]
-    sizes = (C.c_int64 * 3)()
+    L.d2f_pressure_hvp_f64.argtypes = [P, P, P, P, P, i32, f64, P, P]
+    L.d2f_pressure_dp_f64.argtypes = [P, P, P, P, P, i32, f64, P, P]
+    L.d2f_minres_begin_f64.argtypes = [C.POINTER(Minres), P]
+    L.d2f_minres_step_f64.argtypes = [C.POINTER(Minres), P]
+    sizes = (C.c_int64 * 4)()
     L.d2f_struct_sizes(sizes)
-    want = (C.sizeof(Scene), C.sizeof(Params), C.sizeof(Collider))
+    want = (C.sizeof(Scene), C.sizeof(Params), C.sizeof(Collider), C.sizeof(Minres))
     if tuple(sizes) != want:
         raise RuntimeError(f"{path}: struct sizes {tuple(sizes)} do not match the binding {want}; "
                            f"rebuild the library")

@@ -8,16 +8,21 @@ trajectory x_0 .. x_T is differentiated by the damped backward-Euler recursion
     H_k lam_k = dJ/dx_k + (1 + drag) M lam_{k+1} - drag M lam_{k+2}
     dJ/drho_e = sum_k -(1 - Emin) lam_k . [h^2 f_unit_e(x_k)]
 
-with H_k = M + h^2 (K_elastic + K_contact) at x_k, restricted to the free degrees of freedom,
-the contact active set frozen at x_k. The operator holds inertia, elasticity and the
-colliders' barriers, and with use_self the "node" contact's penalty between bodies. Friction,
-IPC contact between bodies, pressure, tendons, springs, active strain and rigid regions have no
-tangent in it (transient_adjoint refuses a model with any of them except collider friction, as
-the reference does).
+with H_k = M + h^2 (K_elastic + K_contact + K_pressure) at x_k, restricted to the free degrees of
+freedom, the contact active set frozen at x_k. The operator holds inertia, elasticity, the
+colliders' barriers, with use_self the "node" contact's penalty between bodies, and — beyond
+the reference, whose adjoint refuses them — pressure chambers, whose tangent is exact (the
+pressure gradient is linear in x). With chambers the recursion also gives dJ/dp per chamber
+and step (dJdp_steps): dJ/dp_c,k = -lam_k . dR_k/dp_c. Friction, IPC contact between bodies,
+tendons, springs, active strain and rigid regions have no tangent in it (transient_adjoint
+refuses a model with any of them except collider friction, as the reference does).
 
-Everything runs on the GPU: the Hessian-vector products and dJ/drho are HIP kernels
-(d2f_elastic_hvp_f64, d2f_collider_hvp_f64, d2f_node_hvp_f64, d2f_elastic_drho_f64), and the preconditioned
-MINRES iteration keeps its vectors in device memory; only its scalars come to the host.
+Everything runs on the GPU: the Hessian-vector products, dJ/drho and dJ/dp are HIP kernels
+(d2f_elastic_hvp_f64, d2f_collider_hvp_f64, d2f_node_hvp_f64, d2f_pressure_hvp_f64,
+d2f_elastic_drho_f64, d2f_pressure_dp_f64), and each adjoint solve is a preconditioned MINRES
+with per-scene Krylov scalars on the device (d2f_minres_step_f64): the scenes of a batch do not
+couple through shared scalars, and the host reads only the scenes' status flags, every
+`check_every` iterations.
 """
 from __future__ import annotations
 
@@ -91,11 +96,10 @@ class AdjointPNCG2D:
     def _node_on(self, use_self):
         return bool(use_self) and self.s.node is not None
 
-    def _hvp(self, xw, vw, use_disc=False, use_sdf=False, include_mass=True, use_self=False):
+    def _kv(self, xw, vw, y, use_disc=False, use_sdf=False, use_self=False):
+        """y += h^2 K v at xw (elastic, selected colliders, node contact, pressure)."""
         L = _lib.lib()
         s = self.s
-        y = self._y
-        y.zero_()
         _lib.check(L.d2f_elastic_hvp_f64(self._p(xw), self._p(vw), self._p(s.tri), self._p(s.Bm),
                                          self._p(s.area), self._p(s.mu), self._p(s.lam), self.n_tri,
                                          self.h2, self._p(y), self._stream()), "d2f_elastic_hvp_f64")
@@ -110,6 +114,16 @@ class AdjointPNCG2D:
             _lib.check(L.d2f_node_hvp_f64(self._p(xw), self._p(vw), self._p(s.bnd), self._p(s.bnd_start), s.B,
                                           self._p(s.excl), self._p(s.excl_off), nd["r"], nd["k"], self.h2,
                                           self._p(y), self._stream()), "d2f_node_hvp_f64")
+        if s.pressure is not None:
+            _lib.check(L.d2f_pressure_hvp_f64(self._p(vw), self._p(s.pe_a), self._p(s.pe_b), self._p(s.pe_ch),
+                                              self._p(s.ch_p), s.pressure["n_pe"], self.h2, self._p(y),
+                                              self._stream()), "d2f_pressure_hvp_f64")
+        return y
+
+    def _hvp(self, xw, vw, use_disc=False, use_sdf=False, include_mass=True, use_self=False):
+        y = self._y
+        y.zero_()
+        self._kv(xw, vw, y, use_disc, use_sdf, use_self)
         return self._mass2 * vw + y if include_mass else y.clone()
 
     def apply_H(self, xstar, v, use_self=False, use_disc=False, use_sdf=False, include_mass=True):
@@ -154,81 +168,71 @@ class AdjointPNCG2D:
         return torch.clamp(d, min=1e-30)
 
     # ---- Krylov solves on the free degrees of freedom
-    def _minres(self, op, b, dinv, rtol, maxiter):
-        """Preconditioned MINRES (Paige and Saunders, SIAM J. Numer. Anal. 12, 1975) for the
-        symmetric, possibly indefinite op; the preconditioner is the diagonal dinv (zero on
-        held degrees of freedom, so every iterate stays on the free ones). Stops when the
-        preconditioned residual norm falls to rtol times that of b. Returns (x, info, iters):
-        info 0 converged, 1 iteration limit, -1 breakdown (a non-positive preconditioned
-        norm)."""
-        x = torch.zeros_like(b)
-        r1 = b.clone()
-        y = dinv * r1
-        beta1 = float((r1 * y).sum())
-        if beta1 < 0.0:
-            return x, -1, 0
-        beta1 = math.sqrt(beta1)
-        if beta1 == 0.0:
-            return x, 0, 0
-        r2 = r1.clone()
-        w = torch.zeros_like(b)
-        w2 = torch.zeros_like(b)
-        oldb, beta, dbar, epsln, phibar, cs, sn = 0.0, beta1, 0.0, 0.0, beta1, -1.0, 0.0
-        tiny = 1e-300
-        for it in range(1, maxiter + 1):
-            v = y / beta
-            y = op(v)
-            if it >= 2:
-                y = y - (beta / oldb) * r1
-            alfa = float((v * y).sum())
-            y = y - (alfa / beta) * r2
-            r1, r2 = r2, y
-            y = dinv * r2
-            oldb = beta
-            bb = float((r2 * y).sum())
-            if bb < 0.0:
-                return x, -1, it
-            beta = math.sqrt(bb)
-            oldeps = epsln
-            delta = cs * dbar + sn * alfa
-            gbar = sn * dbar - cs * alfa
-            epsln = sn * beta
-            dbar = -cs * beta
-            gamma = max(math.hypot(gbar, beta), tiny)
-            cs, sn = gbar / gamma, beta / gamma
-            phi = cs * phibar
-            phibar = sn * phibar
-            w1, w2 = w2, w
-            w = (v - oldeps * w1 - delta * w2) / gamma
-            x = x + phi * w
-            if phibar <= rtol * beta1:
-                return x, 0, it
-            if beta == 0.0:
-                return x, 0, it
-        return x, 1, maxiter
+    def _workspace(self):
+        ws = getattr(self, "_ws", None)
+        if ws is None:
+            z = lambda: torch.zeros((self.N, 2), dtype=F64, device=self.dev)
+            ws = {k: z() for k in ("dinv", "b", "x", "v", "r1", "r2", "w1", "w2", "Kv")}
+            ws["scal"] = torch.zeros(8 * self.s.B, dtype=F64, device=self.dev)
+            ws["status"] = torch.zeros(2 * self.s.B, dtype=torch.int32, device=self.dev)
+            ws["mass"] = self.s.mass.to(F64).contiguous()
+            self._ws = ws
+        return ws
+
+    def _minres(self, xw, b, dinv, rtol, maxiter, use_disc=False, use_sdf=False, use_self=False,
+                check_every=16):
+        """Preconditioned MINRES (Paige and Saunders, SIAM J. Numer. Anal. 12, 1975) for every
+        scene at once, each with its own Krylov scalars, on the device (d2f_minres_*). The
+        operator H v = free (M v + h^2 K v) is symmetric, possibly indefinite; the preconditioner
+        is the diagonal dinv (zero on held degrees of freedom, so iterates stay on the free ones).
+        A scene stops when its preconditioned residual norm falls to rtol times that of its b.
+        Returns (x, per-scene status (0 running at the limit, 1 converged, 2 iteration limit,
+        -1 breakdown), per-scene iterations)."""
+        L = _lib.lib()
+        ws = self._workspace()
+        ws["mass"].copy_(self.s.mass)
+        ws["b"].copy_(b)
+        ws["dinv"].copy_(dinv)
+        m = _lib.Minres()
+        m.B, m.maxiter, m.rtol = self.s.B, int(maxiter), float(rtol)
+        m.node_start, m.mass, m.fmask = self._p(self.s.node_start), self._p(ws["mass"]), self._p(self._free)
+        for k in ("dinv", "b", "x", "v", "r1", "r2", "w1", "w2", "Kv", "scal", "status"):
+            setattr(m, k, self._p(ws[k]))
+        st = self._stream()
+        _lib.check(L.d2f_minres_begin_f64(C.byref(m), st), "d2f_minres_begin_f64")
+        done = 0
+        while done < maxiter:
+            for _ in range(min(check_every, maxiter - done)):
+                ws["Kv"].zero_()
+                self._kv(xw, ws["v"], ws["Kv"], use_disc, use_sdf, use_self)
+                _lib.check(L.d2f_minres_step_f64(C.byref(m), st), "d2f_minres_step_f64")
+            done += check_every
+            if not bool((ws["status"][0::2] == 0).any()):
+                break
+        status = ws["status"].view(-1, 2).cpu().numpy()
+        return ws["x"].clone(), status[:, 0].copy(), status[:, 1].copy()
 
     def adjoint_solve(self, xk, rhs, use_disc=False, use_self=False, use_sdf=False, rtol=1e-9,
-                      maxiter=5000):
-        """One adjoint solve H_k lam = rhs on the free degrees of freedom, MINRES with the Jacobi
-        preconditioner (reference implicit/adjoint.py:375-404). Records minres_calls,
-        minres_fail, minres_last_info (and minres_resid when minres_track_resid is set)."""
+                      maxiter=5000, as_tensor=False):
+        """One adjoint solve H_k lam = rhs on the free degrees of freedom, per-scene MINRES with the
+        Jacobi preconditioner (reference implicit/adjoint.py:339-369). Records minres_calls,
+        minres_fail (scenes that did not converge), minres_last_info, minres_iters (the largest
+        per-scene count), and minres_resid when minres_track_resid is set."""
         xw = self._t(xk)
         b = self._t(rhs) * self._free
         dinv = self._free / self._diag_precond(xw, use_disc, use_sdf, use_self=use_self)
-
-        def op(v):
-            return self._hvp(xw, v, use_disc, use_sdf, use_self=use_self) * self._free
-        lam, info, its = self._minres(op, b, dinv, float(rtol), int(maxiter))
+        lam, status, its = self._minres(xw, b, dinv, float(rtol), int(maxiter), use_disc, use_sdf, use_self)
         self.minres_calls = getattr(self, "minres_calls", 0) + 1
-        self.minres_iters = its
-        if info != 0:
-            self.minres_fail = getattr(self, "minres_fail", 0) + 1
-            self.minres_last_info = int(info)
+        self.minres_iters = int(its.max(initial=0))
+        bad = status != 1
+        if bad.any():
+            self.minres_fail = getattr(self, "minres_fail", 0) + int(bad.sum())
+            self.minres_last_info = int(status[bad][0])
         if getattr(self, "minres_track_resid", False):
-            nb = float(torch.linalg.vector_norm(b))
-            r = float(torch.linalg.vector_norm(op(lam) - b)) / max(nb, 1e-300)
-            self.minres_resid = max(getattr(self, "minres_resid", 0.0), r)
-        return lam.cpu().numpy()
+            r = self._hvp(xw, lam, use_disc, use_sdf, use_self=use_self) * self._free - b
+            rel = float(torch.linalg.vector_norm(r)) / max(float(torch.linalg.vector_norm(b)), 1e-300)
+            self.minres_resid = max(getattr(self, "minres_resid", 0.0), rel)
+        return lam if as_tensor else lam.cpu().numpy()
 
     def adjoint_cg(self, xstar, b, diagH, use_self=False, include_mass=True, iters=8000, tol=1e-9):
         """Jacobi-preconditioned CG on the free degrees of freedom (for a positive definite H);
@@ -307,8 +311,6 @@ class AdjointPNCG2D:
         missing = []
         if s.ipc is not None:
             missing.append("IPCContact")
-        if s.pressure is not None:
-            missing.append("Pressure")
         if s.tendon is not None:
             missing.append("Tendon")
         if s.affine is not None:
@@ -327,24 +329,61 @@ class AdjointPNCG2D:
                    "model.body_contact = 'node', which the adjoint does have a tangent for."
                    if "IPCContact" in missing else ""))
 
+    # ---- dJ/dp of one adjoint vector (pressure chambers)
+    def _dp_dev(self, xw, lamw):
+        """-lam . dR/dp per chamber, in the model's chamber order."""
+        L = _lib.lib()
+        s = self.s
+        out = torch.zeros(s.pressure["n_ch"], dtype=F64, device=self.dev)
+        _lib.check(L.d2f_pressure_dp_f64(self._p(xw), self._p(lamw), self._p(s.pe_a), self._p(s.pe_b),
+                                         self._p(s.pe_ch), s.pressure["n_pe"], self.h2, self._p(out),
+                                         self._stream()), "d2f_pressure_dp_f64")
+        model_order = torch.empty_like(out)
+        model_order[torch.as_tensor(s.ch_order, device=self.dev)] = out
+        return model_order
+
+    def adjoint_step(self, xk, lam1, lam2, seed_k=None, drag=None, use_disc=False, use_self=False,
+                     use_sdf=False, **minres_kw):
+        """One step of the damped backward-Euler recursion on device tensors: solves
+        H_k lam_k = seed_k + (1 + drag) M lam_{k+1} - drag M lam_{k+2} and returns (lam_k,
+        dJ/drho contribution, dJ/dp contribution or None)."""
+        dr = float(self.s.drag) if drag is None else float(drag)
+        rhs = (1.0 + dr) * self._mass2 * lam1 - dr * self._mass2 * lam2
+        if seed_k is not None:
+            rhs = rhs + self._t(seed_k)
+        xw = self._t(xk)
+        lam = self.adjoint_solve(xw, rhs, use_disc=use_disc, use_self=use_self, use_sdf=use_sdf,
+                                 as_tensor=True, **minres_kw)
+        g = self._drho_dev(xw, lam).clone()
+        gp = self._dp_dev(xw, lam) if self.s.pressure is not None else None
+        return lam, g, gp
+
     def transient_adjoint(self, xs, seed, drag=None, use_disc=False, use_self=False, use_sdf=False,
-                          **minres_kw):
+                          pressures=None, **minres_kw):
         """xs = [x_0 .. x_T] checkpointed states; seed = {k: dJ/dx_k (N, 2)}. Returns dJ/drho
-        (n_tri,) accumulated over the trajectory."""
+        (n_tri,) accumulated over the trajectory. With pressure chambers, dJ/dp per step and
+        chamber (model order) lands in self.dJdp_steps (T, n_chambers); `pressures` (T,
+        n_chambers) gives the pressures each step was taken with (default: the current ones)."""
         self._check_terms()
         T = len(xs) - 1
-        dr = float(self.s.drag) if drag is None else float(drag)
         self.refresh_mass()
-        m2 = self._mass2
         zero = torch.zeros((self.N, 2), dtype=F64, device=self.dev)
         lam = {T + 1: zero, T + 2: zero}
         g = torch.zeros(self.n_tri, dtype=F64, device=self.dev)
-        for k in range(T, 0, -1):
-            rhs = (1.0 + dr) * m2 * lam[k + 1] - dr * m2 * lam[k + 2]
-            if k in seed:
-                rhs = rhs + self._t(seed[k])
-            lam[k] = self._t(self.adjoint_solve(xs[k], rhs, use_disc=use_disc, use_self=use_self,
-                                                use_sdf=use_sdf, **minres_kw))
-            g += self._drho_dev(self._t(xs[k]), lam[k])
-            lam.pop(k + 2, None)
+        s = self.s
+        keep_p = None if s.pressure is None else s.ch_p.clone()
+        self.dJdp_steps = None if s.pressure is None else np.zeros((T, s.pressure["n_ch"]))
+        try:
+            for k in range(T, 0, -1):
+                if pressures is not None and s.pressure is not None:
+                    s.set_pressure(np.asarray(pressures, np.float64)[k - 1])
+                lam[k], gk, gp = self.adjoint_step(xs[k], lam[k + 1], lam[k + 2], seed.get(k), drag,
+                                                   use_disc, use_self, use_sdf, **minres_kw)
+                g += gk
+                if gp is not None:
+                    self.dJdp_steps[k - 1] = gp.cpu().numpy()
+                lam.pop(k + 2, None)
+        finally:
+            if keep_p is not None:
+                s.ch_p.copy_(keep_p)
         return g.cpu().numpy()

@@ -93,6 +93,157 @@ __global__ void k_elastic_drho(const double *x, const double *lam, const int32_t
   out[t] += -(1.0 - Emin) * h2 * s;
 }
 
+// ---- pressure (not in the reference's adjoint, which refuses pressure chambers): the gradient
+// of E = -p A(x) puts -1/2 h2 p J(x_b - x_a) on both ends of every chamber edge, J(u) =
+// (u_y, -u_x) (reference implicit/actuation.py:126-151); it is linear in x.
+// y += h2 H_pressure v: -1/2 h2 p J(v_b - v_a) on both ends.
+__global__ void k_pressure_hvp(const double *v, const int32_t *pe_a, const int32_t *pe_b, const int32_t *pe_ch,
+                               const double *ch_p, int n_pe, double h2, double *y) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_pe) return;
+  const int a = pe_a[k], b = pe_b[k];
+  const V2<double> u = ld2(v, b) - ld2(v, a);
+  const double c = -0.5 * h2 * ch_p[pe_ch[k]];
+  const V2<double> w = {c * u.y, -c * u.x};
+  atomic_add2(y, a, w);
+  atomic_add2(y, b, w);
+}
+
+// out[c] += -lam . dR/dp_c = 1/2 h2 sum over chamber c's edges of (lam_a + lam_b) . J(x_b - x_a).
+__global__ void k_pressure_dp(const double *x, const double *lam, const int32_t *pe_a, const int32_t *pe_b,
+                              const int32_t *pe_ch, int n_pe, double h2, double *out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_pe) return;
+  const int a = pe_a[k], b = pe_b[k];
+  const V2<double> t = ld2(x, b) - ld2(x, a);
+  const V2<double> l = ld2(lam, a) + ld2(lam, b);
+  atomicAdd(out + pe_ch[k], 0.5 * h2 * (l.x * t.y - l.y * t.x));
+}
+
+// ---- MINRES with per-scene Krylov scalars (Paige and Saunders, SIAM J. Numer. Anal. 12,
+// 1975), for the adjoint solve H lam = b of every scene at once: scene e's degrees of freedom
+// are [2 node_start[e], 2 node_start[e+1]); each scene has its own Lanczos process, so scenes
+// do not couple through shared scalars. One workgroup per scene; its scalars and status stay
+// on the device between launches, so an iteration needs no host round trip. The operator is
+// applied outside: the caller fills Kv = h2 K v (the Hessian-vector kernels, v as left by the
+// previous launch), and d2f_minres_step_f64 forms H v = fmask (mass v + Kv).
+constexpr int MR_TPB = 256;
+constexpr int MR_NS = 8;  // beta1, oldb, beta, dbar, epsln, phibar, cs, sn
+
+__device__ __forceinline__ double mr_block_sum(double v, double *red) {
+  for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
+  __syncthreads();
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int k = 0; k < MR_TPB / WAVE; ++k) t += red[k];
+  return t;
+}
+
+__global__ __launch_bounds__(MR_TPB) void k_minres_begin(d2f_minres m) {
+  __shared__ double red[MR_TPB / WAVE];
+  const int e = blockIdx.x;
+  const int d0 = 2 * m.node_start[e], d1 = 2 * m.node_start[e + 1];
+  double s = 0.0;
+  for (int i = d0 + threadIdx.x; i < d1; i += MR_TPB) {
+    const double r = m.b[i] * m.fmask[i];
+    const double y = m.dinv[i] * r;
+    m.r1[i] = r;
+    m.r2[i] = r;
+    m.Kv[i] = y;
+    m.x[i] = 0.0;
+    m.w1[i] = 0.0;
+    m.w2[i] = 0.0;
+    s += r * y;
+  }
+  const double bb = mr_block_sum(s, red);
+  const double beta1 = bb > 0.0 ? sqrt(bb) : 0.0;
+  for (int i = d0 + threadIdx.x; i < d1; i += MR_TPB) m.v[i] = beta1 > 0.0 ? m.Kv[i] / beta1 : 0.0;
+  if (threadIdx.x == 0) {
+    double *q = m.scal + MR_NS * e;
+    q[0] = beta1;
+    q[1] = 0.0;
+    q[2] = beta1;
+    q[3] = 0.0;
+    q[4] = 0.0;
+    q[5] = beta1;
+    q[6] = -1.0;
+    q[7] = 0.0;
+    m.status[2 * e] = bb < 0.0 ? -1 : (beta1 == 0.0 ? 1 : 0);
+    m.status[2 * e + 1] = 0;
+  }
+}
+
+__global__ __launch_bounds__(MR_TPB) void k_minres_step(d2f_minres m) {
+  __shared__ double red[MR_TPB / WAVE];
+  const int e = blockIdx.x;
+  if (m.status[2 * e] != 0) return;
+  const int d0 = 2 * m.node_start[e], d1 = 2 * m.node_start[e + 1];
+  double *q = m.scal + MR_NS * e;
+  const double beta1 = q[0], oldb = q[1], beta = q[2], dbar = q[3], epsln = q[4], phibar = q[5], cs = q[6],
+               sn = q[7];
+  const int it = m.status[2 * e + 1] + 1;
+  const double c1 = it >= 2 ? beta / oldb : 0.0;
+  // y = H v - (beta / oldb) r1; alfa = v . y
+  double s = 0.0;
+  for (int i = d0 + threadIdx.x; i < d1; i += MR_TPB) {
+    const double y = m.fmask[i] * (m.mass[i >> 1] * m.v[i] + m.Kv[i]) - c1 * m.r1[i];
+    m.Kv[i] = y;
+    s += m.v[i] * y;
+  }
+  const double alfa = mr_block_sum(s, red);
+  // y -= (alfa / beta) r2; r1 <- r2; r2 <- y; y <- dinv r2; beta' = sqrt(r2 . y)
+  s = 0.0;
+  const double c2 = alfa / beta;
+  for (int i = d0 + threadIdx.x; i < d1; i += MR_TPB) {
+    const double y = m.Kv[i] - c2 * m.r2[i];
+    m.r1[i] = m.r2[i];
+    m.r2[i] = y;
+    const double z = m.dinv[i] * y;
+    m.Kv[i] = z;
+    s += y * z;
+  }
+  const double bb = mr_block_sum(s, red);
+  if (bb < 0.0) {
+    if (threadIdx.x == 0) {
+      m.status[2 * e] = -1;
+      m.status[2 * e + 1] = it;
+    }
+    return;
+  }
+  const double nb = sqrt(bb);
+  const double oldeps = epsln;
+  const double delta = cs * dbar + sn * alfa;
+  const double gbar = sn * dbar - cs * alfa;
+  const double eps2 = sn * nb;
+  const double dbar2 = -cs * nb;
+  const double gamma = fmax(sqrt(gbar * gbar + nb * nb), 1.0e-300);
+  const double cs2 = gbar / gamma, sn2 = nb / gamma;
+  const double phi = cs2 * phibar;
+  const double phibar2 = sn2 * phibar;
+  // w <- (v - oldeps w1 - delta w2) / gamma, (w1, w2) <- (w2, w); x += phi w; v <- y / beta'
+  for (int i = d0 + threadIdx.x; i < d1; i += MR_TPB) {
+    const double w = (m.v[i] - oldeps * m.w1[i] - delta * m.w2[i]) / gamma;
+    m.w1[i] = m.w2[i];
+    m.w2[i] = w;
+    m.x[i] += phi * w;
+    m.v[i] = nb > 0.0 ? m.Kv[i] / nb : 0.0;
+  }
+  if (threadIdx.x == 0) {
+    q[1] = beta;
+    q[2] = nb;
+    q[3] = dbar2;
+    q[4] = eps2;
+    q[5] = phibar2;
+    q[6] = cs2;
+    q[7] = sn2;
+    m.status[2 * e] = (phibar2 <= m.rtol * beta1 || nb == 0.0) ? 1 : (it >= m.maxiter ? 2 : 0);
+    m.status[2 * e + 1] = it;
+  }
+}
+
 static ColSet colset(const d2f_collider *cols, int n_col) {
   ColSet cs{};
   cs.n = n_col;
@@ -184,6 +335,52 @@ extern "C" int d2f_elastic_drho_f64(const double *x, const double *lam, const in
   if (T == 0) return 0;
   hipLaunchKernelGGL(k_elastic_drho, dim3((T + 255) / 256), dim3(256), 0, (hipStream_t)stream, x, lam, tri,
                      Bm, area, mu0, lam0, T, h2, Emin, out);
+  D2F_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" int d2f_pressure_hvp_f64(const double *v, const int32_t *pe_a, const int32_t *pe_b, const int32_t *pe_ch,
+                                    const double *ch_p, int32_t n_pe, double h2, double *y, void *stream) {
+  D2F_CHECK(n_pe >= 0, "d2f_pressure_hvp_f64: bad argument");
+  D2F_CHECK(n_pe == 0 || (v && pe_a && pe_b && pe_ch && ch_p && y), "d2f_pressure_hvp_f64: null pointer");
+  if (n_pe == 0) return 0;
+  hipLaunchKernelGGL(k_pressure_hvp, dim3((n_pe + 255) / 256), dim3(256), 0, (hipStream_t)stream, v, pe_a, pe_b,
+                     pe_ch, ch_p, n_pe, h2, y);
+  D2F_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" int d2f_pressure_dp_f64(const double *x, const double *lam, const int32_t *pe_a, const int32_t *pe_b,
+                                   const int32_t *pe_ch, int32_t n_pe, double h2, double *out, void *stream) {
+  D2F_CHECK(n_pe >= 0, "d2f_pressure_dp_f64: bad argument");
+  D2F_CHECK(n_pe == 0 || (x && lam && pe_a && pe_b && pe_ch && out), "d2f_pressure_dp_f64: null pointer");
+  if (n_pe == 0) return 0;
+  hipLaunchKernelGGL(k_pressure_dp, dim3((n_pe + 255) / 256), dim3(256), 0, (hipStream_t)stream, x, lam, pe_a,
+                     pe_b, pe_ch, n_pe, h2, out);
+  D2F_HIP(hipGetLastError());
+  return 0;
+}
+
+static int minres_check(const d2f_minres *m, const char *who) {
+  D2F_CHECK(m && m->B >= 0, "%s: bad argument", who);
+  D2F_CHECK(m->B == 0 || (m->node_start && m->mass && m->fmask && m->dinv && m->b && m->x && m->v && m->r1 &&
+                          m->r2 && m->w1 && m->w2 && m->Kv && m->scal && m->status),
+            "%s: null pointer", who);
+  return 0;
+}
+
+extern "C" int d2f_minres_begin_f64(const d2f_minres *m, void *stream) {
+  if (int rc = minres_check(m, "d2f_minres_begin_f64")) return rc;
+  if (m->B == 0) return 0;
+  hipLaunchKernelGGL(k_minres_begin, dim3(m->B), dim3(MR_TPB), 0, (hipStream_t)stream, *m);
+  D2F_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" int d2f_minres_step_f64(const d2f_minres *m, void *stream) {
+  if (int rc = minres_check(m, "d2f_minres_step_f64")) return rc;
+  if (m->B == 0) return 0;
+  hipLaunchKernelGGL(k_minres_step, dim3(m->B), dim3(MR_TPB), 0, (hipStream_t)stream, *m);
   D2F_HIP(hipGetLastError());
   return 0;
 }

@@ -1614,6 +1614,7 @@ void d2f_struct_sizes(int64_t *out) {
   out[0] = (int64_t)sizeof(d2f_scene);
   out[1] = (int64_t)sizeof(d2f_params);
   out[2] = (int64_t)sizeof(d2f_collider);
+  out[3] = (int64_t)sizeof(d2f_minres);
 }
 
 int d2f_elastic_grad_diag_f64(const double *x, const int32_t *tri, const double *Bm, const double *area,

@@ -184,8 +184,8 @@ typedef struct d2f_params {
 int d2f_abi_version(void);
 /* Text of the calling thread's last failure (empty if none). */
 const char *d2f_last_error(void);
-/* sizeof(d2f_scene), sizeof(d2f_params), sizeof(d2f_collider) into out[0..2], so a binding can
- * check its mirror of the structs. */
+/* sizeof(d2f_scene), sizeof(d2f_params), sizeof(d2f_collider), sizeof(d2f_minres) into out[0..3],
+ * so a binding can check its mirror of the structs. */
 void d2f_struct_sizes(int64_t *out);
 
 /* ---- per-kernel entry points (float64), one per reference kernel family ------------------ */
@@ -321,6 +321,39 @@ int d2f_node_hvp_f64(const double *x, const double *v, const int32_t *bnd, const
 int d2f_elastic_drho_f64(const double *x, const double *lam, const int32_t *tri, const double *Bm,
                          const double *area, const double *mu0, const double *lam0, int32_t T,
                          double h2, double Emin, double *out, void *stream);
+
+/* ---- adjoint of pressure chambers (float64; beyond the reference, whose adjoint refuses
+ * pressure: implicit/adjoint.py:376-385; the pressure term is implicit/actuation.py:126-151) --- */
+/* y += h2 H_pressure v: each chamber edge (pe_a -> pe_b) adds -1/2 h2 p J(v_b - v_a) to both
+ * ends, J(u) = (u_y, -u_x), p = ch_p[pe_ch[k]] (the gradient's derivative; exact, it is linear). */
+int d2f_pressure_hvp_f64(const double *v, const int32_t *pe_a, const int32_t *pe_b, const int32_t *pe_ch,
+                         const double *ch_p, int32_t n_pe, double h2, double *y, void *stream);
+/* out[c] += -lam . dR/dp_c = 1/2 h2 sum over chamber c's edges (lam_a + lam_b) . J(x_b - x_a). */
+int d2f_pressure_dp_f64(const double *x, const double *lam, const int32_t *pe_a, const int32_t *pe_b,
+                        const int32_t *pe_ch, int32_t n_pe, double h2, double *out, void *stream);
+
+/* ---- MINRES with per-scene Krylov scalars, on the device (replaces the host scipy MINRES of
+ * AdjointPNCG2D.adjoint_solve, implicit/adjoint.py:339-369). Scene e solves for the degrees of
+ * freedom [2 node_start[e], 2 node_start[e+1]) with its own Lanczos process. All vectors are
+ * 2N float64, caller-owned workspace. Protocol: d2f_minres_begin_f64 (x = 0, v = the first
+ * Lanczos vector); then per iteration the caller zeroes Kv, adds h2 K v into it with the
+ * Hessian-vector entry points, and calls d2f_minres_step_f64, which forms
+ * H v = fmask (mass v + Kv) and advances every scene still running. status[2e] is 0 running,
+ * 1 converged (preconditioned residual <= rtol times the initial one), 2 iteration limit,
+ * -1 breakdown (a negative preconditioned norm); status[2e+1] its iterations. */
+typedef struct d2f_minres {
+  int32_t B, maxiter;
+  const int32_t *node_start;          /* B+1 */
+  const double *mass;                 /* N */
+  const double *fmask, *dinv, *b;     /* 2N: 1/0 free mask, Jacobi preconditioner (0 off free), rhs */
+  double *x, *v, *r1, *r2, *w1, *w2;  /* 2N each: solution, Lanczos vector, recurrences */
+  double *Kv;                         /* 2N: h2 K v on entry to a step; scratch inside */
+  double *scal;                       /* 8B Krylov scalars */
+  int32_t *status;                    /* 2B */
+  double rtol;
+} d2f_minres;
+int d2f_minres_begin_f64(const d2f_minres *m, void *stream);
+int d2f_minres_step_f64(const d2f_minres *m, void *stream);
 
 #ifdef __cplusplus
 }

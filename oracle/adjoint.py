@@ -5,6 +5,12 @@ over an OracleSolver: the operator H = M + h^2 (K_elastic + K_colliders [+ K_nod
 use_self]) restricted to the free degrees of freedom, its Jacobi preconditioner, scipy's MINRES, dJ/drho per element and
 the damped backward-Euler recursion over a trajectory. The product computes the same on the
 GPU (diff_2d_fem_amd.adjoint); this is its checker.
+
+Beyond the reference (whose adjoint refuses pressure chambers): with a Pressure term on the
+solver, its Hessian joins the operator (oracle.bodies.Pressure.hvp) and the recursion also
+accumulates dJ/dp per chamber and step (Pressure.dp) into self.dJdp_steps. Pinned by the
+FD checks of tests/test_oracle.py (Pressure.hvp against differences of its gradient,
+Pressure.dp against differences in p) and by central differences of whole oracle rollouts.
 """
 from __future__ import annotations
 
@@ -29,6 +35,8 @@ class OracleAdjoint:
         if self.free.shape[1] == 1:
             self.free = np.repeat(self.free, 2, axis=1)
         self.mass = np.asarray(solver.mass, np.float64)
+        from .bodies import Pressure
+        self.pressure = next((t for t in getattr(solver, "terms", []) if isinstance(t, Pressure)), None)
 
     # reference implicit/adjoint.py:41-50
     def set_density(self, rho):
@@ -47,6 +55,8 @@ class OracleAdjoint:
             K.collider_hvp(c, x, v, self.s.bnd, self.s.dhat, self.s.kappa, self.h2, y)
         if use_self and self.s.node is not None:
             self.s.node.hvp(x, v, self.h2, y)
+        if self.pressure is not None:
+            self.pressure.hvp(x, v, self.h2, y)
         return self.mass[:, None] * v + y if include_mass else y
 
     # reference implicit/adjoint.py:356-372
@@ -95,10 +105,13 @@ class OracleAdjoint:
         m2 = self.mass[:, None]
         lam = {T + 1: np.zeros((self.N, 2)), T + 2: np.zeros((self.N, 2))}
         g = np.zeros(self.n_tri)
+        self.dJdp_steps = None if self.pressure is None else np.zeros((T, len(self.pressure.p)))
         for k in range(T, 0, -1):
             rhs = (1.0 + drag) * m2 * lam[k + 1] - drag * m2 * lam[k + 2]
             if k in seed:
                 rhs = rhs + seed[k]
             lam[k] = self.adjoint_solve(xs[k], rhs, use_disc, use_sdf, use_self=use_self, **kw)
             g += self.drho(xs[k], lam[k])
+            if self.pressure is not None:
+                self.dJdp_steps[k - 1] = self.pressure.dp(xs[k], lam[k], self.h2)
         return g

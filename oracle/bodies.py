@@ -362,6 +362,27 @@ class Pressure:
         rb = s.x[self.eb] - self.cref[c]
         np.add.at(buf, self.ch2env[c], -s.h2 * self.p[c] * 0.5 * (ra[:, 0] * rb[:, 1] - ra[:, 1] * rb[:, 0]))
 
+    # The adjoint's pieces. Not in the reference, whose adjoint refuses Pressure
+    # (implicit/adjoint.py:376-385); derived from its gradient above: each edge adds
+    # -1/2 h2 p J(x_b - x_a) to both ends, J(u) = (u_y, -u_x).
+    def hvp(self, x, v, h2, y):
+        """y += h2 H_pressure v: the directional derivative of the gradient along v (linear in
+        x, so exact), w = -1/2 h2 p J(v_b - v_a) on both ends of every edge."""
+        u = v[self.eb] - v[self.ea]
+        w = (-0.5 * h2 * self.p[self.ech])[:, None] * np.stack([u[:, 1], -u[:, 0]], 1)
+        np.add.at(y, self.ea, w)
+        np.add.at(y, self.eb, w)
+
+    def dp(self, x, lam, h2):
+        """-lam . dR/dp per chamber, R the gradient of the incremental potential: dR/dp_c puts
+        -1/2 h2 J(x_b - x_a) on both ends of chamber c's edges, so this is
+        1/2 h2 sum_e (lam_a + lam_b) . J(x_b - x_a)."""
+        t = x[self.eb] - x[self.ea]
+        Jt = np.stack([t[:, 1], -t[:, 0]], 1)
+        out = np.zeros(len(self.ch2env))
+        np.add.at(out, self.ech, 0.5 * h2 * ((lam[self.ea] + lam[self.eb]) * Jt).sum(1))
+        return out
+
 
 def _seg_dir(xa, xb):
     t = xb - xa

@@ -449,6 +449,108 @@ def test_pressure_derivatives():
     assert abs(-buf[0] / (9e-6) - 1.0) < 1e-12                 # 3 x 3 px cavity of 1 mm pixels
 
 
+def test_pressure_adjoint_pieces():
+    """The pressure tangent the adjoint adds (Pressure.hvp) is the derivative of the pressure
+    gradient, is symmetric, and Pressure.dp is -lam . dR/dp by differences in p."""
+    ring = np.ones((7, 9), np.uint8)
+    ring[2:5, 3:6] = 0
+    m = Model()
+    h = m.add_soft_body(ring, physical_size=(9e-3, 7e-3), center=(0.0, 0.0), E=2e5, nu=0.4)
+    m.finalize()
+    m.set_pressure(m.add_pressure_chambers(h)[0], 3.0e4)
+    m.set_pressure(m.add_ambient_pressure(h), 1.1e4)
+    lay = m.actuation.pressure_layout(m.body_env)
+    term = Pressure(None, lay["ea"], lay["eb"], lay["chamber"], [0, 0], lay["ref"])
+    p0 = np.array([3.0e4, 1.1e4])
+    term.set(p0)
+    rng = np.random.default_rng(8)
+    x = m.me.rest_pos + 1e-4 * rng.standard_normal(m.me.rest_pos.shape)
+
+    def grad(xx, p=p0):
+        term.set(p)
+        s = _Sol(m.me, xx)
+        term.grad_diag(s)
+        term.set(p0)
+        return s.grad
+    v, u = rng.standard_normal(x.shape), rng.standard_normal(x.shape)
+    Hv = np.zeros_like(x)
+    term.hvp(x, v, 1.0, Hv)
+    hh = 1e-6
+    fd = (grad(x + hh * v) - grad(x - hh * v)) / (2 * hh)
+    assert np.abs(Hv - fd).max() < 1e-7 * np.abs(Hv).max()
+    Hu = np.zeros_like(x)
+    term.hvp(x, u, 1.0, Hu)
+    assert abs((u * Hv).sum() - (v * Hu).sum()) < 1e-12 * np.abs(u * Hv).sum()
+    lam = rng.standard_normal(x.shape)
+    dp = term.dp(x, lam, 1.0)
+    for c in range(2):
+        e = np.zeros(2)
+        e[c] = 1.0
+        fdp = (grad(x, p0 + e) - grad(x, p0 - e)) / 2.0
+        assert abs(dp[c] + (lam * fdp).sum()) < 1e-9 * abs(dp[c])
+
+
+def _finger(n_ch_px=((2, 3, 2, 3), (2, 12, 2, 4))):
+    """A small held finger (6 x 24 px of 0.5 mm) with two sealed chambers."""
+    mask = np.ones((6, 24), np.uint8)
+    for r, c, hh, ww in n_ch_px:
+        mask[r:r + hh, c:c + ww] = 0
+    m = Model()
+    h = m.add_soft_body(mask, physical_size=(12e-3, 3e-3), center=(0.0, 0.0), E=2e5, nu=0.4,
+                        density=1100.0)
+    m.finalize()
+    X = m.rest_x(h)
+    m.set_kinematic_constraint(h, np.nonzero(X[:, 0] < X[:, 0].min() + 1e-9)[0])
+    ch = m.add_pressure_chambers(h)
+    return m, h, ch
+
+
+def test_oracle_adjoint_with_pressure_matches_finite_differences():
+    """The oracle adjoint with the pressure tangent: dJ/dp (constant pressures over the rollout)
+    and dJ/drho against central differences of converged oracle rollouts."""
+    from oracle.adjoint import OracleAdjoint
+    T, dt = 8, 2e-4
+    kw = dict(max_iter=4000, eps=1e-15, tol_x=1e-16, conv_check=8, self_collision=False, friction=False)
+    m, h, ch = _finger()
+    p0 = np.array([2.0e4, 3.5e4])
+    rng = np.random.default_rng(3)
+    X = m.rest_x(h)
+    c = np.zeros((m.n_nodes, 2))                       # J: the tip's height and reach
+    tip = np.nonzero(X[:, 0] > X[:, 0].max() - 1e-9)[0]
+    c[tip, 1], c[tip, 0] = 1.0 / len(tip), 0.3 / len(tip)
+    oi = OracleIntegrator(**kw)
+    oi.attach(m, dt)
+    oa = OracleAdjoint(oi.solver, m.me)
+    rho0 = 0.6 + 0.3 * rng.random(m.n_elements)
+
+    def rollout(p, rho):
+        for k, pk in zip(ch, p):
+            m.set_pressure(k, pk)
+        oa.set_density(rho)
+        s = oi.solver
+        s.x = np.asarray(m.me.rest_pos, np.float64).copy()
+        s.v[:] = 0.0
+        xs = [s.x.copy()]
+        for _ in range(T):
+            oi.simulate(m, dt)
+            xs.append(oi.solver.x.copy())
+        return xs
+    xs = rollout(p0, rho0)
+    oa = OracleAdjoint(oi.solver, m.me)
+    oa.set_density(rho0)
+    g = oa.transient_adjoint(xs, {T: c}, oi.solver.drag, rtol=1e-14)
+    gp = oa.dJdp_steps.sum(0)
+    J = lambda p, rho: float((c * rollout(p, rho)[-1]).sum())
+    for k in range(2):
+        e = np.zeros(2)
+        e[k] = 50.0                     # central differences converge as h^2: 7e-6 at 50 Pa
+        fd = (J(p0 + e, rho0) - J(p0 - e, rho0)) / 100.0
+        assert abs(gp[k] - fd) < 1e-4 * abs(fd), (k, gp[k], fd)
+    d = g / np.abs(g).max()
+    fd = (J(p0, rho0 + 1e-3 * d) - J(p0, rho0 - 1e-3 * d)) / 2e-3
+    assert abs(g @ d - fd) < 1e-4 * abs(fd), (g @ d, fd)
+
+
 @pytest.mark.parametrize("mode", ["tension", "taut"])
 def test_tendon_derivatives(mode):
     m = Model()
