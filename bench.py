@@ -18,8 +18,9 @@ from the identical state, over each scene's block size; `roofline` is the domina
 bytes over its event-timed duration against HBM peak; `element_roofline` is the float64
 elastic gradient+diagonal element kernel alone on this workload's mesh, and
 `element_roofline_large` the same kernel at the size of the star-mesh configuration (45 M
-triangles, working set far beyond the Infinity Cache); `cpu_baseline` times the float64 CPU oracle
-on a bounded sample of the same workload.
+triangles, working set far beyond the Infinity Cache); `adjoint_c4` the differentiable rollout of
+the adjoint configuration (512 pneumatic fingers: forward and backward ms per step);
+`cpu_baseline` times the float64 CPU oracle on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -47,6 +48,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-large", action="store_true", help="skip the 45 M-triangle element-kernel leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the 512-finger adjoint leg")
     return ap.parse_args()
 
 
@@ -276,6 +278,45 @@ def element_roofline_large(n_meshes=32, copies=128, seed=0):
                 owned_frac=round(float(et["owned"].float().mean()), 4))
 
 
+def adjoint_leg(n_fingers=512, T=10, every=5):
+    """The adjoint configuration: 512 pneumatic fingers (diff_2d_fem_amd.workloads,
+    pneumatic_fingers), loss |tip - target|^2 after T steps of dt 2 ms, gradients with respect
+    to every chamber pressure and every element's modulus scale through the autograd rollout
+    (checkpoints every `every` steps, recomputation in the backward pass, float64, per-scene
+    device MINRES at the reference's rtol 1e-9). Times the forward pass and the backward pass
+    (recomputation + one adjoint solve per step) with synchronisation on both sides."""
+    import torch
+    from diff_2d_fem_amd import ImplicitIntegrator
+    from diff_2d_fem_amd import workloads as W
+    from diff_2d_fem_amd.autograd import Rollout
+    w = W.pneumatic_fingers(n_fingers)
+    m = w["model"]
+    tips = torch.as_tensor([int(m.body_nodes(h)[np.argmax(m.rest_x(h)[:, 0])]) for h in w["bodies"]],
+                           device="cuda")
+    targets = torch.as_tensor(w["targets"], device="cuda")
+    integ = ImplicitIntegrator(path="scene", precision="double", self_collision=False)
+    roll = Rollout(m, integ, w["dt"], T, every=every, adjoint_kw=dict(rtol=1e-9, maxiter=5000))
+    scale = torch.ones(m.n_elements, dtype=torch.float64, requires_grad=True)
+    p = torch.tensor([c.pressure for c in m.actuation.chambers], requires_grad=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss = ((roll(scale, p)[tips] - targets) ** 2).sum()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    it = integ.stats(m)["iters"]
+    return dict(workload="pneumatic_fingers", scenes=n_fingers, nodes=m.n_nodes, triangles=m.n_elements,
+                chambers=len(m.actuation.chambers), steps=T, dt=w["dt"], checkpoint_every=every,
+                forward_ms_per_step=round((t1 - t0) / T * 1e3, 3),
+                backward_ms_per_step=round((t2 - t1) / T * 1e3, 3),
+                recomputed_steps=roll.recomputed_steps, pncg_iters_last_step=int(it.max()),
+                minres_iters_last_solve=int(getattr(roll.adj, "minres_iters", 0)),
+                minres_unconverged=int(getattr(roll.adj, "minres_fail", 0)),
+                grad_norm_scale=float(scale.grad.norm()), grad_norm_pressure=float(p.grad.norm()))
+
+
 def cpu_baseline(args, steps):
     """The float64 numpy oracle on one scene of the same workload, from its first step."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
@@ -339,6 +380,8 @@ def main():
         pv = B * args.steps / plain["t"]
         out["plain_fp64"] = dict(value=round(pv, 2), ms_per_step=round(plain["t"] / args.steps * 1e3, 3))
         out["speedup_vs_plain_fp64"] = round(value / pv, 3)
+    if not args.no_adjoint:
+        out["adjoint_c4"] = adjoint_leg()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, args.cpu_steps)
     print(json.dumps(out), flush=True)

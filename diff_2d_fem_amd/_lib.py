@@ -58,6 +58,7 @@ _SCENE_FIELDS = (
     + [("el", ElemTiles)]
     + [("n_sp", C.c_int32), ("max_ag", C.c_int32)]
     + [(k, P) for k in ("sp_a", "sp_b", "sp2env", "sp_anchor", "sp_k", "sp_L0", "sp_start")]
+    + [("env_skip", P), ("sc_list", P), ("n_list", C.c_int32), ("pad4_", C.c_int32)]
 )
 
 

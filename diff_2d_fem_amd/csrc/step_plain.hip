@@ -33,12 +33,15 @@ constexpr int TPB = 256;
     for (int i = base_ + threadIdx.x, once_ = 1; once_; once_ = 0)
 
 __device__ __forceinline__ double fr_of(const S &s, int i) { return s.free_[i] ? 1.0 : 0.0; }
+// scene e is stepped elsewhere (the batch's routing): leave its state alone
+__device__ __forceinline__ bool skip_env(const S &s, int e) { return s.env_skip && s.env_skip[e]; }
+__device__ __forceinline__ bool skip_node(const S &s, int i) { return s.env_skip && s.env_skip[s.node2env[i]]; }
 
 // ------------------------------------------------------------------------- step setup
 // reference implicit/kernels.py:42-68, 438-441
 __global__ void k_init_step(S s, double dt, double drag) {
   GRID_LOOP(i, s.N) {
-    if (i < s.N) {
+    if (i < s.N && !skip_node(s, i)) {
       const V2<double> x = ld2(s.x, i);
       st2(s.xn, i, x);
       st2(s.xhat, i, x + (dt * drag) * ld2(s.v, i));
@@ -47,7 +50,7 @@ __global__ void k_init_step(S s, double dt, double drag) {
 }
 __global__ void k_apply_bc(S s) {
   GRID_LOOP(b, s.n_bc) {
-    if (b < s.n_bc) {
+    if (b < s.n_bc && !skip_node(s, s.bc_idx[b])) {
       const int n = s.bc_idx[b];
       const V2<double> t = ld2(s.bc_target, b);
       st2(s.x, n, t);
@@ -59,14 +62,18 @@ __global__ void k_reset_step(S s) {
   GRID_LOOP(e, s.B) {
     if (e < s.B) {
       s.dE0[e] = 0.0;
-      s.active[e] = 1;
-      s.iters[e] = 0;
+      if (skip_env(s, e)) {
+        s.active[e] = 0;     // its iteration count stays as its own path left it
+      } else {
+        s.active[e] = 1;
+        s.iters[e] = 0;
+      }
     }
   }
 }
 __global__ void k_commit_v(S s, double inv_dt) {
   GRID_LOOP(i, s.N) {
-    if (i < s.N) st2(s.v, i, inv_dt * (ld2(s.x, i) - ld2(s.xn, i)));
+    if (i < s.N && !skip_node(s, i)) st2(s.v, i, inv_dt * (ld2(s.x, i) - ld2(s.xn, i)));
   }
 }
 
@@ -548,7 +555,7 @@ __global__ void k_far_cap(int B, const double *p_inf, double reach, double *alph
 // Kinematic sweep (reference implicit/ipc_contact.py:308-340)
 __global__ void k_bc_motion(S s) {
   GRID_LOOP(b, s.n_bc) {
-    const bool v = b < s.n_bc;
+    const bool v = b < s.n_bc && !skip_node(s, s.bc_idx[b]);
     double l = 0.0;
     int e = 0;
     if (v) {
@@ -563,7 +570,7 @@ __global__ void k_bc_motion(S s) {
 }
 __global__ void k_bc_advance(S s) {
   GRID_LOOP(b, s.n_bc) {
-    if (b < s.n_bc) {
+    if (b < s.n_bc && !skip_node(s, s.bc_idx[b])) {
       const int i = s.bc_idx[b];
       const double t = s.sweep_beta[s.node2env[i]];
       const V2<double> tg = ld2(s.bc_target, b);
@@ -575,7 +582,7 @@ __global__ void k_bc_advance(S s) {
 }
 __global__ void k_bc_wake(S s) {
   GRID_LOOP(e, s.B) {
-    if (e < s.B && s.q_inf[e] > 0.0) s.active[e] = 1;
+    if (e < s.B && s.q_inf[e] > 0.0 && !skip_env(s, e)) s.active[e] = 1;
   }
 }
 __global__ void k_fill(double *a, int n, double v) {
@@ -848,7 +855,7 @@ __global__ void k_affine_update(S s, int which) {
 
 __global__ void k_affine_expand(S s, const double *v6, double *out) {
   GRID_LOOP(k, s.n_an) {
-    if (k < s.n_an) {
+    if (k < s.n_an && !(out == s.x && skip_node(s, s.an_node[k]))) {
       const double *w = s.an_w + 3 * k;
       const double *v = v6 + 6 * s.an_grp[k];
       st2(out, s.an_node[k], V2<double>{w[0] * v[0] + w[1] * v[1] + w[2] * v[2],

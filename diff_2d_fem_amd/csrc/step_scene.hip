@@ -366,7 +366,7 @@ struct Step {
 
   __device__ __forceinline__ Step(S &s_, KPm &p_, LdsScalars &sc_, LdsD *lds)
       : s(s_), p(p_), sc(sc_) {
-    e = blockIdx.x;
+    e = s.sc_list ? s.sc_list[blockIdx.x] : blockIdx.x;
     tid = threadIdx.x;
     blk.red = lds;
     bpos = lds + Block<TPB>::RED_SIZE;
@@ -1860,7 +1860,9 @@ static int launch(const SH &s, const Pm &p, hipStream_t stream) {
   if (lds > 64 * 1024)
     D2F_HIP(hipFuncSetAttribute((const void *)k_scene<R, TPB>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL((k_scene<R, TPB>), dim3(s.B), dim3(TPB), lds, stream, s, p);
+  const int n_wg = s.sc_list ? s.n_list : s.B;
+  if (n_wg == 0) return 0;
+  hipLaunchKernelGGL((k_scene<R, TPB>), dim3(n_wg), dim3(TPB), lds, stream, s, p);
   D2F_HIP(hipGetLastError());
   return 0;
 }
@@ -1895,6 +1897,7 @@ extern "C" int d2f_step_scene(const d2f_scene *s, const d2f_params *p, void *str
   D2F_CHECK(s && p, "d2f_step_scene: null scene or params");
   D2F_CHECK(s->B > 0 && s->N > 0 && s->sc_tri && s->sc_cs && s->sc_tri_start,
             "d2f_step_scene: the scene has no per-scene layout");
+  D2F_CHECK(!s->sc_list || (s->n_list >= 0 && s->n_list <= s->B), "d2f_step_scene: bad scene list");
   D2F_CHECK(s->max_cells >= 1 && s->max_nb >= 0, "d2f_step_scene: bad grid capacity");
   D2F_CHECK(s->excl_bits_lds >= 0 && s->max_pairs >= 0, "d2f_step_scene: negative LDS capacity");
   D2F_CHECK(s->n_ag == 0 || (s->max_ag > 0 && s->ag_start),

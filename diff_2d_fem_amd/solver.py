@@ -73,6 +73,9 @@ class DeviceSolver:
         self.node2env = t(self.node2env_np, I32)
         self.block_size = 512              # scene path: 2 waves per SIMD, no register spills
         self._layout = None
+        self._route_t = None
+        self.scene_envs = np.arange(self.B)
+        self.plain_envs = np.zeros(0, np.int64)
         self.tri2env = t(np.asarray(me.tri2env), I32)
         from .layout import element_tiles
         self.el_tiles = element_tiles(me.tri, self.N, device=self.dev)
@@ -459,6 +462,7 @@ class DeviceSolver:
             if (bl[ends] < 0).any():
                 raise ValueError("a pressure edge has an end that is not a boundary node")
         self.max_cells = int(max_cells)
+        self._route()
         self._plan_lds()
         self._apply_lds_overrides()
         self._layout = True
@@ -467,26 +471,54 @@ class DeviceSolver:
     EXCL_LDS_MAX = 8192              # largest exclusion bitset staged in LDS, 32-bit words
     SKIN = 0.5                       # pair-list margin, in units of the contact reach
 
-    def _plan_lds(self):
-        """Split each scene workgroup's LDS budget (for the current block_size): fixed scratch
-        (reductions, boundary positions and accumulators, grid), then per-node element
-        accumulators, then the bit-packed exclusion matrix, then the candidate-pair list (bit
-        jp * n_b_e + j of a scene's bitset is its exclusion of boundary nodes jp and j)."""
-        self.excl_bits_lds, self.max_pairs, self.nacc_lds, self.xc_lds = 0, 0, 0, 0
-        self.excl_bits = self.excl_bits_off = None
-        nn = np.diff(np.asarray(self.me.node_starts, np.int64))
-        self.max_nn = int(nn.max(initial=0))
+    def _fixed_lds(self, nb, ag):
+        """Bytes of LDS a scene workgroup cannot do without (for the current block_size), for a
+        scene of nb boundary nodes and ag affine groups: block-reduction partials, boundary
+        positions and accumulators, affine group state, the grid and the boundary list."""
         nw = int(self.block_size) // 64            # waves per workgroup
         red = 20 * 4 * nw + 20                     # block reductions: 20 values x 4 rows per wave, totals
+        return 8 * (red + 8 * nb + 26 * ag) + 4 * (2 * self.max_cells + 1 + 2 * nb)
+
+    def _route(self):
+        """Which scenes the scene kernel takes: those whose fixed LDS fits one workgroup's budget.
+        The others are stepped by the per-kernel path, in float64, in the same call (their
+        indices in self.plain_envs; stats() reports them)."""
+        nb = np.asarray(self.me.boundary_per_env, np.int64)
+        ag = (np.diff(self.ag_start.cpu().numpy()).astype(np.int64) if self.affine is not None
+              else np.zeros(self.B, np.int64))
+        fits = np.array([self._fixed_lds(int(nb[e]), int(ag[e])) <= self.LDS_BUDGET for e in range(self.B)], bool)
+        self.scene_envs = np.flatnonzero(fits)
+        self.plain_envs = np.flatnonzero(~fits)
+        if len(self.plain_envs) and not getattr(self, "_warned_route", False):
+            self._warned_route = True
+            warnings.warn(f"scenes {self.plain_envs[:8].tolist()}{' ...' if len(self.plain_envs) > 8 else ''} "
+                          f"are too large for one workgroup's LDS: they take the per-kernel path "
+                          f"(float64); the other {len(self.scene_envs)} keep the scene path.", stacklevel=4)
+        self._route_t = None
+        if len(self.plain_envs):
+            self._route_t = dict(skip=self._t(fits.astype(np.uint8), U8),
+                                 list=self._t(self.scene_envs.astype(np.int32), I32))
+
+    def _plan_lds(self):
+        """Split each scene workgroup's LDS budget (for the current block_size) over the scenes
+        the scene kernel takes: fixed scratch (reductions, boundary positions and accumulators,
+        grid), then per-node element accumulators, then the bit-packed exclusion matrix, then
+        the candidate-pair list (bit jp * n_b_e + j of a scene's bitset is its exclusion of
+        boundary nodes jp and j)."""
+        self.excl_bits_lds, self.max_pairs, self.nacc_lds, self.xc_lds = 0, 0, 0, 0
+        self.excl_bits = self.excl_bits_off = None
+        sub = self.scene_envs
+        nn = np.diff(np.asarray(self.me.node_starts, np.int64))[sub]
+        self.max_nn = int(nn.max(initial=0))
+        self.sc_max_nb = int(np.asarray(self.me.boundary_per_env, np.int64)[sub].max(initial=0))
         self.max_ag = 0                            # affine groups of the largest scene: 26 doubles each
         if self.affine is not None:
-            self.max_ag = int(np.diff(self.ag_start.cpu().numpy()).max(initial=0))
-        base = (8 * (red + 8 * self.max_nb + 26 * self.max_ag)
-                + 4 * (2 * self.max_cells + 1 + 2 * self.max_nb))
+            self.max_ag = int(np.diff(self.ag_start.cpu().numpy())[sub].max(initial=0))
+        base = self._fixed_lds(self.sc_max_nb, self.max_ag)
         if base + 32 * self.max_nn <= self.LDS_BUDGET:
             self.nacc_lds = 1
             base += 32 * self.max_nn
-        if base + 16 * self.max_nn + 4 * 4 * self.max_nb <= self.LDS_BUDGET:
+        if base + 16 * self.max_nn + 4 * 4 * self.sc_max_nb <= self.LDS_BUDGET:
             self.xc_lds = 1                  # positions, if a modest pair list still fits
             base += 16 * self.max_nn
         if (self.ipc is None or self.n_e == 0) and self.node is None:
@@ -503,15 +535,15 @@ class DeviceSolver:
             woff.append(woff[-1] + len(bits) // 4)
         self.excl_bits = self._t(np.concatenate(words).view(np.int32), I32)
         self.excl_bits_off = self._t(np.asarray(woff, np.int64), I64)
-        wmax = int(np.max(np.diff(woff)))
+        wmax = int(np.max(np.diff(woff)[sub], initial=0))
         if wmax <= self.EXCL_LDS_MAX and base + 4 * wmax <= self.LDS_BUDGET:
             self.excl_bits_lds = wmax
         if self.ipc is None:             # node contact: partners come from the grid itself
             return
         left = (self.LDS_BUDGET - base - 4 * self.excl_bits_lds) // 4
-        e_per = np.bincount(self.e2env.cpu().numpy(), minlength=self.B)
-        if self.max_nb < 65536 and e_per.max(initial=0) < 32768:
-            self.max_pairs = int(max(0, min(left, 16 * self.max_nb)))
+        e_per = np.bincount(self.e2env.cpu().numpy(), minlength=self.B)[sub]
+        if self.sc_max_nb < 65536 and e_per.max(initial=0) < 32768:
+            self.max_pairs = int(max(0, min(left, 16 * self.sc_max_nb)))
 
     def _apply_lds_overrides(self):
         """Caps on the LDS plan, set through `lds_overrides` (a dict with any of nacc_lds,
@@ -549,7 +581,9 @@ class DeviceSolver:
         st.max_nodes = et["max_nodes"]
         return st
 
-    def scene_struct(self):
+    def scene_struct(self, part=None):
+        """The d2f_scene of this batch. part "plain": the per-kernel path's share of a routed
+        batch (the scene kernel's scenes skipped); part "scene": the scene kernel's share."""
         s = _lib.Scene()
         s.B, s.N, s.T, s.n_b, s.n_bc = self.B, self.N, self.n_tri, self.n_b, self.n_bc
         s.n_e = self.n_e if self.ipc is not None else 0
@@ -604,7 +638,7 @@ class DeviceSolver:
                 s.pe_start = p("pe_start")
             if self.tendon is not None:
                 s.td_start, s.ey_start = p("td_start"), p("ey_start")
-            s.max_nb, s.max_cells = self.max_nb, self.max_cells
+            s.max_nb, s.max_cells = (self.max_nb if part == "plain" else self.sc_max_nb), self.max_cells
             if self.excl_bits is not None:
                 s.excl_bits, s.excl_bits_off = p("excl_bits"), p("excl_bits_off")
             s.excl_bits_lds, s.max_pairs = self.excl_bits_lds, self.max_pairs
@@ -614,6 +648,11 @@ class DeviceSolver:
             s.n_sp = self.springs["n"]
             for k in ("sp_a", "sp_b", "sp2env", "sp_anchor", "sp_k", "sp_L0", "sp_start"):
                 setattr(s, k, p(k))
+        if part is not None and self._route_t is not None:
+            if part == "plain":
+                s.env_skip = self._ptr(self._route_t["skip"])
+            else:
+                s.sc_list, s.n_list = self._ptr(self._route_t["list"]), len(self.scene_envs)
         s.n_col = len(self.colliders)
         for i, c in enumerate(self.colliders):
             s.col[i].kind = int(c["kind"])
@@ -662,18 +701,30 @@ class DeviceSolver:
         stream = C.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
         if self.path == "plain":
             _lib.check(L.d2f_step_plain(C.byref(s), C.byref(q), stream), "d2f_step_plain")
+        elif self._route_t is None:
+            _lib.check(L.d2f_step_scene(C.byref(s), C.byref(q), stream), "d2f_step_scene")
         else:
-            if not hasattr(L, "d2f_step_scene"):
-                raise RuntimeError("libd2fem_hip.so has no d2f_step_scene (the scene-persistent path)")
-            rc = L.d2f_step_scene(C.byref(s), C.byref(q), stream)
-            if rc == _lib.D2F_ECAP:
-                # a scene too large for one workgroup's LDS: the per-kernel path takes any size
-                msg = _lib.lib().d2f_last_error().decode(errors="replace")
-                warnings.warn(f"{msg}. Stepping with path='plain' (float64) from here on.", stacklevel=3)
-                self.path, self.precision = "plain", "double"
-                _lib.check(L.d2f_step_plain(C.byref(s), C.byref(self.params_struct()), stream), "d2f_step_plain")
-                return
-            _lib.check(rc, "d2f_step_scene")
+            # scenes too large for the scene kernel first, on the per-kernel path (the others'
+            # nodes and scalars untouched; their affine group state is put back, since the
+            # group kernels run over every group), then the scene kernel on its scenes
+            keep = [getattr(self, k).clone() for k in self._GROUP_STATE if getattr(self, k, None) is not None]
+            sp = self.scene_struct("plain")
+            _lib.check(L.d2f_step_plain(C.byref(sp), C.byref(q), stream), "d2f_step_plain")
+            if keep:
+                g = self._scene_groups()
+                for k, v in zip([k for k in self._GROUP_STATE if getattr(self, k, None) is not None], keep):
+                    getattr(self, k)[g] = v[g]
+            ss = self.scene_struct("scene")
+            _lib.check(L.d2f_step_scene(C.byref(ss), C.byref(q), stream), "d2f_step_scene")
+
+    _GROUP_STATE = ("ag_q", "ag_pq", "ag_gq", "ag_gq_prev", "ag_zq", "ag_Hx", "ag_Hy")
+
+    def _scene_groups(self):
+        """Affine groups of the scene kernel's scenes (a device index tensor)."""
+        if getattr(self, "_sg", None) is None:
+            env = self.ag2env.cpu().numpy()
+            self._sg = torch.as_tensor(np.flatnonzero(np.isin(env, self.scene_envs)), device=self.dev)
+        return self._sg
 
     def min_distance(self):
         """Smallest node-edge distance per environment (1e30 where none is within reach)."""

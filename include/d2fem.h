@@ -156,6 +156,13 @@ typedef struct d2f_scene {
   const double *sp_anchor;              /* 2 n_sp */
   const double *sp_k, *sp_L0;           /* n_sp */
   const int32_t *sp_start;              /* B+1 springs per scene (scene path) */
+  /* routing of a batch between the two paths (both NULL: every scene on the called path).
+   * env_skip[e] = 1: d2f_step_plain leaves scene e untouched (its nodes, targets, scalars and
+   * iteration count); sc_list: d2f_step_scene steps only the n_list scenes listed, one
+   * workgroup each (the LDS plan fields then describe the listed scenes). */
+  const uint8_t *env_skip;              /* B, or NULL */
+  const int32_t *sc_list;               /* n_list, or NULL */
+  int32_t n_list, pad4_;
 } d2f_scene;
 
 /* Solver options and step constants (reference PNCG2DSolver.__init__, implicit/solver.py:97-226,

@@ -239,27 +239,49 @@ def test_scene_fallback_paths(override, precision):
     assert err < (1e-9 if precision == "double" else 1e-5)
 
 
-def test_scene_too_large_for_lds_falls_back_to_plain():
-    """A scene whose boundary overflows one workgroup's LDS (a 2 x 3000 px strip: about 6,000
-    boundary nodes) makes d2f_step_scene return D2F_ECAP; the solver warns and steps it with the
-    per-kernel path: the plain path's result, up to the order of its float64 atomics."""
-    def make():
-        from diff_2d_fem_amd import Model
+def test_scene_too_large_for_lds_takes_the_plain_path_alone():
+    """A batch with one scene whose boundary overflows one workgroup's LDS (a 2 x 3000 px strip:
+    about 6,000 boundary nodes) and two small ones: the large scene is stepped by the per-kernel
+    path (float64) and the others keep the scene path and its precision. Each environment
+    matches the same environment stepped by its own path in a batch of one kind (the order of
+    float64 atomics aside)."""
+    from diff_2d_fem_amd import Model
+
+    def make(big=True):
         m = Model()
         m.add_ground(0.0, mu=0.5)
-        m.add_soft_body(np.ones((2, 3000), np.uint8), physical_size=(1.5, 0.001), center=(0.0, 0.0008),
-                        E=0.5e6, nu=0.40, density=1100.0)
+        for i in range(3):
+            if i == 1 and big:
+                m.add_soft_body(np.ones((2, 3000), np.uint8), physical_size=(1.5, 0.001),
+                                center=(0.0, 0.0008), E=0.5e6, nu=0.40, density=1100.0, env_id=i)
+            else:
+                m.add_soft_body(np.ones((4, 12), np.uint8), physical_size=(0.012, 0.004),
+                                center=(0.0, 0.0025), E=0.5e6, nu=0.40, density=1100.0, env_id=i)
         m.finalize()
         m.gravity = (0.0, -9.81)
         return m
     kw = dict(max_iter=20, eps=0.0, conv_check=0, self_collision=False)
-    ms, mp = make(), make()
-    gs = ImplicitIntegrator(path="scene", precision="mixed", **kw)
+    ms, mp, mr = make(), make(), make()
+    gs = ImplicitIntegrator(path="scene", precision="double", **kw)
     gp = ImplicitIntegrator(path="plain", precision="double", **kw)
-    with pytest.warns(UserWarning, match="path='plain'"):
+    with pytest.warns(UserWarning, match="per-kernel path"):
         gs.simulate(ms, 1e-3)
     gp.simulate(mp, 1e-3)
-    assert ms._solver.path == "plain" and ms._solver.precision == "double"
-    gs.simulate(ms, 1e-3)
-    gp.simulate(mp, 1e-3)
-    assert np.abs(ms.get_x() - mp.get_x()).max() < 1e-12
+    s = ms._solver
+    assert s.path == "scene" and list(s.plain_envs) == [1] and list(s.scene_envs) == [0, 2]
+    assert list(gs.stats(ms)["plain_envs"]) == [1]
+    for _ in range(2):
+        gs.simulate(ms, 1e-3)
+        gp.simulate(mp, 1e-3)
+    env = np.asarray(ms.me.node2env)
+    xs, xp = ms.get_x(), mp.get_x()
+    assert np.abs(xs[env == 1] - xp[env == 1]).max() < 1e-12
+    # the small scenes against a batch that holds only small scenes, all on the scene path
+    small = make(big=False)
+    gq = ImplicitIntegrator(path="scene", precision="double", **kw)
+    for _ in range(3):
+        gq.simulate(small, 1e-3)
+    envq = np.asarray(small.me.node2env)
+    for e in (0, 2):
+        assert np.abs(xs[env == e] - small.get_x()[envq == e]).max() < 1e-12
+    assert (gs.stats(ms)["iters"] == 20).all()
