@@ -111,6 +111,7 @@ def lib():
     L.d2f_struct_sizes.argtypes = [C.POINTER(C.c_int64)]
     L.d2f_step_plain.argtypes = [C.POINTER(Scene), C.POINTER(Params), P]
     L.d2f_ipc_min_distance.argtypes = [C.POINTER(Scene), C.POINTER(Params), P]
+    L.d2f_ipc_pairs.argtypes = [C.POINTER(Scene), C.POINTER(Params), P, C.c_int32, P]
     if hasattr(L, "d2f_step_scene"):
         L.d2f_step_scene.argtypes = [C.POINTER(Scene), C.POINTER(Params), P]
     i32, f64 = C.c_int32, C.c_double

@@ -204,9 +204,12 @@ static int launch_elastic_grad_diag(const double *x, const int32_t *tri, const d
   if (tiles && tiles->node_off && tiles->max_nodes > 0 && tiles->max_nodes <= D2F_ELEM_NODES_MAX) {
     D2F_CHECK(tiles->nodes && tiles->owned && tiles->tri_local, "elastic grad/diag: incomplete tiles");
     const int blocks = (T + D2F_ELEM_CHUNK - 1) / D2F_ELEM_CHUNK;
-    hipLaunchKernelGGL(k_elastic_grad_diag_tiled, dim3(blocks), dim3(ELEM_TPB),
-                       sizeof(double) * 6 * (size_t)tiles->max_nodes, st, x, Bm, area, mu, lam, T, h2,
-                       clamp, *tiles, grad, diag);
+    const size_t lds = sizeof(double) * 6 * (size_t)tiles->max_nodes;
+    if (lds > 64 * 1024)   // beyond the default dynamic LDS limit (up to 96 KiB at D2F_ELEM_NODES_MAX)
+      D2F_HIP(hipFuncSetAttribute((const void *)k_elastic_grad_diag_tiled,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_elastic_grad_diag_tiled, dim3(blocks), dim3(ELEM_TPB), lds, st, x, Bm, area, mu, lam,
+                       T, h2, clamp, *tiles, grad, diag);
   } else {
     const int blocks = (int)std::min<long>((T + 255) / 256, 65536L);
     hipLaunchKernelGGL(k_elastic_grad_diag, dim3(blocks), dim3(256), 0, st, x, tri, Bm, area, mu, lam, T, h2,
@@ -261,18 +264,32 @@ __global__ void k_extforce_grad(S s, double h2) {
 struct PairCtx {
   double dhat, kappa, mu, epsv, inv_dt, h2, reach, gap, floor_;
   int per_body;
+  int *plist;  // mode 4: plist[0] counts the pairs, (edge, node) pairs from plist[2] on
+  int pcap;    // capacity of plist in pairs
 };
 
-// One barrier-and-friction term (reference implicit/ipc_contact.py:131-171).
-__device__ inline void ipc_term(const S &s, int mode, const PairCtx &c, double coef, int ip, int ia,
-                                int ib, double wa, double wb, double d, V2<double> n, double ddd,
-                                int e, double mu, const double *pdir, double *grad, double *diag,
-                                const uint8_t *freev, double *pHp, double *E) {
+// The scene's boundary nodes as a pair pass reads them: positions, previous positions and the
+// pass's direction from LDS copies indexed by local boundary index when staged (x, xn, pd not
+// null), else from global memory by node id; exclusion from an LDS bitset when staged (bits
+// not null: bit jp * n_be + j excludes local nodes jp and j), else from the byte matrix.
+struct BView {
+  const double *x, *xn, *pd;
+  const uint32_t *bits;
+};
+
+// One barrier-and-friction term (reference implicit/ipc_contact.py:131-171) of node ip against
+// the edge point wa x_a + wb x_b. sp, sa, sb: the three nodes' slips x - xn; qp, qa, qb: their
+// direction (mode 1). Mode 0 adds into grad / diag; modes 1 (pHp) and 2 (energy) add the
+// term's scalar to acc, which the caller reduces into the environment's total.
+__device__ __forceinline__ void ipc_term(int mode, const PairCtx &c, double coef, int ip, int ia, int ib,
+                                         double wa, double wb, double d, V2<double> n, double ddd, int e,
+                                         double mu, V2<double> sp, V2<double> sa, V2<double> sb,
+                                         V2<double> qp, V2<double> qa, V2<double> qb, double *grad,
+                                         double *diag, const uint8_t *freev, double &acc) {
   const double bp = barrier_bp(d, c.dhat);
   const double f_n = -c.kappa * bp;
   const V2<double> tn = {-n.y, n.x};
-  const V2<double> u = (ld2(s.x, ip) - ld2(s.xn, ip)) - wa * (ld2(s.x, ia) - ld2(s.xn, ia)) -
-                       wb * (ld2(s.x, ib) - ld2(s.xn, ib));
+  const V2<double> u = sp - wa * sa - wb * sb;
   const double ut = dot(u, tn);
   const double vt = fabs(ut) * c.inv_dt;
   const double kt = mu * f_n * c.inv_dt / fmax(vt, c.epsv);
@@ -290,155 +307,422 @@ __device__ inline void ipc_term(const S &s, int mode, const PairCtx &c, double c
     atomic_add2(diag, ib, (coef * wb * wb) * hd);
   } else if (mode == 1) {
     const double bpp = barrier_bpp(d, c.dhat);
-    const V2<double> dq = ld2(pdir, ip) - wa * ld2(pdir, ia) - wb * ld2(pdir, ib);
+    const V2<double> dq = qp - wa * qa - wb * qb;
     const double dn = dot(dq, n), dt_ = dot(dq, tn);
     const double val = fmax(c.kappa * (bpp * dn * dn + bp * ddd), 0.0);
-    atomicAdd(pHp + e, coef * c.h2 * (val + kt * dt_ * dt_));
+    acc += coef * c.h2 * (val + kt * dt_ * dt_);
   } else {
     const double asl = c.epsv / c.inv_dt;
     const double au = fabs(ut);
     const double f0 = au < asl ? au * au / (2.0 * asl) : au - 0.5 * asl;
-    atomicAdd(E + e, coef * c.h2 * (c.kappa * barrier_b(d, c.dhat) + mu * f_n * f0));
+    acc += coef * c.h2 * (c.kappa * barrier_b(d, c.dhat) + mu * f_n * f0);
   }
 }
 
-// Every boundary node near one edge (reference implicit/ipc_contact.py:175-296), with exact
-// enumeration over the edge's environment in place of the hash-grid query. mode 0 grad+diag,
-// 1 pHp + clearance + step limit, 2 energy, 3 clearance + step limit along pdir.
-// bp, bn: when not null, the environment's boundary positions and node ids staged in LDS
-// (indexed by local boundary index); the distance filter then runs before the exclusion
-// lookup (both are pure filters, so the pair set is the same).
-__device__ inline void ipc_edge(const S &s, int k, int mode, const PairCtx &c, const double *pdir,
-                                double *grad, double *diag, const uint8_t *freev, double *pHp,
-                                double *E, double *clr, double *alpha, double t_c, const double *mu_node,
-                                const double *bp = nullptr, const int *bn = nullptr) {
-  const int a = s.ea[k], b = s.eb[k], e = s.e2env[k];
-  const int start = s.bnd_start[e], n_be = s.bnd_start[e + 1] - start;
-  const int64_t eoff = s.excl_off[e];
-  const V2<double> xa = ld2(s.x, a), xb = ld2(s.x, b);
-  const V2<double> ab = xb - xa;
-  const double L2 = fmax(dot(ab, ab), 1.0e-30);
-  const double L = sqrt(L2);
-  const double dlo = 1.0e-9 * c.dhat, onset = 0.02 * c.dhat;
-  const int la = s.ela[k], lb = s.elb[k];
-  double amin = BIG, cmin = BIG;
-  for (int jl = 0; jl < n_be; ++jl) {
-    const int pn = bn ? bn[jl] : s.bnd[start + jl];
-    if (pn == a || pn == b) continue;
-    const uint8_t *row = s.excl + eoff + (int64_t)jl * n_be;
-    if (!bp && (row[la] || row[lb])) continue;
-    const V2<double> xp = bp ? V2<double>{bp[2 * jl], bp[2 * jl + 1]} : ld2(s.x, pn);
-    const V2<double> w = xp - xa;
-    const double tt = dot(w, ab) / L2;
-    V2<double> dvec = w - tt * ab;
-    if (tt <= 0.0) dvec = w;
-    else if (tt >= 1.0) dvec = xp - xb;
-    double d = len(dvec);
-    if (!(d < c.reach)) continue;
-    if (bp && (row[la] || row[lb])) continue;
-    if (mode == 1 || mode == 3) {
-      cmin = fmin(cmin, d);
-      double keep = fmax(c.gap * d, c.dhat - onset);
-      if (d < c.dhat) keep = d - fmin(2.0 * (c.dhat - d) + onset, 0.5 * d);
-      if (mode == 3) keep = fmin(d, fmax(c.gap * d, c.floor_));
-      amin = fmin(amin, accd(xp, xa, xb, ld2(pdir, pn), ld2(pdir, a), ld2(pdir, b), keep, t_c));
-    }
-    if (mode == 3) continue;
-    double mu_p = c.mu;
-    if (c.per_body) {
-      const double mi = mu_node[pn], mj = mu_node[a], sm = mi + mj;
-      mu_p = sm > 0.0 ? 2.0 * mi * mj / sm : 0.0;
-    }
-    if (tt > 0.0 && tt < 1.0 && d < c.dhat) {
-      d = fmax(d, dlo);
-      double ddd = 0.0;
-      if (mode == 1) {
-        const V2<double> de = ld2(pdir, b) - ld2(pdir, a);
-        const V2<double> dw = ld2(pdir, pn) - ld2(pdir, a);
-        const double c0 = cross(ab, w);
-        const double c1 = cross(de, w) + cross(ab, dw);
-        const double c2 = 2.0 * cross(de, dw);
-        const double L1 = dot(ab, de) / L;
-        const double Lq = (dot(de, de) - L1 * L1) / L;
-        const double s0 = c0 / L;
-        const double s1 = (c1 - s0 * L1) / L;
-        ddd = (c2 - 2.0 * s1 * L1 - s0 * Lq) / L;
-        if (c0 < 0.0) ddd = -ddd;
-      }
-      ipc_term(s, mode, c, 1.0, pn, a, b, 1.0 - tt, tt, d, (1.0 / d) * dvec, ddd, e, mu_p, pdir,
-               grad, diag, freev, pHp, E);
-    }
-    double coef_a = -s.ca[k];
-    if (tt <= 0.0) coef_a += 1.0;
-    double d_a = len(w);
-    if (coef_a != 0.0 && d_a < c.dhat) {
-      d_a = fmax(d_a, dlo);
-      const V2<double> n_a = (1.0 / d_a) * w;
-      double ddd = 0.0;
-      if (mode == 1) {
-        const V2<double> dw = ld2(pdir, pn) - ld2(pdir, a);
-        const double dn = dot(dw, n_a);
-        ddd = (dot(dw, dw) - dn * dn) / d_a;
-      }
-      ipc_term(s, mode, c, coef_a, pn, a, b, 1.0, 0.0, d_a, n_a, ddd, e, mu_p, pdir, grad, diag,
-               freev, pHp, E);
-    }
-    double coef_b = -s.cb[k];
-    if (tt >= 1.0) coef_b += 1.0;
-    const V2<double> wb_ = xp - xb;
-    double d_b = len(wb_);
-    if (coef_b != 0.0 && d_b < c.dhat) {
-      d_b = fmax(d_b, dlo);
-      const V2<double> n_b = (1.0 / d_b) * wb_;
-      double ddd = 0.0;
-      if (mode == 1) {
-        const V2<double> dw = ld2(pdir, pn) - ld2(pdir, b);
-        const double dn = dot(dw, n_b);
-        ddd = (dot(dw, dw) - dn * dn) / d_b;
-      }
-      ipc_term(s, mode, c, coef_b, pn, a, b, 0.0, 1.0, d_b, n_b, ddd, e, mu_p, pdir, grad, diag,
-               freev, pHp, E);
-    }
+// One boundary edge as the pair terms read it (positions, slips x - xn and direction of its
+// ends as the mode needs them).
+struct EdgeData {
+  int a, b, e, la, lb;
+  V2<double> xa, xb, sa, sb, qa, qb, ab;
+  double L2, L, ca, cb;
+};
+
+__device__ __forceinline__ V2<double> bv_x(const S &s, const BView &bv, int jl, int node) {
+  return bv.x ? V2<double>{bv.x[2 * jl], bv.x[2 * jl + 1]} : ld2(s.x, node);
+}
+__device__ __forceinline__ V2<double> bv_xn(const S &s, const BView &bv, int jl, int node) {
+  return bv.xn ? V2<double>{bv.xn[2 * jl], bv.xn[2 * jl + 1]} : ld2(s.xn, node);
+}
+__device__ __forceinline__ V2<double> bv_pd(const BView &bv, const double *pdir, int jl, int node) {
+  return bv.pd ? V2<double>{bv.pd[2 * jl], bv.pd[2 * jl + 1]} : ld2(pdir, node);
+}
+
+__device__ __forceinline__ EdgeData edge_data(const S &s, int k, int mode, const BView &bv, const double *pdir) {
+  EdgeData d;
+  d.a = s.ea[k];
+  d.b = s.eb[k];
+  d.e = s.e2env[k];
+  d.la = s.ela[k];
+  d.lb = s.elb[k];
+  d.ca = s.ca[k];
+  d.cb = s.cb[k];
+  d.xa = bv_x(s, bv, d.la, d.a);
+  d.xb = bv_x(s, bv, d.lb, d.b);
+  const bool slips = mode == 0 || mode == 1 || mode == 2;
+  const bool dirs = mode == 1 || mode == 3;
+  d.sa = slips ? d.xa - bv_xn(s, bv, d.la, d.a) : V2<double>{0.0, 0.0};
+  d.sb = slips ? d.xb - bv_xn(s, bv, d.lb, d.b) : V2<double>{0.0, 0.0};
+  d.qa = dirs ? bv_pd(bv, pdir, d.la, d.a) : V2<double>{0.0, 0.0};
+  d.qb = dirs ? bv_pd(bv, pdir, d.lb, d.b) : V2<double>{0.0, 0.0};
+  d.ab = d.xb - d.xa;
+  d.L2 = fmax(dot(d.ab, d.ab), 1.0e-30);
+  d.L = sqrt(d.L2);
+  return d;
+}
+
+// The pair filter (reference implicit/ipc_contact.py:212-235): node jl of the edge's scene is
+// not an end, is closer than r to the edge, and is not excluded against either end.
+__device__ __forceinline__ bool pair_in(const S &s, const EdgeData &ed, int jl, V2<double> xp, double r,
+                                        int n_be, int64_t eoff, const BView &bv) {
+  if (jl == ed.la || jl == ed.lb) return false;
+  const V2<double> w = xp - ed.xa;
+  const double tt = dot(w, ed.ab) / ed.L2;
+  V2<double> dvec = w - tt * ed.ab;
+  if (tt <= 0.0) dvec = w;
+  else if (tt >= 1.0) dvec = xp - ed.xb;
+  if (!(len(dvec) < r)) return false;
+  if (bv.bits) {
+    const int64_t i0 = (int64_t)jl * n_be + ed.la, i1 = (int64_t)jl * n_be + ed.lb;
+    return !(((bv.bits[i0 >> 5] >> (i0 & 31)) & 1u) || ((bv.bits[i1 >> 5] >> (i1 & 31)) & 1u));
   }
-  if (mode == 1 || mode == 3) {
-    if (cmin < BIG) atomic_min_d(clr + e, cmin);
-    if (amin < BIG) atomic_min_d(alpha + e, amin);
+  const uint8_t *row = s.excl + eoff + (int64_t)jl * n_be;
+  return row[ed.la] == 0 && row[ed.lb] == 0;
+}
+
+// The terms of one pair that passed the filter within the reach (reference
+// implicit/ipc_contact.py:175-296): clearance and step limit (modes 1, 3), then the interior
+// term and the endpoint terms with their vertex-correction coefficients. amin / cmin take the
+// minimum with the pair's step limit and clearance, acc adds its pHp (mode 1) or energy
+// (mode 2). ip: the node's id (-1: read it from the scene's boundary list).
+__device__ __forceinline__ void pair_body(const S &s, int k, int mode, const PairCtx &c, const EdgeData &ed,
+                                          int jl, int ip, V2<double> xp, const double *pdir, double *grad,
+                                          double *diag, const uint8_t *freev, double t_c,
+                                          const double *mu_node, const BView &bv, double &amin,
+                                          double &cmin, double &acc) {
+  const bool dirs = mode == 1 || mode == 3;
+  if (ip < 0) ip = s.bnd[s.bnd_start[ed.e] + jl];
+  if (mode == 4) {
+    const int slot = atomicAdd(c.plist, 1);
+    if (slot < c.pcap) {
+      c.plist[2 + 2 * slot] = k;
+      c.plist[3 + 2 * slot] = ip;
+    }
+    return;
+  }
+  const double dlo = 1.0e-9 * c.dhat, onset = 0.02 * c.dhat;
+  const V2<double> w = xp - ed.xa;
+  const double tt = dot(w, ed.ab) / ed.L2;
+  V2<double> dvec = w - tt * ed.ab;
+  if (tt <= 0.0) dvec = w;
+  else if (tt >= 1.0) dvec = xp - ed.xb;
+  double d = len(dvec);
+  const V2<double> qp = dirs ? bv_pd(bv, pdir, jl, ip) : V2<double>{0.0, 0.0};
+  if (dirs) {
+    cmin = fmin(cmin, d);
+    double keep = fmax(c.gap * d, c.dhat - onset);
+    if (d < c.dhat) keep = d - fmin(2.0 * (c.dhat - d) + onset, 0.5 * d);
+    if (mode == 3) keep = fmin(d, fmax(c.gap * d, c.floor_));
+    amin = fmin(amin, accd(xp, ed.xa, ed.xb, qp, ed.qa, ed.qb, keep, t_c));
+  }
+  if (mode == 3) return;
+  if (!(d < c.dhat || len(w) < c.dhat || len(xp - ed.xb) < c.dhat)) return;
+  const V2<double> sp = xp - bv_xn(s, bv, jl, ip);
+  double mu_p = c.mu;
+  if (c.per_body) {
+    const double mi = mu_node[ip], mj = mu_node[ed.a], sm = mi + mj;
+    mu_p = sm > 0.0 ? 2.0 * mi * mj / sm : 0.0;
+  }
+  const int a = ed.a, b = ed.b, e = ed.e;
+  if (tt > 0.0 && tt < 1.0 && d < c.dhat) {
+    d = fmax(d, dlo);
+    double ddd = 0.0;
+    if (mode == 1) {
+      const V2<double> de = ed.qb - ed.qa;
+      const V2<double> dw = qp - ed.qa;
+      const double c0 = cross(ed.ab, w);
+      const double c1 = cross(de, w) + cross(ed.ab, dw);
+      const double c2 = 2.0 * cross(de, dw);
+      const double L1 = dot(ed.ab, de) / ed.L;
+      const double Lq = (dot(de, de) - L1 * L1) / ed.L;
+      const double s0 = c0 / ed.L;
+      const double s1 = (c1 - s0 * L1) / ed.L;
+      ddd = (c2 - 2.0 * s1 * L1 - s0 * Lq) / ed.L;
+      if (c0 < 0.0) ddd = -ddd;
+    }
+    ipc_term(mode, c, 1.0, ip, a, b, 1.0 - tt, tt, d, (1.0 / d) * dvec, ddd, e, mu_p, sp, ed.sa, ed.sb, qp,
+             ed.qa, ed.qb, grad, diag, freev, acc);
+  }
+  double coef_a = -ed.ca;
+  if (tt <= 0.0) coef_a += 1.0;
+  double d_a = len(w);
+  if (coef_a != 0.0 && d_a < c.dhat) {
+    d_a = fmax(d_a, dlo);
+    const V2<double> n_a = (1.0 / d_a) * w;
+    double ddd = 0.0;
+    if (mode == 1) {
+      const V2<double> dw = qp - ed.qa;
+      const double dn = dot(dw, n_a);
+      ddd = (dot(dw, dw) - dn * dn) / d_a;
+    }
+    ipc_term(mode, c, coef_a, ip, a, b, 1.0, 0.0, d_a, n_a, ddd, e, mu_p, sp, ed.sa, ed.sb, qp, ed.qa, ed.qb,
+             grad, diag, freev, acc);
+  }
+  double coef_b = -ed.cb;
+  if (tt >= 1.0) coef_b += 1.0;
+  const V2<double> wb_ = xp - ed.xb;
+  double d_b = len(wb_);
+  if (coef_b != 0.0 && d_b < c.dhat) {
+    d_b = fmax(d_b, dlo);
+    const V2<double> n_b = (1.0 / d_b) * wb_;
+    double ddd = 0.0;
+    if (mode == 1) {
+      const V2<double> dw = qp - ed.qb;
+      const double dn = dot(dw, n_b);
+      ddd = (dot(dw, dw) - dn * dn) / d_b;
+    }
+    ipc_term(mode, c, coef_b, ip, a, b, 0.0, 1.0, d_b, n_b, ddd, e, mu_p, sp, ed.sa, ed.sb, qp, ed.qa, ed.qb,
+             grad, diag, freev, acc);
   }
 }
 
 // t_c (the step the ACCD may stop at) is each environment's alpha at launch, read from
-// a snapshot so that every pair sees the same value.
+// a snapshot so that every pair sees the same value. One thread per edge, every boundary node of
+// its scene a candidate.
 __global__ void k_ipc(S s, int mode, PairCtx c, const double *pdir, double *grad, double *diag,
                       const uint8_t *freev, double *E, double *clr, double *alpha, const double *tc) {
   GRID_LOOP(k, s.n_e) {
-    if (k < s.n_e)
-      ipc_edge(s, k, mode, c, pdir, grad, diag, freev, s.pHp, E, clr, alpha,
-               (mode == 1 || mode == 3) ? tc[s.e2env[k]] : 0.0, s.mu_node);
+    if (k < s.n_e) {
+      const BView bv{nullptr, nullptr, nullptr, nullptr};
+      const EdgeData ed = edge_data(s, k, mode, bv, pdir);
+      const int start = s.bnd_start[ed.e], n_be = s.bnd_start[ed.e + 1] - start;
+      const int64_t eoff = s.excl_off[ed.e];
+      const double t_c = (mode == 1 || mode == 3) ? tc[ed.e] : 0.0;
+      double amin = BIG, cmin = BIG, acc = 0.0;
+      for (int jl = 0; jl < n_be; ++jl) {
+        const int pn = s.bnd[start + jl];
+        const V2<double> xp = ld2(s.x, pn);
+        if (pair_in(s, ed, jl, xp, c.reach, n_be, eoff, bv))
+          pair_body(s, k, mode, c, ed, jl, pn, xp, pdir, grad, diag, freev, t_c, s.mu_node, bv, amin, cmin, acc);
+      }
+      if (cmin < BIG) atomic_min_d(clr + ed.e, cmin);
+      if (amin < BIG) atomic_min_d(alpha + ed.e, amin);
+      if (acc != 0.0) atomicAdd((mode == 1 ? s.pHp : E) + ed.e, acc);
+    }
   }
 }
 
-// The same pass with one workgroup per environment: its boundary positions and node ids are
-// staged in LDS (24 bytes per boundary node) and its edges are spread over the threads.
-__global__ __launch_bounds__(256) void k_ipc_env(S s, int mode, PairCtx c, const double *pdir,
-                                                 double *grad, double *diag, const uint8_t *freev,
-                                                 double *E, double *clr, double *alpha,
-                                                 const double *tc) {
+// The same pass with workgroups per environment. The environment's boundary positions
+// (previous positions and direction as the mode needs them, and its exclusion bitset when it
+// fits) are staged in LDS, and a uniform grid over them is built there by counting sort (cells
+// at least the contact reach wide, at most IPC_CELLS of them): an edge visits the nodes of the
+// cells its box, widened by the reach, overlaps (the reference queries a hash grid,
+// implicit/ipc_contact.py:380-391). LDS: 8 + 16 per staged vector bytes per boundary node, 4 per
+// cell, 4 per bitset word.
+constexpr int IPC_CELLS = 2048;
+constexpr int IPC_TPB = 256;
+constexpr int IPC_TEAM = 8;  // lanes per edge in the candidate search
+// workgroups per environment, each building the grid and taking every IPC_PARTS-th round of
+// edges: more resident workgroups per CU than one per environment (the kernel's registers
+// allow two waves per SIMD)
+constexpr int IPC_PARTS = 4;
+// the gradient and energy passes keep only pairs within dhat, few: one workgroup per
+// environment builds its grid once and works them all
+__host__ __device__ constexpr int ipc_parts(int mode) { return (mode == 1 || mode == 3) ? IPC_PARTS : 1; }
+
+// LDS bytes of k_ipc_env for a mode: bitset words excluded
+__host__ __device__ inline size_t ipc_env_lds(int mode, int max_nb) {
+  const int vecs = 1 + ((mode == 0 || mode == 1 || mode == 2) ? 1 : 0) + ((mode == 1 || mode == 3) ? 1 : 0);
+  return (16 * (size_t)vecs + 8) * (size_t)max_nb + sizeof(int) * (IPC_CELLS + 2);
+}
+
+template <int mode>
+__global__ __launch_bounds__(IPC_TPB) void k_ipc_env(S s, PairCtx c, const double *pdir, double *grad,
+                                                     double *diag, const uint8_t *freev, double *E,
+                                                     double *clr, double *alpha, const double *tc,
+                                                     int bits_words, int plist_cap) {
   extern __shared__ double sh[];
+  constexpr bool slips = mode == 0 || mode == 1 || mode == 2;
+  constexpr bool dirs = mode == 1 || mode == 3;
   double *bp = sh;
-  int *bn = reinterpret_cast<int *>(sh + 2 * s.max_nb);
+  double *bxn = bp + 2 * s.max_nb;
+  double *bpd = bxn + (slips ? 2 * s.max_nb : 0);
+  int *citem = reinterpret_cast<int *>(bpd + (dirs ? 2 * s.max_nb : 0));
+  int *cstart = citem + s.max_nb;                  // IPC_CELLS + 2
+  uint32_t *bits = reinterpret_cast<uint32_t *>(cstart + IPC_CELLS + 2);
+  uint32_t *plist = bits + bits_words;             // plist_cap packed pairs
+  __shared__ double lo[2], cell;
+  __shared__ int ncx, ncy, use_bits;
+  __shared__ double red[4 * (IPC_TPB / WAVE)];
   const int e = blockIdx.x;
   const int start = s.bnd_start[e], n_be = s.bnd_start[e + 1] - start;
-  for (int j = threadIdx.x; j < n_be; j += blockDim.x) {
+  if (s.e_start[e + 1] == s.e_start[e]) return;    // no edges in this environment
+  double mnx = BIG, mny = BIG, mxx = -BIG, mxy = -BIG;
+  for (int j = threadIdx.x; j < n_be; j += IPC_TPB) {
     const int n = s.bnd[start + j];
     const V2<double> x = ld2(s.x, n);
-    bn[j] = n;
     bp[2 * j] = x.x;
     bp[2 * j + 1] = x.y;
+    if (slips) {
+      const V2<double> q = ld2(s.xn, n);
+      bxn[2 * j] = q.x;
+      bxn[2 * j + 1] = q.y;
+    }
+    if (dirs) {
+      const V2<double> q = ld2(pdir, n);
+      bpd[2 * j] = q.x;
+      bpd[2 * j + 1] = q.y;
+    }
+    mnx = fmin(mnx, x.x);
+    mny = fmin(mny, x.y);
+    mxx = fmax(mxx, x.x);
+    mxy = fmax(mxy, x.y);
+  }
+  // the exclusion bitset, when this scene's fits the words the launch reserved
+  const int64_t w0 = (bits_words > 0 && s.excl_bits) ? s.excl_bits_off[e] : 0;
+  const int64_t nw = (bits_words > 0 && s.excl_bits) ? s.excl_bits_off[e + 1] - w0 : 0;
+  const bool have_bits = nw > 0 && nw <= bits_words;
+  if (have_bits)
+    for (int64_t i = threadIdx.x; i < nw; i += IPC_TPB) bits[i] = s.excl_bits[w0 + i];
+  for (int k = threadIdx.x; k < IPC_CELLS + 2; k += IPC_TPB) cstart[k] = 0;
+  for (int o = WAVE / 2; o > 0; o >>= 1) {
+    mnx = fmin(mnx, __shfl_xor(mnx, o));
+    mny = fmin(mny, __shfl_xor(mny, o));
+    mxx = fmax(mxx, __shfl_xor(mxx, o));
+    mxy = fmax(mxy, __shfl_xor(mxy, o));
+  }
+  const int w = threadIdx.x / WAVE;
+  if (threadIdx.x % WAVE == 0) {
+    red[4 * w] = mnx;
+    red[4 * w + 1] = mny;
+    red[4 * w + 2] = mxx;
+    red[4 * w + 3] = mxy;
   }
   __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < IPC_TPB / WAVE; ++k) {
+      mnx = fmin(mnx, red[4 * k]);
+      mny = fmin(mny, red[4 * k + 1]);
+      mxx = fmax(mxx, red[4 * k + 2]);
+      mxy = fmax(mxy, red[4 * k + 3]);
+    }
+    const double ex = fmax(mxx - mnx, 0.0), ey = fmax(mxy - mny, 0.0);
+    double h = c.reach;
+    while ((floor(ex / h) + 1.0) * (floor(ey / h) + 1.0) > double(IPC_CELLS)) h *= 1.4142135623730951;
+    lo[0] = mnx;
+    lo[1] = mny;
+    cell = h;
+    ncx = int(floor(ex / h)) + 1;
+    ncy = int(floor(ey / h)) + 1;
+  }
+  __syncthreads();
+  const double h = cell, x0 = lo[0], y0 = lo[1];
+  const int nx = ncx, ny = ncy;
+  // cell coordinates clamped in floating point first (a far or non-finite position lands in an
+  // edge cell instead of overflowing the conversion)
+  const auto cx_of = [&](double v) { return int(fmin(fmax(floor((v - x0) / h), 0.0), double(nx - 1))); };
+  const auto cy_of = [&](double v) { return int(fmin(fmax(floor((v - y0) / h), 0.0), double(ny - 1))); };
+  // counting sort of the boundary nodes by cell: counts into cstart[cell + 1], an inclusive
+  // block scan, then each node counts its cell's end down to its slot, which leaves the start of
+  // cell k in cstart[k + 1] (and n_be in cstart[ncell + 1]): cell k is [cstart[k+1], cstart[k+2]).
+  const int ncell = nx * ny;
+  for (int j = threadIdx.x; j < n_be; j += IPC_TPB)
+    atomicAdd(cstart + cy_of(bp[2 * j + 1]) * nx + cx_of(bp[2 * j]) + 1, 1);
+  __syncthreads();
+  {
+    __shared__ int wsum[IPC_TPB / WAVE];
+    const int per = (ncell + IPC_TPB - 1) / IPC_TPB;
+    const int k0 = 1 + threadIdx.x * per, k1 = min(k0 + per, ncell + 1);
+    int sum = 0;
+    for (int k = k0; k < k1; ++k) sum += cstart[k];
+    const int lane = threadIdx.x % WAVE;
+    int incl = sum;
+    for (int o = 1; o < WAVE; o <<= 1) {
+      const int t = __shfl_up(incl, o);
+      if (lane >= o) incl += t;
+    }
+    if (lane == WAVE - 1) wsum[w] = incl;
+    __syncthreads();
+    int run = incl - sum;
+    for (int q = 0; q < w; ++q) run += wsum[q];
+    for (int k = k0; k < k1; ++k) {
+      run += cstart[k];
+      cstart[k] = run;
+    }
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < n_be; j += IPC_TPB) {
+    const int cc = cy_of(bp[2 * j + 1]) * nx + cx_of(bp[2 * j]);
+    citem[atomicAdd(cstart + cc + 1, -1) - 1] = j;
+  }
+  if (threadIdx.x == 0) cstart[ncell + 1] = n_be;
+  __syncthreads();
+  const BView bv{bp, slips ? bxn : nullptr, dirs ? bpd : nullptr, have_bits ? bits : nullptr};
   const double t_c = (mode == 1 || mode == 3) ? tc[e] : 0.0;
-  for (int k = s.e_start[e] + threadIdx.x; k < s.e_start[e + 1]; k += blockDim.x)
-    ipc_edge(s, k, mode, c, pdir, grad, diag, freev, s.pHp, E, clr, alpha, t_c, s.mu_node, bp, bn);
+  // the gradient and energy passes (modes 0, 2) add terms only for pairs closer than dhat (a
+  // pair's end distance is never below its segment distance), so they keep those; the
+  // clearance and step-limit passes need every pair within the reach
+  const double r = (mode == 1 || mode == 3 || mode == 4) ? c.reach : c.dhat;
+  const int64_t eoff = s.excl_off[e];
+  const int ke0 = s.e_start[e], ne = s.e_start[e + 1] - ke0;
+  double amin = BIG, cmin = BIG, acc = 0.0;     // this lane's, over all its pairs
+  // Phase 1: this workgroup's edges (every ipc_parts(mode)-th block of IPC_TPB / IPC_TEAM), a
+  // team of IPC_TEAM lanes per edge splitting the candidates of the cells the edge's box,
+  // widened by r, overlaps; the pairs that pass the filter go to the LDS pair list (edge and
+  // node packed in 16 bits each). When the list is full a lane works its pair at once.
+  const int lane = threadIdx.x % IPC_TEAM, team = threadIdx.x / IPC_TEAM;
+  __shared__ int npairs;
+  if (threadIdx.x == 0) npairs = 0;
+  __syncthreads();
+  for (int base = blockIdx.y * (IPC_TPB / IPC_TEAM); base < ne; base += ipc_parts(mode) * (IPC_TPB / IPC_TEAM)) {
+    const int kk = base + team;
+    if (kk >= ne) continue;
+    const EdgeData ed = edge_data(s, ke0 + kk, mode, bv, pdir);
+    const int ax = cx_of(fmin(ed.xa.x, ed.xb.x) - r), bx = cx_of(fmax(ed.xa.x, ed.xb.x) + r);
+    const int ay = cy_of(fmin(ed.xa.y, ed.xb.y) - r), by = cy_of(fmax(ed.xa.y, ed.xb.y) + r);
+    int seen = 0;    // candidates in the cells before this one
+    for (int gy = ay; gy <= by; ++gy)
+      for (int gx = ax; gx <= bx; ++gx) {
+        const int cc = gy * nx + gx;
+        const int q0 = cstart[cc + 1], q1 = cstart[cc + 2];
+        for (int q = q0 + ((lane - seen) % IPC_TEAM + IPC_TEAM) % IPC_TEAM; q < q1; q += IPC_TEAM) {
+          const int jl = citem[q];
+          const V2<double> xp = {bp[2 * jl], bp[2 * jl + 1]};
+          if (!pair_in(s, ed, jl, xp, r, n_be, eoff, bv)) continue;
+          const int slot = kk < 65536 ? atomicAdd(&npairs, 1) : plist_cap;
+          if (slot < plist_cap) plist[slot] = (uint32_t(kk) << 16) | uint32_t(jl);
+          else pair_body(s, ke0 + kk, mode, c, ed, jl, -1, xp, pdir, grad, diag, freev, t_c, s.mu_node, bv,
+                         amin, cmin, acc);
+        }
+        seen += q1 - q0;
+      }
+  }
+  __syncthreads();
+  // Phase 2: the listed pairs, spread evenly over every lane of the workgroup.
+  const int nl = min(npairs, plist_cap);
+  for (int i = threadIdx.x; i < nl; i += IPC_TPB) {
+    const uint32_t pk = plist[i];
+    const int kk = int(pk >> 16), jl = int(pk & 0xFFFFu);
+    const EdgeData ed = edge_data(s, ke0 + kk, mode, bv, pdir);
+    const V2<double> xp = {bp[2 * jl], bp[2 * jl + 1]};
+    pair_body(s, ke0 + kk, mode, c, ed, jl, -1, xp, pdir, grad, diag, freev, t_c, s.mu_node, bv, amin, cmin, acc);
+  }
+  // the workgroup's minima and sum, then one atomic each per workgroup
+  if (mode == 1 || mode == 2 || mode == 3) {
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+      amin = fmin(amin, __shfl_xor(amin, o));
+      cmin = fmin(cmin, __shfl_xor(cmin, o));
+      acc += __shfl_xor(acc, o);
+    }
+    __syncthreads();
+    if (threadIdx.x % WAVE == 0) {
+      red[4 * w] = amin;
+      red[4 * w + 1] = cmin;
+      red[4 * w + 2] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int k = 1; k < IPC_TPB / WAVE; ++k) {
+        amin = fmin(amin, red[4 * k]);
+        cmin = fmin(cmin, red[4 * k + 1]);
+        acc += red[4 * k + 2];
+      }
+      if (mode != 2) {
+        if (cmin < BIG) atomic_min_d(clr + e, cmin);
+        if (amin < BIG) atomic_min_d(alpha + e, amin);
+      }
+      if (mode != 3 && acc != 0.0) atomicAdd((mode == 1 ? s.pHp : E) + e, acc);
+    }
+  }
 }
 
 // ------------------------------------------------------------------------- "node" contact
@@ -1299,7 +1583,7 @@ struct Plain {
 
   Plain(const S &s_, const Pm &p_, hipStream_t st_) : s(s_), p(p_), st(st_) {
     pc = PairCtx{p.dhat, p.ipc_kappa, p.ipc_mu, p.epsv, p.inv_dt, p.h2, p.ipc_reach, p.ipc_gap, 0.0,
-                 p.ipc_per_body};
+                 p.ipc_per_body, nullptr, 0};
   }
 
   int ipc_pass(int mode, const double *pdir, double *grad, double *diag, const uint8_t *freev,
@@ -1312,10 +1596,25 @@ struct Plain {
       // t_c snapshot: the ACCD of every pair stops at the alpha the pass started from
       D2F_HIP(hipMemcpyAsync(s.tc, alpha, sizeof(double) * s.B, hipMemcpyDeviceToDevice, st));
     }
-    const size_t lds = (2 * sizeof(double) + sizeof(int)) * (size_t)s.max_nb;
-    if (s.e_start && s.max_nb > 0 && lds <= 48 * 1024) {
-      hipLaunchKernelGGL(k_ipc_env, dim3(s.B), dim3(256), lds, st, s, mode, c, pdir, grad, diag, freev, E,
-                         clr, alpha, s.tc);
+    const size_t base = ipc_env_lds(mode, s.max_nb);
+    // 64 KiB with the static LDS; edge and node indices packed in 16 bits in the pair list
+    if (s.e_start && s.max_nb > 0 && s.max_nb < 65536 && base <= 52 * 1024) {
+      // the largest exclusion bitset that still fits, staged by the scenes whose bitset it holds;
+      // the rest of the 60 KiB holds the pair list (at least 1024 pairs)
+      const bool reach_pass = mode == 1 || mode == 3 || mode == 4;   // many candidates: stage the bitset
+      const int words = (s.excl_bits && reach_pass) ? (int)std::min<size_t>((56 * 1024 - base) / 4, (size_t)s.excl_bits_lds) : 0;
+      const int cap = (int)((60 * 1024 - base - 4 * (size_t)words) / 4);
+      const size_t lds = base + 4 * (size_t)words + 4 * (size_t)cap;
+      const dim3 grid(s.B, ipc_parts(mode));
+#define D2F_IPC_ENV(M)                                                                                  \
+  hipLaunchKernelGGL(k_ipc_env<M>, grid, dim3(IPC_TPB), lds, st, s, c, pdir, grad, diag, freev, E, clr, \
+                     alpha, s.tc, words, cap)
+      if (mode == 0) D2F_IPC_ENV(0);
+      else if (mode == 1) D2F_IPC_ENV(1);
+      else if (mode == 2) D2F_IPC_ENV(2);
+      else if (mode == 3) D2F_IPC_ENV(3);
+      else D2F_IPC_ENV(4);
+#undef D2F_IPC_ENV
       D2F_HIP(hipGetLastError());
     } else {
       LAUNCH(k_ipc, s.n_e, s, mode, c, pdir, grad, diag, freev, E, clr, alpha, s.tc);
@@ -1682,6 +1981,19 @@ int d2f_ipc_min_distance(const d2f_scene *s, const d2f_params *p, void *stream) 
   LAUNCH(k_fill, s->B, s->sweep_beta, s->B, 1.0);
   LAUNCH(k_fill, s->B, s->clr_ipc, s->B, BIG);
   int r = run.ipc_pass(3, s->x0, s->grad, s->diag, s->free_, s->Ener, s->clr_ipc, s->sweep_beta, p->h2, p->ipc_mu);
+  if (r) return r;
+  D2F_HIP(hipGetLastError());
+  return 0;
+}
+
+int d2f_ipc_pairs(const d2f_scene *s, const d2f_params *p, int32_t *pairs, int32_t cap, void *stream) {
+  D2F_CHECK(s && p && p->ipc_on && pairs && cap >= 0, "d2f_ipc_pairs: needs a scene with IPC contact");
+  hipStream_t st = (hipStream_t)stream;
+  D2F_HIP(hipMemsetAsync(pairs, 0, sizeof(int32_t), st));
+  Plain run(*s, *p, st);
+  run.pc.plist = pairs;
+  run.pc.pcap = cap;
+  int r = run.ipc_pass(4, s->x0, s->grad, s->diag, s->free_, s->Ener, s->clr_ipc, s->sweep_beta, p->h2, p->ipc_mu);
   if (r) return r;
   D2F_HIP(hipGetLastError());
   return 0;

@@ -74,6 +74,8 @@ class DeviceSolver:
         self.block_size = 512              # scene path: 2 waves per SIMD, no register spills
         self._layout = None
         self._route_t = None
+        self.excl_bits = self.excl_bits_off = None     # bit-packed exclusion (with contact)
+        self.excl_words, self.excl_words_max = np.zeros(self.B, np.int64), 0
         self.scene_envs = np.arange(self.B)
         self.plain_envs = np.zeros(0, np.int64)
         self.tri2env = t(np.asarray(me.tri2env), I32)
@@ -145,7 +147,27 @@ class DeviceSolver:
             raise RuntimeError("boundary edges are not grouped by environment")
         self.e_start = t(np.searchsorted(env, np.arange(self.B + 1)), I32)
         self.max_nb = int(np.max(np.asarray(me.boundary_per_env))) if self.B else 0
+        self._pack_exclusion()
         self._layout = None                 # the scene layout plans LDS for the contact lists
+
+    def _pack_exclusion(self):
+        """Each scene's exclusion matrix bit-packed, 32 per word (bit jp * n_b_e + j of scene e's
+        words: boundary nodes jp and j do not contact): excl_bits, excl_bits_off, and the
+        largest scene's word count excl_words_max."""
+        me = self.me
+        flat = np.asarray(me.excl_flat, np.uint8)
+        off = np.asarray(me.excl_offset, np.int64)
+        nbe = np.asarray(me.boundary_per_env, np.int64)
+        words, woff = [], [0]
+        for e in range(self.B):
+            bits = np.packbits(flat[off[e]:off[e] + nbe[e] * nbe[e]] != 0, bitorder="little")
+            bits = np.concatenate([bits, np.zeros((-len(bits)) % 4, np.uint8)])
+            words.append(bits.view("<u4"))
+            woff.append(woff[-1] + len(bits) // 4)
+        self.excl_bits = self._t(np.concatenate(words).view(np.int32), I32)
+        self.excl_bits_off = self._t(np.asarray(woff, np.int64), I64)
+        self.excl_words = np.diff(woff)
+        self.excl_words_max = int(self.excl_words.max(initial=0))
 
     def setup_node(self, stiffness, radius, mu, mu_node=None, friction=False):
         """The "node" penalty contact between boundary nodes of one environment, with Coulomb
@@ -159,10 +181,12 @@ class DeviceSolver:
         self.excl_off = t(np.asarray(me.excl_offset, np.int64), I64)
         self.mu_node = None if mu_node is None else t(mu_node, F64)
         self.max_nb = int(np.max(np.asarray(me.boundary_per_env))) if self.B else 0
+        self._pack_exclusion()
 
     def refresh_exclusion(self):
         if self.ipc is not None or self.node is not None:
             self.excl = self._t(np.asarray(self.me.excl_flat, np.uint8), U8)
+            self._pack_exclusion()
             self._layout = None
 
     # ------------------------------------------------------------------ rigid bodies
@@ -506,7 +530,6 @@ class DeviceSolver:
         the candidate-pair list (bit jp * n_b_e + j of a scene's bitset is its exclusion of
         boundary nodes jp and j)."""
         self.excl_bits_lds, self.max_pairs, self.nacc_lds, self.xc_lds = 0, 0, 0, 0
-        self.excl_bits = self.excl_bits_off = None
         sub = self.scene_envs
         nn = np.diff(np.asarray(self.me.node_starts, np.int64))[sub]
         self.max_nn = int(nn.max(initial=0))
@@ -523,19 +546,7 @@ class DeviceSolver:
             base += 16 * self.max_nn
         if (self.ipc is None or self.n_e == 0) and self.node is None:
             return
-        me = self.me
-        flat = np.asarray(me.excl_flat, np.uint8)
-        off = np.asarray(me.excl_offset, np.int64)
-        nbe = np.asarray(me.boundary_per_env, np.int64)
-        words, woff = [], [0]
-        for e in range(self.B):
-            bits = np.packbits(flat[off[e]:off[e] + nbe[e] * nbe[e]] != 0, bitorder="little")
-            bits = np.concatenate([bits, np.zeros((-len(bits)) % 4, np.uint8)])
-            words.append(bits.view("<u4"))
-            woff.append(woff[-1] + len(bits) // 4)
-        self.excl_bits = self._t(np.concatenate(words).view(np.int32), I32)
-        self.excl_bits_off = self._t(np.asarray(woff, np.int64), I64)
-        wmax = int(np.max(np.diff(woff)[sub], initial=0))
+        wmax = int(self.excl_words[sub].max(initial=0))
         if wmax <= self.EXCL_LDS_MAX and base + 4 * wmax <= self.LDS_BUDGET:
             self.excl_bits_lds = wmax
         if self.ipc is None:             # node contact: partners come from the grid itself
@@ -601,6 +612,8 @@ class DeviceSolver:
                 setattr(s, k, p(k))
             s.mu_node = self._ptr(self.mu_node)
             s.e_start, s.max_nb = p("e_start"), self.max_nb
+            s.excl_bits, s.excl_bits_off = p("excl_bits"), p("excl_bits_off")
+            s.excl_bits_lds = self.excl_words_max          # the per-kernel path's reading
         if self.node is not None:
             s.excl, s.excl_off = p("excl"), p("excl_off")
             s.mu_node = self._ptr(self.mu_node)
@@ -642,6 +655,8 @@ class DeviceSolver:
             if self.excl_bits is not None:
                 s.excl_bits, s.excl_bits_off = p("excl_bits"), p("excl_bits_off")
             s.excl_bits_lds, s.max_pairs = self.excl_bits_lds, self.max_pairs
+            if part == "plain" or self.path == "plain":
+                s.excl_bits_lds = self.excl_words_max
             s.max_nn, s.nacc_lds, s.xc_lds = self.max_nn, self.nacc_lds, self.xc_lds
             s.max_ag = self.max_ag
         if self.springs is not None:
@@ -735,6 +750,23 @@ class DeviceSolver:
         stream = C.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
         _lib.check(L.d2f_ipc_min_distance(C.byref(s), C.byref(q), stream), "d2f_ipc_min_distance")
         return self.clr_ipc.cpu().numpy().copy()
+
+    def ipc_pairs(self, cap=None):
+        """The (edge, node) pairs the per-kernel path's contact passes evaluate at the current
+        positions, sorted (d2f_ipc_pairs)."""
+        if self.ipc is None:
+            return np.zeros((0, 2), np.int64)
+        cap = int(cap or 64 * max(self.n_e, 1))
+        buf = torch.zeros(2 + 2 * cap, dtype=I32, device=self.dev)
+        L = _lib.lib()
+        s, q = self.scene_struct("plain"), self.params_struct()
+        _lib.check(L.d2f_ipc_pairs(C.byref(s), C.byref(q), self._ptr(buf), cap, self._stream()), "d2f_ipc_pairs")
+        h = buf.cpu().numpy()
+        n = int(h[0])
+        if n > cap:
+            return self.ipc_pairs(n)
+        P = h[2:2 + 2 * n].reshape(-1, 2).astype(np.int64)
+        return P[np.lexsort((P[:, 1], P[:, 0]))]
 
     # ------------------------------------------------------------------ contact read-outs
     def _stream(self):

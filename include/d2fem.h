@@ -138,7 +138,10 @@ typedef struct d2f_scene {
    * when excl_bits_lds is set; pair-list capacity per scene (0: traverse the grid each pass) */
   const uint32_t *excl_bits;          /* sum_e ceil(n_b_e^2 / 32) words */
   const int64_t *excl_bits_off;       /* B+1 word offsets */
-  int32_t excl_bits_lds, max_pairs;   /* words of LDS for the bitset (0: read bytes); pairs cap */
+  int32_t excl_bits_lds, max_pairs;   /* words of LDS for the bitset (0: read bytes); pairs cap.
+                                       * d2f_step_plain reads excl_bits_lds as the largest scene's
+                                       * bitset in words (staged in LDS by its contact pass when
+                                       * it fits there) */
   /* largest per-scene node count; nacc_lds = 1: the element pass accumulates every node's
    * gradient and diagonal in LDS (32 bytes per node), 0: colour-by-colour read-modify-write */
   int32_t max_nn, nacc_lds;
@@ -235,6 +238,13 @@ int d2f_step_plain(const d2f_scene *s, const d2f_params *prm, void *stream);
  * reach); uses s->x0 as scratch. Replaces IPCContact.min_distance (implicit/ipc_contact.py:441-447),
  * the "bodies must start apart" check of ImplicitIntegrator._build. */
 int d2f_ipc_min_distance(const d2f_scene *s, const d2f_params *prm, void *stream);
+
+/* The node-edge pairs the per-kernel path's contact passes evaluate at s->x: every (boundary
+ * edge k, boundary node n) of one scene with n not an end of k, not excluded against either
+ * end, and closer than prm->ipc_reach to the edge (the candidates of IPCContact's hash-grid
+ * query after its filters, implicit/ipc_contact.py:212-235). pairs[0] receives the count;
+ * pairs[2 + 2i], pairs[3 + 2i] the edge and node of pair i < cap, in no particular order. */
+int d2f_ipc_pairs(const d2f_scene *s, const d2f_params *prm, int32_t *pairs, int32_t cap, void *stream);
 
 /* The same step on the scene-persistent path: one workgroup per environment runs the whole
  * iteration loop in one launch, with no host synchronisation; prm->precision 0 = float64,

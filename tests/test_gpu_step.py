@@ -285,3 +285,85 @@ def test_scene_too_large_for_lds_takes_the_plain_path_alone():
     for e in (0, 2):
         assert np.abs(xs[env == e] - small.get_x()[envq == e]).max() < 1e-12
     assert (gs.stats(ms)["iters"] == 20).all()
+
+
+def _far_grip():
+    """The grip scene with one more soft block 0.2 m away in the same environment: the
+    environment's extent forces the plain path's contact grid to cells much wider than the
+    reach."""
+    from diff_2d_fem_amd import Model
+    px, block = 0.5e-3, 8
+    m = Model()
+    m.body_friction = 0.8
+    m.add_ground(0.0, mu=0.5)
+    m.add_affine_body(np.ones((block, block), np.uint8), physical_size=(block * px, block * px),
+                      center=(0.0, block * px / 2 + 0.9 * px / 8), density=600.0)
+    pads = [m.add_soft_body(np.ones((8, 4), np.uint8), physical_size=(4 * px, 8 * px),
+                            center=(sg * (block * px / 2 + 2 * px + 2 * px), 4 * px + 0.8e-3),
+                            E=1.0e6, nu=0.40, density=1100.0) for sg in (-1, 1)]
+    m.add_soft_body(np.ones((4, 4), np.uint8), physical_size=(4 * px, 4 * px), center=(0.2, 0.0021),
+                    E=1.0e6, nu=0.40, density=1100.0)
+    m.finalize()
+    m.gravity = (0.0, -9.81)
+    for sg, p in zip((-1, 1), pads):
+        X = m.rest_x(p)
+        back = X[:, 0].min() if sg < 0 else X[:, 0].max()
+        m.set_kinematic_constraint(p, np.where(np.abs(X[:, 0] - back) < 1e-9)[0])
+    return m, [pads]
+
+
+@pytest.mark.parametrize("scene", ["grip", "far"])
+def test_ipc_pairs_equal_brute_force(scene):
+    """The per-kernel path's grid broadphase finds exactly the oracle's brute-force pair set
+    (every boundary node against every edge of its scene, same filters), at several states of
+    a closing grip; "far" adds a body 0.2 m away, which coarsens the grid."""
+    kw = dict(max_iter=60, eps=0.0, conv_check=0)
+    m, pads = grip(n_env=2) if scene == "grip" else _far_grip()
+    gi = ImplicitIntegrator(path="plain", precision="double", **kw)
+    oi = OracleIntegrator(**kw)
+    checked = 0
+    for k in range(1, 13):
+        close_jaws(m, pads, k, n_close=8)
+        gi.simulate(m, 1e-3)
+        if k % 3:
+            continue
+        s = m._solver
+        got = s.ipc_pairs()
+        if oi.solver is None:
+            oi.attach(m, 1e-3)
+        ipc = oi.solver.ipc
+        K, JB = ipc.pairs(s.get_x())[:2]
+        want = np.stack([K, ipc.bnd[JB]], 1)
+        want = want[np.lexsort((want[:, 1], want[:, 0]))]
+        assert len(want) > 0
+        assert got.shape == want.shape and np.array_equal(got, want)
+        checked += len(want)
+    print(f"{scene}: {checked} pairs checked")
+
+
+def test_elastic_tiles_beyond_the_default_lds():
+    """The tiled element kernel with chunks touching more than 1365 nodes (512 single-pixel
+    bodies: 2048 nodes per 1024-triangle chunk, 96 KiB of LDS) against the oracle."""
+    from diff_2d_fem_amd.mesh import build_scene
+    me = build_scene([dict(mask_xy=np.ones((1, 1), bool), size=(1e-3, 1e-3), center=(0.0, 0.0),
+                           young=1e6, nu=0.4, rho=1000.0, env=i) for i in range(600)])
+    rng = np.random.default_rng(4)
+    x = me.rest_pos + rng.normal(scale=5e-5, size=me.rest_pos.shape)
+    d = torch.device("cuda")
+    et = element_tiles(me.tri, len(x), device=d)
+    assert et["max_nodes"] > 1365
+    tiles = _lib.ElemTiles()
+    for k in ("node_off", "nodes", "owned", "tri_local"):
+        setattr(tiles, k, et[k].data_ptr())
+    tiles.max_nodes = et["max_nodes"]
+    xt = torch.tensor(x, device=d)
+    tri = torch.tensor(me.tri, dtype=torch.int32, device=d)
+    Bm = torch.tensor(me.D_m_inv.reshape(-1, 4), device=d)
+    A, mu, lam = (torch.tensor(v, device=d) for v in (me.rest_area, me.mu, me.lam))
+    g, dg = torch.zeros_like(xt), torch.zeros_like(xt)
+    L = _lib.lib()
+    _lib.check(L.d2f_elastic_grad_diag_f64(_p(xt), _p(tri), _p(Bm), _p(A), _p(mu), _p(lam), len(me.tri), 4e-6,
+                                           1, C.byref(tiles), _p(g), _p(dg), _stream()), "grad_diag")
+    go, do = np.zeros_like(x), np.zeros_like(x)
+    OT.elastic_grad_diag(x, me.tri, me.D_m_inv, me.rest_area, me.mu, me.lam, 4e-6, 1, go, do)
+    assert _rel(g.cpu().numpy(), go) < 1e-12 and _rel(dg.cpu().numpy(), do) < 1e-12
