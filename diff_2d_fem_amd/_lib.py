@@ -13,6 +13,7 @@ LIB_NAME = "libd2fem_hip.so"
 LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
 
 MAX_COLLIDERS = 8
+NOALLOC_MAX_NODES = 131072     # nodes of d2f_step_scene_noalloc's static work arrays
 MAX_COLORS = 24
 P = C.c_void_p
 
@@ -114,6 +115,7 @@ def lib():
     L.d2f_ipc_pairs.argtypes = [C.POINTER(Scene), C.POINTER(Params), P, C.c_int32, P]
     if hasattr(L, "d2f_step_scene"):
         L.d2f_step_scene.argtypes = [C.POINTER(Scene), C.POINTER(Params), P]
+        L.d2f_step_scene_noalloc.argtypes = [C.POINTER(Scene), C.POINTER(Params), P]
     i32, f64 = C.c_int32, C.c_double
     L.d2f_elastic_grad_diag_f64.argtypes = [P, P, P, P, P, P, i32, f64, i32, C.POINTER(ElemTiles), P, P, P]
     L.d2f_elastic_pHp_f64.argtypes = [P, P, P, P, P, P, P, P, i32, f64, i32, P, P]

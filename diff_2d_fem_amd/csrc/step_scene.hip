@@ -13,6 +13,9 @@
 // Template R is the element arithmetic: double (float64 path) or float (mixed: float32 element
 // stress and curvature from a displacement gradient formed in float64; float64 state, dot
 // products and energies).
+#include <algorithm>
+#include <mutex>
+
 #include "common.h"
 #include "node_contact.h"
 #include "scene_dev.h"
@@ -1917,5 +1920,62 @@ extern "C" int d2f_step_scene(const d2f_scene *s, const d2f_params *p, void *str
   }
   D2F_CHECK(s->sc_Bm && s->sc_Amu && s->sc_Alam, "d2f_step_scene: float64 constants missing");
   return scene_launch_f64(*s, *p, tpb, st);
+}
+
+// ---- d2f_step_scene_noalloc: the scene kernel's node work arrays in a static device array
+namespace d2f {
+// xn, xhat, grad, grad_prev, p, diag, diag_el, x0, bc_q: 2 N doubles each, each array starting
+// on a 256-byte boundary. Every device holds its own copy until the process ends.
+constexpr int NOALLOC_ARRAYS = 9;
+static __device__ __align__(256) double scratch[NOALLOC_ARRAYS * 2 * (size_t)D2F_NOALLOC_MAX_NODES];
+
+__global__ void k_fill_zero(double *a, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = 0.0;
+}
+
+constexpr int NOALLOC_DEVICES = 64;
+static std::mutex scratch_mu;
+static double *scratch_dev[NOALLOC_DEVICES];   // scratch's address on device d, from its first call
+
+// The current device's address of `scratch`, looked up once per device with that device current.
+static int scratch_of_current_device(double **out) {
+  int dev = 0;
+  D2F_HIP(hipGetDevice(&dev));
+  D2F_CHECK(dev >= 0 && dev < NOALLOC_DEVICES, "d2f_step_scene_noalloc: device %d beyond %d", dev,
+            NOALLOC_DEVICES);
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_dev[dev]) {
+    void *ptr = nullptr;
+    D2F_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    scratch_dev[dev] = (double *)ptr;
+  }
+  *out = scratch_dev[dev];
+  return 0;
+}
+}  // namespace d2f
+
+extern "C" int d2f_step_scene_noalloc(const d2f_scene *s, const d2f_params *p, void *stream) {
+  D2F_CHECK(s && p, "d2f_step_scene_noalloc: null scene or params");
+  D2F_CHECK(s->N > 0 && s->N <= D2F_NOALLOC_MAX_NODES,
+            "d2f_step_scene_noalloc: %d nodes; the static work arrays hold at most %d "
+            "(use d2f_step_scene with caller-owned work arrays)", s->N, D2F_NOALLOC_MAX_NODES);
+  double *w = nullptr;
+  if (const int rc = scratch_of_current_device(&w)) return rc;
+  const size_t n2 = (2 * (size_t)s->N + 31) & ~(size_t)31;   // 256-byte aligned stride
+  d2f_scene t = *s;
+  double **arr[NOALLOC_ARRAYS] = {&t.xn, &t.xhat, &t.grad, &t.grad_prev, &t.p,
+                                  &t.diag, &t.diag_el, &t.x0, &t.bc_q};
+  for (int k = 0; k < NOALLOC_ARRAYS; ++k) *arr[k] = w + k * n2;
+  // the sweep direction must be zero off the kinematic nodes, whatever the last call left there;
+  // the kernel writes every other array before it reads it
+  hipStream_t st = (hipStream_t)stream;
+  if (s->n_bc > 0) {
+    const int64_t n = 2 * (int64_t)s->N;
+    hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 1024)), dim3(256), 0,
+                       st, t.bc_q, n);
+    D2F_HIP(hipGetLastError());
+  }
+  return d2f_step_scene(&t, p, stream);
 }
 #endif

@@ -74,6 +74,9 @@ class DeviceSolver:
         self.block_size = 512              # scene path: 2 waves per SIMD, no register spills
         self._layout = None
         self._route_t = None
+        # scene path: step through d2f_step_scene_noalloc (the library's static work arrays, no
+        # per-call allocation; at most NOALLOC_MAX_NODES nodes, one solver per device at a time)
+        self.noalloc = False
         self.excl_bits = self.excl_bits_off = None     # bit-packed exclusion (with contact)
         self.excl_words, self.excl_words_max = np.zeros(self.B, np.int64), 0
         self.scene_envs = np.arange(self.B)
@@ -717,8 +720,14 @@ class DeviceSolver:
         if self.path == "plain":
             _lib.check(L.d2f_step_plain(C.byref(s), C.byref(q), stream), "d2f_step_plain")
         elif self._route_t is None:
-            _lib.check(L.d2f_step_scene(C.byref(s), C.byref(q), stream), "d2f_step_scene")
+            if self.noalloc:
+                _lib.check(L.d2f_step_scene_noalloc(C.byref(s), C.byref(q), stream), "d2f_step_scene_noalloc")
+            else:
+                _lib.check(L.d2f_step_scene(C.byref(s), C.byref(q), stream), "d2f_step_scene")
         else:
+            if self.noalloc:
+                raise ValueError("noalloc steps the scene kernel alone; this batch has scenes routed "
+                                 "to the per-kernel path")
             # scenes too large for the scene kernel first, on the per-kernel path (the others'
             # nodes and scalars untouched; their affine group state is put back, since the
             # group kernels run over every group), then the scene kernel on its scenes

@@ -372,6 +372,20 @@ typedef struct d2f_minres {
 int d2f_minres_begin_f64(const d2f_minres *m, void *stream);
 int d2f_minres_step_f64(const d2f_minres *m, void *stream);
 
+/* ---- allocation-free step -----------------------------------------------------------------
+ * No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). d2f_step_scene with
+ * the scene kernel's node work arrays (xn, xhat, grad, grad_prev, p, diag, diag_el, x0, bc_q)
+ * taken from a static device array of the library instead of the scene's pointers, which are
+ * neither read nor written (they may be NULL). Everything else — inputs, x and v, the
+ * per-scene scalars, the affine state — is the caller's, as for d2f_step_scene. The array holds
+ * D2F_NOALLOC_MAX_NODES nodes (9 x 16 bytes per node: 18 MiB), on every device the library runs
+ * on, until the process ends; a larger scene returns D2F_EINVAL. The first call on a device looks
+ * the array up (hipGetSymbolAddress); no call allocates. Every call on one device shares that
+ * device's array: calls that overlap on one device (two streams or two threads) race. */
+#define D2F_NOALLOC_MAX_NODES 131072
+int d2f_step_scene_noalloc(const d2f_scene *s, const d2f_params *p, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
