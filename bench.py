@@ -1,7 +1,8 @@
 """Throughput of the batched implicit step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3] [--path scene|plain]
-                    [--precision mixed|double] [--no-plain] [--no-cpu]
+                    [--precision mixed|double] [--no-plain] [--no-cpu] [--no-large]
+                    [--no-adjoint] [--no-graph] [--no-tune]
 
 The workload is the contact-heavy configuration: 256 scenes of two soft pads on kinematic
 jaws gripping and lifting a rigid block (diff_2d_fem_amd.workloads.pads_lift_block). W
@@ -19,8 +20,10 @@ bytes over its event-timed duration against HBM peak; `element_roofline` is the 
 elastic gradient+diagonal element kernel alone on this workload's mesh, and
 `element_roofline_large` the same kernel at the size of the star-mesh configuration (45 M
 triangles, working set far beyond the Infinity Cache); `adjoint_c4` the differentiable rollout of
-the adjoint configuration (512 pneumatic fingers: forward and backward ms per step);
-`cpu_baseline` times the float64 CPU oracle on a bounded sample of the same workload.
+the adjoint configuration (512 pneumatic fingers: forward and backward ms per step); `graph`
+the scene path's K steps captured as one HIP graph and replayed (staged targets, one graph
+launch) beside the eager per-step loop; `tuning` the scene kernel's plan that the autotuner
+chose when the solver was built (before the warm-up; --no-tune keeps the default); `cpu_baseline` times the float64 CPU oracle on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -33,6 +36,43 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+
+
+def graph_leg(res, steps, per_replay=5):
+    """The same steps as a HIP graph (diff_2d_fem_amd.graph.StepGraph): each replay holds the
+    staged targets' host-to-device copy and `per_replay` steps; the host stages the next replay
+    while one runs. One untimed replay, then `steps` timed steps continuing the measured
+    trajectory (host clock, and HIP events around each replay on its stream)."""
+    import torch
+    from diff_2d_fem_amd.graph import StepGraph
+    m, w, integ = res["m"], res["w"], res["integ"]
+    g = StepGraph(integ, m, w["dt"], per_replay)
+    step = res["step"]
+    g.run(w["command"], step + 1)
+    step += per_replay
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    n = max(steps // per_replay, 1)
+    ev = []
+    t0 = time.perf_counter()
+    for r in range(n):
+        for k in range(per_replay):
+            w["command"](step + 1 + k)
+            g.stage(k)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        g.replay()
+        e1.record(st)
+        ev.append((e0, e1))
+        step += per_replay
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    res["step"] = step
+    return dict(steps_per_replay=per_replay, steps=n * per_replay,
+                ms_per_step=round(t / (n * per_replay) * 1e3, 3),
+                replay_ms_per_step=round(sum(a.elapsed_time(b) for a, b in ev) / (n * per_replay), 3),
+                eager_ms_per_step=round(res["t"] / res["n_steps"] * 1e3, 3),
+                eager_launch_ms=round(res["launch_ms"], 3))
 
 
 def parse():
@@ -49,6 +89,8 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-large", action="store_true", help="skip the 45 M-triangle element-kernel leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 512-finger adjoint leg")
+    ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay leg")
+    ap.add_argument("--no-tune", action="store_true", help="scene path: keep the default plan")
     return ap.parse_args()
 
 
@@ -65,7 +107,8 @@ def run_gpu(args, path, precision, steps, warmup):
     from diff_2d_fem_amd import ImplicitIntegrator
     w = build(args.config, args.envs)
     m = w["model"]
-    integ = ImplicitIntegrator(path=path, precision=precision)
+    integ = ImplicitIntegrator(path=path, precision=precision,
+                               autotune=(path == "scene" and not getattr(args, "no_tune", False)))
     step = 0
     for _ in range(warmup):
         step += 1
@@ -366,6 +409,11 @@ def main():
         "converged_frac": round(float(np.mean(main_run["stats"]["converged"])), 4),
     }
     main_run["n_steps"] = args.steps
+    tuned = getattr(main_run["integ"], "tuned", None)
+    if tuned is not None:
+        out["tuning"] = {k: tuned[k] for k in ("block_size", "max_cells", "skin", "lds_order", "nacc_lds",
+                                               "xc_lds", "excl_bits_lds", "max_pairs", "cached")}
+        out["tuning"]["trials_ms_per_step"] = [t["ms"] for t in tuned["trials"]]
     if path == "scene":
         out["roofline"] = scene_roofline(main_run, precision)
     out["element_roofline"] = elastic_roofline(main_run)
@@ -375,6 +423,8 @@ def main():
         out["roofline"] = dict(out["element_roofline"])
     if not (path == "scene" and precision == "double"):
         out["pos_err_1step"] = one_step_error(main_run)
+    if path == "scene" and not args.no_graph:
+        out["graph"] = graph_leg(main_run, args.steps)
     if not args.no_plain and not (path == "plain" and precision == "double"):
         plain = run_gpu(args, "plain", "double", args.steps, args.warmup)
         pv = B * args.steps / plain["t"]

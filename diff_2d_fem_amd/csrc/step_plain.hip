@@ -79,8 +79,17 @@ __global__ void k_commit_v(S s, double inv_dt) {
 
 // ------------------------------------------------------------------------- gradient + diagonal
 // reference implicit/kernels.py:73-82 (+ the external force, implicit/actuation.py:37-41)
+// Also clears the direction pass's per-scene dot-product accumulators (k_mask_dk and the mode
+// reductions add into them later in the same iteration; N >= B).
 __global__ void k_grad_diag_mass(S s, double h2) {
   GRID_LOOP(i, s.N) {
+    if (i < s.B) {
+      s.gPg[i] = 0.0;
+      s.g_p[i] = 0.0;
+      s.g_Py[i] = 0.0;
+      s.y_p[i] = 0.0;
+      s.y_Py[i] = 0.0;
+    }
     if (i < s.N) {
       const V2<double> g = ld2(s.grad, i);
       st2(s.grad_prev, i, g);
@@ -874,6 +883,46 @@ __global__ void k_fill(double *a, int n, double v) {
     if (i < n) a[i] = v;
   }
 }
+// Start of a contact pass that caps a step (modes 1 and 3): reset the cap to 1 when asked (the
+// sweep), snapshot it into t_c (every pair's ACCD stops at the cap the pass started from) and
+// clear the pass's minimum distance.
+__global__ void k_ipc_begin(int B, double *tc, double *alpha, double *clr, int reset) {
+  GRID_LOOP(e, B) {
+    if (e < B) {
+      if (reset) alpha[e] = 1.0;
+      tc[e] = alpha[e];
+      clr[e] = BIG;
+    }
+  }
+}
+// Per-iteration reset of the affine groups' reductions: gq_prev = gq, gq = Hx = Hy = 0.
+__global__ void k_affine_reset(S s) {
+  GRID_LOOP(j, 24 * s.n_ag) {
+    if (j < 24 * s.n_ag) {
+      const int g = j / 24, r = j % 24;
+      if (r < 6) {
+        s.ag_gq_prev[6 * g + r] = s.ag_gq[6 * g + r];
+        s.ag_gq[6 * g + r] = 0.0;
+      } else if (r < 15) {
+        s.ag_Hx[9 * g + r - 6] = 0.0;
+      } else {
+        s.ag_Hy[9 * g + r - 15] = 0.0;
+      }
+    }
+  }
+}
+// Per-iteration reset of the rigid-mode groups' reductions.
+__global__ void k_rigid_reset(S s) {
+  GRID_LOOP(j, 18 * s.n_rg) {
+    if (j < 18 * s.n_rg) {
+      const int g = j / 18, r = j % 18;
+      if (r < 3) s.rg_acc[3 * g + r] = 0.0;
+      else if (r < 6) s.rg_gq[3 * g + r - 3] = 0.0;
+      else if (r < 9) s.rg_gq_prev[3 * g + r - 6] = 0.0;
+      else s.rg_H[9 * g + r - 9] = 0.0;
+    }
+  }
+}
 
 // ------------------------------------------------------------------------- pressure, tendons
 // reference implicit/actuation.py:89-123
@@ -1592,10 +1641,7 @@ struct Plain {
     c.h2 = h2;
     c.mu = mu;
     c.floor_ = mode == 3 ? 0.25 * p.dhat : 0.0;
-    if (mode == 1 || mode == 3) {
-      // t_c snapshot: the ACCD of every pair stops at the alpha the pass started from
-      D2F_HIP(hipMemcpyAsync(s.tc, alpha, sizeof(double) * s.B, hipMemcpyDeviceToDevice, st));
-    }
+    // modes 1 and 3 read the t_c snapshot that k_ipc_begin took before the pass
     const size_t base = ipc_env_lds(mode, s.max_nb);
     // 64 KiB with the static LDS; edge and node indices packed in 16 bits in the pair list
     if (s.e_start && s.max_nb > 0 && s.max_nb < 65536 && base <= 52 * 1024) {
@@ -1652,8 +1698,7 @@ struct Plain {
   }
 
   int ccd_sweep() {  // IPCContact.ccd along bc_q into sweep_beta (reference ipc_contact.py:432-439)
-    LAUNCH(k_fill, s.B, s.sweep_beta, s.B, 1.0);
-    LAUNCH(k_fill, s.B, s.clr_ipc, s.B, BIG);
+    LAUNCH(k_ipc_begin, s.B, s.B, s.tc, s.sweep_beta, s.clr_ipc, 1);
     int r = ipc_pass(3, s.bc_q, s.grad, s.diag, s.free_, s.Ener, s.clr_ipc, s.sweep_beta, p.h2, p.ipc_mu);
     if (r) return r;
     LAUNCH(k_far_cap, s.B, s.B, s.q_inf, p.ipc_reach, s.sweep_beta);
@@ -1661,7 +1706,7 @@ struct Plain {
   }
 
   int sweep() {
-    D2F_HIP(hipMemsetAsync(s.q_inf, 0, sizeof(double) * s.B, st));
+    LAUNCH(k_fill, s.B, s.q_inf, s.B, 0.0);
     LAUNCH(k_bc_motion, s.n_bc, s);
     int r = ccd_sweep();
     if (r) return r;
@@ -1702,24 +1747,18 @@ struct Plain {
     if (s.n_sp > 0) LAUNCH(k_springs, s.n_sp, s, 0, p.h2, s.Ener);
     if (s.n_pe > 0) LAUNCH(k_pressure, s.n_pe, s, 0, p.h2, s.Ener);
     if (s.n_td > 0) {
-      D2F_HIP(hipMemsetAsync(s.td_L, 0, sizeof(double) * s.n_td, st));
+      LAUNCH(k_fill, s.n_td, s.td_L, s.n_td, 0.0);
       LAUNCH(k_tendon_length, s.n_ey, s);
       LAUNCH(k_tendon_state, s.n_td, s);
       LAUNCH(k_tendon_grad, s.n_ey, s, p.h2);
     }
     // modes: affine gather, rigid gather
     if (s.n_ag > 0) {
-      D2F_HIP(hipMemcpyAsync(s.ag_gq_prev, s.ag_gq, sizeof(double) * 6 * s.n_ag, hipMemcpyDeviceToDevice, st));
-      D2F_HIP(hipMemsetAsync(s.ag_gq, 0, sizeof(double) * 6 * s.n_ag, st));
-      D2F_HIP(hipMemsetAsync(s.ag_Hx, 0, sizeof(double) * 9 * s.n_ag, st));
-      D2F_HIP(hipMemsetAsync(s.ag_Hy, 0, sizeof(double) * 9 * s.n_ag, st));
+      LAUNCH(k_affine_reset, 24 * s.n_ag, s);
       LAUNCH(k_affine_gather, s.n_an, s);
     }
     if (s.n_rg > 0) {
-      D2F_HIP(hipMemsetAsync(s.rg_acc, 0, sizeof(double) * 3 * s.n_rg, st));
-      D2F_HIP(hipMemsetAsync(s.rg_gq, 0, sizeof(double) * 3 * s.n_rg, st));
-      D2F_HIP(hipMemsetAsync(s.rg_gq_prev, 0, sizeof(double) * 3 * s.n_rg, st));
-      D2F_HIP(hipMemsetAsync(s.rg_H, 0, sizeof(double) * 9 * s.n_rg, st));
+      LAUNCH(k_rigid_reset, 18 * s.n_rg, s);
       LAUNCH(k_rigid_centroid, s.n_rn, s);
       LAUNCH(k_rigid_gather, s.n_rn, s);
     }
@@ -1727,8 +1766,7 @@ struct Plain {
   }
 
   int direction(int it) {
-    for (double *a : {s.gPg, s.g_p, s.g_Py, s.y_p, s.y_Py})
-      D2F_HIP(hipMemsetAsync(a, 0, sizeof(double) * s.B, st));
+    // gPg, g_p, g_Py, y_p, y_Py were cleared by k_grad_diag_mass
     LAUNCH(k_mask_dk, s.N, s);
     if (s.n_ag > 0) LAUNCH(k_affine_reduce, s.n_ag, s, p.h2);
     if (s.n_rg > 0) LAUNCH(k_rigid_reduce, s.n_rg, s);
@@ -1753,7 +1791,7 @@ struct Plain {
     }
     if (s.n_ag > 0) LAUNCH(k_affine_pHp_energy, s.n_ag, s, 1, p.h2, s.Ener);
     if (p.ipc_on) {
-      LAUNCH(k_fill, s.B, s.clr_ipc, s.B, BIG);
+      LAUNCH(k_ipc_begin, s.B, s.B, s.tc, s.alpha, s.clr_ipc, 0);
       int r = ipc_pass(1, s.p, s.grad, s.diag, s.free_, s.Ener, s.clr_ipc, s.alpha, p.h2, p.ipc_mu);
       if (r) return r;
       LAUNCH(k_far_cap, s.B, s.B, s.p_inf, p.ipc_reach, s.alpha);
@@ -1761,7 +1799,7 @@ struct Plain {
     if (s.n_sp > 0) LAUNCH(k_springs, s.n_sp, s, 1, p.h2, s.Ener);
     if (s.n_pe > 0) LAUNCH(k_pressure, s.n_pe, s, 1, p.h2, s.Ener);
     if (s.n_td > 0) {
-      D2F_HIP(hipMemsetAsync(s.td_dL, 0, sizeof(double) * s.n_td, st));
+      LAUNCH(k_fill, s.n_td, s.td_dL, s.n_td, 0.0);
       LAUNCH(k_tendon_pHp, s.n_ey, s, p.h2);
       LAUNCH(k_tendon_taut, s.n_td, s, p.h2);
     }
@@ -1772,7 +1810,7 @@ struct Plain {
   }
 
   int energy_into(double *E) {
-    D2F_HIP(hipMemsetAsync(E, 0, sizeof(double) * s.B, st));
+    LAUNCH(k_fill, s.B, E, s.B, 0.0);
     LAUNCH(k_node_energy, s.N, s, p.h2, E);
     LAUNCH(k_elastic_energy, s.T, s.x, s.tri, s.Bm, s.area, s.mu, s.lam, s.tri2env, s.T, p.h2, E);
     if (s.n_col > 0) LAUNCH(k_colliders_energy, s.n_b, s, p, E);
@@ -1788,7 +1826,7 @@ struct Plain {
     if (s.n_sp > 0) LAUNCH(k_springs, s.n_sp, s, 2, p.h2, E);
     if (s.n_pe > 0) LAUNCH(k_pressure, s.n_pe, s, 2, p.h2, E);
     if (s.n_td > 0) {
-      D2F_HIP(hipMemsetAsync(s.td_L, 0, sizeof(double) * s.n_td, st));
+      LAUNCH(k_fill, s.n_td, s.td_L, s.n_td, 0.0);
       LAUNCH(k_tendon_length, s.n_ey, s);
       LAUNCH(k_tendon_energy, s.n_td, s, p.h2, E);
     }
@@ -1835,8 +1873,8 @@ struct Plain {
       if (r) return r;
     }
     LAUNCH(k_reset_step, s.B, s);
-    D2F_HIP(hipMemsetAsync(s.p, 0, sizeof(double) * 2 * s.N, st));
-    D2F_HIP(hipMemsetAsync(s.grad, 0, sizeof(double) * 2 * s.N, st));
+    LAUNCH(k_fill, 2 * s.N, s.p, 2 * s.N, 0.0);
+    LAUNCH(k_fill, 2 * s.N, s.grad, 2 * s.N, 0.0);
     for (int it = 0; it < p.iter_max; ++it) {
       if (p.ipc_on && pending) {
         int r = sweep();
@@ -1975,11 +2013,10 @@ int d2f_ipc_min_distance(const d2f_scene *s, const d2f_params *p, void *stream) 
   hipStream_t st = (hipStream_t)stream;
   // IPCContact.min_distance (reference implicit/ipc_contact.py:441-447): a sweep along a zero
   // direction leaves the smallest pair distance in clr_ipc.
-  D2F_HIP(hipMemsetAsync(s->x0, 0, sizeof(double) * 2 * s->N, st));
-  D2F_HIP(hipMemsetAsync(s->q_inf, 0, sizeof(double) * s->B, st));
   Plain run(*s, *p, st);
-  LAUNCH(k_fill, s->B, s->sweep_beta, s->B, 1.0);
-  LAUNCH(k_fill, s->B, s->clr_ipc, s->B, BIG);
+  LAUNCH(k_fill, 2 * s->N, s->x0, 2 * s->N, 0.0);
+  LAUNCH(k_fill, s->B, s->q_inf, s->B, 0.0);
+  LAUNCH(k_ipc_begin, s->B, s->B, s->tc, s->sweep_beta, s->clr_ipc, 1);
   int r = run.ipc_pass(3, s->x0, s->grad, s->diag, s->free_, s->Ener, s->clr_ipc, s->sweep_beta, p->h2, p->ipc_mu);
   if (r) return r;
   D2F_HIP(hipGetLastError());

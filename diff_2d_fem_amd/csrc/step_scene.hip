@@ -14,6 +14,7 @@
 // stress and curvature from a displacement gradient formed in float64; float64 state, dot
 // products and energies).
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 
 #include "common.h"
@@ -1860,9 +1861,17 @@ static int launch(const SH &s, const Pm &p, hipStream_t stream) {
               "budget; use the per-kernel path for scenes this large", lds, s.max_nb, s.max_cells);
     return D2F_ECAP;
   }
-  if (lds > 64 * 1024)
-    D2F_HIP(hipFuncSetAttribute((const void *)k_scene<R, TPB>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  if (lds > 64 * 1024) {
+    // once per device and size, so that a launch under graph capture makes no attribute call
+    static std::atomic<int> lds_set[64];
+    int dev = 0;
+    D2F_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64 || lds_set[dev].load() < (int)lds) {
+      D2F_HIP(hipFuncSetAttribute((const void *)k_scene<R, TPB>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      if (dev >= 0 && dev < 64) lds_set[dev].store((int)lds);
+    }
+  }
   const int n_wg = s.sc_list ? s.n_list : s.B;
   if (n_wg == 0) return 0;
   hipLaunchKernelGGL((k_scene<R, TPB>), dim3(n_wg), dim3(TPB), lds, stream, s, p);

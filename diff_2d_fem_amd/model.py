@@ -491,6 +491,11 @@ class Model:
         """Where a body's nodes are to be at the end of the next step ((n_body_nodes, 2))."""
         self._targets[self._node_range[handle]] = np.asarray(targets, np.float64)
 
+    def set_node_targets(self, nodes, targets):
+        """Targets of any nodes, by global id ((n, 2)), in one call: set_kinematic_target for
+        many bodies at once. Only the prescribed nodes' targets are read by the step."""
+        self._targets[np.asarray(nodes, np.int64)] = np.asarray(targets, np.float64)
+
     def set_kinematic_pose(self, handle, translation=(0.0, 0.0), angle=0.0, about=None):
         self.set_kinematic_target(handle, self.rigid_motion(handle, translation, angle, about))
 

@@ -72,6 +72,8 @@ class DeviceSolver:
         self.node2env_np = np.asarray(me.node2env, np.int64)
         self.node2env = t(self.node2env_np, I32)
         self.block_size = 512              # scene path: 2 waves per SIMD, no register spills
+        self.skin = self.SKIN              # scene path: pair-list margin (x the contact reach)
+        self.lds_order = self.LDS_ORDER    # scene path: which optional LDS parts come first
         self._layout = None
         self._route_t = None
         # scene path: step through d2f_step_scene_noalloc (the library's static work arrays, no
@@ -494,9 +496,44 @@ class DeviceSolver:
         self._apply_lds_overrides()
         self._layout = True
 
+    def set_plan(self, block_size=None, max_cells=None, skin=None, lds_order=None):
+        """Launch shape and LDS plan of the scene kernel (what the autotuner chooses): threads
+        per workgroup (256, 512 or 1024), grid cells per scene, pair-list margin and the order
+        in which the optional LDS parts are granted. Re-plans routing and LDS."""
+        if block_size is not None:
+            if int(block_size) not in (256, 512, 1024):
+                raise ValueError("block_size must be 256, 512 or 1024")
+            self.block_size = int(block_size)
+        if skin is not None:
+            if not float(skin) >= 0.0:
+                raise ValueError("skin must be >= 0")
+            self.skin = float(skin)
+        if lds_order is not None:
+            if sorted(lds_order) != sorted(self.LDS_ORDER):
+                raise ValueError(f"lds_order must order {self.LDS_ORDER}")
+            self.lds_order = tuple(lds_order)
+        if max_cells is not None:
+            if int(max_cells) < 1:
+                raise ValueError("max_cells must be >= 1")
+            self.max_cells = int(max_cells)
+        if self._layout is not None:
+            self._route()
+            self._plan_lds()
+            self._apply_lds_overrides()
+
+    def plan(self):
+        """The current launch shape and LDS plan."""
+        return dict(block_size=self.block_size, max_cells=self.max_cells, skin=self.skin,
+                    lds_order=list(self.lds_order), nacc_lds=self.nacc_lds, xc_lds=self.xc_lds,
+                    excl_bits_lds=self.excl_bits_lds, max_pairs=self.max_pairs,
+                    plain_envs=len(self.plain_envs))
+
     LDS_BUDGET = 150 * 1024          # bytes of LDS one scene's workgroup may take
     EXCL_LDS_MAX = 8192              # largest exclusion bitset staged in LDS, 32-bit words
     SKIN = 0.5                       # pair-list margin, in units of the contact reach
+    # optional LDS parts, granted in this order while the budget lasts: per-node element
+    # accumulators, positions, the bit-packed exclusion matrix, the candidate-pair list
+    LDS_ORDER = ("nacc", "xc", "bits", "pairs")
 
     def _fixed_lds(self, nb, ag):
         """Bytes of LDS a scene workgroup cannot do without (for the current block_size), for a
@@ -529,9 +566,10 @@ class DeviceSolver:
     def _plan_lds(self):
         """Split each scene workgroup's LDS budget (for the current block_size) over the scenes
         the scene kernel takes: fixed scratch (reductions, boundary positions and accumulators,
-        grid), then per-node element accumulators, then the bit-packed exclusion matrix, then
-        the candidate-pair list (bit jp * n_b_e + j of a scene's bitset is its exclusion of
-        boundary nodes jp and j)."""
+        grid), then the optional parts in the order of lds_order while they fit: per-node
+        element accumulators ("nacc"), positions ("xc"), the bit-packed exclusion matrix
+        ("bits"; bit jp * n_b_e + j of a scene's bitset is its exclusion of boundary nodes jp
+        and j) and the candidate-pair list ("pairs")."""
         self.excl_bits_lds, self.max_pairs, self.nacc_lds, self.xc_lds = 0, 0, 0, 0
         sub = self.scene_envs
         nn = np.diff(np.asarray(self.me.node_starts, np.int64))[sub]
@@ -541,23 +579,26 @@ class DeviceSolver:
         if self.affine is not None:
             self.max_ag = int(np.diff(self.ag_start.cpu().numpy())[sub].max(initial=0))
         base = self._fixed_lds(self.sc_max_nb, self.max_ag)
-        if base + 32 * self.max_nn <= self.LDS_BUDGET:
-            self.nacc_lds = 1
-            base += 32 * self.max_nn
-        if base + 16 * self.max_nn + 4 * 4 * self.sc_max_nb <= self.LDS_BUDGET:
-            self.xc_lds = 1                  # positions, if a modest pair list still fits
-            base += 16 * self.max_nn
-        if (self.ipc is None or self.n_e == 0) and self.node is None:
-            return
-        wmax = int(self.excl_words[sub].max(initial=0))
-        if wmax <= self.EXCL_LDS_MAX and base + 4 * wmax <= self.LDS_BUDGET:
-            self.excl_bits_lds = wmax
-        if self.ipc is None:             # node contact: partners come from the grid itself
-            return
-        left = (self.LDS_BUDGET - base - 4 * self.excl_bits_lds) // 4
-        e_per = np.bincount(self.e2env.cpu().numpy(), minlength=self.B)[sub]
-        if self.sc_max_nb < 65536 and e_per.max(initial=0) < 32768:
-            self.max_pairs = int(max(0, min(left, 16 * self.sc_max_nb)))
+        contact = not ((self.ipc is None or self.n_e == 0) and self.node is None)
+        wmax = int(self.excl_words[sub].max(initial=0)) if contact else 0
+        pairs_ok = False
+        if contact and self.ipc is not None:   # node contact: partners come from the grid itself
+            e_per = np.bincount(self.e2env.cpu().numpy(), minlength=self.B)[sub]
+            pairs_ok = self.sc_max_nb < 65536 and e_per.max(initial=0) < 32768
+        budget = self.LDS_BUDGET
+        for part in self.lds_order:
+            if part == "nacc" and base + 32 * self.max_nn <= budget:
+                self.nacc_lds = 1
+                base += 32 * self.max_nn
+            elif part == "xc" and base + 16 * self.max_nn + 4 * 4 * self.sc_max_nb <= budget:
+                self.xc_lds = 1                  # positions, if a modest pair list still fits
+                base += 16 * self.max_nn
+            elif part == "bits" and contact and wmax <= self.EXCL_LDS_MAX and base + 4 * wmax <= budget:
+                self.excl_bits_lds = wmax
+                base += 4 * wmax
+            elif part == "pairs" and pairs_ok:
+                self.max_pairs = int(max(0, min((budget - base) // 4, 16 * self.sc_max_nb)))
+                base += 4 * self.max_pairs
 
     def _apply_lds_overrides(self):
         """Caps on the LDS plan, set through `lds_overrides` (a dict with any of nacc_lds,
@@ -691,13 +732,13 @@ class DeviceSolver:
             q.ipc_on = 1
             q.ipc_kappa, q.ipc_mu = self.ipc["kappa"], self.ipc["mu"]
             q.ipc_reach, q.ipc_gap = self.ipc["reach"], self.ipc["gap"]
-            q.ipc_skin = self.SKIN * self.ipc["reach"]
+            q.ipc_skin = self.skin * self.ipc["reach"]
             q.ipc_per_body = int(self.ipc["per_body"])
         if self.node is not None:
             nd = self.node
             q.node_on, q.node_fric = 1, int(nd["fric"])
             q.node_k, q.node_r, q.node_mu = nd["k"], nd["r"], nd["mu"]
-            q.ipc_skin = self.SKIN * nd["r"]
+            q.ipc_skin = self.skin * nd["r"]
             q.ipc_per_body = int(nd["per_body"])
         q.rigid_modes_on = int(self.rigid is not None)
         q.precision = 1 if self.precision == "mixed" else 0

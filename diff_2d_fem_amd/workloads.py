@@ -116,28 +116,30 @@ def pads_lift_block(n_env=256, seed=0, n_close=10, n_lift=200, scene_ids=None):
                                      E=3.0e5, nu=0.45, density=1100.0, env_id=i) for s in (-1, 1)])
     m.finalize()
     m.gravity = (0.0, -9.81)
-    rest = []
-    for pair in pads:
-        r = []
+    # the pads' driven back faces, all scenes at once: global ids, rest positions, side (-1
+    # left, +1 right) and scene of every driven node (the step reads no other node's target)
+    nodes, X0, side, env = [], [], [], []
+    for i, pair in enumerate(pads):
         for s, p in zip((-1, 1), pair):
             X = m.rest_x(p)
             back = X[:, 0].min() if s < 0 else X[:, 0].max()
-            m.set_kinematic_constraint(p, np.where(np.abs(X[:, 0] - back) < 1e-9)[0])
-            r.append(X)
-        rest.append(r)
+            driven = np.where(np.abs(X[:, 0] - back) < 1e-9)[0]
+            m.set_kinematic_constraint(p, driven)
+            nodes.append(m.body_nodes(p)[driven])
+            X0.append(X[driven])
+            side.append(np.full(len(driven), float(s)))
+            env.append(np.full(len(driven), i))
+    nodes, X0, side, env = (np.concatenate(a) for a in (nodes, X0, side, env))
     close = np.array([2.25 * px for px in pxs])             # to a quarter pixel past touching
-
     lift = np.array([8.0 * px for px in pxs])
 
     def command(step):
-        for i, (pair, xs) in enumerate(zip(pads, rest)):
-            c = close[i] * min(step / n_close, 1.0)
-            up = lift[i] * min(max(step - n_close, 0) / n_lift, 1.0)
-            for s, p, X in zip((-1, 1), pair, xs):
-                Y = X.copy()
-                Y[:, 0] -= s * c
-                Y[:, 1] += up
-                m.set_kinematic_target(p, Y)
+        c = close * min(step / n_close, 1.0)
+        up = lift * min(max(step - n_close, 0) / n_lift, 1.0)
+        Y = X0.copy()
+        Y[:, 0] -= side * c[env]
+        Y[:, 1] += up[env]
+        m.set_node_targets(nodes, Y)
     return dict(model=m, dt=1e-3, command=command, bodies=blocks, pads=pads,
                 size=float(np.mean([16 * px for px in pxs])),
                 sides=np.array([16 * px for px in pxs]), name="pads_lift_block")
