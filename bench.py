@@ -2,7 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3] [--path scene|plain]
                     [--precision mixed|double] [--no-plain] [--no-cpu] [--no-large]
-                    [--no-adjoint] [--no-graph] [--no-tune]
+                    [--no-adjoint] [--no-graph] [--no-tune] [--no-c5]
 
 The workload is the contact-heavy configuration: 256 scenes of two soft pads on kinematic
 jaws gripping and lifting a rigid block (diff_2d_fem_amd.workloads.pads_lift_block). W
@@ -23,7 +23,10 @@ triangles, working set far beyond the Infinity Cache); `adjoint_c4` the differen
 the adjoint configuration (512 pneumatic fingers: forward and backward ms per step); `graph`
 the scene path's K steps captured as one HIP graph and replayed (staged targets, one graph
 launch) beside the eager per-step loop; `tuning` the scene kernel's plan that the autotuner
-chose when the solver was built (before the warm-up; --no-tune keeps the default); `cpu_baseline` times the float64 CPU oracle on a bounded sample of the same workload.
+chose from the warmed-up state (--no-tune keeps the default); `c5` the large uneven batch at 256
+star-mesh scenes (2k-20k triangles each; mixed precision, tuned, eager and graph-replayed, and
+the per-kernel float64 path on the same steps: a batch whose scenes converge at different
+iterations); `cpu_baseline` times the float64 CPU oracle on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -75,6 +78,38 @@ def graph_leg(res, steps, per_replay=5):
                 eager_launch_ms=round(res["launch_ms"], 3))
 
 
+def c5_leg(n_scenes=256, steps=10, warmup=3, plain=True):
+    """The large uneven batch (star meshes of 2k-20k triangles with affine regions, tendons and
+    the ground; mixed precision) at a stated scene count: warm-up, the autotuner, `steps` eager
+    steps (ms/step, PNCG iterations/s, the step kernel's roofline), then the same number of
+    steps replayed as HIP graphs. Scenes of this batch converge at different iterations (the
+    small meshes long before their budget): `plain_fp64` runs the same steps on the per-kernel
+    float64 path, which stops when every scene has converged."""
+    t0 = time.perf_counter()
+    res = run_gpu("c5", n_scenes, "scene", "mixed", steps, warmup, tune=True)
+    m = res["m"]
+    out = dict(workload=res["w"]["name"], scenes=m.n_envs, nodes=m.n_nodes, triangles=m.n_elements,
+               dt=res["w"]["dt"], setup_and_warmup_s=round(time.perf_counter() - t0 - res["t"], 1),
+               ms_per_step=round(res["t"] / steps * 1e3, 3),
+               scene_steps_per_s=round(m.n_envs * steps / res["t"], 1),
+               pncg_iters_per_s=round(res["iters"] / res["t"], 1),
+               iters_per_scene_per_step=round(res["iters"] / m.n_envs / steps, 1),
+               converged_frac=round(float(np.mean(res["stats"]["converged"])), 4),
+               max_iter=int(res["stats"]["max_iter"]),
+               element_lds=bool(m._solver.nacc_lds),
+               roofline=scene_roofline(res, "mixed"))
+    if res["tuned"] is not None:
+        out["tuning"] = {k: res["tuned"][k] for k in ("block_size", "max_cells", "skin", "lds_order")}
+    out["graph"] = graph_leg(res, steps)
+    if plain:
+        pr = run_gpu("c5", n_scenes, "plain", "double", steps, warmup)
+        out["plain_fp64"] = dict(ms_per_step=round(pr["t"] / steps * 1e3, 3),
+                                 iters_per_scene_per_step=round(pr["iters"] / n_scenes / steps, 1),
+                                 converged_frac=round(float(np.mean(pr["stats"]["converged"])), 4))
+        out["speedup_vs_plain_fp64"] = round(pr["t"] / res["t"], 3)
+    return out
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +126,7 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 512-finger adjoint leg")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay leg")
     ap.add_argument("--no-tune", action="store_true", help="scene path: keep the default plan")
+    ap.add_argument("--no-c5", action="store_true", help="skip the 256-scene star-mesh leg")
     return ap.parse_args()
 
 
@@ -100,21 +136,25 @@ def build(config, n_env):
     return fn() if n_env is None else fn(n_env)
 
 
-def run_gpu(args, path, precision, steps, warmup):
-    """W warm-up steps, then K timed steps. HIP events recorded on the launch stream right
-    around each solver launch give the step kernel's own duration."""
+def run_gpu(config, envs, path, precision, steps, warmup, tune=False):
+    """W warm-up steps, the autotuner (scene path, when asked: it measures its candidates from
+    the warmed-up state and puts the state back), then K timed steps. HIP events recorded on the
+    launch stream right around each solver launch give the step kernel's own duration."""
     import torch
     from diff_2d_fem_amd import ImplicitIntegrator
-    w = build(args.config, args.envs)
+    w = build(config, envs)
     m = w["model"]
-    integ = ImplicitIntegrator(path=path, precision=precision,
-                               autotune=(path == "scene" and not getattr(args, "no_tune", False)))
+    integ = ImplicitIntegrator(path=path, precision=precision)
     step = 0
     for _ in range(warmup):
         step += 1
         w["command"](step)
         integ.simulate(m, w["dt"])
     torch.cuda.synchronize()
+    tuned = None
+    if tune and path == "scene":
+        from diff_2d_fem_amd.tune import autotune
+        tuned = autotune(integ, m, w["dt"], cache=False)
     s = m._solver
     events = []
     launch = s.step
@@ -140,7 +180,8 @@ def run_gpu(args, path, precision, steps, warmup):
     launch_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in events]))
     st = integ.stats(m)
     iters = int(acc.sum().item())
-    return dict(t=t, m=m, w=w, integ=integ, stats=st, iters=iters, step=step, launch_ms=launch_ms)
+    return dict(t=t, m=m, w=w, integ=integ, stats=st, iters=iters, step=step, launch_ms=launch_ms,
+                n_steps=steps, tuned=tuned)
 
 
 def _snapshot(s):
@@ -205,12 +246,13 @@ def measured_traffic(kernel, res):
 def scene_roofline(res, precision):
     """The step kernel of the scene path (one launch per step): algorithmic bytes per launch
     = sum over scenes of iterations x per-iteration bytes, where one iteration reads each
-    scene's element constants twice (gradient+diagonal and curvature: T (12 + 7 s_e)) and
-    touches node state for 272 bytes per node; s_e = 4 in mixed, 8 in float64."""
+    scene's element constants twice (gradient+diagonal and curvature: T (12 + 6 s_e) for the
+    three node indices, D_m^-1 and the area-weighted moduli) and touches node state for 272
+    bytes per node; s_e = 4 in mixed, 8 in float64."""
     s = res["m"]._solver
     se = 4 if precision == "mixed" else 8
     it_per_env = res["iters"] / res["m"].n_envs / res["n_steps"]
-    bytes_ = it_per_env * (2.0 * (12 + 7 * se) * s.n_tri + 272.0 * s.N)
+    bytes_ = it_per_env * (2.0 * (12 + 6 * se) * s.n_tri + 272.0 * s.N)
     dur = res["launch_ms"] * 1e-3
     ach = bytes_ / dur / 1e9
     name = f"d2f::k_scene<{'float' if precision == 'mixed' else 'double'}, {s.block_size}>"
@@ -390,7 +432,8 @@ def main():
     has_scene = hasattr(_lib.lib(), "d2f_step_scene")
     path = args.path or ("scene" if has_scene else "plain")
     precision = args.precision or ("mixed" if path == "scene" else "double")
-    main_run = run_gpu(args, path, precision, args.steps, args.warmup)
+    main_run = run_gpu(args.config, args.envs, path, precision, args.steps, args.warmup,
+                       tune=not args.no_tune)
     m = main_run["m"]
     B = m.n_envs
     value = B * args.steps / main_run["t"]
@@ -408,8 +451,7 @@ def main():
         "max_iter": int(main_run["stats"]["max_iter"]),
         "converged_frac": round(float(np.mean(main_run["stats"]["converged"])), 4),
     }
-    main_run["n_steps"] = args.steps
-    tuned = getattr(main_run["integ"], "tuned", None)
+    tuned = main_run["tuned"]
     if tuned is not None:
         out["tuning"] = {k: tuned[k] for k in ("block_size", "max_cells", "skin", "lds_order", "nacc_lds",
                                                "xc_lds", "excl_bits_lds", "max_pairs", "cached")}
@@ -426,12 +468,14 @@ def main():
     if path == "scene" and not args.no_graph:
         out["graph"] = graph_leg(main_run, args.steps)
     if not args.no_plain and not (path == "plain" and precision == "double"):
-        plain = run_gpu(args, "plain", "double", args.steps, args.warmup)
+        plain = run_gpu(args.config, args.envs, "plain", "double", args.steps, args.warmup)
         pv = B * args.steps / plain["t"]
         out["plain_fp64"] = dict(value=round(pv, 2), ms_per_step=round(plain["t"] / args.steps * 1e3, 3))
         out["speedup_vs_plain_fp64"] = round(value / pv, 3)
     if not args.no_adjoint:
         out["adjoint_c4"] = adjoint_leg()
+    if not args.no_c5:
+        out["c5"] = c5_leg(plain=not args.no_plain)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, args.cpu_steps)
     print(json.dumps(out), flush=True)
