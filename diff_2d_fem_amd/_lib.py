@@ -139,6 +139,10 @@ def lib():
     L.d2f_elastic_drho_f64.argtypes = [P, P, P, P, P, P, P, i32, f64, f64, P, P]
     L.d2f_pressure_hvp_f64.argtypes = [P, P, P, P, P, i32, f64, P, P]
     L.d2f_pressure_dp_f64.argtypes = [P, P, P, P, P, i32, f64, P, P]
+    L.d2f_ipc_hvp_f64.argtypes = [P, P, P, i32, P, P, P, P, f64, f64, f64, P, P]
+    L.d2f_ipc_grad_diag_f64.argtypes = [P, P, i32, P, P, P, P, f64, f64, f64, P, P, P]
+    L.d2f_affine_expand_f64.argtypes = [P, P, P, P, P, i32, P, P]
+    L.d2f_affine_reduce_f64.argtypes = [P, P, P, P, P, P, P, P, i32, i32, i32, P, P]
     L.d2f_minres_begin_f64.argtypes = [C.POINTER(Minres), P]
     L.d2f_minres_step_f64.argtypes = [C.POINTER(Minres), P]
     sizes = (C.c_int64 * 4)()

@@ -11,15 +11,27 @@ trajectory x_0 .. x_T is differentiated by the damped backward-Euler recursion
 with H_k = M + h^2 (K_elastic + K_contact + K_pressure) at x_k, restricted to the free degrees of
 freedom, the contact active set frozen at x_k. The operator holds inertia, elasticity, the
 colliders' barriers, with use_self the "node" contact's penalty between bodies, and — beyond
-the reference, whose adjoint refuses them — pressure chambers, whose tangent is exact (the
-pressure gradient is linear in x). With chambers the recursion also gives dJ/dp per chamber
-and step (dJdp_steps): dJ/dp_c,k = -lam_k . dR_k/dp_c. Friction, IPC contact between bodies,
-tendons, springs, active strain and rigid regions have no tangent in it (transient_adjoint
-refuses a model with any of them except collider friction, as the reference does).
+the reference, whose adjoint refuses them:
+* pressure chambers, whose tangent is exact (the pressure gradient is linear in x); the
+  recursion then also gives dJ/dp per chamber and step (dJdp_steps):
+  dJ/dp_c,k = -lam_k . dR_k/dp_c;
+* the IPC barrier between bodies (use_ipc, on by default when the model has IPC contact): the
+  Hessian of every node-edge barrier term the gradient pass evaluates at x_k, the terms and
+  their vertex-correction coefficients frozen there;
+* affine bodies: their slaved nodes follow x_i = J_i q, so the step's optimality conditions are
+  those of the energy in (free soft nodes, q). Each solve is P^T H P lam_r = P^T rhs with P that
+  map, and lam = P lam_r feeds dJ/drho and the recursion. A region's own elements carry their
+  moduli in H, so at the slaved positions their elastic Hessian is that of the region energy
+  in q, and dJ/drho covers them.
+Friction (colliders: no tangent, as in the reference; between bodies: refused), tendons, springs
+and active strain have no tangent in it (transient_adjoint refuses a model with any of them
+except collider friction, as the reference does). Rigid modes only precondition the forward
+solve and do not enter.
 
 Everything runs on the GPU: the Hessian-vector products, dJ/drho and dJ/dp are HIP kernels
-(d2f_elastic_hvp_f64, d2f_collider_hvp_f64, d2f_node_hvp_f64, d2f_pressure_hvp_f64,
-d2f_elastic_drho_f64, d2f_pressure_dp_f64), and each adjoint solve is a preconditioned MINRES
+(d2f_elastic_hvp_f64, d2f_collider_hvp_f64, d2f_node_hvp_f64, d2f_ipc_hvp_f64,
+d2f_pressure_hvp_f64, d2f_elastic_drho_f64, d2f_pressure_dp_f64), the affine reduction is
+d2f_affine_expand_f64 / d2f_affine_reduce_f64, and each adjoint solve is a preconditioned MINRES
 with per-scene Krylov scalars on the device (d2f_minres_step_f64): the scenes of a batch do not
 couple through shared scalars, and the host reads only the scenes' status flags, every
 `check_every` iterations.
@@ -62,6 +74,40 @@ class AdjointPNCG2D:
                 setattr(self._cols[i], k, float(c[k]))
         self._n_col = len(cols)
         self._tiles = solver.elem_tiles_struct()
+        # moduli of the operator: the solver's, with each rigid region's own elements (zero in
+        # the solver's nodal passes) carrying the moduli its region energy sums
+        mu_el, lam_el = np.asarray(solver.mu_np, np.float64).copy(), np.asarray(solver.lam_np, np.float64).copy()
+        self.affine = solver.affine
+        if self.affine is not None:
+            for el in self.affine["group_elems"]:
+                mu_el[el] = np.asarray(me.mu, np.float64)[el]
+                lam_el[el] = np.asarray(me.lam, np.float64)[el]
+            self._setup_reduction()
+        self._mu_el, self._lam_el = self._t(mu_el), self._t(lam_el)
+        self._pairs = None                   # IPC pair list at the state being solved
+        self._n_pairs = 0
+
+    def _setup_reduction(self):
+        """Index maps of the affine reduction: slave_of[i] (entry of node i in the an_* lists, or
+        -1), three slot nodes per group, the reduced free mask and the soft-node mass."""
+        a, s = self.affine, self.s
+        node = a["node_np"]
+        slave_of = np.full(self.N, -1, np.int32)
+        slave_of[node] = np.arange(len(node), dtype=np.int32)
+        first = a["first"]
+        slot = np.stack([node[first[:-1] + k] for k in range(3)], 1).reshape(-1).astype(np.int32)
+        fm = self.free_np.copy()
+        fm[node] = 0.0
+        fm[slot] = 1.0
+        I32 = torch.int32
+        self._slave_of = torch.as_tensor(slave_of, device=self.dev)
+        self._slot = torch.as_tensor(slot, device=self.dev)
+        self._an_first = torch.as_tensor(np.asarray(first, np.int32), device=self.dev).to(I32)
+        self._fmask_red = self._t(fm)
+        self._slaved = torch.as_tensor(a["slaved"], device=self.dev)
+        self._n_groups = int(a["n"])
+        self._vfull = torch.zeros((self.N, 2), dtype=F64, device=self.dev)
+        self._kfull = torch.zeros((self.N, 2), dtype=F64, device=self.dev)
 
     # ---- plumbing
     def _t(self, a):
@@ -80,6 +126,7 @@ class AdjointPNCG2D:
         """Re-read the solver's nodal mass (call after changing solver.mass)."""
         self.mass_np = np.asarray(self.s.mass.cpu().numpy(), np.float64)
         self._mass2 = self.s.mass.to(F64)[:, None]
+        self._mass_full = self.s.mass.to(F64).clone()
         return self.mass_np
 
     # ---- density (reference implicit/adjoint.py:41-50, 225-228)
@@ -88,21 +135,49 @@ class AdjointPNCG2D:
         if rho.shape[0] != self.n_tri:
             raise ValueError(f"density has {rho.shape[0]} entries, the mesh {self.n_tri} elements")
         E = self.Emin + rho * (1.0 - self.Emin)
-        self.s.mu_np = E * np.asarray(self.s.me.mu, np.float64)
-        self.s.lam_np = E * np.asarray(self.s.me.lam, np.float64)
-        self.s.refresh_moduli()
+        mu_el = E * np.asarray(self.s.me.mu, np.float64)
+        lam_el = E * np.asarray(self.s.me.lam, np.float64)
+        self._mu_el, self._lam_el = self._t(mu_el), self._t(lam_el)
+        if self.affine is not None:          # regions: summed moduli, own elements zero in the solver
+            self.s.set_affine_moduli(mu_el, lam_el)
+        else:
+            self.s.mu_np, self.s.lam_np = mu_el, lam_el
+            self.s.refresh_moduli()
 
     # ---- operator H v = M v + h^2 (K_elastic + K_contact) v (reference :231-256)
     def _node_on(self, use_self):
         return bool(use_self) and self.s.node is not None
 
-    def _kv(self, xw, vw, y, use_disc=False, use_sdf=False, use_self=False):
-        """y += h^2 K v at xw (elastic, selected colliders, node contact, pressure)."""
+    def _ipc_on(self, use_ipc):
+        return self.s.ipc is not None and (use_ipc is None or bool(use_ipc))
+
+    def _set_pairs(self, xw):
+        """The IPC pair list at xw (d2f_ipc_pairs), kept for the solve's products."""
+        s = self.s
+        L = _lib.lib()
+        cap = max(int(getattr(self, "_pair_cap", 0)), 16 * max(s.n_e, 1))
+        while True:
+            if self._pairs is None or self._pairs.numel() < 2 + 2 * cap:
+                self._pairs = torch.zeros(2 + 2 * cap, dtype=torch.int32, device=self.dev)
+            sc, q = s.scene_struct("plain"), s.params_struct()
+            sc.x = self._p(xw)
+            _lib.check(L.d2f_ipc_pairs(C.byref(sc), C.byref(q), self._p(self._pairs), cap, self._stream()),
+                       "d2f_ipc_pairs")
+            n = int(self._pairs[0].item())
+            if n <= cap:
+                break
+            cap = 2 * n
+        self._pair_cap, self._n_pairs = cap, n
+
+    def _kv(self, xw, vw, y, use_disc=False, use_sdf=False, use_self=False, use_ipc=None):
+        """y += h^2 K v at xw (elastic, selected colliders, node contact, IPC barrier between
+        bodies with the pairs of the last _set_pairs, pressure)."""
         L = _lib.lib()
         s = self.s
         _lib.check(L.d2f_elastic_hvp_f64(self._p(xw), self._p(vw), self._p(s.tri), self._p(s.Bm),
-                                         self._p(s.area), self._p(s.mu), self._p(s.lam), self.n_tri,
-                                         self.h2, self._p(y), self._stream()), "d2f_elastic_hvp_f64")
+                                         self._p(s.area), self._p(self._mu_el), self._p(self._lam_el),
+                                         self.n_tri, self.h2, self._p(y), self._stream()),
+                   "d2f_elastic_hvp_f64")
         which = 2 if (use_disc and use_sdf) else (0 if use_disc else 1)
         if (use_disc or use_sdf) and self._n_col:
             _lib.check(L.d2f_collider_hvp_f64(self._p(xw), self._p(vw), self._p(s.bnd), s.n_b, self._cols,
@@ -114,22 +189,59 @@ class AdjointPNCG2D:
             _lib.check(L.d2f_node_hvp_f64(self._p(xw), self._p(vw), self._p(s.bnd), self._p(s.bnd_start), s.B,
                                           self._p(s.excl), self._p(s.excl_off), nd["r"], nd["k"], self.h2,
                                           self._p(y), self._stream()), "d2f_node_hvp_f64")
+        if self._ipc_on(use_ipc) and self._n_pairs:
+            ip = s.ipc
+            _lib.check(L.d2f_ipc_hvp_f64(self._p(xw), self._p(vw), self._p(self._pairs), self._n_pairs,
+                                         self._p(s.ea), self._p(s.eb), self._p(s.ca), self._p(s.cb), ip["dhat"],
+                                         ip["kappa"], self.h2, self._p(y), self._stream()), "d2f_ipc_hvp_f64")
         if s.pressure is not None:
             _lib.check(L.d2f_pressure_hvp_f64(self._p(vw), self._p(s.pe_a), self._p(s.pe_b), self._p(s.pe_ch),
                                               self._p(s.ch_p), s.pressure["n_pe"], self.h2, self._p(y),
                                               self._stream()), "d2f_pressure_hvp_f64")
         return y
 
-    def _hvp(self, xw, vw, use_disc=False, use_sdf=False, include_mass=True, use_self=False):
+    # ---- the affine reduction (reduced vectors: soft entries in place, q in three slot nodes)
+    def _expand(self, vr, out):
+        L = _lib.lib()
+        s = self.s
+        _lib.check(L.d2f_affine_expand_f64(self._p(vr), self._p(self._slave_of), self._p(s.an_grp),
+                                           self._p(s.an_w), self._p(self._slot), self.N, self._p(out),
+                                           self._stream()), "d2f_affine_expand_f64")
+        return out
+
+    def _reduce(self, yf, out, vf=None, square=False):
+        L = _lib.lib()
+        s = self.s
+        _lib.check(L.d2f_affine_reduce_f64(self._p(yf), None if vf is None else self._p(vf),
+                                           self._p(self._mass_full), self._p(self._slave_of),
+                                           self._p(s.an_node), self._p(s.an_w), self._p(self._an_first),
+                                           self._p(self._slot), self.N, self._n_groups, int(bool(square)),
+                                           self._p(out), self._stream()), "d2f_affine_reduce_f64")
+        return out
+
+    def _kv_reduced(self, xw, vr, out, **use):
+        """out = P^T (h^2 K + M_slaved) P vr: the reduced operator's part beyond the soft nodes'
+        diagonal mass (the Krylov step adds mass v with the slaved nodes' mass zero)."""
+        vf, kf = self._vfull, self._kfull
+        self._expand(vr, vf)
+        kf.zero_()
+        self._kv(xw, vf, kf, **use)
+        return self._reduce(kf, out, vf=vf)
+
+    def _hvp(self, xw, vw, use_disc=False, use_sdf=False, include_mass=True, use_self=False, use_ipc=None):
         y = self._y
         y.zero_()
-        self._kv(xw, vw, y, use_disc, use_sdf, use_self)
+        self._kv(xw, vw, y, use_disc, use_sdf, use_self, use_ipc)
         return self._mass2 * vw + y if include_mass else y.clone()
 
-    def apply_H(self, xstar, v, use_self=False, use_disc=False, use_sdf=False, include_mass=True):
-        """H v at x* as an (N, 2) array (the reference's matrix-free product)."""
-        return self._hvp(self._t(xstar), self._t(v), use_disc, use_sdf, include_mass,
-                         use_self=use_self).cpu().numpy()
+    def apply_H(self, xstar, v, use_self=False, use_disc=False, use_sdf=False, include_mass=True, use_ipc=None):
+        """H v at x* as an (N, 2) array (the reference's matrix-free product; node space, the rigid
+        regions' elements with their moduli)."""
+        xw = self._t(xstar)
+        if self._ipc_on(use_ipc):
+            self._set_pairs(xw)
+        return self._hvp(xw, self._t(v), use_disc, use_sdf, include_mass, use_self=use_self,
+                         use_ipc=use_ipc).cpu().numpy()
 
     def spd_probe(self, xstar, use_self=False, include_mass=True, n=16, seed=0):
         """min and max Rayleigh quotient p^T H p / p^T p over random free directions."""
@@ -143,15 +255,21 @@ class AdjointPNCG2D:
         return min(vals), max(vals)
 
     # ---- Jacobi preconditioner diag(M + h^2 (K_elastic + contact)) (reference :356-372)
-    def _diag_precond(self, xw, use_disc=False, use_sdf=False, use_self=False):
+    def _diag_precond(self, xw, use_disc=False, use_sdf=False, use_self=False, use_ipc=None):
         L = _lib.lib()
         s = self.s
         g = torch.zeros((self.N, 2), dtype=F64, device=self.dev)
         d = self._mass2.expand(self.N, 2).clone()
         _lib.check(L.d2f_elastic_grad_diag_f64(self._p(xw), self._p(s.tri), self._p(s.Bm), self._p(s.area),
-                                               self._p(s.mu), self._p(s.lam), self.n_tri, self.h2, 1,
-                                               C.byref(self._tiles), self._p(g), self._p(d), self._stream()),
-                   "elastic grad/diag")
+                                               self._p(self._mu_el), self._p(self._lam_el), self.n_tri,
+                                               self.h2, 1, C.byref(self._tiles), self._p(g), self._p(d),
+                                               self._stream()), "elastic grad/diag")
+        if self._ipc_on(use_ipc) and self._n_pairs:
+            ip = s.ipc
+            _lib.check(L.d2f_ipc_grad_diag_f64(self._p(xw), self._p(self._pairs), self._n_pairs, self._p(s.ea),
+                                               self._p(s.eb), self._p(s.ca), self._p(s.cb), ip["dhat"],
+                                               ip["kappa"], self.h2, self._p(g), self._p(d), self._stream()),
+                       "d2f_ipc_grad_diag_f64")
         if (use_disc or use_sdf) and self._n_col:
             which = 2 if (use_disc and use_sdf) else (0 if use_disc else 1)
             _lib.check(L.d2f_collider_grad_diag_f64(self._p(xw), self._p(s.bnd), s.n_b, self._cols,
@@ -175,12 +293,11 @@ class AdjointPNCG2D:
             ws = {k: z() for k in ("dinv", "b", "x", "v", "r1", "r2", "w1", "w2", "Kv")}
             ws["scal"] = torch.zeros(8 * self.s.B, dtype=F64, device=self.dev)
             ws["status"] = torch.zeros(2 * self.s.B, dtype=torch.int32, device=self.dev)
-            ws["mass"] = self.s.mass.to(F64).contiguous()
+            ws["mass"] = self.s.mass.to(F64).clone()            # owned: the solve masks it
             self._ws = ws
         return ws
 
-    def _minres(self, xw, b, dinv, rtol, maxiter, use_disc=False, use_sdf=False, use_self=False,
-                check_every=16):
+    def _minres(self, xw, b, dinv, rtol, maxiter, check_every=16, **use):
         """Preconditioned MINRES (Paige and Saunders, SIAM J. Numer. Anal. 12, 1975) for every
         scene at once, each with its own Krylov scalars, on the device (d2f_minres_*). The
         operator H v = free (M v + h^2 K v) is symmetric, possibly indefinite; the preconditioner
@@ -190,12 +307,16 @@ class AdjointPNCG2D:
         -1 breakdown), per-scene iterations)."""
         L = _lib.lib()
         ws = self._workspace()
+        red = self.affine is not None
         ws["mass"].copy_(self.s.mass)
+        if red:                              # the slaved nodes' inertia is in the reduced product
+            ws["mass"].masked_fill_(self._slaved, 0.0)
         ws["b"].copy_(b)
         ws["dinv"].copy_(dinv)
         m = _lib.Minres()
         m.B, m.maxiter, m.rtol = self.s.B, int(maxiter), float(rtol)
-        m.node_start, m.mass, m.fmask = self._p(self.s.node_start), self._p(ws["mass"]), self._p(self._free)
+        fmask = self._fmask_red if red else self._free
+        m.node_start, m.mass, m.fmask = self._p(self.s.node_start), self._p(ws["mass"]), self._p(fmask)
         for k in ("dinv", "b", "x", "v", "r1", "r2", "w1", "w2", "Kv", "scal", "status"):
             setattr(m, k, self._p(ws[k]))
         st = self._stream()
@@ -203,8 +324,11 @@ class AdjointPNCG2D:
         done = 0
         while done < maxiter:
             for _ in range(min(check_every, maxiter - done)):
-                ws["Kv"].zero_()
-                self._kv(xw, ws["v"], ws["Kv"], use_disc, use_sdf, use_self)
+                if red:
+                    self._kv_reduced(xw, ws["v"], ws["Kv"], **use)
+                else:
+                    ws["Kv"].zero_()
+                    self._kv(xw, ws["v"], ws["Kv"], **use)
                 _lib.check(L.d2f_minres_step_f64(C.byref(m), st), "d2f_minres_step_f64")
             done += check_every
             if not bool((ws["status"][0::2] == 0).any()):
@@ -213,15 +337,28 @@ class AdjointPNCG2D:
         return ws["x"].clone(), status[:, 0].copy(), status[:, 1].copy()
 
     def adjoint_solve(self, xk, rhs, use_disc=False, use_self=False, use_sdf=False, rtol=1e-9,
-                      maxiter=5000, as_tensor=False):
+                      maxiter=5000, as_tensor=False, use_ipc=None):
         """One adjoint solve H_k lam = rhs on the free degrees of freedom, per-scene MINRES with the
-        Jacobi preconditioner (reference implicit/adjoint.py:339-369). Records minres_calls,
+        Jacobi preconditioner (reference implicit/adjoint.py:339-369); with affine bodies the
+        reduced system P^T H P lam_r = P^T rhs, returning lam = P lam_r. Records minres_calls,
         minres_fail (scenes that did not converge), minres_last_info, minres_iters (the largest
         per-scene count), and minres_resid when minres_track_resid is set."""
         xw = self._t(xk)
-        b = self._t(rhs) * self._free
-        dinv = self._free / self._diag_precond(xw, use_disc, use_sdf, use_self=use_self)
-        lam, status, its = self._minres(xw, b, dinv, float(rtol), int(maxiter), use_disc, use_sdf, use_self)
+        use = dict(use_disc=use_disc, use_sdf=use_sdf, use_self=use_self, use_ipc=use_ipc)
+        if self._ipc_on(use_ipc):
+            self._set_pairs(xw)
+        rhs_t = self._t(rhs)
+        diag = self._diag_precond(xw, **use)
+        if self.affine is not None:
+            b = self._reduce(rhs_t, torch.empty_like(rhs_t)) * self._fmask_red
+            dred = self._reduce(diag, torch.empty_like(diag), square=True)
+            dinv = self._fmask_red / torch.clamp(dred, min=1e-30)
+        else:
+            b = rhs_t * self._free
+            dinv = self._free / diag
+        lam, status, its = self._minres(xw, b, dinv, float(rtol), int(maxiter), **use)
+        if self.affine is not None:
+            lam = self._expand(lam, torch.empty_like(lam))
         self.minres_calls = getattr(self, "minres_calls", 0) + 1
         self.minres_iters = int(its.max(initial=0))
         bad = status != 1
@@ -229,7 +366,13 @@ class AdjointPNCG2D:
             self.minres_fail = getattr(self, "minres_fail", 0) + int(bad.sum())
             self.minres_last_info = int(status[bad][0])
         if getattr(self, "minres_track_resid", False):
-            r = self._hvp(xw, lam, use_disc, use_sdf, use_self=use_self) * self._free - b
+            if self.affine is not None:       # P^T H P lam_r - b, with P lam_r = lam
+                kf = torch.zeros_like(lam)
+                self._kv(xw, lam, kf, **use)
+                r = self._reduce(kf, torch.empty_like(kf), vf=lam) + self._workspace()["mass"][:, None] * lam
+                r = r * self._fmask_red - b
+            else:
+                r = self._hvp(xw, lam, **use) * self._free - b
             rel = float(torch.linalg.vector_norm(r)) / max(float(torch.linalg.vector_norm(b)), 1e-300)
             self.minres_resid = max(getattr(self, "minres_resid", 0.0), rel)
         return lam if as_tensor else lam.cpu().numpy()
@@ -302,20 +445,18 @@ class AdjointPNCG2D:
         w = np.zeros((self.N, 2))
         w[nodes] = d
         J = float((self.elastic_force(x)[nodes] * d).sum())
-        dJdx = -self.apply_H(x, w, include_mass=False) / self.h2
+        dJdx = -self.apply_H(x, w, include_mass=False, use_ipc=False) / self.h2
         return J, dJdx, self.drho(x, w / self.h2)
 
     # ---- the trajectory (reference :373-399)
     def _check_terms(self):
         s = self.s
         missing = []
-        if s.ipc is not None:
-            missing.append("IPCContact")
+        ipc_fric = s.ipc is not None and s.friction and (
+            s.ipc["mu"] > 0.0 or (s.mu_node is not None and bool((s.mu_node > 0.0).any())))
         if s.tendon is not None:
             missing.append("Tendon")
-        if s.affine is not None:
-            missing.append("AffineBodies")
-        if s.node is not None and s.node["fric"]:
+        if (s.node is not None and s.node["fric"]) or ipc_fric:
             missing.append("BodyFriction")
         if s.springs is not None:
             missing.append("Springs")
@@ -325,9 +466,8 @@ class AdjointPNCG2D:
             raise NotImplementedError(
                 f"the adjoint operator has no tangent for {', '.join(sorted(missing))}: the gradient "
                 f"would be of a different system. Differentiate without these actuators."
-                + (" For contact between bodies, use ImplicitIntegrator(self_collision=False) or "
-                   "model.body_contact = 'node', which the adjoint does have a tangent for."
-                   if "IPCContact" in missing else ""))
+                + (" Friction between bodies has no tangent: set model.body_friction = 0 or "
+                   "ImplicitIntegrator(friction=False)." if "BodyFriction" in missing else ""))
 
     # ---- dJ/dp of one adjoint vector (pressure chambers)
     def _dp_dev(self, xw, lamw):

@@ -34,13 +34,13 @@ F64 = torch.float64
 
 class Rollout:
     def __init__(self, model, integrator, dt, T, every=10, command=None, adjoint_kw=None,
-                 use_disc=False, use_sdf=False, use_self=False):
+                 use_disc=False, use_sdf=False, use_self=False, use_ipc=None):
         if T < 1 or every < 1:
             raise ValueError("T and every must be >= 1")
         self.model, self.integ, self.dt, self.T, self.every = model, integrator, float(dt), int(T), int(every)
         self.command = command
         self.adjoint_kw = dict(rtol=1e-12, maxiter=20000) if adjoint_kw is None else dict(adjoint_kw)
-        self.use = dict(use_disc=use_disc, use_sdf=use_sdf, use_self=use_self)
+        self.use = dict(use_disc=use_disc, use_sdf=use_sdf, use_self=use_self, use_ipc=use_ipc)
         self.solver = integrator.attach(model, dt)
         # a density of rho = scale with Emin = 0: moduli = scale x the model's own
         self.adj = AdjointPNCG2D(self.solver, Emin=0.0)
@@ -76,15 +76,20 @@ class Rollout:
             m.actuation.commanded()
         self.integ.simulate(m, self.dt)
 
+    _AFFINE_STATE = ("ag_q", "ag_q0", "ag_pq", "ag_gq", "ag_gq_prev", "ag_zq")
+
     def _state(self):
         s = self.solver
-        return s.x.clone(), s.v.clone(), self.model.time
+        groups = tuple(getattr(s, k).clone() for k in self._AFFINE_STATE) if s.affine is not None else ()
+        return s.x.clone(), s.v.clone(), self.model.time, groups
 
     def _restore(self, st):
         s = self.solver
         s.x.copy_(st[0])
         s.v.copy_(st[1])
         self.model.time = st[2]
+        for k, t in zip(self._AFFINE_STATE, st[3]):
+            getattr(s, k).copy_(t)
 
     def forward(self, scale, pressure):
         """T steps from the model's current state; returns x_T (N, 2)."""

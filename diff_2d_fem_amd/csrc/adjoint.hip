@@ -109,6 +109,204 @@ __global__ void k_pressure_hvp(const double *v, const int32_t *pe_a, const int32
   atomic_add2(y, b, w);
 }
 
+// ---- IPC contact between bodies (not in the reference's adjoint, which refuses IPCContact).
+// The barrier terms of one (edge, node) pair are those the gradient pass evaluates (reference
+// implicit/ipc_contact.py:131-296): the interior term when the foot is inside the edge and
+// d < dhat, and an end term per edge end whose vertex-correction coefficient is non-zero and
+// whose distance to the node is below dhat. The set and the coefficients are frozen at x.
+struct PairGeom {
+  int p, a, b;
+  V2<double> xp, xa, xb;
+  double tt;        // foot parameter along a -> b
+  double d;         // distance to the segment
+  double coef_a, coef_b;
+};
+
+__device__ __forceinline__ PairGeom pair_geom(const double *x, const int32_t *pairs, int i, const int32_t *ea,
+                                              const int32_t *eb, const double *ca, const double *cb) {
+  PairGeom g;
+  const int k = pairs[2 + 2 * i];
+  g.p = pairs[3 + 2 * i];
+  g.a = ea[k];
+  g.b = eb[k];
+  g.xp = ld2(x, g.p);
+  g.xa = ld2(x, g.a);
+  g.xb = ld2(x, g.b);
+  const V2<double> ab = g.xb - g.xa, w = g.xp - g.xa;
+  g.tt = dot(w, ab) / fmax(dot(ab, ab), 1.0e-30);
+  V2<double> dv = w - g.tt * ab;
+  if (g.tt <= 0.0) dv = w;
+  else if (g.tt >= 1.0) dv = g.xp - g.xb;
+  g.d = len(dv);
+  g.coef_a = -ca[k] + (g.tt <= 0.0 ? 1.0 : 0.0);
+  g.coef_b = -cb[k] + (g.tt >= 1.0 ? 1.0 : 0.0);
+  return g;
+}
+
+__device__ __forceinline__ V2<double> perp(V2<double> u) { return {-u.y, u.x}; }
+
+// y += h2 (Hessian of the pair's barrier energy) v, UNCLAMPED, friction left out. Interior term:
+// d = |s|, s = (e x w) / L with e = x_b - x_a, w = x_p - x_a, L = |e|; grad s is perp(e) / L on
+// p and -perp(w) / L - s e / L^2 on b (minus their sum on a), and the product takes the
+// directional derivative of grad s along v: H v = kappa (b'' (grad s . v) grad s + b' sign(s)
+// D(grad s)[v]). End terms: d = |x_p - x_end|, Hess d = (I - n n^T) / d.
+__global__ void k_ipc_hvp(const double *x, const double *v, const int32_t *pairs, int n_pairs, const int32_t *ea,
+                          const int32_t *eb, const double *ca, const double *cb, double dhat, double kappa,
+                          double h2, double *y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_pairs) return;
+  const PairGeom g = pair_geom(x, pairs, i, ea, eb, ca, cb);
+  const double dlo = 1.0e-9 * dhat;
+  const V2<double> vp = ld2(v, g.p), va = ld2(v, g.a), vb = ld2(v, g.b);
+  V2<double> yp = {0.0, 0.0}, ya = {0.0, 0.0}, yb = {0.0, 0.0};
+  if (g.tt > 0.0 && g.tt < 1.0 && g.d < dhat) {
+    const V2<double> e = g.xb - g.xa, w = g.xp - g.xa, de = vb - va, dw = vp - va;
+    const double L2 = fmax(dot(e, e), 1.0e-30), L = sqrt(L2), L3 = L * L2;
+    const double c = cross(e, w);
+    const double d = fmax(fabs(c) / L, dlo);
+    const double sg = c < 0.0 ? -1.0 : 1.0;
+    const V2<double> gw = (1.0 / L) * perp(e);
+    const V2<double> ge = (-1.0 / L) * perp(w) - (c / L3) * e;
+    const double sv = dot(gw, dw) + dot(ge, de);
+    const double eL = dot(e, de);
+    const double dc = cross(de, w) + cross(e, dw);
+    const V2<double> dgw = (1.0 / L) * perp(de) - (eL / L3) * perp(e);
+    const V2<double> dge = (-1.0 / L) * perp(dw) + (eL / L3) * perp(w) - (dc / L3) * e - (c / L3) * de +
+                           (3.0 * c * eL / (L2 * L3)) * e;
+    const double k = kappa * h2;
+    const double bp = barrier_bp(d, dhat) * sg, bpp = barrier_bpp(d, dhat) * sv;
+    const V2<double> Hw = k * (bpp * gw + bp * dgw);
+    const V2<double> He = k * (bpp * ge + bp * dge);
+    yp = yp + Hw;
+    yb = yb + He;
+    ya = ya - (Hw + He);
+  }
+  for (int end = 0; end < 2; ++end) {
+    const double coef = end == 0 ? g.coef_a : g.coef_b;
+    const V2<double> w = end == 0 ? g.xp - g.xa : g.xp - g.xb;
+    const double dd = len(w);
+    if (coef == 0.0 || !(dd < dhat)) continue;
+    const double d = fmax(dd, dlo);
+    const V2<double> n = (1.0 / d) * w;
+    const V2<double> dq = vp - (end == 0 ? va : vb);
+    const double dn = dot(dq, n);
+    const V2<double> H = (coef * kappa * h2) *
+                         ((barrier_bpp(d, dhat) * dn) * n + (barrier_bp(d, dhat) / d) * (dq - dn * n));
+    yp = yp + H;
+    if (end == 0) ya = ya - H;
+    else yb = yb - H;
+  }
+  atomic_add2(y, g.p, yp);
+  atomic_add2(y, g.a, ya);
+  atomic_add2(y, g.b, yb);
+}
+
+// grad += h2 kappa b' grad d and diag += h2 kappa b'' (w grad d)^2 per term (the gradient pass
+// with friction off, nothing masked): the contact part of the adjoint's Jacobi preconditioner.
+__global__ void k_ipc_grad_diag(const double *x, const int32_t *pairs, int n_pairs, const int32_t *ea,
+                                const int32_t *eb, const double *ca, const double *cb, double dhat, double kappa,
+                                double h2, double *gout, double *dout) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_pairs) return;
+  const PairGeom g = pair_geom(x, pairs, i, ea, eb, ca, cb);
+  const double dlo = 1.0e-9 * dhat;
+  auto term = [&](double coef, double wa, double wb, double d, V2<double> n) {
+    const V2<double> f = (coef * h2 * kappa * barrier_bp(d, dhat)) * n;
+    const double c2 = coef * h2 * kappa * barrier_bpp(d, dhat);
+    const V2<double> hd = {c2 * n.x * n.x, c2 * n.y * n.y};
+    atomic_add2(gout, g.p, f);
+    atomic_add2(dout, g.p, hd);
+    if (wa != 0.0) {
+      atomic_add2(gout, g.a, (-wa) * f);
+      atomic_add2(dout, g.a, (wa * wa) * hd);
+    }
+    if (wb != 0.0) {
+      atomic_add2(gout, g.b, (-wb) * f);
+      atomic_add2(dout, g.b, (wb * wb) * hd);
+    }
+  };
+  if (g.tt > 0.0 && g.tt < 1.0 && g.d < dhat) {
+    const V2<double> ab = g.xb - g.xa, w = g.xp - g.xa;
+    const double d = fmax(g.d, dlo);
+    term(1.0, 1.0 - g.tt, g.tt, d, (1.0 / d) * (w - g.tt * ab));
+  }
+  const V2<double> wa = g.xp - g.xa, wb = g.xp - g.xb;
+  const double da = len(wa), db = len(wb);
+  if (g.coef_a != 0.0 && da < dhat) term(g.coef_a, 1.0, 0.0, fmax(da, dlo), (1.0 / fmax(da, dlo)) * wa);
+  if (g.coef_b != 0.0 && db < dhat) term(g.coef_b, 0.0, 1.0, fmax(db, dlo), (1.0 / fmax(db, dlo)) * wb);
+}
+
+// ---- affine bodies in the adjoint (not in the reference's adjoint, which refuses
+// AffineBodies). A slaved node follows x_i = J_i q, J_i = diag(w_i^T, w_i^T), w_i = (1, X_i - c),
+// q = (t_x, A11, A12, t_y, A21, A22) (reference implicit/affine.py:216-224). The adjoint solves
+// in (free soft nodes, q per group); a reduced vector keeps the soft nodes' entries in place and
+// group g's six q entries in the (x, y) slots of three of its nodes, slot[3g .. 3g + 2].
+// vf = P vr: soft nodes copied, slaved nodes J_i q.
+__global__ void k_affine_expand(const double *vr, const int32_t *slave_of, const int32_t *an_grp,
+                                const double *an_w, const int32_t *slot, int N, double *vf) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int j = slave_of[i];
+  if (j < 0) {
+    st2(vf, i, ld2(vr, i));
+    return;
+  }
+  const int g = an_grp[j];
+  const V2<double> q01 = ld2(vr, slot[3 * g]), q23 = ld2(vr, slot[3 * g + 1]), q45 = ld2(vr, slot[3 * g + 2]);
+  const double w0 = an_w[3 * j], w1 = an_w[3 * j + 1], w2 = an_w[3 * j + 2];
+  st2(vf, i, V2<double>{w0 * q01.x + w1 * q01.y + w2 * q23.x, w0 * q23.y + w1 * q45.x + w2 * q45.y});
+}
+
+// yr = y on soft nodes, 0 on slaved ones (the groups' slots are written by k_affine_reduce_groups).
+__global__ void k_affine_reduce_nodes(const double *yf, const int32_t *slave_of, int N, double *yr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  st2(yr, i, slave_of[i] < 0 ? ld2(yf, i) : V2<double>{0.0, 0.0});
+}
+
+constexpr int AR_TPB = 256;
+
+// Group g's slots = sum over its nodes of (w_i u_x, w_i u_y), u = yf + mass vf when vf is given
+// (the slaved nodes' inertia, which the Krylov solve's diagonal mass leaves out), or of
+// (w_i^2 y_x, w_i^2 y_y) with square (the diagonal of J^T diag(y) J). One workgroup per group.
+__global__ __launch_bounds__(AR_TPB) void k_affine_reduce_groups(const double *yf, const double *vf,
+                                                                 const double *mass, const int32_t *an_node,
+                                                                 const double *an_w, const int32_t *an_first,
+                                                                 const int32_t *slot, int square, double *yr) {
+  __shared__ double part[AR_TPB / WAVE][6];
+  const int g = blockIdx.x;
+  double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int j = an_first[g] + threadIdx.x; j < an_first[g + 1]; j += AR_TPB) {
+    const int i = an_node[j];
+    V2<double> u = ld2(yf, i);
+    if (vf) u = u + mass[i] * ld2(vf, i);
+    double w[3] = {an_w[3 * j], an_w[3 * j + 1], an_w[3 * j + 2]};
+    if (square)
+      for (int k = 0; k < 3; ++k) w[k] *= w[k];
+    for (int k = 0; k < 3; ++k) {
+      s[k] += w[k] * u.x;
+      s[3 + k] += w[k] * u.y;
+    }
+  }
+  const int lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+  for (int k = 0; k < 6; ++k) {
+    double t = s[k];
+    for (int o = WAVE / 2; o > 0; o >>= 1) t += __shfl_down(t, o, WAVE);
+    if (lane == 0) part[wv][k] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double q[6];
+    for (int k = 0; k < 6; ++k) {
+      q[k] = 0.0;
+      for (int w = 0; w < AR_TPB / WAVE; ++w) q[k] += part[w][k];
+    }
+    st2(yr, slot[3 * g], V2<double>{q[0], q[1]});
+    st2(yr, slot[3 * g + 1], V2<double>{q[2], q[3]});
+    st2(yr, slot[3 * g + 2], V2<double>{q[4], q[5]});
+  }
+}
+
 // out[c] += -lam . dR/dp_c = 1/2 h2 sum over chamber c's edges of (lam_a + lam_b) . J(x_b - x_a).
 __global__ void k_pressure_dp(const double *x, const double *lam, const int32_t *pe_a, const int32_t *pe_b,
                               const int32_t *pe_ch, int n_pe, double h2, double *out) {
@@ -357,6 +555,59 @@ extern "C" int d2f_pressure_dp_f64(const double *x, const double *lam, const int
   if (n_pe == 0) return 0;
   hipLaunchKernelGGL(k_pressure_dp, dim3((n_pe + 255) / 256), dim3(256), 0, (hipStream_t)stream, x, lam, pe_a,
                      pe_b, pe_ch, n_pe, h2, out);
+  D2F_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" int d2f_ipc_hvp_f64(const double *x, const double *v, const int32_t *pairs, int32_t n_pairs,
+                               const int32_t *ea, const int32_t *eb, const double *ca, const double *cb,
+                               double dhat, double kappa, double h2, double *y, void *stream) {
+  D2F_CHECK(n_pairs >= 0 && dhat > 0.0, "d2f_ipc_hvp_f64: bad argument");
+  if (n_pairs == 0) return 0;
+  D2F_CHECK(x && v && pairs && ea && eb && ca && cb && y, "d2f_ipc_hvp_f64: null pointer");
+  hipLaunchKernelGGL(k_ipc_hvp, dim3((n_pairs + 255) / 256), dim3(256), 0, (hipStream_t)stream, x, v, pairs,
+                     n_pairs, ea, eb, ca, cb, dhat, kappa, h2, y);
+  D2F_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" int d2f_ipc_grad_diag_f64(const double *x, const int32_t *pairs, int32_t n_pairs, const int32_t *ea,
+                                     const int32_t *eb, const double *ca, const double *cb, double dhat,
+                                     double kappa, double h2, double *g, double *dg, void *stream) {
+  D2F_CHECK(n_pairs >= 0 && dhat > 0.0, "d2f_ipc_grad_diag_f64: bad argument");
+  if (n_pairs == 0) return 0;
+  D2F_CHECK(x && pairs && ea && eb && ca && cb && g && dg, "d2f_ipc_grad_diag_f64: null pointer");
+  hipLaunchKernelGGL(k_ipc_grad_diag, dim3((n_pairs + 255) / 256), dim3(256), 0, (hipStream_t)stream, x, pairs,
+                     n_pairs, ea, eb, ca, cb, dhat, kappa, h2, g, dg);
+  D2F_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" int d2f_affine_expand_f64(const double *vr, const int32_t *slave_of, const int32_t *an_grp,
+                                     const double *an_w, const int32_t *slot, int32_t N, double *vf,
+                                     void *stream) {
+  D2F_CHECK(N >= 0, "d2f_affine_expand_f64: bad argument");
+  if (N == 0) return 0;
+  D2F_CHECK(vr && slave_of && an_grp && an_w && slot && vf && vr != vf, "d2f_affine_expand_f64: null or aliased pointer");
+  hipLaunchKernelGGL(k_affine_expand, dim3((N + 255) / 256), dim3(256), 0, (hipStream_t)stream, vr, slave_of,
+                     an_grp, an_w, slot, N, vf);
+  D2F_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" int d2f_affine_reduce_f64(const double *yf, const double *vf, const double *mass,
+                                     const int32_t *slave_of, const int32_t *an_node, const double *an_w,
+                                     const int32_t *an_first, const int32_t *slot, int32_t N, int32_t n_groups,
+                                     int32_t square, double *yr, void *stream) {
+  D2F_CHECK(N >= 0 && n_groups >= 0, "d2f_affine_reduce_f64: bad argument");
+  if (N == 0) return 0;
+  D2F_CHECK(yf && slave_of && yr && yf != yr && (!vf || mass) && (n_groups == 0 || (an_node && an_w && an_first && slot)),
+            "d2f_affine_reduce_f64: null or aliased pointer");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_affine_reduce_nodes, dim3((N + 255) / 256), dim3(256), 0, st, yf, slave_of, N, yr);
+  if (n_groups > 0)
+    hipLaunchKernelGGL(k_affine_reduce_groups, dim3(n_groups), dim3(AR_TPB), 0, st, yf, vf, mass, an_node, an_w,
+                       an_first, slot, square, yr);
   D2F_HIP(hipGetLastError());
   return 0;
 }

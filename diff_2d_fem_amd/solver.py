@@ -201,7 +201,7 @@ class DeviceSolver:
         tri = np.asarray(me.tri)
         X = np.asarray(me.rest_pos, np.float64)
         area = np.asarray(me.rest_area, np.float64)
-        node, grp, w, Mu, La, env = [], [], [], [], [], []
+        node, grp, w, Mu, La, env, gel = [], [], [], [], [], [], []
         owner = np.full(self.N, -1)
         genv = [int(np.asarray(me.tri2env)[np.asarray(g["elems"], np.int64)[0]]) for g in groups]
         order = np.argsort(genv, kind="stable")          # groups by environment
@@ -217,6 +217,7 @@ class DeviceSolver:
                 raise ValueError("a rigid region contains kinematically constrained nodes. A body "
                                  "whose motion is PRESCRIBED needs no rigid region")
             owner[nd] = b
+            gel.append(el)
             c = (self.mass_np[nd, None] * X[nd]).sum(0) / self.mass_np[nd].sum()
             node += list(nd)
             grp += [b] * len(nd)
@@ -233,7 +234,7 @@ class DeviceSolver:
         a = dict(n=n, node_np=np.asarray(node, np.int64), grp_np=np.asarray(grp, np.int64),
                  elems=np.concatenate([np.asarray(g["elems"], np.int64) for g in groups]),
                  w_np=np.concatenate(w, 0), first=np.concatenate([[0], np.cumsum([len(v) for v in w])]),
-                 slaved=owner >= 0)
+                 slaved=owner >= 0, group_elems=gel, owner=owner)
         self.affine = a
         self.an_node, self.an_grp = t(a["node_np"], I32), t(a["grp_np"], I32)
         self.an_w = t(a["w_np"], F64)
@@ -245,6 +246,22 @@ class DeviceSolver:
         self.ag_Hx, self.ag_Hy = z(9), z(9)
         self.sync_affine(None)
         self._layout = None                 # the scene layout sorts rigid triangles and sizes their LDS
+
+    def set_affine_moduli(self, mu_full, lam_full):
+        """Element moduli with rigid regions: each region's energy takes the area-weighted sums of
+        its elements' moduli, and its elements stay zero in the nodal passes."""
+        a = self.affine
+        mu, lam = np.asarray(mu_full, np.float64).copy(), np.asarray(lam_full, np.float64).copy()
+        area = np.asarray(self.me.rest_area, np.float64)
+        Mu = np.array([(area[el] * mu[el]).sum() for el in a["group_elems"]])
+        La = np.array([(area[el] * lam[el]).sum() for el in a["group_elems"]])
+        for el in a["group_elems"]:
+            mu[el] = 0.0
+            lam[el] = 0.0
+        self.mu_np, self.lam_np = mu, lam
+        self.ag_Mu.copy_(torch.as_tensor(Mu))
+        self.ag_La.copy_(torch.as_tensor(La))
+        self.refresh_moduli()
 
     def sync_affine(self, regions=None):
         """q from the current node positions by least squares (reference implicit/affine.py:331-344).

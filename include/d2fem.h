@@ -349,6 +349,40 @@ int d2f_pressure_hvp_f64(const double *v, const int32_t *pe_a, const int32_t *pe
 int d2f_pressure_dp_f64(const double *x, const double *lam, const int32_t *pe_a, const int32_t *pe_b,
                         const int32_t *pe_ch, int32_t n_pe, double h2, double *out, void *stream);
 
+/* ---- adjoint of IPC contact between bodies (float64; beyond the reference, whose adjoint
+ * refuses IPCContact: implicit/adjoint.py:376-385; the barrier terms are those of
+ * ipc_pair_kernel, implicit/ipc_contact.py:131-296). pairs: the list d2f_ipc_pairs writes at the
+ * adjoint's state x (pairs[2 + 2i] edge, pairs[3 + 2i] node id, i < n_pairs); the terms of each
+ * pair (interior foot, end corrections with ca / cb) are selected at x and held fixed. */
+/* y += h2 (Hessian of the barrier energy) v, UNCLAMPED, friction left out (its tangent is absent,
+ * as for the colliders in the reference's adjoint). */
+int d2f_ipc_hvp_f64(const double *x, const double *v, const int32_t *pairs, int32_t n_pairs,
+                    const int32_t *ea, const int32_t *eb, const double *ca, const double *cb,
+                    double dhat, double kappa, double h2, double *y, void *stream);
+/* g += h2 barrier gradient, dg += h2 barrier curvature of every term (friction off, nothing
+ * masked): the contact part of the adjoint's Jacobi preconditioner (the gradient pass of
+ * ipc_pair_kernel mode 0, implicit/ipc_contact.py:131-171). */
+int d2f_ipc_grad_diag_f64(const double *x, const int32_t *pairs, int32_t n_pairs, const int32_t *ea,
+                          const int32_t *eb, const double *ca, const double *cb, double dhat,
+                          double kappa, double h2, double *g, double *dg, void *stream);
+
+/* ---- adjoint with affine bodies (beyond the reference, whose adjoint refuses AffineBodies;
+ * the reduction is that of implicit/affine.py:157-224). Slaved node i (slave_of[i] = j >= 0, an
+ * index into the scene's an_* lists) follows x_i = J_i q with w_j = an_w[3j .. 3j+2]; the solve
+ * runs in reduced vectors (2N doubles): soft nodes' entries in place, group g's
+ * q = (t_x, A11, A12, t_y, A21, A22) in the (x, y) slots of nodes slot[3g], slot[3g+1], slot[3g+2]
+ * (three of the group's own nodes), every other slaved entry zero. */
+/* vf = P vr: soft nodes copied, slaved nodes J_i q. vr and vf must differ. */
+int d2f_affine_expand_f64(const double *vr, const int32_t *slave_of, const int32_t *an_grp,
+                          const double *an_w, const int32_t *slot, int32_t N, double *vf, void *stream);
+/* yr = P^T u with u = yf + mass vf on the slaved nodes when vf is given (the inertia the Krylov
+ * solve's diagonal mass leaves out there), or, with square, the diagonal of P^T diag(yf) P.
+ * an_first: n_groups + 1 offsets of each group's entries in an_node / an_w. yf and yr must differ. */
+int d2f_affine_reduce_f64(const double *yf, const double *vf, const double *mass, const int32_t *slave_of,
+                          const int32_t *an_node, const double *an_w, const int32_t *an_first,
+                          const int32_t *slot, int32_t N, int32_t n_groups, int32_t square, double *yr,
+                          void *stream);
+
 /* ---- MINRES with per-scene Krylov scalars, on the device (replaces the host scipy MINRES of
  * AdjointPNCG2D.adjoint_solve, implicit/adjoint.py:339-369). Scene e solves for the degrees of
  * freedom [2 node_start[e], 2 node_start[e+1]) with its own Lanczos process. All vectors are

@@ -94,6 +94,8 @@ class AffineBodies:
         area = np.asarray(me.rest_area, np.float64)
         node, grp, w, Mu, La, env = [], [], [], [], [], []
         owner = np.full(X.shape[0], -1)
+        self.elems = [np.asarray(g["elems"], np.int64) for g in groups]
+        self.area = area
         for b, g in enumerate(groups):
             el = np.asarray(g["elems"], np.int64)
             nd = np.unique(tri[el].reshape(-1))
@@ -147,6 +149,33 @@ class AffineBodies:
 
     def after_elastic(self, s):
         pass
+
+    # ---- the adjoint's reduction (no counterpart in the reference, whose adjoint refuses
+    # affine bodies). The slaved nodes follow x_i = J_i q, so the implicit step's optimality
+    # conditions are those of the energy in (free soft nodes, q): its Hessian is P^T H P with
+    # P the map from those coordinates to node positions.
+    def set_moduli(self, s, mu_full, lam_full):
+        """Region energies from per-element moduli (the area-weighted sums); the region's own
+        elements stay zero in the nodal passes."""
+        for b, el in enumerate(self.elems):
+            self.Mu[b] = float((self.area[el] * mu_full[el]).sum())
+            self.La[b] = float((self.area[el] * lam_full[el]).sum())
+            s.mu[el] = 0.0
+            s.lam[el] = 0.0
+
+    def expand_into(self, q6, out):
+        """out[slaved nodes] = J q (q: (n, 6))."""
+        self._expand(q6, out)
+        return out
+
+    def reduce_vec(self, y, square=False):
+        """P^T y per group ((n, 6)): sum over the group's nodes of w y_x, w y_y; with square,
+        the diagonal of J^T diag(y) J (weights squared)."""
+        w = self.w * self.w if square else self.w
+        yn = y[self.node]
+        out = np.zeros((self.n, 6))
+        np.add.at(out, self.grp, np.concatenate([yn[:, 0:1] * w, yn[:, 1:2] * w], 1))
+        return out
 
     def gather(self, s):
         self.gq_prev = self.gq.copy()

@@ -271,6 +271,89 @@ class IPCContact:
         self.ccd(s, self.q0, self.q_inf, self.beta)
         return self.clr.copy()
 
+    # ---- the adjoint's tangent (no counterpart in the reference, whose adjoint refuses IPC)
+    def terms(self, x):
+        """The barrier terms the gradient pass evaluates at x, as arrays over terms: kind (0
+        interior foot, 1 end a, 2 end b), coefficient, node, edge ends a and b. The same
+        selection as _run: interior when 0 < t < 1 and d < dhat; an end when its
+        vertex-correction coefficient is non-zero and the node is within dhat of it."""
+        K, JB, tt, dvec, d = self.pairs(x)
+        pn, a, b = self.bnd[JB], self.ea[K], self.eb[K]
+        xp = x[pn]
+        out = []
+        sel = (tt > 0.0) & (tt < 1.0) & (d < self.dhat)
+        out.append((np.zeros(sel.sum(), np.int64), np.ones(sel.sum()), pn[sel], a[sel], b[sel]))
+        for kind, end, share, onend in ((1, a, self.ca[K], tt <= 0.0), (2, b, self.cb[K], tt >= 1.0)):
+            coef = -share + onend.astype(np.float64)
+            dd = np.linalg.norm(xp - x[end], axis=1)
+            sel = (coef != 0.0) & (dd < self.dhat)
+            out.append((np.full(sel.sum(), kind), coef[sel], pn[sel], a[sel], b[sel]))
+        return tuple(np.concatenate([o[i] for o in out]) for i in range(5))
+
+    def hvp(self, x, v, h2, y):
+        """y += h2 (Hessian of the barrier energy) v at x, the terms and their coefficients frozen
+        at x, friction left out, UNCLAMPED. Interior term: d = |s|, s = (e x w) / |e| with
+        e = x_b - x_a, w = x_p - x_a, so grad s = (R(e) / L on p; -R(w) / L - s e / L^2 on b;
+        minus their sum on a), R(u) = (-u_y, u_x), and the product is the directional derivative
+        of grad s along v. End terms: d = |x_p - x_end|, Hess d = (I - n n^T) / d."""
+        kind, coef, pn, a, b = self.terms(x)
+        if kind.size == 0:
+            return y
+        dlo = 1.0e-9 * self.dhat
+        R = lambda u: np.stack([-u[:, 1], u[:, 0]], 1)
+        dot = lambda u, w: (u * w).sum(1)
+        xp, xa, xb = x[pn], x[a], x[b]
+        vp, va, vb = v[pn], v[a], v[b]
+        yp, ya, yb = np.zeros_like(xp), np.zeros_like(xp), np.zeros_like(xp)
+        it = kind == 0
+        if it.any():
+            e, w = (xb - xa)[it], (xp - xa)[it]
+            de, dw = (vb - va)[it], (vp - va)[it]
+            L2 = np.maximum(dot(e, e), 1.0e-30)
+            L = np.sqrt(L2)
+            c = cross2(e, w)
+            d = np.maximum(np.abs(c) / L, dlo)
+            sg = np.where(c < 0.0, -1.0, 1.0)
+            gw = R(e) / L[:, None]
+            ge = -R(w) / L[:, None] - (c / (L * L2))[:, None] * e
+            sv = dot(gw, dw) + dot(ge, de)
+            eL = dot(e, de)
+            dc = cross2(de, w) + cross2(e, dw)
+            dgw = R(de) / L[:, None] - (eL / (L * L2))[:, None] * R(e)
+            dge = (-R(dw) / L[:, None] + (eL / (L * L2))[:, None] * R(w) - (dc / (L * L2))[:, None] * e
+                   - (c / (L * L2))[:, None] * de + (3.0 * c * eL / (L2 * L2 * L))[:, None] * e)
+            bp, bpp = barrier_bp(d, self.dhat), barrier_bpp(d, self.dhat)
+            k = self.kappa * coef[it] * h2
+            Hw = k[:, None] * (bpp[:, None] * sv[:, None] * gw + (bp * sg)[:, None] * dgw)
+            He = k[:, None] * (bpp[:, None] * sv[:, None] * ge + (bp * sg)[:, None] * dge)
+            yp[it] += Hw
+            yb[it] += He
+            ya[it] -= Hw + He
+        for kk, xe, ve, ye in ((1, xa, va, ya), (2, xb, vb, yb)):
+            m = kind == kk
+            if not m.any():
+                continue
+            w = (xp - xe)[m]
+            d = np.maximum(np.linalg.norm(w, axis=1), dlo)
+            n = w / d[:, None]
+            dq = (vp - ve)[m]
+            dn = dot(dq, n)
+            bp, bpp = barrier_bp(d, self.dhat), barrier_bpp(d, self.dhat)
+            Hq = (self.kappa * coef[m] * h2)[:, None] * (bpp[:, None] * dn[:, None] * n +
+                                                         (bp / d)[:, None] * (dq - dn[:, None] * n))
+            yp[m] += Hq
+            ye[m] -= Hq
+        np.add.at(y, pn, yp)
+        np.add.at(y, a, ya)
+        np.add.at(y, b, yb)
+        return y
+
+    def barrier_grad_diag(self, s, x, h2, g, d):
+        """The barrier part of the gradient pass (friction off) at x into g, d, nothing masked:
+        the adjoint preconditioner's contact diagonal."""
+        self._run(s, 0, x, np.zeros_like(x), np.ones_like(x), g, d, None, None, None, None, 0.0,
+                  False, h2, 0.0)
+
     def _alloc(self, s):
         self.q = np.zeros((s.N, 2))
         self.q0 = np.zeros((s.N, 2))

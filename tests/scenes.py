@@ -151,6 +151,41 @@ def node_press_targets(x0, top, t, T, px=0.5e-3):
     return x
 
 
+def ipc_press(px=0.5e-3, gap=0.5, region=False):
+    """A 6 x 10 px soft block standing on a soft slab (its bottom row held) through the IPC barrier
+    between bodies, frictionless: the block's lower face starts `gap` dhat above the slab's
+    (dhat = px / 8, so the same node-edge terms act throughout), its top row held and driven:
+    pressed by 0.2 dhat and sheared by 0.1 px over the steps (ipc_press_targets). region: rows
+    3-6 of the block are one rigid (affine) region with the block's own modulus. The adjoint's
+    scene for the IPC and affine tangents. Returns (model, block handle, the block's rest
+    positions, its top nodes)."""
+    m = Model()
+    m.body_contact = "ipc"
+    m.body_friction = 0.0
+    slab = m.add_soft_body(np.ones((4, 20), np.uint8), physical_size=(20 * px, 4 * px),
+                           center=(0.0, -2.0 * px), E=1.0e6, nu=0.40, density=1100.0)
+    h = m.add_soft_body(np.ones((10, 6), np.uint8), physical_size=(6 * px, 10 * px),
+                        center=(0.0, 5.0 * px + gap * px / 8), E=1.0e6, nu=0.40, density=1100.0)
+    m.finalize()
+    if region:
+        mask = np.zeros((10, 6), bool)
+        mask[3:7, :] = True
+        m.add_affine_region(h, mask, E=1.0e6, nu=0.40)
+    Xs = m.rest_x(slab)
+    m.set_kinematic_constraint(slab, np.where(Xs[:, 1] < Xs[:, 1].min() + 1e-9)[0])
+    x0 = m.get_body_x(h)
+    top = np.where(x0[:, 1] > x0[:, 1].max() - 1e-9)[0]
+    m.set_kinematic_constraint(h, top)
+    return m, h, x0, top
+
+
+def ipc_press_targets(x0, top, t, T, px=0.5e-3):
+    x = x0.copy()
+    x[top, 1] = x0[top, 1] - 0.2 * (px / 8) * t / T
+    x[top, 0] = x0[top, 0] + 0.1 * px * t / T
+    return x
+
+
 def spring_rig(n_env=2, px=0.5e-3):
     """Springs and pins (reference implicit/actuation.py:368-476, sim_model.py:988-1060): per
     scene a soft block pinned to the world at its top-left corner, a spring with a rest length

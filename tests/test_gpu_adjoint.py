@@ -134,3 +134,124 @@ def test_contact_read_outs(path):
     assert abs(F[:, 1]).min() > 0.0
     bcf = integ.body_contact_force(m, pads[0][0])
     assert np.allclose(bcf, f_all[m.body_nodes(pads[0][0])].sum(0))
+
+
+# ---- IPC contact between bodies and affine bodies in the adjoint (beyond the reference, whose
+# adjoint refuses both): the pressed block on a soft slab through the IPC barrier, frictionless,
+# with and without a rigid region in the block (tests.scenes.ipc_press).
+IPC_T, IPC_DT = 5, 2.0e-4
+IPC_KW = dict(max_iter=6000, eps=1e-14, tol_x=1e-13, conv_check=8, friction=False)
+
+
+def _ipc_setup(region, path="scene"):
+    from tests.scenes import ipc_press, ipc_press_targets
+    m, h, x0, top = ipc_press(region=region)
+    integ = ImplicitIntegrator(path=path, precision="double", **IPC_KW)
+    adj = integ.adjoint(m, IPC_DT)
+    rng = np.random.default_rng(0)
+    rho0 = 0.45 + 0.5 * rng.random(adj.n_tri)
+    low = m.body_nodes(h)[np.where(x0[:, 1] < np.median(x0[:, 1]))[0]]
+    c = np.zeros((m.n_nodes, 2))
+    c[low, 0] = 1.0 / len(low)
+    c[low, 1] = 0.5 / len(low)
+
+    def command(model, t):
+        model.set_kinematic_target(h, ipc_press_targets(x0, top, t, IPC_T))
+
+    def rollout(rho):
+        adj.set_density(rho)
+        m.reset()
+        xs = [m.get_x()]
+        for t in range(1, IPC_T + 1):
+            command(m, t)
+            integ.simulate(m, IPC_DT)
+            xs.append(m.get_x())
+        return xs
+    return m, integ, adj, rho0, c, rollout, rng, command
+
+
+def _ipc_oracle(m, rho0):
+    oi = OracleIntegrator(**IPC_KW)
+    s = oi.attach(m, IPC_DT)
+    oa = OracleAdjoint(s, m.me)
+    oa.set_density(rho0)
+    return s, oa
+
+
+@pytest.mark.parametrize("region", [False, True])
+def test_ipc_and_affine_operator_equals_oracle(region):
+    """H v with the IPC tangent, and the IPC part of the preconditioner's diagonal, at the end
+    state of a rollout, against the oracle's (node space, rigid-region elements with their
+    moduli)."""
+    m, integ, adj, rho0, c, rollout, rng, _ = _ipc_setup(region)
+    xs = rollout(rho0)
+    s, oa = _ipc_oracle(m, rho0)
+    x = xs[-1]
+    assert s.ipc.terms(x)[0].size > 0, "no contact"
+    v = rng.standard_normal(x.shape)
+    for use_ipc in (True, False):
+        hv = adj.apply_H(x, v, use_ipc=use_ipc)
+        ho = oa.apply_H(x, v, use_ipc=use_ipc)
+        assert np.abs(hv - ho).max() <= 1e-12 * np.abs(ho).max(), use_ipc
+    import torch
+    xw = adj._t(x)
+    adj._set_pairs(xw)
+    assert adj._n_pairs == len(m._solver.ipc_pairs())
+    d_on = adj._diag_precond(xw, use_ipc=True).cpu().numpy()
+    d_off = adj._diag_precond(xw, use_ipc=False).cpu().numpy()
+    go, do = np.zeros_like(x), np.zeros_like(x)
+    s.ipc.barrier_grad_diag(s, x, adj.h2, go, do)
+    assert np.abs((d_on - d_off) - do).max() <= 1e-9 * np.abs(do).max()
+
+
+@pytest.mark.parametrize("region", [False, True])
+def test_ipc_and_affine_adjoint_equals_oracle(region):
+    m, integ, adj, rho0, c, rollout, rng, _ = _ipc_setup(region)
+    xs = rollout(rho0)
+    g = adj.transient_adjoint(xs, seed={IPC_T: c}, rtol=1e-13, maxiter=20000)
+    assert getattr(adj, "minres_fail", 0) == 0
+    s, oa = _ipc_oracle(m, rho0)
+    go = oa.transient_adjoint(xs, {IPC_T: c}, s.drag, rtol=1e-13)
+    err = np.abs(g - go).max() / np.abs(go).max()
+    print(f"region={region}: |g_gpu - g_oracle| / |g| = {err:.2e}")
+    assert err < 1e-7
+
+
+@pytest.mark.parametrize("region", [False, True])
+def test_ipc_and_affine_adjoint_matches_finite_differences(region):
+    m, integ, adj, rho0, c, rollout, rng, _ = _ipc_setup(region)
+    xs = rollout(rho0)
+    g = adj.transient_adjoint(xs, seed={IPC_T: c}, rtol=1e-13, maxiter=20000)
+    J = lambda rho: float((c * rollout(rho)[-1]).sum())
+    hfd = 0.002
+    d = g / np.abs(g).max()
+    fd = (J(rho0 + hfd * d) - J(rho0 - hfd * d)) / (2 * hfd)
+    rel = abs(g @ d - fd) / abs(fd)
+    d2 = rng.standard_normal(adj.n_tri)
+    fd2 = (J(rho0 + hfd * d2) - J(rho0 - hfd * d2)) / (2 * hfd)
+    nerr = abs(g @ d2 - fd2) / (np.linalg.norm(g) * np.linalg.norm(d2))
+    print(f"region={region}: along the gradient {rel:.2e}, random direction {nerr:.2e}")
+    assert rel < 1e-4
+    assert nerr < 1e-5
+
+
+def test_ipc_and_affine_rollout_checkpointing():
+    """Rollout (torch.autograd.Function) with checkpoints every 2 steps gives the gradient of
+    transient_adjoint over the whole trajectory, with a rigid region and IPC contact: the
+    recomputed segments restore the affine groups' state."""
+    import torch
+    from diff_2d_fem_amd.autograd import Rollout
+    m, integ, adj, rho0, c, rollout, rng, command = _ipc_setup(True)
+    xs = rollout(rho0)
+    g = adj.transient_adjoint(xs, seed={IPC_T: c}, rtol=1e-13, maxiter=20000)
+    m.reset()
+    roll = Rollout(m, integ, IPC_DT, IPC_T, every=2, command=command, adjoint_kw=dict(rtol=1e-13, maxiter=20000))
+    roll.adj.Emin = adj.Emin                     # density rho = Emin + scale (1 - Emin)
+    scale = torch.tensor(rho0, requires_grad=True)
+    x = roll(scale)
+    assert np.abs(x.detach().cpu().numpy() - xs[-1]).max() <= 1e-9 * np.abs(xs[-1]).max()
+    (x * torch.as_tensor(c, device=x.device)).sum().backward()
+    assert roll.recomputed_steps == IPC_T
+    err = np.abs(scale.grad.numpy() - g).max() / np.abs(g).max()
+    print(f"checkpointed rollout vs whole trajectory: {err:.2e}")
+    assert err < 1e-9
