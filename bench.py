@@ -307,7 +307,7 @@ def element_roofline_large(n_meshes=32, copies=128, seed=0):
     import ctypes as C
     import torch
     from diff_2d_fem_amd import _lib
-    from diff_2d_fem_amd.layout import element_tiles
+    from diff_2d_fem_amd.layout import element_tiles, tiles_struct
     from diff_2d_fem_amd.mesh import lame
     from diff_2d_fem_amd.workloads import _star_mesh
     Vs, Ts, Bs, As, MUs, LAs = [], [], [], [], [], []
@@ -331,10 +331,7 @@ def element_roofline_large(n_meshes=32, copies=128, seed=0):
     T, N = len(tri), len(x)
     dev = torch.device("cuda")
     et = element_tiles(tri, N, device=dev)
-    tiles = _lib.ElemTiles()
-    for k in ("node_off", "nodes", "owned", "tri_local"):
-        setattr(tiles, k, et[k].data_ptr())
-    tiles.max_nodes = et["max_nodes"]
+    tiles = tiles_struct(et)
     tt = lambda a, dt=torch.float64: torch.as_tensor(a, dtype=dt, device=dev)
     xd, trid, Bd, Ad, mud, lad = tt(x), tt(tri, torch.int32), tt(Bm), tt(area), tt(mu), tt(lam)
     g = torch.zeros_like(xd)

@@ -19,8 +19,10 @@ P = C.c_void_p
 
 
 class ElemTiles(C.Structure):
-    _fields_ = [("node_off", C.c_void_p), ("nodes", C.c_void_p), ("owned", C.c_void_p),
-                ("tri_local", C.c_void_p), ("max_nodes", C.c_int32), ("pad_", C.c_int32)]
+    _fields_ = [("node_off", C.c_void_p), ("nodes", C.c_void_p), ("halo_off", C.c_void_p),
+                ("tri_local", C.c_void_p), ("hnode", C.c_void_p), ("hnode_off", C.c_void_p),
+                ("hdest", C.c_void_p), ("part", C.c_void_p), ("max_nodes", C.c_int32),
+                ("n_hnodes", C.c_int32)]
 
 
 class Collider(C.Structure):

@@ -111,11 +111,19 @@ __device__ __forceinline__ M2<R> load_B(const double *Bm, int t) {
 // takes triangles [c * D2F_ELEM_CHUNK, (c + 1) * D2F_ELEM_CHUNK), ELEM_PER_THREAD per thread.
 // It gathers the positions of the chunk's nodes (the tile's node list) into LDS, sums every
 // triangle's contributions per node in LDS (the triangles address the list through their
-// 16-bit local indices), then adds each node once: plain read-modify-write where the chunk
-// owns the node (no other workgroup writes it), an atomic otherwise.
-constexpr int ELEM_TPB = 256;   // 4 triangles per thread: 4 waves per workgroup, up to 5 workgroups per CU
+// 10-bit local indices), then adds each node it owns once (plain read-modify-write: no other
+// workgroup touches it). Its sums at halo nodes (shared with other chunks):
+// * PARTIAL: written to their partial slots, which k_elastic_halo adds per node in chunk order.
+//   Used when the launch's working set exceeds the Infinity Cache: there the device-scope
+//   float64 atomics on the halo nodes took about a quarter of the launch (46 M triangles:
+//   1.46 -> 1.37 ms).
+// * otherwise: device-scope atomics, which resolve in cache at these sizes and save the second
+//   launch (0.9 M triangles: 19.9 against 23.2 us).
+constexpr int ELEM_TPB = 256;   // 4 triangles per thread: 4 waves per workgroup
 constexpr int ELEM_PER_THREAD = D2F_ELEM_CHUNK / ELEM_TPB;
+constexpr int ELEM_PARTIAL_MIN_T = 1 << 22;   // 4.2 M triangles: 285 MB of float64 constants
 
+template <bool PARTIAL>
 __global__ __launch_bounds__(ELEM_TPB) void k_elastic_grad_diag_tiled(
     const double *x, const double *Bm, const double *area, const double *mu, const double *lam, int T,
     double h2, int clamp, d2f_elem_tiles tl, double *grad, double *diag) {
@@ -127,6 +135,7 @@ __global__ __launch_bounds__(ELEM_TPB) void k_elastic_grad_diag_tiled(
   const int nbk = gridDim.x, q8 = nbk / 8, r8 = nbk % 8, xcd = blockIdx.x % 8;
   const int c = xcd * q8 + min(xcd, r8) + blockIdx.x / 8;
   const int n0 = tl.node_off[c], nn = tl.node_off[c + 1] - n0;
+  const int h0 = tl.halo_off[c], no = nn - (tl.halo_off[c + 1] - h0);   // owned entries first
   const int M = tl.max_nodes;
   double *xs = sh, *ys = sh + M, *acc = sh + 2 * M;
   for (int j = threadIdx.x; j < nn; j += ELEM_TPB) {
@@ -141,8 +150,8 @@ __global__ __launch_bounds__(ELEM_TPB) void k_elastic_grad_diag_tiled(
   for (int k = 0; k < ELEM_PER_THREAD; ++k) {
     const int t = base + k * ELEM_TPB;
     if (t >= T) continue;
-    const ushort4 l = reinterpret_cast<const ushort4 *>(tl.tri_local)[t];
-    const int li[3] = {l.x, l.y, l.z};
+    const uint32_t l = tl.tri_local[t];
+    const int li[3] = {int(l & 1023u), int((l >> 10) & 1023u), int((l >> 20) & 1023u)};
     const V2<double> p0 = {xs[li[0]], ys[li[0]]}, p1 = {xs[li[1]], ys[li[1]]}, p2 = {xs[li[2]], ys[li[2]]};
     const M2<double> B = load_B<double>(Bm, t);
     const double m_ = mu[t], l_ = lam[t], A = area[t];
@@ -165,15 +174,36 @@ __global__ __launch_bounds__(ELEM_TPB) void k_elastic_grad_diag_tiled(
   }
   __syncthreads();
   for (int j = threadIdx.x; j < nn; j += ELEM_TPB) {
-    const int n = tl.nodes[n0 + j];
     const double *a = acc + j;
     const V2<double> gg = {h2 * a[0], h2 * a[M]}, dd = {h2 * a[2 * M], h2 * a[3 * M]};
-    if (tl.owned[n0 + j]) {
+    if (j < no) {
+      const int n = tl.nodes[n0 + j];
       st2(grad, n, ld2(grad, n) + gg);
       st2(diag, n, ld2(diag, n) + dd);
+    } else if (PARTIAL) {
+      reinterpret_cast<double4 *>(tl.part)[tl.hdest[h0 + j - no]] = make_double4(gg.x, gg.y, dd.x, dd.y);
     } else {
+      const int n = tl.nodes[n0 + j];
       atomic_add2(grad, n, gg);
       atomic_add2(diag, n, dd);
+    }
+  }
+}
+
+// The halo nodes of the tiled pass: node hnode[k] adds its chunks' partial sums (contiguous,
+// in chunk order).
+__global__ void k_elastic_halo(d2f_elem_tiles tl, double *grad, double *diag) {
+  GRID_LOOP(k, tl.n_hnodes) {
+    if (k < tl.n_hnodes) {
+      const int n = tl.hnode[k];
+      V2<double> g = ld2(grad, n), d = ld2(diag, n);
+      for (int i = tl.hnode_off[k]; i < tl.hnode_off[k + 1]; ++i) {
+        const double4 q = reinterpret_cast<const double4 *>(tl.part)[i];
+        g = g + V2<double>{q.x, q.y};
+        d = d + V2<double>{q.z, q.w};
+      }
+      st2(grad, n, g);
+      st2(diag, n, d);
     }
   }
 }
@@ -211,14 +241,22 @@ static int launch_elastic_grad_diag(const double *x, const int32_t *tri, const d
                                     double *diag, hipStream_t st) {
   if (T == 0) return 0;
   if (tiles && tiles->node_off && tiles->max_nodes > 0 && tiles->max_nodes <= D2F_ELEM_NODES_MAX) {
-    D2F_CHECK(tiles->nodes && tiles->owned && tiles->tri_local, "elastic grad/diag: incomplete tiles");
+    D2F_CHECK(tiles->nodes && tiles->halo_off && tiles->tri_local && tiles->part && tiles->n_hnodes >= 0 &&
+                  (tiles->n_hnodes == 0 || (tiles->hnode && tiles->hnode_off && tiles->hdest)),
+              "elastic grad/diag: incomplete tiles");
     const int blocks = (T + D2F_ELEM_CHUNK - 1) / D2F_ELEM_CHUNK;
-    const size_t lds = sizeof(double) * 6 * (size_t)tiles->max_nodes;
-    if (lds > 64 * 1024)   // beyond the default dynamic LDS limit (up to 96 KiB at D2F_ELEM_NODES_MAX)
-      D2F_HIP(hipFuncSetAttribute((const void *)k_elastic_grad_diag_tiled,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_elastic_grad_diag_tiled, dim3(blocks), dim3(ELEM_TPB), lds, st, x, Bm, area, mu, lam,
-                       T, h2, clamp, *tiles, grad, diag);
+    const size_t lds = sizeof(double) * 6 * (size_t)tiles->max_nodes;   // at most 48 KiB
+    const bool partial = T >= ELEM_PARTIAL_MIN_T;
+    if (partial)
+      hipLaunchKernelGGL(k_elastic_grad_diag_tiled<true>, dim3(blocks), dim3(ELEM_TPB), lds, st, x, Bm, area, mu,
+                         lam, T, h2, clamp, *tiles, grad, diag);
+    else
+      hipLaunchKernelGGL(k_elastic_grad_diag_tiled<false>, dim3(blocks), dim3(ELEM_TPB), lds, st, x, Bm, area, mu,
+                         lam, T, h2, clamp, *tiles, grad, diag);
+    if (partial && tiles->n_hnodes > 0) {
+      const int hb = (int)std::min<long>((tiles->n_hnodes + 255) / 256, 65536L);
+      hipLaunchKernelGGL(k_elastic_halo, dim3(hb), dim3(256), 0, st, *tiles, grad, diag);
+    }
   } else {
     const int blocks = (int)std::min<long>((T + 255) / 256, 65536L);
     hipLaunchKernelGGL(k_elastic_grad_diag, dim3(blocks), dim3(256), 0, st, x, tri, Bm, area, mu, lam, T, h2,

@@ -68,18 +68,30 @@ def starts_of(keys, n):
 
 
 ELEM_CHUNK = 1024          # triangles per workgroup of the tiled element kernel (D2F_ELEM_CHUNK)
-ELEM_NODES_MAX = 2048      # longest node list one chunk sums in LDS (D2F_ELEM_NODES_MAX)
+ELEM_NODES_MAX = 1024      # longest node list one chunk sums in LDS (D2F_ELEM_NODES_MAX; 10-bit indices)
 
 
 def element_tiles(tri, n_nodes, chunk=ELEM_CHUNK, device="cpu"):
-    """Node tiles of the tiled element kernel. Triangle t belongs to chunk t // chunk. Returns
-    torch tensors on `device`:
-      node_off  (n_chunks + 1,) int32   chunk c's nodes are nodes[node_off[c]:node_off[c + 1]]
-      nodes     int32                   ascending within a chunk
-      owned     uint8                   1 when every triangle at that node is in that chunk
-      tri_local (T, 4) int16 (read as uint16)  positions of each triangle's nodes in its
-                                        chunk's list (column 3 is padding)
-    and max_nodes, the longest list."""
+    """Node tiles of the tiled element kernel. Triangle t belongs to chunk t // chunk. A node
+    is owned by a chunk when every triangle at it is in that chunk; the others are halo nodes,
+    and every chunk touching one sums its part into a slot of a partial buffer that a second
+    pass adds up per node, in chunk order (no global atomics). Returns torch tensors on `device`:
+      node_off  (n_chunks + 1,) int32   chunk c's list is nodes[node_off[c]:node_off[c + 1]]
+      nodes     int32                   per chunk its owned nodes, then its halo nodes, each
+                                        part ascending
+      halo_off  (n_chunks + 1,) int32   chunk c's halo entries (the last halo_off[c + 1] -
+                                        halo_off[c] of its list) are partial slots
+                                        halo_off[c] .. halo_off[c + 1]
+      tri_local (T,) int32              positions of triangle t's nodes in its chunk's list,
+                                        10 bits each (bits 0-9, 10-19, 20-29)
+      hnode, hnode_off int32            halo node k = hnode[k] sums the partials
+                                        part[hnode_off[k]:hnode_off[k + 1]] (chunk order)
+      hdest     int32                   chunk-order slot s of a halo entry -> its position in
+                                        that node-sorted partial buffer
+      part      float64 (4 n_slots,)    the partial buffer (grad x, grad y, diag x, diag y)
+      owned     uint8                   1 per owned list entry (diagnostics)
+    and max_nodes (the longest list), n_hnodes. When a list exceeds ELEM_NODES_MAX nodes the
+    kernel is not tiled (tri_local is then left zero)."""
     import torch
     t = torch.as_tensor(np.asarray(tri, np.int64).reshape(-1, 3), device=device)
     T = t.shape[0]
@@ -89,14 +101,44 @@ def element_tiles(tri, n_nodes, chunk=ELEM_CHUNK, device="cpu"):
     uniq, inv = torch.unique(key, sorted=True, return_inverse=True)
     ucid = uniq.div(n_nodes, rounding_mode="floor")
     node = uniq - ucid * n_nodes
-    off = torch.searchsorted(ucid, torch.arange(nch + 1, device=device))
-    local = inv.reshape(T, 3) - off[cid][:, None]
     count = torch.bincount(node, minlength=n_nodes)
-    owned = (count[node] == 1).to(torch.uint8)
-    tl = torch.zeros((T, 4), dtype=torch.int32, device=device)
-    tl[:, :3] = local.to(torch.int32)
+    halo = count[node] > 1
+    # owned entries first, halo entries after, within each chunk (stable: ascending node ids)
+    perm = torch.argsort(ucid * 2 + halo.to(torch.int64), stable=True)
+    pos = torch.empty_like(perm)
+    pos[perm] = torch.arange(perm.numel(), device=device)
+    node, ucid, halo = node[perm], ucid[perm], halo[perm]
+    ar = torch.arange(nch + 1, device=device)
+    off = torch.searchsorted(ucid, ar)
+    local = pos[inv].reshape(T, 3) - off[cid][:, None]
     max_nodes = int((off[1:] - off[:-1]).max()) if nch else 0
-    if max_nodes > 65535:
-        raise ValueError("a chunk touches more than 65535 nodes")
-    return dict(node_off=off.to(torch.int32), nodes=node.to(torch.int32), owned=owned,
-                tri_local=tl.to(torch.int16), max_nodes=max_nodes)
+    if max_nodes <= ELEM_NODES_MAX:
+        packed = local[:, 0] | (local[:, 1] << 10) | (local[:, 2] << 20)
+    else:
+        packed = torch.zeros(T, dtype=torch.int64, device=device)
+    hcount = torch.bincount(ucid[halo], minlength=nch)
+    halo_off = torch.zeros(nch + 1, dtype=torch.int64, device=device)
+    halo_off[1:] = torch.cumsum(hcount, 0)
+    hnodes_slot = node[halo]                          # node of partial slot s (slots in chunk order)
+    order = torch.argsort(hnodes_slot, stable=True)   # per node, its slots in chunk order
+    hnode, hcnt = torch.unique_consecutive(hnodes_slot[order], return_counts=True)
+    hnode_off = torch.zeros(hnode.numel() + 1, dtype=torch.int64, device=device)
+    hnode_off[1:] = torch.cumsum(hcnt, 0)
+    hdest = torch.empty_like(order)
+    hdest[order] = torch.arange(order.numel(), device=device)
+    i32 = torch.int32
+    return dict(node_off=off.to(i32), nodes=node.to(i32), halo_off=halo_off.to(i32),
+                tri_local=packed.to(i32), hnode=hnode.to(i32), hnode_off=hnode_off.to(i32),
+                hdest=hdest.to(i32), part=torch.zeros(4 * max(int(halo_off[-1]), 1), dtype=torch.float64,
+                                                      device=device),
+                owned=(~halo).to(torch.uint8), max_nodes=max_nodes, n_hnodes=int(hnode.numel()))
+
+
+def tiles_struct(et):
+    """The d2f_elem_tiles struct of element_tiles' tensors."""
+    from ._lib import ElemTiles
+    st = ElemTiles()
+    for k in ("node_off", "nodes", "halo_off", "tri_local", "hnode", "hnode_off", "hdest", "part"):
+        setattr(st, k, et[k].data_ptr())
+    st.max_nodes, st.n_hnodes = et["max_nodes"], et["n_hnodes"]
+    return st

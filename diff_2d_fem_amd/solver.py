@@ -646,12 +646,8 @@ class DeviceSolver:
 
     def elem_tiles_struct(self):
         """The element kernel's node tiles as the C struct (pointers into el_tiles)."""
-        et = self.el_tiles
-        st = _lib.ElemTiles()
-        for k in ("node_off", "nodes", "owned", "tri_local"):
-            setattr(st, k, et[k].data_ptr())
-        st.max_nodes = et["max_nodes"]
-        return st
+        from .layout import tiles_struct
+        return tiles_struct(self.el_tiles)
 
     def scene_struct(self, part=None):
         """The d2f_scene of this batch. part "plain": the per-kernel path's share of a routed

@@ -36,19 +36,29 @@ extern "C" {
 #define D2F_MAX_COLLIDERS 8
 #define D2F_MAX_COLORS 24
 #define D2F_ELEM_CHUNK 1024       /* triangles per workgroup of the tiled element kernel */
-#define D2F_ELEM_NODES_MAX 2048   /* largest node list one chunk sums in LDS */
+#define D2F_ELEM_NODES_MAX 1024   /* longest node list one chunk sums in LDS (10-bit local indices) */
 
 /* Node tiles of the element kernel. Triangle t belongs to chunk t / D2F_ELEM_CHUNK; chunk c
- * touches the nodes nodes[node_off[c] .. node_off[c+1]) (ascending); owned[k] = 1 when every
- * triangle at nodes[k] belongs to chunk c (the chunk then adds into it without an atomic);
- * tri_local[4t .. 4t+2] are the positions of triangle t's nodes in its chunk's list (the
- * fourth entry is padding). max_nodes = the longest list. */
+ * touches the nodes nodes[node_off[c] .. node_off[c+1]): first the nodes it owns (every
+ * triangle at them is in chunk c), then its halo nodes (shared with other chunks), each part
+ * ascending. tri_local[t] holds the positions of triangle t's nodes in its chunk's list, 10 bits
+ * each (bits 0-9, 10-19, 20-29). The chunk adds into its owned nodes directly and writes its sums
+ * at its halo nodes, halo entries s = halo_off[c] .. halo_off[c+1], into part[hdest[s]] (4
+ * doubles each: grad x, grad y, diag x, diag y), where part is ordered by node; a second pass
+ * adds, for each halo node hnode[k], part[hnode_off[k] .. hnode_off[k+1]) in chunk order: no
+ * global atomics (within a chunk the sums are LDS atomics).
+ * part is workspace: launches that share one tiles struct must not overlap. max_nodes = the
+ * longest list; a list longer than D2F_ELEM_NODES_MAX makes the launch take the untiled kernel. */
 typedef struct d2f_elem_tiles {
   const int32_t *node_off;            /* n_chunks + 1 */
   const int32_t *nodes;               /* node_off[n_chunks] */
-  const uint8_t *owned;               /* node_off[n_chunks] */
-  const uint16_t *tri_local;          /* 4 T */
-  int32_t max_nodes, pad_;
+  const int32_t *halo_off;            /* n_chunks + 1 */
+  const uint32_t *tri_local;          /* T */
+  const int32_t *hnode;               /* n_hnodes */
+  const int32_t *hnode_off;           /* n_hnodes + 1 */
+  const int32_t *hdest;               /* halo_off[n_chunks] */
+  double *part;                       /* 4 halo_off[n_chunks] */
+  int32_t max_nodes, n_hnodes;
 } d2f_elem_tiles;
 
 /* A rigid collider (reference sim_model.py:167-174; kinds core/objects.py:22-24):
@@ -202,9 +212,10 @@ void d2f_struct_sizes(int64_t *out);
 /* grad += h2 dPsi/dx, diag += h2 max(diag d2Psi/dx2, 0) over all triangles.
  * Replaces grad_diag_elastic_kernel (implicit/kernels.py:86-128). With tiles (see
  * d2f_elem_tiles; max_nodes <= D2F_ELEM_NODES_MAX) chunk c gathers its nodes' positions into
- * LDS, sums its triangles' contributions there and adds them once per node: plain
- * read-modify-write for the nodes it owns, atomics for the rest. tiles NULL: one atomic per
- * contribution. */
+ * LDS, sums its triangles' contributions there and adds them once per node it owns (plain
+ * read-modify-write). Its sums at halo nodes are atomics below 4,194,304 triangles; from there
+ * on they go to the tiles' partial slots, which a second launch adds per node in chunk order
+ * (no global atomics). tiles NULL: one atomic per contribution. */
 int d2f_elastic_grad_diag_f64(const double *x, const int32_t *tri, const double *Bm,
                               const double *area, const double *mu, const double *lam, int32_t T,
                               double h2, int32_t clamp, const d2f_elem_tiles *tiles,

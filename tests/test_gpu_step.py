@@ -13,7 +13,7 @@ import pytest
 import torch
 
 from diff_2d_fem_amd import ImplicitIntegrator, _lib
-from diff_2d_fem_amd.layout import element_tiles
+from diff_2d_fem_amd.layout import ELEM_NODES_MAX, element_tiles, tiles_struct
 from oracle import terms as OT
 from oracle.pncg import OracleIntegrator
 from tests.scenes import strip, grip, close_jaws
@@ -54,11 +54,8 @@ def test_elastic_grad_diag(deformed):
     Bm = torch.tensor(me.D_m_inv.reshape(-1, 4), device=d)
     A, mu, lam = (torch.tensor(v, device=d) for v in (me.rest_area, me.mu, me.lam))
     et = element_tiles(me.tri, len(x), device=d)
-    assert et["max_nodes"] > 1 and et["owned"].bool().any() and not et["owned"].bool().all()
-    tiles = _lib.ElemTiles()
-    for k in ("node_off", "nodes", "owned", "tri_local"):
-        setattr(tiles, k, et[k].data_ptr())
-    tiles.max_nodes = et["max_nodes"]
+    assert et["max_nodes"] > 1 and et["n_hnodes"] > 0 and et["owned"].bool().any()
+    tiles = tiles_struct(et)
     # atomics only (no tiles) and tiled
     for tl in (None, tiles):
         for clamp in (1, 0):
@@ -341,9 +338,9 @@ def test_ipc_pairs_equal_brute_force(scene):
     print(f"{scene}: {checked} pairs checked")
 
 
-def test_elastic_tiles_beyond_the_default_lds():
-    """The tiled element kernel with chunks touching more than 1365 nodes (512 single-pixel
-    bodies: 2048 nodes per 1024-triangle chunk, 96 KiB of LDS) against the oracle."""
+def test_elastic_chunks_beyond_the_tile_limit():
+    """Chunks whose node lists exceed D2F_ELEM_NODES_MAX (512 single-pixel bodies: 2048 nodes
+    per 1024-triangle chunk) take the untiled kernel, against the oracle."""
     from diff_2d_fem_amd.mesh import build_scene
     me = build_scene([dict(mask_xy=np.ones((1, 1), bool), size=(1e-3, 1e-3), center=(0.0, 0.0),
                            young=1e6, nu=0.4, rho=1000.0, env=i) for i in range(600)])
@@ -351,11 +348,8 @@ def test_elastic_tiles_beyond_the_default_lds():
     x = me.rest_pos + rng.normal(scale=5e-5, size=me.rest_pos.shape)
     d = torch.device("cuda")
     et = element_tiles(me.tri, len(x), device=d)
-    assert et["max_nodes"] > 1365
-    tiles = _lib.ElemTiles()
-    for k in ("node_off", "nodes", "owned", "tri_local"):
-        setattr(tiles, k, et[k].data_ptr())
-    tiles.max_nodes = et["max_nodes"]
+    assert et["max_nodes"] > ELEM_NODES_MAX
+    tiles = tiles_struct(et)
     xt = torch.tensor(x, device=d)
     tri = torch.tensor(me.tri, dtype=torch.int32, device=d)
     Bm = torch.tensor(me.D_m_inv.reshape(-1, 4), device=d)
@@ -366,4 +360,43 @@ def test_elastic_tiles_beyond_the_default_lds():
                                            1, C.byref(tiles), _p(g), _p(dg), _stream()), "grad_diag")
     go, do = np.zeros_like(x), np.zeros_like(x)
     OT.elastic_grad_diag(x, me.tri, me.D_m_inv, me.rest_area, me.mu, me.lam, 4e-6, 1, go, do)
+    assert _rel(g.cpu().numpy(), go) < 1e-12 and _rel(dg.cpu().numpy(), do) < 1e-12
+
+
+def test_elastic_grad_diag_with_halo_partials():
+    """From 4,194,304 triangles on, the tiled element kernel writes its halo sums to partial
+    slots and a second launch adds them per node (no global atomics): 4.3 M triangles of tiled
+    star meshes against the oracle."""
+    from diff_2d_fem_amd.workloads import _star_mesh
+    Vs, Ts = [], []
+    for i in range(6):
+        V, Tm, _ = _star_mesh(np.random.default_rng([7, i]), 10e-3, 12000)
+        Vs.append(V)
+        Ts.append(Tm)
+    reps = 4_300_000 // sum(len(t) for t in Ts) + 1
+    nv = np.array([len(v) for v in Vs] * reps)
+    off = np.concatenate([[0], np.cumsum(nv)[:-1]])
+    tri = np.concatenate([Ts[k % 6] + off[k] for k in range(6 * reps)])
+    X = np.concatenate([Vs[k % 6] for k in range(6 * reps)])
+    T = len(tri)
+    assert T >= 1 << 22
+    D = np.stack([X[tri[:, 1]] - X[tri[:, 0]], X[tri[:, 2]] - X[tri[:, 0]]], 2)
+    B = np.linalg.inv(D)
+    area = 0.5 * np.abs(np.linalg.det(D))
+    rng = np.random.default_rng(8)
+    mu, lam = 1e5 * (1 + rng.random(T)), 4e5 * (1 + rng.random(T))
+    x = X + rng.normal(scale=2e-5, size=X.shape)
+    d = torch.device("cuda")
+    et = element_tiles(tri, len(x), device=d)
+    assert et["max_nodes"] <= ELEM_NODES_MAX and et["n_hnodes"] > 0
+    tiles = tiles_struct(et)
+    tt = lambda a, dt=torch.float64: torch.as_tensor(a, dtype=dt, device=d)
+    g0 = rng.normal(scale=1e-7, size=x.shape)
+    g, dg = tt(g0), tt(g0)
+    args = [tt(x), tt(tri, torch.int32), tt(B.reshape(-1, 4)), tt(area), tt(mu), tt(lam)]   # kept alive
+    L = _lib.lib()
+    _lib.check(L.d2f_elastic_grad_diag_f64(*map(_p, args), T, 4e-6, 1, C.byref(tiles), _p(g), _p(dg), _stream()),
+               "grad_diag")
+    go, do = g0.copy(), g0.copy()
+    OT.elastic_grad_diag(x, tri, B, area, mu, lam, 4e-6, 1, go, do)
     assert _rel(g.cpu().numpy(), go) < 1e-12 and _rel(dg.cpu().numpy(), do) < 1e-12
