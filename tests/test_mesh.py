@@ -94,9 +94,10 @@ def test_model_refuses_cpu_device():
 
 
 def test_element_tiles_layout():
-    """Node tiles of the element kernel, checked by brute force: each chunk's list is exactly
-    the sorted set of nodes its triangles touch, a node is owned by a chunk iff no other
-    chunk touches it, and the local indices point back at the triangle's nodes."""
+    """Node tiles of the element kernel on a multi-body scene, checked by brute force: each
+    chunk's list is exactly the set of nodes its triangles touch (owned ones first, then the
+    halo, each ascending), a node is owned by a chunk iff no other chunk touches it, and the
+    local indices point back at the triangle's nodes."""
     from diff_2d_fem_amd.layout import element_tiles
     me = build_scene([dict(b) for b in scenes()["multi"]])
     tri = np.asarray(me.tri)
@@ -104,7 +105,9 @@ def test_element_tiles_layout():
     ch = 37
     et = element_tiles(tri, N, chunk=ch)
     off, nodes = et["node_off"].numpy(), et["nodes"].numpy()
-    owned, loc = et["owned"].numpy(), et["tri_local"].numpy().astype(np.int64) & 0xFFFF
+    owned, hoff = et["owned"].numpy(), et["halo_off"].numpy()
+    tl = et["tri_local"].numpy().astype(np.int64)
+    loc = np.stack([tl & 1023, (tl >> 10) & 1023, (tl >> 20) & 1023], 1)
     nch = (len(tri) + ch - 1) // ch
     assert len(off) == nch + 1
     chunks_at = [set() for _ in range(N)]
@@ -114,8 +117,9 @@ def test_element_tiles_layout():
     for c in range(nch):
         sub = tri[c * ch:(c + 1) * ch]
         lst = nodes[off[c]:off[c + 1]]
-        assert np.array_equal(lst, np.unique(sub))
+        assert np.array_equal(np.sort(lst), np.unique(sub))
+        no = len(lst) - (hoff[c + 1] - hoff[c])
         for k, n in enumerate(lst):
-            assert owned[off[c] + k] == (chunks_at[n] == {c})
-        assert np.array_equal(lst[loc[c * ch:(c + 1) * ch, :3]], sub)
+            assert owned[off[c] + k] == (chunks_at[n] == {c}) == (k < no)
+        assert np.array_equal(lst[loc[c * ch:(c + 1) * ch]], sub)
     assert et["max_nodes"] == int(np.diff(off).max())
