@@ -36,7 +36,7 @@ _SCENE_FIELDS = (
                         "mass", "free_", "node2env", "node_start",
                         "tri", "Bm", "area", "mu", "lam", "tri2env",
                         "bnd", "b2e", "bnd_start", "ea", "eb", "ela", "elb", "e2env", "ca", "cb",
-                        "excl", "excl_off", "mu_node",
+                        "mu_node",
                         "bc_idx", "bc_target", "bc_q",
                         "gPg", "g_p", "g_Py", "y_p", "y_Py", "beta", "gTp", "pHp", "alpha", "p_inf",
                         "clr_min", "dE", "dE0", "Ener", "Eprev", "clr_ipc", "q_inf", "sweep_beta", "tc",
@@ -133,8 +133,8 @@ def lib():
         getattr(L, name).restype = i64
     L.d2f_unique_pairs.argtypes = [P, i64, P, P, P, P]
     L.d2f_unique_triples.argtypes = [P, i64, P, P, P]
-    L.d2f_hop_exclusion.argtypes = [P, i64, i64, P, i64, i32, P]
-    L.d2f_material_exclusion.argtypes = [P, i64, P, i64, P, i64, f64, P]
+    L.d2f_exclusion_csr.restype = i64
+    L.d2f_exclusion_csr.argtypes = [P, i64, i64, P, P, i64, i32, f64, P, P, i64]
     L.d2f_apply_activation_f64.argtypes = [P, P, i32, P, P, P, P, C.POINTER(ElemConsts), P]
     L.d2f_node_grad_diag_f64.argtypes = [P, P, P, i32, P, P, f64, f64, f64, i32, P, P, P]
     L.d2f_node_hvp_f64.argtypes = [P, P, P, P, i32, P, P, f64, f64, f64, P, P]

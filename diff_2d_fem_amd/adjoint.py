@@ -187,7 +187,7 @@ class AdjointPNCG2D:
         if self._node_on(use_self):           # reference :239-245, the pairs frozen at x
             nd = s.node
             _lib.check(L.d2f_node_hvp_f64(self._p(xw), self._p(vw), self._p(s.bnd), self._p(s.bnd_start), s.B,
-                                          self._p(s.excl), self._p(s.excl_off), nd["r"], nd["k"], self.h2,
+                                          self._p(s.excl_bits), self._p(s.excl_bits_off), nd["r"], nd["k"], self.h2,
                                           self._p(y), self._stream()), "d2f_node_hvp_f64")
         if self._ipc_on(use_ipc) and self._n_pairs:
             ip = s.ipc
@@ -280,7 +280,7 @@ class AdjointPNCG2D:
         if self._node_on(use_self):           # reference :332-335: clamped, as in the solve
             nd = s.node
             _lib.check(L.d2f_node_grad_diag_f64(self._p(xw), self._p(s.bnd), self._p(s.bnd_start), s.B,
-                                                self._p(s.excl), self._p(s.excl_off), nd["r"], nd["k"],
+                                                self._p(s.excl_bits), self._p(s.excl_bits_off), nd["r"], nd["k"],
                                                 self.h2, 1, self._p(g), self._p(d), self._stream()),
                        "d2f_node_grad_diag_f64")
         return torch.clamp(d, min=1e-30)

@@ -483,7 +483,7 @@ extern "C" int d2f_collider_grad_diag_f64(const double *x, const int32_t *bnd, i
 // The "node" contact's penalty on its own (reference implicit/kernels.py:772-808 as
 // implicit/adjoint.py:319-336 calls it, and implicit/adjoint.py:115-143).
 static NodeArgs node_args(const double *x, const double *v, const int32_t *bnd, const int32_t *bnd_start,
-                          int32_t B, const uint8_t *excl, const int64_t *excl_off, double radius,
+                          int32_t B, const uint32_t *excl_bits, const int64_t *excl_bits_off, double radius,
                           double stiffness, double h2, int32_t clamp, double *g, double *dg) {
   NodeArgs a{};
   a.x = x;
@@ -492,8 +492,8 @@ static NodeArgs node_args(const double *x, const double *v, const int32_t *bnd, 
   a.bnd_start = bnd_start;
   a.B = B;
   a.max_nb = 0;  // sizes unknown on the host: partners read from global memory
-  a.excl = excl;
-  a.excl_off = excl_off;
+  a.excl_bits = excl_bits;
+  a.excl_bits_off = excl_bits_off;
   a.k = stiffness;
   a.r = radius;
   a.h2 = h2;
@@ -504,23 +504,23 @@ static NodeArgs node_args(const double *x, const double *v, const int32_t *bnd, 
 }
 
 extern "C" int d2f_node_grad_diag_f64(const double *x, const int32_t *bnd, const int32_t *bnd_start, int32_t B,
-                                      const uint8_t *excl, const int64_t *excl_off, double radius,
+                                      const uint32_t *excl_bits, const int64_t *excl_bits_off, double radius,
                                       double stiffness, double h2, int32_t clamp, double *g, double *dg,
                                       void *stream) {
   D2F_CHECK(B >= 0 && radius >= 0.0, "d2f_node_grad_diag_f64: bad argument");
-  D2F_CHECK(B == 0 || (x && bnd && bnd_start && excl && excl_off && g && dg),
+  D2F_CHECK(B == 0 || (x && bnd && bnd_start && excl_bits && excl_bits_off && g && dg),
             "d2f_node_grad_diag_f64: null pointer");
-  return launch_node_env(0, node_args(x, nullptr, bnd, bnd_start, B, excl, excl_off, radius, stiffness, h2,
+  return launch_node_env(0, node_args(x, nullptr, bnd, bnd_start, B, excl_bits, excl_bits_off, radius, stiffness, h2,
                                       clamp, g, dg),
                          (hipStream_t)stream);
 }
 
 extern "C" int d2f_node_hvp_f64(const double *x, const double *v, const int32_t *bnd, const int32_t *bnd_start,
-                                int32_t B, const uint8_t *excl, const int64_t *excl_off, double radius,
+                                int32_t B, const uint32_t *excl_bits, const int64_t *excl_bits_off, double radius,
                                 double stiffness, double h2, double *y, void *stream) {
   D2F_CHECK(B >= 0 && radius >= 0.0, "d2f_node_hvp_f64: bad argument");
-  D2F_CHECK(B == 0 || (x && v && bnd && bnd_start && excl && excl_off && y), "d2f_node_hvp_f64: null pointer");
-  return launch_node_env(3, node_args(x, v, bnd, bnd_start, B, excl, excl_off, radius, stiffness, h2, 0, y,
+  D2F_CHECK(B == 0 || (x && v && bnd && bnd_start && excl_bits && excl_bits_off && y), "d2f_node_hvp_f64: null pointer");
+  return launch_node_env(3, node_args(x, v, bnd, bnd_start, B, excl_bits, excl_bits_off, radius, stiffness, h2, 0, y,
                                       nullptr),
                          (hipStream_t)stream);
 }

@@ -146,20 +146,28 @@ int64_t d2f_unique_triples(const int64_t *R, int64_t m, int64_t *key, int64_t *f
   return unique_rows<3>(R, m, key, first, count, nullptr);
 }
 
-int d2f_hop_exclusion(const int64_t *tri, int64_t T, int64_t n_nodes, const int64_t *boundary, int64_t nb,
-                      int32_t k, uint8_t *out) {
-  if (T < 0 || n_nodes < 0 || nb < 0 || k < 0 || (nb > 0 && (!boundary || !out)) || (T > 0 && !tri))
+// Self-contact exclusion of one body as sorted lists (CSR over its boundary nodes): boundary
+// node s excludes every boundary node within k mesh-edge hops (reference core/mesh.py:133-153)
+// and, when radius > 0, every one closer than radius along mesh edges at rest (reference
+// core/mesh.py:163-186), and the relation is made symmetric (reference core/mesh.py:157-158).
+// The same sets as d2f_hop_exclusion | d2f_material_exclusion, without the nb x nb matrix.
+// row_ptr[nb + 1] and cols (capacity cap) receive the lists; returns the number of entries
+// (when it exceeds cap, nothing is written beyond cap and the caller retries), or < 0.
+int64_t d2f_exclusion_csr(const int64_t *tri, int64_t T, int64_t n_nodes, const double *pos,
+                          const int64_t *boundary, int64_t nb, int32_t k, double radius, int64_t *row_ptr,
+                          int32_t *cols, int64_t cap) {
+  if (T < 0 || n_nodes < 0 || nb < 0 || k < 0 || !(radius >= 0.0) || !row_ptr || cap < 0 ||
+      (nb > 0 && !boundary) || (T > 0 && !tri) || (radius > 0.0 && !pos) || (cap > 0 && !cols))
     return D2F_EINVAL;
   if (!valid_nodes(tri, 3 * T, n_nodes) || !valid_nodes(boundary, nb, n_nodes)) return D2F_EINVAL;
-  std::memset(out, 0, (size_t)nb * (size_t)nb);
   std::vector<int64_t> off, nbr;
   adjacency(tri, T, n_nodes, off, nbr);
   std::vector<int64_t> bidx((size_t)n_nodes, -1);
   for (int64_t i = 0; i < nb; ++i) bidx[(size_t)boundary[i]] = i;
+  std::vector<std::vector<int32_t>> rows((size_t)nb);
   std::vector<int32_t> hop((size_t)n_nodes, -1);
   std::vector<int64_t> seen, q;
-  for (int64_t s = 0; s < nb; ++s) {
-    // breadth-first search from boundary node s, up to k edges (reference core/mesh.py:133-153)
+  for (int64_t s = 0; s < nb; ++s) {          // breadth-first search up to k edges
     const int64_t src = boundary[s];
     q.assign(1, src);
     seen.assign(1, src);
@@ -167,7 +175,7 @@ int d2f_hop_exclusion(const int64_t *tri, int64_t T, int64_t n_nodes, const int6
     for (size_t h = 0; h < q.size(); ++h) {
       const int64_t u = q[h];
       const int32_t d = hop[(size_t)u];
-      if (bidx[(size_t)u] >= 0) out[(size_t)s * nb + bidx[(size_t)u]] = 1;
+      if (bidx[(size_t)u] >= 0) rows[(size_t)s].push_back((int32_t)bidx[(size_t)u]);
       if (d == k) continue;
       for (int64_t j = off[(size_t)u]; j < off[(size_t)u + 1]; ++j) {
         const int64_t w = nbr[(size_t)j];
@@ -180,74 +188,65 @@ int d2f_hop_exclusion(const int64_t *tri, int64_t T, int64_t n_nodes, const int6
     }
     for (int64_t w : seen) hop[(size_t)w] = -1;
   }
-  return 0;
-}
-
-int d2f_material_exclusion(const double *pos, int64_t n_nodes, const int64_t *edges, int64_t m,
-                           const int64_t *boundary, int64_t nb, double radius, uint8_t *out) {
-  if (n_nodes < 0 || m < 0 || nb < 0 || !(radius >= 0.0) || (nb > 0 && (!boundary || !out)) ||
-      (m > 0 && (!edges || !pos)))
-    return D2F_EINVAL;
-  if (!valid_nodes(edges, 2 * m, n_nodes) || !valid_nodes(boundary, nb, n_nodes)) return D2F_EINVAL;
-  // the undirected edge graph weighted by rest length (reference core/mesh.py:163-186)
-  std::vector<int64_t> off((size_t)n_nodes + 1, 0), nbr((size_t)(2 * m));
-  std::vector<double> wt((size_t)(2 * m));
-  for (int64_t i = 0; i < m; ++i) {
-    ++off[(size_t)edges[2 * i] + 1];
-    ++off[(size_t)edges[2 * i + 1] + 1];
-  }
-  for (int64_t i = 0; i < n_nodes; ++i) off[(size_t)i + 1] += off[(size_t)i];
-  std::vector<int64_t> fill(off.begin(), off.end() - 1);
-  for (int64_t i = 0; i < m; ++i) {
-    const int64_t a = edges[2 * i], b = edges[2 * i + 1];
-    const double dx = pos[2 * a] - pos[2 * b], dy = pos[2 * a + 1] - pos[2 * b + 1];
-    const double w = std::sqrt(dx * dx + dy * dy);
-    nbr[(size_t)fill[(size_t)a]] = b;
-    wt[(size_t)fill[(size_t)a]++] = w;
-    nbr[(size_t)fill[(size_t)b]] = a;
-    wt[(size_t)fill[(size_t)b]++] = w;
-  }
-  std::vector<int64_t> bidx((size_t)n_nodes, -1);
-  for (int64_t i = 0; i < nb; ++i) bidx[(size_t)boundary[i]] = i;
-  const double inf = INFINITY;
-  std::vector<double> dist((size_t)n_nodes, inf);
-  std::vector<uint8_t> done((size_t)n_nodes, 0);
-  std::vector<int64_t> touched;
-  using Item = std::pair<double, int64_t>;
-  for (int64_t s = 0; s < nb; ++s) {
-    // Dijkstra from boundary node s; every node closer than the radius is settled before the
-    // first node at or beyond it leaves the queue, and its distance is the smallest
-    // d(u) + w(u, v) over its settled neighbours, whatever the queue's order
-    std::priority_queue<Item, std::vector<Item>, std::greater<Item>> pq;
-    const int64_t src = boundary[s];
-    dist[(size_t)src] = 0.0;
-    touched.assign(1, src);
-    pq.push({0.0, src});
-    while (!pq.empty()) {
-      const Item it = pq.top();
-      pq.pop();
-      const int64_t u = it.second;
-      if (done[(size_t)u] || it.first > dist[(size_t)u]) continue;
-      if (!(it.first < radius)) break;
-      done[(size_t)u] = 1;
-      if (bidx[(size_t)u] >= 0) out[(size_t)s * nb + bidx[(size_t)u]] = 1;
-      for (int64_t j = off[(size_t)u]; j < off[(size_t)u + 1]; ++j) {
-        const int64_t v = nbr[(size_t)j];
-        if (done[(size_t)v]) continue;
-        const double nd = dist[(size_t)u] + wt[(size_t)j];
-        if (nd < dist[(size_t)v]) {
-          if (dist[(size_t)v] == inf) touched.push_back(v);
-          dist[(size_t)v] = nd;
-          pq.push({nd, v});
+  if (radius > 0.0) {                         // Dijkstra over rest edge lengths, stopped at radius
+    const double inf = INFINITY;
+    std::vector<double> dist((size_t)n_nodes, inf);
+    std::vector<uint8_t> done((size_t)n_nodes, 0);
+    std::vector<int64_t> touched;
+    using Item = std::pair<double, int64_t>;
+    for (int64_t s = 0; s < nb; ++s) {
+      std::priority_queue<Item, std::vector<Item>, std::greater<Item>> pq;
+      const int64_t src = boundary[s];
+      dist[(size_t)src] = 0.0;
+      touched.assign(1, src);
+      pq.push({0.0, src});
+      while (!pq.empty()) {
+        const Item it = pq.top();
+        pq.pop();
+        const int64_t u = it.second;
+        if (done[(size_t)u] || it.first > dist[(size_t)u]) continue;
+        if (!(it.first < radius)) break;
+        done[(size_t)u] = 1;
+        if (bidx[(size_t)u] >= 0) rows[(size_t)s].push_back((int32_t)bidx[(size_t)u]);
+        for (int64_t j = off[(size_t)u]; j < off[(size_t)u + 1]; ++j) {
+          const int64_t v = nbr[(size_t)j];
+          if (done[(size_t)v]) continue;
+          const double dx = pos[2 * u] - pos[2 * v], dy = pos[2 * u + 1] - pos[2 * v + 1];
+          const double nd = dist[(size_t)u] + std::sqrt(dx * dx + dy * dy);
+          if (nd < dist[(size_t)v]) {
+            if (dist[(size_t)v] == inf) touched.push_back(v);
+            dist[(size_t)v] = nd;
+            pq.push({nd, v});
+          }
         }
       }
-    }
-    for (int64_t v : touched) {
-      dist[(size_t)v] = inf;
-      done[(size_t)v] = 0;
+      for (int64_t v : touched) {
+        dist[(size_t)v] = inf;
+        done[(size_t)v] = 0;
+      }
     }
   }
-  return 0;
+  // symmetric closure, then each list sorted without repeats
+  for (int64_t s = 0; s < nb; ++s) {
+    const size_t n0 = rows[(size_t)s].size();
+    for (size_t i = 0; i < n0; ++i) {
+      const int32_t t = rows[(size_t)s][i];
+      if (t != s) rows[(size_t)t].push_back((int32_t)s);
+    }
+  }
+  int64_t nnz = 0;
+  row_ptr[0] = 0;
+  for (int64_t s = 0; s < nb; ++s) {
+    auto &r = rows[(size_t)s];
+    std::sort(r.begin(), r.end());
+    r.erase(std::unique(r.begin(), r.end()), r.end());
+    for (int32_t t : r) {
+      if (nnz < cap) cols[nnz] = t;
+      ++nnz;
+    }
+    row_ptr[s + 1] = nnz;
+  }
+  return nnz;
 }
 
 }  // extern "C"

@@ -38,10 +38,13 @@ __device__ __forceinline__ void atomic_add2(double *a, int i, V2<double> v) {
   atomicAdd(a + 2 * i, v.x);
   atomicAdd(a + 2 * i + 1, v.y);
 }
+// Bit i of a bit-packed set (32 per word, little-endian within the word).
+__device__ __forceinline__ bool bit_of(const uint32_t *w, int64_t i) { return (w[i >> 5] >> (i & 31)) & 1u; }
 // The same on pointers known to be in global memory (address space 1).
 #ifndef D2F_GLOBAL
 #define D2F_GLOBAL __attribute__((address_space(1)))
 #endif
+__device__ __forceinline__ bool bit_of(const D2F_GLOBAL uint32_t *w, int64_t i) { return (w[i >> 5] >> (i & 31)) & 1u; }
 typedef double d2f_dvec2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ V2<double> ld2(const D2F_GLOBAL double *a, int i) {
   const d2f_dvec2 t = reinterpret_cast<const D2F_GLOBAL d2f_dvec2 *>(a)[i];

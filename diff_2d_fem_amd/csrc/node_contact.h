@@ -118,8 +118,8 @@ struct NodeArgs {
   const double *x, *xn, *pdir;
   const int32_t *bnd, *bnd_start;
   int32_t B, max_nb;
-  const uint8_t *excl;
-  const int64_t *excl_off;
+  const uint32_t *excl_bits;     // bit i * n_be + j of scene e's words: nodes i, j excluded
+  const int64_t *excl_bits_off;  // B + 1 word offsets
   const uint8_t *freev;      // friction's free mask per node (NULL: every node free)
   const double *mu_node;     // per-node coefficient when per_body
   double k, r, mu, epsv, inv_dt, h2;

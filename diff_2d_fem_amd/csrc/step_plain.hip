@@ -422,8 +422,8 @@ __device__ __forceinline__ bool pair_in(const S &s, const EdgeData &ed, int jl, 
     const int64_t i0 = (int64_t)jl * n_be + ed.la, i1 = (int64_t)jl * n_be + ed.lb;
     return !(((bv.bits[i0 >> 5] >> (i0 & 31)) & 1u) || ((bv.bits[i1 >> 5] >> (i1 & 31)) & 1u));
   }
-  const uint8_t *row = s.excl + eoff + (int64_t)jl * n_be;
-  return row[ed.la] == 0 && row[ed.lb] == 0;
+  const uint32_t *xw = s.excl_bits + eoff;
+  return !(bit_of(xw, (int64_t)jl * n_be + ed.la) || bit_of(xw, (int64_t)jl * n_be + ed.lb));
 }
 
 // The terms of one pair that passed the filter within the reach (reference
@@ -532,7 +532,7 @@ __global__ void k_ipc(S s, int mode, PairCtx c, const double *pdir, double *grad
       const BView bv{nullptr, nullptr, nullptr, nullptr};
       const EdgeData ed = edge_data(s, k, mode, bv, pdir);
       const int start = s.bnd_start[ed.e], n_be = s.bnd_start[ed.e + 1] - start;
-      const int64_t eoff = s.excl_off[ed.e];
+      const int64_t eoff = s.excl_bits_off[ed.e];
       const double t_c = (mode == 1 || mode == 3) ? tc[ed.e] : 0.0;
       double amin = BIG, cmin = BIG, acc = 0.0;
       for (int jl = 0; jl < n_be; ++jl) {
@@ -699,7 +699,7 @@ __global__ __launch_bounds__(IPC_TPB) void k_ipc_env(S s, PairCtx c, const doubl
   // pair's end distance is never below its segment distance), so they keep those; the
   // clearance and step-limit passes need every pair within the reach
   const double r = (mode == 1 || mode == 3 || mode == 4) ? c.reach : c.dhat;
-  const int64_t eoff = s.excl_off[e];
+  const int64_t eoff = s.excl_bits_off[e];
   const int ke0 = s.e_start[e], ne = s.e_start[e + 1] - ke0;
   double amin = BIG, cmin = BIG, acc = 0.0;     // this lane's, over all its pairs
   // Phase 1: this workgroup's edges (every ipc_parts(mode)-th block of IPC_TPB / IPC_TEAM), a
@@ -817,7 +817,7 @@ __global__ __launch_bounds__(256) void k_node_env(int mode, NodeArgs a) {
     if (STAGED) return {sp[2 * j], sp[2 * j + 1]};
     return ld2(a.pdir, a.bnd[start + j]);
   };
-  const int64_t eoff = a.excl_off[e];
+  const uint32_t *xw = a.excl_bits + a.excl_bits_off[e];
   double acc = 0.0;
   for (int i = threadIdx.x; i < n_be; i += blockDim.x) {
     const int n = a.bnd[start + i];
@@ -826,9 +826,9 @@ __global__ __launch_bounds__(256) void k_node_env(int mode, NodeArgs a) {
     const V2<double> pa = dir ? P(i) : V2<double>{0.0, 0.0};
     V2<double> g = {0.0, 0.0}, gf = {0.0, 0.0};
     double hx = 0.0, hy = 0.0, fx = 0.0, fy = 0.0, val = 0.0, valf = 0.0;
-    const uint8_t *row = a.excl + eoff + (int64_t)i * n_be;
+    const int64_t row = (int64_t)i * n_be;
     for (int jl = 0; jl < n_be; ++jl) {
-      if (jl == i || row[jl]) continue;
+      if (jl == i || bit_of(xw, row + jl)) continue;
       NodeVisit v;
       if (!node_visit(xa, X(jl), a.r, v)) continue;
       if (mode == 0) node_grad_diag(v, a.k, a.clamp, g, hx, hy);
@@ -1716,8 +1716,8 @@ struct Plain {
     a.bnd_start = s.bnd_start;
     a.B = s.B;
     a.max_nb = s.max_nb;
-    a.excl = s.excl;
-    a.excl_off = s.excl_off;
+    a.excl_bits = s.excl_bits;
+    a.excl_bits_off = s.excl_bits_off;
     a.freev = s.free_;
     a.mu_node = s.mu_node;
     a.k = p.node_k;
@@ -2076,8 +2076,8 @@ int d2f_ipc_pairs(const d2f_scene *s, const d2f_params *p, int32_t *pairs, int32
 
 int d2f_step_plain(const d2f_scene *s, const d2f_params *p, void *stream) {
   D2F_CHECK(s && p, "d2f_step_plain: null scene or params");
-  D2F_CHECK(!p->node_on || (!p->ipc_on && s->excl && s->excl_off && p->node_r > 0.0),
-            "d2f_step_plain: node contact needs excl, excl_off, node_r > 0 and no IPC contact");
+  D2F_CHECK(!p->node_on || (!p->ipc_on && s->excl_bits && s->excl_bits_off && p->node_r > 0.0),
+            "d2f_step_plain: node contact needs excl_bits, excl_bits_off, node_r > 0 and no IPC contact");
   D2F_CHECK(s->B > 0 && s->N > 0 && s->T >= 0 && s->n_col >= 0 && s->n_col <= D2F_MAX_COLLIDERS,
             "d2f_step_plain: bad sizes (B=%d N=%d T=%d n_col=%d)", s->B, s->N, s->T, s->n_col);
   D2F_CHECK(p->iter_max >= 0 && p->dt > 0.0, "d2f_step_plain: bad params");

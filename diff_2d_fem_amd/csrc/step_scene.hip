@@ -406,7 +406,7 @@ struct Step {
     td1 = s.n_td > 0 ? s.td_start[e + 1] : 0;
     sp0 = s.n_sp > 0 ? s.sp_start[e] : 0;
     sp1 = s.n_sp > 0 ? s.sp_start[e + 1] : 0;
-    eoff = s.excl_off ? s.excl_off[e] : 0;
+    eoff = s.excl_bits_off ? s.excl_bits_off[e] : 0;
     xcon = s.xc_lds && n1 - n0 <= s.max_nn;
   }
 
@@ -633,13 +633,13 @@ struct Step {
   }
 
   // Is boundary node jp excluded against boundary node j of this scene (mesh neighbours,
-  // nodes of one rigid piece)? From the LDS bitset when it was staged, else the byte matrix.
+  // nodes of one rigid piece)? From the LDS bitset when it was staged, else from global memory.
   __device__ __forceinline__ bool excluded(int jp, int j) const {
     if (sc.excl_lds) {
       const unsigned idx = (unsigned)jp * (unsigned)nb + (unsigned)j;
       return (xbits[idx >> 5] >> (idx & 31u)) & 1u;
     }
-    return s.excl[eoff + (int64_t)jp * nb + j] != 0;
+    return bit_of(s.excl_bits + eoff, (int64_t)jp * nb + j);
   }
 
   // One node-edge pair (reference implicit/ipc_contact.py:175-296, the body of the query loop):
@@ -1919,8 +1919,8 @@ extern "C" int d2f_step_scene(const d2f_scene *s, const d2f_params *p, void *str
   D2F_CHECK(s->max_pairs == 0 || s->max_nb < 65536,
             "d2f_step_scene: the pair list packs boundary indices in 16 bits (max_nb < 65536)");
   D2F_CHECK(!p->ipc_on || (s->bloc && s->e_start), "d2f_step_scene: IPC needs bloc and e_start");
-  D2F_CHECK(!p->node_on || (!p->ipc_on && s->excl && s->excl_off && p->node_r > 0.0 && p->ipc_skin >= 0.0),
-            "d2f_step_scene: node contact needs excl, excl_off, node_r > 0 and no IPC contact");
+  D2F_CHECK(!p->node_on || (!p->ipc_on && s->excl_bits && s->excl_bits_off && p->node_r > 0.0 && p->ipc_skin >= 0.0),
+            "d2f_step_scene: node contact needs excl_bits, excl_bits_off, node_r > 0 and no IPC contact");
   hipStream_t st = (hipStream_t)stream;
   const int tpb = p->block_size > 0 ? p->block_size : 256;
   if (p->precision == 1) {

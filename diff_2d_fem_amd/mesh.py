@@ -14,8 +14,9 @@ exactly (core/mesh.py:74-109, 112-160, 380-479):
   * scenes (environments) are laid out in order, bodies of one scene in handle order.
 
 The edge tables and the self-contact exclusion run in the library's host C++ (csrc/assemble.cpp,
-d2f_unique_pairs / d2f_unique_triples / d2f_hop_exclusion / d2f_material_exclusion); the rest
-is numpy. Nothing here touches the GPU.
+d2f_unique_pairs / d2f_unique_triples / d2f_exclusion_csr); the rest is numpy. The exclusion
+is kept as sorted lists per boundary node (CSR), not the reference's dense n_b_e x n_b_e
+matrices; Mesh builds those views on demand. Nothing here touches the GPU.
 """
 from __future__ import annotations
 
@@ -88,9 +89,10 @@ class Mesh:
     tri_starts: np.ndarray        # (B+1,)
     boundary_starts: np.ndarray   # (B+1,)
     boundary_per_env: np.ndarray  # (B,)
-    excl_flat: np.ndarray         # concatenated per-scene (n_b_e * n_b_e,) uint8
-    excl_offset: np.ndarray       # (B+1,)
-    excluded_pairs_per_env: list  # per scene (n_b_e, n_b_e) bool
+    # self-contact exclusion as sorted lists (CSR over the boundary nodes in layout order):
+    # boundary node b excludes the scene-local boundary indices excl_idx[excl_ptr[b]:excl_ptr[b+1]]
+    excl_ptr: np.ndarray          # (n_b + 1,) int64
+    excl_idx: np.ndarray          # int32 scene-local boundary indices
     tri_quad: np.ndarray = None   # (T,) body-local quad index of a pixel triangle, -1 for mesh bodies
     body_env: np.ndarray = None
     body_node_range: np.ndarray = None
@@ -105,6 +107,49 @@ class Mesh:
     @property
     def sum_n_tri(self):
         return self.tri.shape[0]
+
+    # ---- the reference's dense views of the exclusion (core/mesh.py:452-459), built on demand
+    @property
+    def excl_offset(self):
+        """(B+1,) offsets of each scene's n_b_e x n_b_e block in excl_flat."""
+        n = np.asarray(self.boundary_per_env, np.int64)
+        return np.concatenate([[0], np.cumsum(n * n)]).astype(np.int64)
+
+    def excluded_pairs(self, env):
+        """Scene env's exclusion as an (n_b_e, n_b_e) bool matrix (a fresh array)."""
+        b0, b1 = int(self.boundary_starts[env]), int(self.boundary_starts[env + 1])
+        n = b1 - b0
+        out = np.zeros((n, n), bool)
+        p = self.excl_ptr
+        rows = np.repeat(np.arange(n), np.diff(p[b0:b1 + 1]))
+        out[rows, self.excl_idx[p[b0]:p[b1]]] = True
+        return out
+
+    @property
+    def excluded_pairs_per_env(self):
+        return [self.excluded_pairs(e) for e in range(self.n_envs)]
+
+    @property
+    def excl_flat(self):
+        """Every scene's block, flattened and concatenated, uint8 (reference layout)."""
+        blocks = [self.excluded_pairs(e).reshape(-1).astype(np.uint8) for e in range(self.n_envs)]
+        return np.concatenate(blocks) if blocks else np.zeros(0, np.uint8)
+
+    def add_exclusion(self, env, local):
+        """Exclude every pair of scene env's boundary nodes with local indices `local`
+        (each with itself too)."""
+        local = np.unique(np.asarray(local, np.int64))
+        if local.size == 0:
+            return
+        b0 = int(self.boundary_starts[env])
+        p, idx = self.excl_ptr, self.excl_idx
+        rows = [idx[p[b]:p[b + 1]] for b in range(len(p) - 1)]
+        for j in local:
+            r = rows[b0 + j]
+            rows[b0 + j] = np.union1d(r, local).astype(np.int32)
+        counts = np.array([len(r) for r in rows], np.int64)
+        self.excl_ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        self.excl_idx = np.concatenate(rows).astype(np.int32) if rows else np.zeros(0, np.int32)
 
     def body_node_index(self, body, ix, iy):
         """Global id of the node at grid corner (ix, iy) of a pixel body, -1 if pruned."""
@@ -161,36 +206,33 @@ def _boundary_edges(tri):
     return uniq[single][order]
 
 
-def _hop_exclusion(tri, n_nodes, boundary, k):
-    """Boundary-node pairs within k mesh-edge hops of each other (incl. each node itself),
-    (reference core/mesh.py:133-153), by breadth-first search in d2f_hop_exclusion."""
-    nb = boundary.size
+def exclusion_csr(tri, n_nodes, boundary, k, rest_pos=None, radius=0.0):
+    """Self-contact exclusion of one body as sorted lists over its boundary nodes: pairs within
+    k mesh-edge hops (each node with itself; reference core/mesh.py:133-153), and with
+    radius > 0 pairs closer than radius along mesh edges at rest (reference
+    core/mesh.py:163-186), made symmetric (core/mesh.py:157-158). Returns (ptr (nb+1,) int64,
+    idx int32): node s excludes boundary indices idx[ptr[s]:ptr[s+1]]. Built in
+    d2f_exclusion_csr (breadth-first search and Dijkstra per boundary node) without the
+    nb x nb matrix of the reference."""
+    nb = int(np.asarray(boundary).size)
+    ptr = np.zeros(nb + 1, np.int64)
     if nb == 0:
-        return np.zeros((0, 0), bool)
+        return ptr, np.zeros(0, np.int32)
     tri = np.ascontiguousarray(tri, np.int64)
     boundary = np.ascontiguousarray(boundary, np.int64)
-    out = np.empty((nb, nb), np.uint8)
-    rc = _native().d2f_hop_exclusion(_ptr(tri), len(tri), int(n_nodes), _ptr(boundary), nb, int(k), _ptr(out))
-    if rc != 0:
-        raise ValueError("hop exclusion: bad mesh indices")
-    return out.astype(bool)
-
-
-def material_exclusion(rest_pos, tri, boundary, radius):
-    """Boundary-node pairs closer than `radius` along mesh edges at rest
-    (reference core/mesh.py:163-186: the same edge graph, rest edge lengths as weights, shortest
-    paths stopped at the radius), in d2f_material_exclusion."""
-    n = rest_pos.shape[0]
-    e, _, _, _ = unique_pairs(np.concatenate([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]]))
-    pos = np.ascontiguousarray(rest_pos, np.float64)
-    boundary = np.ascontiguousarray(boundary, np.int64)
-    nb = len(boundary)
-    out = np.zeros((nb, nb), np.uint8)
-    rc = _native().d2f_material_exclusion(_ptr(pos), n, _ptr(e), len(e), _ptr(boundary), nb, float(radius),
-                                          _ptr(out))
-    if rc != 0:
-        raise ValueError("material exclusion: bad mesh indices or radius")
-    return out.astype(bool)
+    pos = None if rest_pos is None else np.ascontiguousarray(rest_pos, np.float64)
+    if radius > 0.0 and pos is None:
+        raise ValueError("material exclusion needs the rest positions")
+    cap = 16 * nb
+    while True:
+        idx = np.empty(cap, np.int32)
+        nnz = _native().d2f_exclusion_csr(_ptr(tri), len(tri), int(n_nodes), None if pos is None else _ptr(pos),
+                                          _ptr(boundary), nb, int(k), float(radius), _ptr(ptr), _ptr(idx), cap)
+        if nnz < 0:
+            raise ValueError("self-contact exclusion: bad mesh indices or radius")
+        if nnz <= cap:
+            return ptr, idx[:nnz].copy()
+        cap = int(nnz)
 
 
 def finish_body(rest_pos, tri, k_excl, excl_radius=0.0):
@@ -205,15 +247,11 @@ def finish_body(rest_pos, tri, k_excl, excl_radius=0.0):
     bedges = _boundary_edges(tri)
     boundary = np.unique(bedges.reshape(-1)).astype(np.int64)
     n = rest_pos.shape[0]
-    excl = _hop_exclusion(tri, n, boundary, k_excl)
-    if boundary.size and excl_radius > 0.0:
-        excl |= material_exclusion(rest_pos, tri, boundary, excl_radius)
-    if boundary.size:
-        excl = excl | excl.T
+    ptr, idx = exclusion_csr(tri, n, boundary, k_excl, rest_pos, excl_radius)
     if bedges.size == 0:
         bedges = np.zeros((0, 2), np.int64)
     return dict(rest_pos=rest_pos, tri=tri, D_m_inv=D_m_inv, rest_area=rest_area,
-                boundary=boundary, bedges=bedges, excl=excl, n_nodes=n)
+                boundary=boundary, bedges=bedges, excl_ptr=ptr, excl_idx=idx, n_nodes=n)
 
 
 # ---------------------------------------------------------------------------- mesh bodies
@@ -499,18 +537,17 @@ def build_scene(bodies, k_excl=2):
     for h in order:
         members[envs[h]].append(h)
     b_pe = np.array([sum(len(per[h]["boundary"]) for h in mem) for mem in members])
-    excls = []
-    for e, mem in enumerate(members):
-        ex = np.zeros((b_pe[e], b_pe[e]), bool)
+    # exclusion lists, block-diagonal per scene: body h's local boundary indices are shifted by
+    # the boundary nodes of the bodies before it in its scene
+    counts, cols = [], []
+    for mem in members:
         o = 0
         for h in mem:
-            k = len(per[h]["boundary"])
-            ex[o:o + k, o:o + k] = per[h]["excl"]
-            o += k
-        excls.append(ex)
-    excl_flat = (np.concatenate([e.reshape(-1).astype(np.uint8) for e in excls])
-                 if excls else np.zeros(0, np.uint8))
-    excl_offset = starts([e.size for e in excls]).astype(np.int64)
+            counts.append(np.diff(per[h]["excl_ptr"]))
+            cols.append(per[h]["excl_idx"] + o)
+            o += len(per[h]["boundary"])
+    excl_ptr = starts(np.concatenate(counts) if counts else np.zeros(0, np.int64)).astype(np.int64)
+    excl_idx = np.concatenate(cols).astype(np.int32) if cols else np.zeros(0, np.int32)
 
     mu = np.zeros(len(rest_area))
     lam = np.zeros(len(rest_area))
@@ -530,6 +567,6 @@ def build_scene(bodies, k_excl=2):
                 node_starts=starts([sum(per[h]["n_nodes"] for h in mem) for mem in members]),
                 tri_starts=starts([sum(len(per[h]["rest_area"]) for h in mem) for mem in members]),
                 boundary_starts=starts(b_pe), boundary_per_env=b_pe,
-                excl_flat=excl_flat, excl_offset=excl_offset, excluded_pairs_per_env=excls,
+                excl_ptr=excl_ptr, excl_idx=excl_idx,
                 tri_quad=tri_quad, body_env=envs, body_node_range=node_rng, body_tri_range=tri_rng,
                 body_grid=grid, grid_to_live_node_per_body=[per[h]["grid_to_live"] for h in range(nbod)])

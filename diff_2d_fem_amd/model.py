@@ -367,10 +367,7 @@ class Model:
         env = self._bodies[handle].env
         b0, b1 = int(me.boundary_starts[env]), int(me.boundary_starts[env + 1])
         mine = np.flatnonzero(np.isin(np.asarray(me.boundary_nodes)[b0:b1], nodes))
-        excl = me.excluded_pairs_per_env[env]
-        excl[mine[:, None], mine[None, :]] = True
-        o = int(me.excl_offset[env])
-        me.excl_flat[o:o + excl.size] = excl.reshape(-1)
+        me.add_exclusion(env, mine)
         self._rigid.append(RigidRegionSpec(body=handle, elems=elems, nodes=nodes))
         return len(self._rigid) - 1
 

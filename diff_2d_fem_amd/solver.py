@@ -144,8 +144,6 @@ class DeviceSolver:
         self.ela, self.elb = t(pos[edges[:, 0]], I32), t(pos[edges[:, 1]], I32)
         self.e2env = t(env, I32)
         self.ca, self.cb = t(share[edges[:, 0]], F64), t(share[edges[:, 1]], F64)
-        self.excl = t(np.asarray(me.excl_flat, np.uint8), U8)
-        self.excl_off = t(np.asarray(me.excl_offset, np.int64), I64)
         self.mu_node = None if mu_node is None else t(mu_node, F64)
         self.n_e = len(edges)
         if len(env) and (np.diff(env) < 0).any():
@@ -156,23 +154,27 @@ class DeviceSolver:
         self._layout = None                 # the scene layout plans LDS for the contact lists
 
     def _pack_exclusion(self):
-        """Each scene's exclusion matrix bit-packed, 32 per word (bit jp * n_b_e + j of scene e's
-        words: boundary nodes jp and j do not contact): excl_bits, excl_bits_off, and the
-        largest scene's word count excl_words_max."""
+        """Each scene's exclusion, bit-packed from the mesh's lists, 32 per word (bit
+        jp * n_b_e + j of scene e's words: boundary nodes jp and j do not contact): excl_bits,
+        excl_bits_off, and the largest scene's word count excl_words_max. The device never
+        holds the n_b_e x n_b_e byte matrices of the reference."""
         me = self.me
-        flat = np.asarray(me.excl_flat, np.uint8)
-        off = np.asarray(me.excl_offset, np.int64)
         nbe = np.asarray(me.boundary_per_env, np.int64)
-        words, woff = [], [0]
-        for e in range(self.B):
-            bits = np.packbits(flat[off[e]:off[e] + nbe[e] * nbe[e]] != 0, bitorder="little")
-            bits = np.concatenate([bits, np.zeros((-len(bits)) % 4, np.uint8)])
-            words.append(bits.view("<u4"))
-            woff.append(woff[-1] + len(bits) // 4)
-        self.excl_bits = self._t(np.concatenate(words).view(np.int32), I32)
-        self.excl_bits_off = self._t(np.asarray(woff, np.int64), I64)
-        self.excl_words = np.diff(woff)
-        self.excl_words_max = int(self.excl_words.max(initial=0))
+        wcount = (nbe * nbe + 31) // 32
+        woff = np.concatenate([[0], np.cumsum(wcount)]).astype(np.int64)
+        ptr = np.asarray(me.excl_ptr, np.int64)
+        idx = np.asarray(me.excl_idx, np.int64)
+        bstart = np.asarray(me.boundary_starts, np.int64)
+        n_b = len(ptr) - 1
+        row = np.repeat(np.arange(n_b), np.diff(ptr))                    # global boundary index
+        env = np.asarray(me.boundary2env, np.int64)[row]
+        bit = woff[env] * 32 + (row - bstart[env]) * nbe[env] + idx         # global bit position
+        words = np.zeros(int(woff[-1]), np.uint32)
+        np.bitwise_or.at(words, bit // 32, (np.uint32(1) << (bit % 32).astype(np.uint32)))
+        self.excl_bits = self._t(words.view(np.int32), I32)
+        self.excl_bits_off = self._t(woff, I64)
+        self.excl_words = wcount
+        self.excl_words_max = int(wcount.max(initial=0))
 
     def setup_node(self, stiffness, radius, mu, mu_node=None, friction=False):
         """The "node" penalty contact between boundary nodes of one environment, with Coulomb
@@ -182,15 +184,12 @@ class DeviceSolver:
         t = self._t
         self.node = dict(k=float(stiffness), r=float(radius), mu=float(mu), fric=bool(friction),
                          per_body=mu_node is not None)
-        self.excl = t(np.asarray(me.excl_flat, np.uint8), U8)
-        self.excl_off = t(np.asarray(me.excl_offset, np.int64), I64)
         self.mu_node = None if mu_node is None else t(mu_node, F64)
         self.max_nb = int(np.max(np.asarray(me.boundary_per_env))) if self.B else 0
         self._pack_exclusion()
 
     def refresh_exclusion(self):
         if self.ipc is not None or self.node is not None:
-            self.excl = self._t(np.asarray(self.me.excl_flat, np.uint8), U8)
             self._pack_exclusion()
             self._layout = None
 
@@ -665,14 +664,14 @@ class DeviceSolver:
         s.free_ = p("free")
         s.el = self.elem_tiles_struct()
         if self.ipc is not None:
-            for k in ("ea", "eb", "ela", "elb", "e2env", "ca", "cb", "excl", "excl_off"):
+            for k in ("ea", "eb", "ela", "elb", "e2env", "ca", "cb"):
                 setattr(s, k, p(k))
             s.mu_node = self._ptr(self.mu_node)
             s.e_start, s.max_nb = p("e_start"), self.max_nb
             s.excl_bits, s.excl_bits_off = p("excl_bits"), p("excl_bits_off")
             s.excl_bits_lds = self.excl_words_max          # the per-kernel path's reading
         if self.node is not None:
-            s.excl, s.excl_off = p("excl"), p("excl_off")
+            s.excl_bits, s.excl_bits_off = p("excl_bits"), p("excl_bits_off")
             s.mu_node = self._ptr(self.mu_node)
             s.max_nb = self.max_nb
         s.fext = self._ptr(self.fext)

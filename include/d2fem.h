@@ -88,8 +88,6 @@ typedef struct d2f_scene {
   const int32_t *bnd_start;           /* B+1 */
   const int32_t *ea, *eb, *ela, *elb, *e2env; /* n_e */
   const double *ca, *cb;              /* n_e: vertex-correction shares */
-  const uint8_t *excl;                /* per-scene n_b_e^2 exclusion matrices */
-  const int64_t *excl_off;            /* B+1 */
   const double *mu_node;              /* N per-body friction, or NULL */
   /* kinematic nodes [n_bc] */
   const int32_t *bc_idx;              /* n_bc */
@@ -144,11 +142,12 @@ typedef struct d2f_scene {
   const int32_t *pe_start;            /* B+1 pressure edges per scene */
   const int32_t *td_start, *ey_start; /* B+1 tendons per scene, n_td+1 eyelets per tendon */
   int32_t max_nb, max_cells;          /* largest per-scene boundary count; grid cell cap */
-  /* per-scene exclusion matrices bit-packed (bit jp*n_b_e + j of scene e's words), staged in LDS
-   * when excl_bits_lds is set; pair-list capacity per scene (0: traverse the grid each pass) */
+  /* per-scene self-contact exclusion, bit-packed (bit jp*n_b_e + j of scene e's words: boundary
+   * nodes jp and j do not contact), staged in LDS when excl_bits_lds is set; pair-list
+   * capacity per scene (0: traverse the grid each pass) */
   const uint32_t *excl_bits;          /* sum_e ceil(n_b_e^2 / 32) words */
   const int64_t *excl_bits_off;       /* B+1 word offsets */
-  int32_t excl_bits_lds, max_pairs;   /* words of LDS for the bitset (0: read bytes); pairs cap.
+  int32_t excl_bits_lds, max_pairs;   /* words of LDS for the bitset (0: read it from global); pairs cap.
                                        * d2f_step_plain reads excl_bits_lds as the largest scene's
                                        * bitset in words (staged in LDS by its contact pass when
                                        * it fits there) */
@@ -194,7 +193,7 @@ typedef struct d2f_params {
    * implicit/body_contact.py): penalty node_k (node_r - d) between boundary nodes of one scene
    * closer than node_r, not excluded against each other; node_fric adds Coulomb friction on the
    * same pairs with coefficient node_mu (per body through scene mu_node when ipc_per_body). Uses
-   * the scene's excl / excl_off; exclusive with ipc_on. ipc_skin is the scene path's grid
+   * the scene's excl_bits / excl_bits_off; exclusive with ipc_on. ipc_skin is the scene path's grid
    * margin for either contact. */
   int32_t node_on, node_fric;
   double node_k, node_r, node_mu;
@@ -287,15 +286,17 @@ int64_t d2f_unique_pairs(const int64_t *E, int64_t m, int64_t *key, int64_t *fir
                          int64_t *inverse);
 /* The same for rows of three (the duplicate-triangle check of core/mesh.py:224). */
 int64_t d2f_unique_triples(const int64_t *R, int64_t m, int64_t *key, int64_t *first, int64_t *count);
-/* out[nb][nb] = 1 where boundary nodes i and j are within k mesh-edge hops (each with itself),
- * 0 elsewhere. Replaces the breadth-first search of core/mesh.py:133-153. */
-int d2f_hop_exclusion(const int64_t *tri, int64_t T, int64_t n_nodes, const int64_t *boundary, int64_t nb,
-                      int32_t k, uint8_t *out);
-/* out[i][j] |= 1 where boundary nodes i and j are closer than radius along the mesh's edges at
- * rest (edges: the distinct edges, pos: rest positions). Replaces _material_excl
- * (core/mesh.py:163-186): shortest paths over the same graph, stopped at the radius. */
-int d2f_material_exclusion(const double *pos, int64_t n_nodes, const int64_t *edges, int64_t m,
-                           const int64_t *boundary, int64_t nb, double radius, uint8_t *out);
+/* Self-contact exclusion of one body as sorted lists over its boundary nodes (CSR): boundary
+ * node s (of boundary[0..nb)) excludes every boundary node within k mesh-edge hops, itself
+ * included (the breadth-first search of core/mesh.py:133-153), and with radius > 0 every one
+ * closer than radius along the mesh's edges at rest (pos: rest positions; the shortest paths
+ * of _material_excl, core/mesh.py:163-186, stopped at the radius); the relation is made
+ * symmetric (core/mesh.py:157-158). The same sets as the reference's nb x nb matrix, which is
+ * never formed. row_ptr[nb + 1] and cols (capacity cap) receive the lists; returns the number of
+ * entries (more than cap: only cap written, call again with a larger cols), or < 0 on bad input. */
+int64_t d2f_exclusion_csr(const int64_t *tri, int64_t T, int64_t n_nodes, const double *pos,
+                          const int64_t *boundary, int64_t nb, int32_t k, double radius, int64_t *row_ptr,
+                          int32_t *cols, int64_t cap);
 
 /* ---- active strain ------------------------------------------------------------------------ */
 /* Element constants an activation rewrites: the reference-order arrays and, when sc_pos is not
@@ -331,18 +332,19 @@ int d2f_collider_grad_diag_f64(const double *x, const int32_t *bnd, int32_t n_b,
                                double kappa, double h2, int32_t clamp, double *g, double *dg,
                                void *stream);
 /* Penalty part of the "node" contact on its own, one pass over every scene's boundary list
- * (bnd_start: B + 1 offsets into bnd; excl / excl_off: the scenes' n_be x n_be exclusion bytes).
+ * (bnd_start: B + 1 offsets into bnd; excl_bits / excl_bits_off: the scenes' bit-packed exclusion,
+ * bit i * n_be + j of scene e's words excludes its boundary nodes i and j).
  * grad += h2 g, diag += h2 diag (each clamped at 0 when clamp) of every boundary node's own
  * visits. Replaces grad_diag_self_bp_kernel (implicit/kernels.py:772-808) as the adjoint's
  * preconditioner calls it (implicit/adjoint.py:319-336). */
 int d2f_node_grad_diag_f64(const double *x, const int32_t *bnd, const int32_t *bnd_start, int32_t B,
-                           const uint8_t *excl, const int64_t *excl_off, double radius,
+                           const uint32_t *excl_bits, const int64_t *excl_bits_off, double radius,
                            double stiffness, double h2, int32_t clamp, double *g, double *dg,
                            void *stream);
 /* y += h2 H_node v at x, own rows, unclamped (the contact set is the pairs closer than radius at
  * x). Replaces self_bp_hvp_kernel (implicit/adjoint.py:115-143). */
 int d2f_node_hvp_f64(const double *x, const double *v, const int32_t *bnd, const int32_t *bnd_start,
-                     int32_t B, const uint8_t *excl, const int64_t *excl_off, double radius,
+                     int32_t B, const uint32_t *excl_bits, const int64_t *excl_bits_off, double radius,
                      double stiffness, double h2, double *y, void *stream);
 /* out[t] += -(1 - Emin) h2 lam . f_t(x) with f_t the element's full-density elastic force (mu0,
  * lam0). Replaces drho_kernel (implicit/adjoint.py:165-178). */

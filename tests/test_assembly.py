@@ -1,6 +1,7 @@
 """The native scene assembly (csrc/assemble.cpp) against numpy / scipy restatements of the
 reference's steps (core/mesh.py:112-186, 224, 247-248): the same edge tables, boundary edges
-and self-contact exclusion sets, bit for bit, on pixel bodies with holes and on Delaunay star
+and self-contact exclusion sets (the native lists against the reference's dense matrices),
+bit for bit, on pixel bodies with holes and on Delaunay star
 meshes; bad indices raise. The pixel-scene fixtures written by the reference's own
 core/mesh.py (test_mesh.py) cover the assembled scenes end to end."""
 import numpy as np
@@ -71,12 +72,23 @@ def test_native_assembly_is_bit_identical(case):
     bed = M._boundary_edges(T)
     assert np.array_equal(bed, _np_boundary_edges(T))
     boundary = np.unique(bed.reshape(-1))
+    nb = len(boundary)
+
+    def dense(ptr, idx):
+        out = np.zeros((nb, nb), bool)
+        out[np.repeat(np.arange(nb), np.diff(ptr)), idx] = True
+        assert all((np.diff(idx[ptr[i]:ptr[i + 1]]) > 0).all() for i in range(nb))   # sorted, no repeats
+        return out
     for k in (0, 1, 2, 3):
-        assert np.array_equal(M._hop_exclusion(T, len(V), boundary, k), _np_hop_exclusion(T, len(V), boundary, k))
+        hop = _np_hop_exclusion(T, len(V), boundary, k)
+        assert np.array_equal(dense(*M.exclusion_csr(T, len(V), boundary, k)), hop | hop.T)
     for r in (1.5 * h, 3.0 * h):
-        got = M.material_exclusion(V, T, boundary, r)
-        assert np.array_equal(got, _np_material_exclusion(V, T, boundary, r))
-        assert got.sum() > len(boundary)                       # some pairs beyond the node itself
+        want = _np_hop_exclusion(T, len(V), boundary, 2) | _np_material_exclusion(V, T, boundary, r)
+        got = dense(*M.exclusion_csr(T, len(V), boundary, 2, V, r))
+        assert np.array_equal(got, want | want.T)
+        mat = dense(*M.exclusion_csr(T, len(V), boundary, 0, V, r))          # the radius alone
+        assert np.array_equal(mat, _np_material_exclusion(V, T, boundary, r) | np.eye(nb, dtype=bool))
+        assert mat.sum() > nb                                   # some pairs beyond the node itself
 
 
 def test_duplicate_rows_and_bad_indices():
@@ -86,6 +98,6 @@ def test_duplicate_rows_and_bad_indices():
     assert inv.tolist() == [1, 1, 0, 0, 1]
     T = np.array([[0, 1, 2], [1, 2, 3]], np.int64)
     with pytest.raises(ValueError):
-        M._hop_exclusion(T, 3, np.array([0, 1]), 2)             # node 3 is out of range
+        M.exclusion_csr(T, 3, np.array([0, 1]), 2)              # node 3 is out of range
     with pytest.raises(ValueError):
-        M.material_exclusion(np.zeros((3, 2)), T, np.array([0, 1]), 1.0)
+        M.exclusion_csr(T, 3, np.array([0, 1]), 2, np.zeros((3, 2)), 1.0)

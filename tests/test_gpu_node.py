@@ -5,7 +5,7 @@ float64 oracle (oracle/node.py).
 * The per-kernel entry points d2f_node_grad_diag_f64 and d2f_node_hvp_f64 agree with the oracle
   to 1e-12 relative (float64, summation order only).
 * Whole steps on both device paths, with the model's or per-body friction and without, and the
-  scene path's variants (exclusion from the byte matrix, rigid block, line search): positions
+  scene path's variants (exclusion bits read from global memory, rigid block, line search): positions
   within 1e-9 of the slab's length after the same fixed iteration count; mixed precision within
   1e-5. The friction coefficients and tilt are ones for which the oracle's own result moves by
   less than 1e-11 of the slab when its start is perturbed by one part in 1e15: where pairs stick
@@ -54,8 +54,8 @@ def test_node_entry_points():
     x = torch.tensor(s.x, device=d)
     bnd = torch.tensor(np.asarray(me.boundary_nodes), dtype=torch.int32, device=d)
     bst = torch.tensor(np.asarray(me.boundary_starts), dtype=torch.int32, device=d)
-    excl = torch.tensor(np.asarray(me.excl_flat, np.uint8), device=d)
-    eoff = torch.tensor(np.asarray(me.excl_offset, np.int64), device=d)
+    sol = ImplicitIntegrator(path="plain", precision="double").attach(m, 1e-3)
+    excl, eoff = sol.excl_bits, sol.excl_bits_off        # bit-packed from the mesh's exclusion lists
     nd = s.node
     rng = np.random.default_rng(2)
     for clamp in (0, 1):
@@ -110,13 +110,13 @@ def test_node_steps_fixed_iterations(path, fric):
     assert (gi.stats(mg)["iters"] == oi.stats()["iters"]).all()
 
 
-@pytest.mark.parametrize("case", ["byte_exclusion", "rigid_block", "line_search", "rigid_line_search",
+@pytest.mark.parametrize("case", ["global_exclusion", "rigid_block", "line_search", "rigid_line_search",
                                   "converged"])
 def test_node_scene_variants(case):
     kw = dict(max_iter=60, eps=0.0, conv_check=0)
     make = lambda: node_stack(n_env=2, mu=0.1, tilt=3.0)
     override = None
-    if case == "byte_exclusion":
+    if case == "global_exclusion":
         override = dict(excl_bits_lds=0)
     elif case == "rigid_block":
         make = lambda: node_stack(n_env=2, mu=0.05, tilt=3.0, rigid=True)
