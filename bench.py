@@ -294,7 +294,7 @@ def elastic_roofline(res):
     ach = bytes_ / dur / 1e9
     return dict(kernel="k_elastic_grad_diag (float64)", bound="hbm", achieved=round(ach, 1),
                 peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
-                traffic=measured_traffic("d2f::k_elastic_grad_diag_tiled", res),
+                traffic=measured_traffic("d2f::k_elastic_grad_diag_tiled<false>", res),
                 avg_us=round(dur * 1e6, 2), bytes_per_launch=int(bytes_))
 
 
@@ -357,7 +357,23 @@ def element_roofline_large(n_meshes=32, copies=128, seed=0):
                 triangles=int(T), nodes=int(N), bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS,
                 unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4), avg_us=round(dur * 1e6, 1),
                 bytes_per_launch=int(bytes_), max_chunk_nodes=et["max_nodes"],
-                owned_frac=round(float(et["owned"].float().mean()), 4))
+                owned_frac=round(float(et["owned"].float().mean()), 4),
+                traffic=large_traffic(f"{n_meshes} star meshes x {copies}", int(T)))
+
+
+def large_traffic(workload, T):
+    """HBM bytes per launch of the element entry point at the large size (its chunk pass and
+    its halo pass) from the committed rocprofv3 PMC summary profiles/element_large_traffic.json,
+    or None when there is no measurement for this workload."""
+    import json
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "element_large_traffic.json")
+    try:
+        d = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    if d.get("workload") != workload or d.get("triangles") != T:
+        return None
+    return int(sum(k["traffic_bytes"] for k in d["kernels"].values()))
 
 
 def adjoint_leg(n_fingers=512, T=10, every=5):
@@ -400,21 +416,45 @@ def adjoint_leg(n_fingers=512, T=10, every=5):
 
 
 def cpu_baseline(args, steps):
-    """The float64 numpy oracle on one scene of the same workload, from its first step."""
+    """The float64 numpy oracle on one scene of the same workload, from its first step. Returns
+    the baseline entry and the oracle's positions after each step (for oracle_error)."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle.pncg import OracleIntegrator
     from diff_2d_fem_amd import workloads as W
     w = W.CONFIGS[args.config](1)
     m = w["model"]
     oi = OracleIntegrator()
-    t0 = time.perf_counter()
+    xs = []
+    t = 0.0
     for k in range(1, steps + 1):
         w["command"](k)
+        t0 = time.perf_counter()
         oi.simulate(m, w["dt"])
-    t = time.perf_counter() - t0
+        t += time.perf_counter() - t0
+        xs.append(oi.solver.x.copy())
     return dict(value=round(steps / t, 4), unit="scene-steps/s", cores=1, kind="port",
                 sample=f"float64 numpy oracle, 1 scene of {args.config}, steps 1-{steps} "
-                       f"({t:.1f} s, {int(oi.stats()['iters'].sum())} PNCG iterations)")
+                       f"({t:.1f} s, {int(oi.stats()['iters'].sum())} PNCG iterations)"), xs
+
+
+def oracle_error(args, xs, path, precision):
+    """Position error of the GPU path against the float64 oracle on the oracle's scene (scene 0
+    of the workload, the same seeded scene), step by step from rest over the oracle's steps:
+    max |x_gpu - x_oracle| over the nodes, over the scene's body size."""
+    from diff_2d_fem_amd import ImplicitIntegrator
+    from diff_2d_fem_amd import workloads as W
+    w = W.CONFIGS[args.config](1)
+    m = w["model"]
+    integ = ImplicitIntegrator(path=path, precision=precision)
+    size = float(np.asarray(w.get("sides", [w.get("size", 1.0)]), np.float64).reshape(-1)[0])
+    errs = []
+    for k, xo in enumerate(xs, start=1):
+        w["command"](k)
+        integ.simulate(m, w["dt"])
+        errs.append(float(np.linalg.norm(m.get_x() - xo, axis=1).max()) / size)
+    return dict(max=max(errs), per_step=[float(f"{e:.3e}") for e in errs], steps=len(xs),
+                reference="float64 numpy oracle, same scene, same commands, from rest",
+                path=path, precision=precision, bound=1e-5)
 
 
 def main():
@@ -474,7 +514,8 @@ def main():
     if not args.no_c5:
         out["c5"] = c5_leg(plain=not args.no_plain)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args, args.cpu_steps)
+        out["cpu_baseline"], xs = cpu_baseline(args, args.cpu_steps)
+        out["pos_err_vs_oracle"] = oracle_error(args, xs, path, precision)
     print(json.dumps(out), flush=True)
 
 
