@@ -415,12 +415,55 @@ def adjoint_leg(n_fingers=512, T=10, every=5):
                 grad_norm_scale=float(scale.grad.norm()), grad_norm_pressure=float(p.grad.norm()))
 
 
-def cpu_baseline(args, steps):
-    """The float64 numpy oracle on one scene of the same workload, from its first step. Returns
-    the baseline entry and the oracle's positions after each step (for oracle_error)."""
-    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+def _compiled_rate(config, n_scenes, warmup, steps, threads):
+    """Scene-steps/s of the compiled oracle step (oracle/cpu_step.cpp) on the first n_scenes
+    scenes of the workload: `warmup` untimed steps, then `steps` timed."""
+    from oracle.cpu import CpuStepper
     from oracle.pncg import OracleIntegrator
     from diff_2d_fem_amd import workloads as W
+    w = W.CONFIGS[config](n_scenes)
+    m = w["model"]
+    cs = CpuStepper(OracleIntegrator(), m, w["dt"], threads=threads)
+    for k in range(1, warmup + 1):
+        w["command"](k)
+        cs.simulate(m, w["dt"])
+    t = 0.0
+    for k in range(warmup + 1, warmup + steps + 1):
+        w["command"](k)
+        t0 = time.perf_counter()
+        cs.simulate(m, w["dt"])
+        t += time.perf_counter() - t0
+    return n_scenes * steps / t, t
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(args, steps):
+    """CPU baselines on the host cores of this run. The compiled restatement of the reference
+    algorithm (oracle/cpu_step.cpp: float64, one environment after another, kernel loops in
+    index order) timed over the same step range as the GPU's timed steps (after args.warmup
+    untimed steps): with OpenMP over environments on every core this process may use (the
+    reported value), and serially on one core. Also the float64 numpy oracle on one scene from
+    its first step, whose positions serve oracle_error. Returns (entry, numpy oracle positions)."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import cpu
+    from oracle.pncg import OracleIntegrator
+    from diff_2d_fem_amd import workloads as W
+    cores = len(os.sched_getaffinity(0))
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
+    cpu.build()
+    tsteps = 3
+    par, tp = _compiled_rate(args.config, 4 * cores, args.warmup, tsteps, cores)
+    ser, ts = _compiled_rate(args.config, 8, args.warmup, tsteps, 1)
     w = W.CONFIGS[args.config](1)
     m = w["model"]
     oi = OracleIntegrator()
@@ -432,9 +475,14 @@ def cpu_baseline(args, steps):
         oi.simulate(m, w["dt"])
         t += time.perf_counter() - t0
         xs.append(oi.solver.x.copy())
-    return dict(value=round(steps / t, 4), unit="scene-steps/s", cores=1, kind="port",
-                sample=f"float64 numpy oracle, 1 scene of {args.config}, steps 1-{steps} "
-                       f"({t:.1f} s, {int(oi.stats()['iters'].sum())} PNCG iterations)"), xs
+    rng = f"steps {args.warmup + 1}-{args.warmup + tsteps}"
+    return dict(value=round(par, 3), unit="scene-steps/s", cores=cores, kind="port",
+                sample=f"compiled float64 restatement (oracle/cpu_step.cpp), OpenMP over {4 * cores} scenes of "
+                       f"{args.config}, {rng} ({tp:.1f} s), {_cpu_model()}",
+                serial=dict(value=round(ser, 4), cores=1,
+                            sample=f"the same, serial, 8 scenes, {rng} ({ts:.1f} s)"),
+                numpy_oracle=dict(value=round(steps / t, 4), cores=1,
+                                  sample=f"float64 numpy oracle, 1 scene, steps 1-{steps} ({t:.1f} s)")), xs
 
 
 def oracle_error(args, xs, path, precision):
