@@ -76,6 +76,9 @@ class DeviceSolver:
         self.lds_order = self.LDS_ORDER    # scene path: which optional LDS parts come first
         self._layout = None
         self._route_t = None
+        self._order_t = None
+        self.longest_first = True           # scene kernel: launch the largest scenes first
+        self.n_cu = torch.cuda.get_device_properties(self.dev).multi_processor_count if torch.cuda.is_available() else 256
         # scene path: step through d2f_step_scene_noalloc (the library's static work arrays, no
         # per-call allocation; at most NOALLOC_MAX_NODES nodes, one solver per device at a time)
         self.noalloc = False
@@ -577,7 +580,20 @@ class DeviceSolver:
         self._route_t = None
         if len(self.plain_envs):
             self._route_t = dict(skip=self._t(fits.astype(np.uint8), U8),
-                                 list=self._t(self.scene_envs.astype(np.int32), I32))
+                                 list=self._t(self._longest_first(self.scene_envs).astype(np.int32), I32))
+        self._order_t = None
+        if not len(self.plain_envs) and self.longest_first and self.B > self.n_cu:
+            self._order_t = self._t(self._longest_first(self.scene_envs).astype(np.int32), I32)
+
+    def _longest_first(self, envs):
+        """The scenes in order of decreasing work per iteration (triangles, nodes and boundary
+        nodes): workgroups are dispatched in launch order, so when there are more scenes than
+        compute units the largest start first and the small ones fill in behind them."""
+        me = self.me
+        work = (np.diff(np.asarray(me.tri_starts, np.int64)) + 0.5 * np.diff(np.asarray(me.node_starts, np.int64))
+                + np.asarray(me.boundary_per_env, np.int64))
+        envs = np.asarray(envs, np.int64)
+        return envs[np.argsort(-work[envs], kind="stable")]
 
     def _plan_lds(self):
         """Split each scene workgroup's LDS budget (for the current block_size) over the scenes
@@ -724,6 +740,9 @@ class DeviceSolver:
                 s.env_skip = self._ptr(self._route_t["skip"])
             else:
                 s.sc_list, s.n_list = self._ptr(self._route_t["list"]), len(self.scene_envs)
+        elif (part != "plain" and self.path == "scene" and self._layout is not None
+              and self._order_t is not None):
+            s.sc_list, s.n_list = self._ptr(self._order_t), self.B
         s.n_col = len(self.colliders)
         for i, c in enumerate(self.colliders):
             s.col[i].kind = int(c["kind"])

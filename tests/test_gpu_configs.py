@@ -97,3 +97,27 @@ def test_c5_full_size(tri_range, n_scenes, path, precision):
     print(f"c5 {tri_range} {path} {precision}: T = {T}, max |dx| / size = {err:.3e} (motion {moved:.2e})")
     assert moved > 1e-5
     assert err < (1e-9 if precision == "double" else 1e-5)
+
+
+def test_longest_first_launch_order():
+    """With more scenes than compute units the scene kernel launches the scenes in order of
+    decreasing size (solver.longest_first): the same steps as the launch in scene order, 1e-9 of
+    the body size (float64), on C2's uneven blobs with the compute-unit count lowered so that the
+    order applies to 12 scenes."""
+    def run(longest_first):
+        w = W.random_blobs(12)
+        m = w["model"]
+        integ = ImplicitIntegrator(path="scene", precision="double", max_iter=60, eps=0.0, conv_check=0)
+        integ.attach(m, w["dt"])
+        s = m._solver
+        s.n_cu, s.longest_first = 4, longest_first
+        for k in range(1, 4):
+            w["command"](k)
+            integ.simulate(m, w["dt"])
+        return m.get_x(), s, w
+    x1, s1, w = run(True)
+    x0, s0, _ = run(False)
+    assert s1._order_t is not None and s0._order_t is None
+    order = s1._order_t.cpu().numpy()
+    assert sorted(order.tolist()) == list(range(12)) and order.tolist() != list(range(12))
+    assert np.abs(x1 - x0).max() <= 1e-9 * w["size"]
