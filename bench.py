@@ -133,7 +133,10 @@ def parse():
 def build(config, n_env):
     from diff_2d_fem_amd import workloads as W
     fn = W.CONFIGS[config]
-    return fn() if n_env is None else fn(n_env)
+    kw = {}
+    if config == "c5":
+        kw["progress"] = lambda k, n, what: print(f"c5 set-up: {k}/{n} scenes, {what}", file=sys.stderr, flush=True)
+    return fn(**kw) if n_env is None else fn(n_env, **kw)
 
 
 def run_gpu(config, envs, path, precision, steps, warmup, tune=False):
@@ -150,6 +153,9 @@ def run_gpu(config, envs, path, precision, steps, warmup, tune=False):
         step += 1
         w["command"](step)
         integ.simulate(m, w["dt"])
+        if config == "c5":                          # minutes per step at thousands of scenes
+            torch.cuda.synchronize()
+            print(f"c5 {path} {precision}: warm-up step {step} done", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     tuned = None
     if tune and path == "scene":

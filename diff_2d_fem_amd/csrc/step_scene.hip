@@ -1003,35 +1003,69 @@ struct Step {
   // triangles of a colour share a node), for scenes whose nodes do not fit in LDS.
   __device__ __forceinline__ void grad_diag_colored() {
     const double h2 = p.h2;
-    for (int i = n0 + tid; i < n1; i += TPB) {
-      const V2<double> g = ld2(s.grad, i);
-      st2(s.grad_prev, i, g);
-      const double m = s.mass[i], f = fr(i);
-      const V2<double> d = X(i) - ld2(s.xhat, i);
-      V2<double> gn = {m * d.x * f, m * d.y * f};
-      if (s.fext) gn = gn - h2 * ld2(s.fext, i);
-      st2(s.grad, i, gn);
-      st2(s.diag, i, V2<double>{m, m});
-    }
+    struct NodeIn {
+      V2<double> g, x, xh, fe;
+      double m, f;
+    };
+    node_batches<2>(
+        [&](int i) {
+          NodeIn q;
+          q.g = ld2(s.grad, i);
+          q.m = s.mass[i];
+          q.f = fr(i);
+          q.x = X(i);
+          q.xh = ld2(s.xhat, i);
+          q.fe = s.fext ? ld2(s.fext, i) : V2<double>{0.0, 0.0};
+          return q;
+        },
+        [&](int i, const NodeIn &q) {
+          st2(s.grad_prev, i, q.g);
+          const V2<double> d = q.x - q.xh;
+          V2<double> gn = {q.m * d.x * q.f, q.m * d.y * q.f};
+          if (s.fext) gn = gn - h2 * q.fe;
+          st2(s.grad, i, gn);
+          st2(s.diag, i, V2<double>{q.m, q.m});
+        });
     for (int j = tid; j < 4 * nb; j += TPB) bacc[j] = 0.0;
     __syncthreads();
-    // elastic, colour by colour (reference implicit/kernels.py:86-128)
+    // elastic, colour by colour (reference implicit/kernels.py:86-128). A triangle's three
+    // nodes are distinct and no other triangle of its colour touches them, so the old sums of
+    // both triangles of a trip are loaded together with their positions before either triangle
+    // adds: one memory round trip per trip instead of one per node update.
+    struct ColIn {
+      Elem<R> el;
+      V2<double> x[3], g[3], d[3];
+    };
+    const ElemPtrs<R> ep = elem_ptrs<R>(s);
     for (int c = 0; c < MAXC; ++c) {
       const int ta = s.sc_cs[c0 + c], tb = s.sc_cs[c0 + c + 1];
       if (ta == tb) continue;
-      for (int t = ta + tid; t < tb; t += TPB) {
-        const Elem<R> el = load_elem<R>(s, t);
-        const NH<R> h = elem_F(el, X(el.i0), X(el.i1), X(el.i2));
-        V2<R> g0, g1, g2, d0, d1, d2;
-        nh_grad(h, el.B, el.Amu, R(1), g0, g1, g2);
-        nh_diag(h, el.B, el.Amu, el.Alam, R(1), d0, d1, d2);
-        st2(s.grad, el.i0, ld2(s.grad, el.i0) + h2 * up(g0));
-        st2(s.grad, el.i1, ld2(s.grad, el.i1) + h2 * up(g1));
-        st2(s.grad, el.i2, ld2(s.grad, el.i2) + h2 * up(g2));
-        st2(s.diag, el.i0, ld2(s.diag, el.i0) + h2 * V2<double>{fmax(double(d0.x), 0.0), fmax(double(d0.y), 0.0)});
-        st2(s.diag, el.i1, ld2(s.diag, el.i1) + h2 * V2<double>{fmax(double(d1.x), 0.0), fmax(double(d1.y), 0.0)});
-        st2(s.diag, el.i2, ld2(s.diag, el.i2) + h2 * V2<double>{fmax(double(d2.x), 0.0), fmax(double(d2.y), 0.0)});
-      }
+      range_batches<2>(
+          ta, tb,
+          [&](int t) {
+            ColIn q;
+            q.el = load_elem<R>(ep, t);
+            const int id[3] = {q.el.i0, q.el.i1, q.el.i2};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              q.x[k] = X(id[k]);
+              q.g[k] = ld2(s.grad, id[k]);
+              q.d[k] = ld2(s.diag, id[k]);
+            }
+            return q;
+          },
+          [&](int, const ColIn &q) {
+            const NH<R> h = elem_F(q.el, q.x[0], q.x[1], q.x[2]);
+            V2<R> g[3], d[3];
+            nh_grad(h, q.el.B, q.el.Amu, R(1), g[0], g[1], g[2]);
+            nh_diag(h, q.el.B, q.el.Amu, q.el.Alam, R(1), d[0], d[1], d[2]);
+            const int id[3] = {q.el.i0, q.el.i1, q.el.i2};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              st2(s.grad, id[k], q.g[k] + h2 * up(g[k]));
+              st2(s.diag, id[k], q.d[k] + h2 * V2<double>{fmax(double(d[k].x), 0.0), fmax(double(d[k].y), 0.0)});
+            }
+          });
       __syncthreads();
     }
   }

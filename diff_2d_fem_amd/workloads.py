@@ -233,7 +233,9 @@ def _star_mesh(rng, R, n_tri):
     return V, T, h
 
 
-def star_meshes(n_env=4096, seed=0, R=10e-3, tri_range=(2000, 20000)):
+def star_meshes(n_env=4096, seed=0, R=10e-3, tri_range=(2000, 20000), progress=None):
+    """`progress(done, n_env, what)`, when given, is called after every 256 meshes and after the
+    model is finalized (building the full batch takes minutes)."""
     m = Model()
     m.add_ground(0.0, mu=0.6)
     hs, extra = [], []
@@ -244,7 +246,11 @@ def star_meshes(n_env=4096, seed=0, R=10e-3, tri_range=(2000, 20000)):
         hs.append(m.add_soft_mesh(V, T, E=float(np.exp(rng.uniform(np.log(2e5), np.log(2e6)))),
                                   nu=float(rng.uniform(0.40, 0.45)), density=1100.0, env_id=i))
         extra.append((rng, V, T, h))
+        if progress is not None and (i + 1) % 256 == 0:
+            progress(i + 1, n_env, "meshes")
     m.finalize()
+    if progress is not None:
+        progress(n_env, n_env, "finalized")
     m.gravity = (0.0, -9.81)
     tendons = []
     for hd, (rng, V, T, h) in zip(hs, extra):
@@ -268,6 +274,8 @@ def star_meshes(n_env=4096, seed=0, R=10e-3, tri_range=(2000, 20000)):
             m.set_tendon_tension(t, float(rng.uniform(0.0, 50.0)))
             tl.append(t)
         tendons.append(tl)
+    if progress is not None:
+        progress(n_env, n_env, "affine regions and tendons")
     return dict(model=m, dt=2e-3, command=lambda step: None, bodies=hs, tendons=tendons, size=2 * R,
                 name="star_meshes")
 
