@@ -123,7 +123,7 @@ def lib():
     L.d2f_elastic_pHp_f64.argtypes = [P, P, P, P, P, P, P, P, i32, f64, i32, P, P]
     L.d2f_inversion_alpha_f64.argtypes = [P, P, P, P, i32, f64, P, P]
     L.d2f_elastic_energy_f64.argtypes = [P, P, P, P, P, P, P, i32, f64, P, P]
-    L.d2f_elastic_hvp_f64.argtypes = [P, P, P, P, P, P, P, i32, f64, P, P]
+    L.d2f_elastic_hvp_f64.argtypes = [P, P, P, P, P, P, P, i32, f64, P, P, P]
     L.d2f_collider_force.argtypes = [C.POINTER(Scene), C.POINTER(Params), i32, i32, P, P]
     L.d2f_contact_force.argtypes = [C.POINTER(Scene), C.POINTER(Params), i32, i32, i32, P, P, P, P]
     L.d2f_collider_hvp_f64.argtypes = [P, P, P, i32, P, i32, i32, f64, f64, f64, P, P]
@@ -153,8 +153,8 @@ def lib():
     if tuple(sizes) != want:
         raise RuntimeError(f"{path}: struct sizes {tuple(sizes)} do not match the binding {want}; "
                            f"rebuild the library")
-    if L.d2f_abi_version() != 1:
-        raise RuntimeError(f"{path}: ABI version {L.d2f_abi_version()}, expected 1")
+    if L.d2f_abi_version() != 2:
+        raise RuntimeError(f"{path}: ABI version {L.d2f_abi_version()}, expected 2")
     _lib = L
     return L
 

@@ -176,7 +176,7 @@ class AdjointPNCG2D:
         s = self.s
         _lib.check(L.d2f_elastic_hvp_f64(self._p(xw), self._p(vw), self._p(s.tri), self._p(s.Bm),
                                          self._p(s.area), self._p(self._mu_el), self._p(self._lam_el),
-                                         self.n_tri, self.h2, self._p(y), self._stream()),
+                                         self.n_tri, self.h2, C.byref(self._tiles), self._p(y), self._stream()),
                    "d2f_elastic_hvp_f64")
         which = 2 if (use_disc and use_sdf) else (0 if use_disc else 1)
         if (use_disc or use_sdf) and self._n_col:

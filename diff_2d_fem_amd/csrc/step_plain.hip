@@ -191,19 +191,87 @@ __global__ __launch_bounds__(ELEM_TPB) void k_elastic_grad_diag_tiled(
 }
 
 // The halo nodes of the tiled pass: node hnode[k] adds its chunks' partial sums (contiguous,
-// in chunk order).
+// in chunk order). diag null: the partials carry only the first two components (the
+// Hessian-vector product).
 __global__ void k_elastic_halo(d2f_elem_tiles tl, double *grad, double *diag) {
   GRID_LOOP(k, tl.n_hnodes) {
     if (k < tl.n_hnodes) {
       const int n = tl.hnode[k];
-      V2<double> g = ld2(grad, n), d = ld2(diag, n);
+      V2<double> g = ld2(grad, n), d = {0.0, 0.0};
+      if (diag) d = ld2(diag, n);
       for (int i = tl.hnode_off[k]; i < tl.hnode_off[k + 1]; ++i) {
         const double4 q = reinterpret_cast<const double4 *>(tl.part)[i];
         g = g + V2<double>{q.x, q.y};
         d = d + V2<double>{q.z, q.w};
       }
       st2(grad, n, g);
-      st2(diag, n, d);
+      if (diag) st2(diag, n, d);
+    }
+  }
+}
+
+// Elastic Hessian-vector product y += h2 (d2Psi/dx2) v (reference implicit/adjoint.py:70-85),
+// tiled as k_elastic_grad_diag_tiled: chunk c gathers its nodes' positions and direction into
+// LDS, sums every triangle's product per node there, then adds each owned node once (plain
+// read-modify-write) and its halo sums by device-scope atomics, or (PARTIAL) through the
+// tiles' partial slots and k_elastic_halo.
+template <bool PARTIAL>
+__global__ __launch_bounds__(ELEM_TPB) void k_elastic_hvp_tiled(const double *x, const double *vv,
+                                                                const double *Bm, const double *area,
+                                                                const double *mu, const double *lam, int T,
+                                                                double h2, d2f_elem_tiles tl, double *y) {
+  // planes of max_nodes each: x, y, v_x, v_y, (Hv)_x, (Hv)_y
+  extern __shared__ double sh[];
+  const int nbk = gridDim.x, q8 = nbk / 8, r8 = nbk % 8, xcd = blockIdx.x % 8;
+  const int c = xcd * q8 + min(xcd, r8) + blockIdx.x / 8;
+  const int n0 = tl.node_off[c], nn = tl.node_off[c + 1] - n0;
+  const int h0 = tl.halo_off[c], no = nn - (tl.halo_off[c + 1] - h0);
+  const int M = tl.max_nodes;
+  double *xs = sh, *ys = sh + M, *vxs = sh + 2 * M, *vys = sh + 3 * M, *acc = sh + 4 * M;
+  for (int j = threadIdx.x; j < nn; j += ELEM_TPB) {
+    const int n = tl.nodes[n0 + j];
+    const V2<double> p = ld2(x, n), q = ld2(vv, n);
+    xs[j] = p.x;
+    ys[j] = p.y;
+    vxs[j] = q.x;
+    vys[j] = q.y;
+    acc[j] = acc[M + j] = 0.0;
+  }
+  __syncthreads();
+  const int base = c * D2F_ELEM_CHUNK + threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < ELEM_PER_THREAD; ++k) {
+    const int t = base + k * ELEM_TPB;
+    if (t >= T) continue;
+    const uint32_t l = tl.tri_local[t];
+    const int li[3] = {int(l & 1023u), int((l >> 10) & 1023u), int((l >> 20) & 1023u)};
+    const V2<double> p0 = {xs[li[0]], ys[li[0]]}, p1 = {xs[li[1]], ys[li[1]]}, p2 = {xs[li[2]], ys[li[2]]};
+    const M2<double> B = load_B<double>(Bm, t);
+    const double m_ = mu[t], l_ = lam[t], A = area[t];
+    const M2<double> Ds = {p1.x - p0.x, p2.x - p0.x, p1.y - p0.y, p2.y - p0.y};
+    const NH<double> h = nh_eval(mat_mul(Ds, B), m_, l_);
+    const M2<double> dF = dF_of(V2<double>{vxs[li[0]], vys[li[0]]}, V2<double>{vxs[li[1]], vys[li[1]]},
+                                V2<double>{vxs[li[2]], vys[li[2]]}, B);
+    const M2<double> dP = nh_d2_dir(h, dF, m_, l_);
+    const V2<double> f1 = {A * (dP.a * B.a + dP.b * B.b), A * (dP.c * B.a + dP.d * B.b)};
+    const V2<double> f2 = {A * (dP.a * B.c + dP.b * B.d), A * (dP.c * B.c + dP.d * B.d)};
+    const V2<double> f[3] = {{-(f1.x + f2.x), -(f1.y + f2.y)}, f1, f2};
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+      atomicAdd(acc + li[v], f[v].x);
+      atomicAdd(acc + M + li[v], f[v].y);
+    }
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < nn; j += ELEM_TPB) {
+    const V2<double> yy = {h2 * acc[j], h2 * acc[M + j]};
+    if (j < no) {
+      const int n = tl.nodes[n0 + j];
+      st2(y, n, ld2(y, n) + yy);
+    } else if (PARTIAL) {
+      reinterpret_cast<double4 *>(tl.part)[tl.hdest[h0 + j - no]] = make_double4(yy.x, yy.y, 0.0, 0.0);
+    } else {
+      atomic_add2(y, tl.nodes[n0 + j], yy);
     }
   }
 }
@@ -2038,10 +2106,30 @@ int d2f_elastic_energy_f64(const double *x, const int32_t *tri, const double *Bm
 
 int d2f_elastic_hvp_f64(const double *x, const double *v, const int32_t *tri, const double *Bm,
                         const double *area, const double *mu, const double *lam, int32_t T, double h2,
-                        double *y, void *stream) {
+                        const d2f_elem_tiles *tiles, double *y, void *stream) {
   D2F_CHECK(T >= 0 && x && v && tri && Bm && area && mu && lam && y, "d2f_elastic_hvp_f64: bad argument");
   hipStream_t st = (hipStream_t)stream;
-  LAUNCH(k_elastic_hvp, T, x, v, tri, Bm, area, mu, lam, T, h2, y);
+  if (T == 0) return 0;
+  if (tiles && tiles->node_off && tiles->max_nodes > 0 && tiles->max_nodes <= D2F_ELEM_NODES_MAX) {
+    D2F_CHECK(tiles->nodes && tiles->halo_off && tiles->tri_local && tiles->part && tiles->n_hnodes >= 0 &&
+                  (tiles->n_hnodes == 0 || (tiles->hnode && tiles->hnode_off && tiles->hdest)),
+              "d2f_elastic_hvp_f64: incomplete tiles");
+    const int blocks = (T + D2F_ELEM_CHUNK - 1) / D2F_ELEM_CHUNK;
+    const size_t lds = sizeof(double) * 6 * (size_t)tiles->max_nodes;   // at most 48 KiB
+    const bool partial = T >= ELEM_PARTIAL_MIN_T;
+    if (partial)
+      hipLaunchKernelGGL(k_elastic_hvp_tiled<true>, dim3(blocks), dim3(ELEM_TPB), lds, st, x, v, Bm, area, mu, lam,
+                         T, h2, *tiles, y);
+    else
+      hipLaunchKernelGGL(k_elastic_hvp_tiled<false>, dim3(blocks), dim3(ELEM_TPB), lds, st, x, v, Bm, area, mu,
+                         lam, T, h2, *tiles, y);
+    if (partial && tiles->n_hnodes > 0) {
+      const int hb = (int)std::min<long>((tiles->n_hnodes + 255) / 256, 65536L);
+      hipLaunchKernelGGL(k_elastic_halo, dim3(hb), dim3(256), 0, st, *tiles, y, (double *)nullptr);
+    }
+  } else {
+    LAUNCH(k_elastic_hvp, T, x, v, tri, Bm, area, mu, lam, T, h2, y);
+  }
   D2F_HIP(hipGetLastError());
   return 0;
 }

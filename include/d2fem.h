@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define D2F_ABI_VERSION 1
+#define D2F_ABI_VERSION 2
 
 #define D2F_OK 0
 #define D2F_EINVAL (-1)   /* bad argument or inconsistent sizes */
@@ -234,10 +234,14 @@ int d2f_inversion_alpha_f64(const double *x, const double *p, const int32_t *tri
 int d2f_elastic_energy_f64(const double *x, const int32_t *tri, const double *Bm,
                            const double *area, const double *mu, const double *lam,
                            const int32_t *tri2env, int32_t T, double h2, double *E, void *stream);
-/* y += h2 (d2Psi/dx2) v, unclamped. Replaces hvp_elastic_kernel (implicit/adjoint.py:70-85). */
+/* y += h2 (d2Psi/dx2) v, unclamped. Replaces hvp_elastic_kernel (implicit/adjoint.py:70-85).
+ * tiles as for d2f_elastic_grad_diag_f64 (chunk c sums its triangles' products in LDS and adds
+ * each node it owns once; halo sums by atomics, or from 4,194,304 triangles on through the
+ * tiles' partial slots and a second launch); tiles NULL: one atomic per contribution. Calls
+ * that share one tiles struct must not overlap (its partial slots are scratch). */
 int d2f_elastic_hvp_f64(const double *x, const double *v, const int32_t *tri, const double *Bm,
                         const double *area, const double *mu, const double *lam, int32_t T,
-                        double h2, double *y, void *stream);
+                        double h2, const d2f_elem_tiles *tiles, double *y, void *stream);
 
 /* ---- whole implicit steps ----------------------------------------------------------------- */
 /* One implicit step of every environment on the per-kernel path: one launch per reference

@@ -30,7 +30,7 @@ def test_library_loads_and_exports_every_declared_symbol():
     raw = ctypes.CDLL(str(_lib.LIB_PATH))
     missing = [n for n in declared() if not hasattr(raw, n)]
     assert not missing, f"declared in d2fem.h but not exported: {missing}"
-    assert L.d2f_abi_version() == 1
+    assert L.d2f_abi_version() == 2
     assert L.d2f_last_error() is not None
 
 

@@ -101,12 +101,17 @@ def test_elastic_curvature_energy_caps_hvp(deformed):
     OT.inversion_alpha(x, p * 40.0, me.tri, me.tri2env, 0.1, ao)
     assert (ao < 3.0).all()
     assert _rel(alpha.cpu().numpy(), ao) < 1e-12
-    y = torch.zeros_like(xt)
-    _lib.check(L.d2f_elastic_hvp_f64(_p(xt), _p(pt), _p(tri), _p(Bm), _p(A), _p(mu), _p(lam), T, 4e-6,
-                                     _p(y), _stream()), "hvp")
+    # the Hessian-vector product accumulates too: atomics only (no tiles), then tiled
+    tiles = tiles_struct(element_tiles(me.tri, len(x), device=d))
     yo = np.zeros_like(x)
     OT.elastic_hvp(x, p, me.tri, me.D_m_inv, me.rest_area, me.mu, me.lam, 4e-6, yo)
-    assert _rel(y.cpu().numpy(), yo) < 1e-12
+    y0 = np.abs(yo).max() * np.random.default_rng(6).normal(size=x.shape)   # at the product's scale
+    yo += y0
+    for tl in (None, tiles):
+        y = torch.tensor(y0, device=d)
+        _lib.check(L.d2f_elastic_hvp_f64(_p(xt), _p(pt), _p(tri), _p(Bm), _p(A), _p(mu), _p(lam), T, 4e-6,
+                                         None if tl is None else C.byref(tl), _p(y), _stream()), "hvp")
+        assert _rel(y.cpu().numpy(), yo) < 1e-12
 
 
 def _run_pair(make, n_steps, dt, path, kw, each=None, precision="double"):
@@ -400,3 +405,14 @@ def test_elastic_grad_diag_with_halo_partials():
     go, do = g0.copy(), g0.copy()
     OT.elastic_grad_diag(x, tri, B, area, mu, lam, 4e-6, 1, go, do)
     assert _rel(g.cpu().numpy(), go) < 1e-12 and _rel(dg.cpu().numpy(), do) < 1e-12
+    # the Hessian-vector product through the same tiles and partial slots
+    v = rng.normal(scale=1e-5, size=x.shape)
+    yo = np.zeros_like(x)
+    OT.elastic_hvp(x, v, tri, B, area, mu, lam, 4e-6, yo)
+    y0 = np.abs(yo).max() * rng.normal(size=x.shape)        # at the product's scale
+    yo += y0
+    y = tt(y0)
+    args.append(tt(v))
+    _lib.check(L.d2f_elastic_hvp_f64(*map(_p, args[:1]), _p(args[-1]), *map(_p, args[1:6]), T, 4e-6, C.byref(tiles),
+                                     _p(y), _stream()), "hvp")
+    assert _rel(y.cpu().numpy(), yo) < 1e-12
