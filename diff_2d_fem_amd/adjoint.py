@@ -326,8 +326,7 @@ class AdjointPNCG2D:
             for _ in range(min(check_every, maxiter - done)):
                 if red:
                     self._kv_reduced(xw, ws["v"], ws["Kv"], **use)
-                else:
-                    ws["Kv"].zero_()
+                else:                        # Kv left cleared by the previous begin / step
                     self._kv(xw, ws["v"], ws["Kv"], **use)
                 _lib.check(L.d2f_minres_step_f64(C.byref(m), st), "d2f_minres_step_f64")
             done += check_every

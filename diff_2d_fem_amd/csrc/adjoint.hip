@@ -358,7 +358,10 @@ __global__ __launch_bounds__(MR_TPB) void k_minres_begin(d2f_minres m) {
   }
   const double bb = mr_block_sum(s, red);
   const double beta1 = bb > 0.0 ? sqrt(bb) : 0.0;
-  for (int i = d0 + threadIdx.x; i < d1; i += MR_TPB) m.v[i] = beta1 > 0.0 ? m.Kv[i] / beta1 : 0.0;
+  for (int i = d0 + threadIdx.x; i < d1; i += MR_TPB) {
+    m.v[i] = beta1 > 0.0 ? m.Kv[i] / beta1 : 0.0;
+    m.Kv[i] = 0.0;                    // cleared for the first product
+  }
   if (threadIdx.x == 0) {
     double *q = m.scal + MR_NS * e;
     q[0] = beta1;
@@ -374,6 +377,13 @@ __global__ __launch_bounds__(MR_TPB) void k_minres_begin(d2f_minres m) {
   }
 }
 
+// Scenes of up to MR_TPB * MR_K degrees of freedom keep each thread's entries of v and of the
+// Lanczos vector in registers between the step's three passes, and each pass loads all of a
+// thread's entries before it computes or stores any (indices past the scene are clamped to its
+// first entry and their values unused); larger scenes pass them through Kv in memory. Both give
+// the same values: every entry's arithmetic and each thread's summation order are the same.
+constexpr int MR_K = 16;
+
 __global__ __launch_bounds__(MR_TPB) void k_minres_step(d2f_minres m) {
   __shared__ double red[MR_TPB / WAVE];
   const int e = blockIdx.x;
@@ -384,24 +394,65 @@ __global__ __launch_bounds__(MR_TPB) void k_minres_step(d2f_minres m) {
                sn = q[7];
   const int it = m.status[2 * e + 1] + 1;
   const double c1 = it >= 2 ? beta / oldb : 0.0;
+  const bool regs = d1 - d0 <= MR_TPB * MR_K;
+  double vk[MR_K], yk[MR_K];
   // y = H v - (beta / oldb) r1; alfa = v . y
   double s = 0.0;
-  for (int i = d0 + threadIdx.x; i < d1; i += MR_TPB) {
-    const double y = m.fmask[i] * (m.mass[i >> 1] * m.v[i] + m.Kv[i]) - c1 * m.r1[i];
-    m.Kv[i] = y;
-    s += m.v[i] * y;
+  if (regs) {
+    double kv[MR_K], r1[MR_K], f[MR_K], ms[MR_K];
+#pragma unroll
+    for (int k = 0; k < MR_K; ++k) {
+      const int i = d0 + threadIdx.x + k * MR_TPB, j = i < d1 ? i : d0;
+      vk[k] = m.v[j];
+      kv[k] = m.Kv[j];
+      r1[k] = m.r1[j];
+      f[k] = m.fmask[j];
+      ms[k] = m.mass[j >> 1];
+    }
+#pragma unroll
+    for (int k = 0; k < MR_K; ++k) {
+      yk[k] = f[k] * (ms[k] * vk[k] + kv[k]) - c1 * r1[k];
+      if (d0 + (int)threadIdx.x + k * MR_TPB < d1) s += vk[k] * yk[k];
+    }
+  } else {
+    for (int i = d0 + threadIdx.x; i < d1; i += MR_TPB) {
+      const double y = m.fmask[i] * (m.mass[i >> 1] * m.v[i] + m.Kv[i]) - c1 * m.r1[i];
+      m.Kv[i] = y;
+      s += m.v[i] * y;
+    }
   }
   const double alfa = mr_block_sum(s, red);
   // y -= (alfa / beta) r2; r1 <- r2; r2 <- y; y <- dinv r2; beta' = sqrt(r2 . y)
   s = 0.0;
   const double c2 = alfa / beta;
-  for (int i = d0 + threadIdx.x; i < d1; i += MR_TPB) {
-    const double y = m.Kv[i] - c2 * m.r2[i];
-    m.r1[i] = m.r2[i];
-    m.r2[i] = y;
-    const double z = m.dinv[i] * y;
-    m.Kv[i] = z;
-    s += y * z;
+  if (regs) {
+    double r2[MR_K], dv[MR_K];
+#pragma unroll
+    for (int k = 0; k < MR_K; ++k) {
+      const int i = d0 + threadIdx.x + k * MR_TPB, j = i < d1 ? i : d0;
+      r2[k] = m.r2[j];
+      dv[k] = m.dinv[j];
+    }
+#pragma unroll
+    for (int k = 0; k < MR_K; ++k) {
+      const int i = d0 + threadIdx.x + k * MR_TPB;
+      const double y = yk[k] - c2 * r2[k];
+      yk[k] = dv[k] * y;
+      if (i < d1) {
+        m.r1[i] = r2[k];
+        m.r2[i] = y;
+        s += y * yk[k];
+      }
+    }
+  } else {
+    for (int i = d0 + threadIdx.x; i < d1; i += MR_TPB) {
+      const double y = m.Kv[i] - c2 * m.r2[i];
+      m.r1[i] = m.r2[i];
+      m.r2[i] = y;
+      const double z = m.dinv[i] * y;
+      m.Kv[i] = z;
+      s += y * z;
+    }
   }
   const double bb = mr_block_sum(s, red);
   if (bb < 0.0) {
@@ -421,13 +472,38 @@ __global__ __launch_bounds__(MR_TPB) void k_minres_step(d2f_minres m) {
   const double cs2 = gbar / gamma, sn2 = nb / gamma;
   const double phi = cs2 * phibar;
   const double phibar2 = sn2 * phibar;
-  // w <- (v - oldeps w1 - delta w2) / gamma, (w1, w2) <- (w2, w); x += phi w; v <- y / beta'
-  for (int i = d0 + threadIdx.x; i < d1; i += MR_TPB) {
-    const double w = (m.v[i] - oldeps * m.w1[i] - delta * m.w2[i]) / gamma;
-    m.w1[i] = m.w2[i];
-    m.w2[i] = w;
-    m.x[i] += phi * w;
-    m.v[i] = nb > 0.0 ? m.Kv[i] / nb : 0.0;
+  // w <- (v - oldeps w1 - delta w2) / gamma, (w1, w2) <- (w2, w); x += phi w; v <- y / beta';
+  // Kv cleared for the next product
+  if (regs) {
+    double w1[MR_K], w2[MR_K], xx[MR_K];
+#pragma unroll
+    for (int k = 0; k < MR_K; ++k) {
+      const int i = d0 + threadIdx.x + k * MR_TPB, j = i < d1 ? i : d0;
+      w1[k] = m.w1[j];
+      w2[k] = m.w2[j];
+      xx[k] = m.x[j];
+    }
+#pragma unroll
+    for (int k = 0; k < MR_K; ++k) {
+      const int i = d0 + threadIdx.x + k * MR_TPB;
+      if (i < d1) {
+        const double w = (vk[k] - oldeps * w1[k] - delta * w2[k]) / gamma;
+        m.w1[i] = w2[k];
+        m.w2[i] = w;
+        m.x[i] = xx[k] + phi * w;
+        m.v[i] = nb > 0.0 ? yk[k] / nb : 0.0;
+        m.Kv[i] = 0.0;
+      }
+    }
+  } else {
+    for (int i = d0 + threadIdx.x; i < d1; i += MR_TPB) {
+      const double w = (m.v[i] - oldeps * m.w1[i] - delta * m.w2[i]) / gamma;
+      m.w1[i] = m.w2[i];
+      m.w2[i] = w;
+      m.x[i] += phi * w;
+      m.v[i] = nb > 0.0 ? m.Kv[i] / nb : 0.0;
+      m.Kv[i] = 0.0;
+    }
   }
   if (threadIdx.x == 0) {
     q[1] = beta;

@@ -404,9 +404,10 @@ int d2f_affine_reduce_f64(const double *yf, const double *vf, const double *mass
  * AdjointPNCG2D.adjoint_solve, implicit/adjoint.py:339-369). Scene e solves for the degrees of
  * freedom [2 node_start[e], 2 node_start[e+1]) with its own Lanczos process. All vectors are
  * 2N float64, caller-owned workspace. Protocol: d2f_minres_begin_f64 (x = 0, v = the first
- * Lanczos vector); then per iteration the caller zeroes Kv, adds h2 K v into it with the
- * Hessian-vector entry points, and calls d2f_minres_step_f64, which forms
- * H v = fmask (mass v + Kv) and advances every scene still running. status[2e] is 0 running,
+ * Lanczos vector, Kv = 0); then per iteration the caller adds h2 K v into Kv with the
+ * Hessian-vector entry points (or writes it), and calls d2f_minres_step_f64, which forms
+ * H v = fmask (mass v + Kv), advances every scene still running and leaves that scene's Kv
+ * zero for the next product (a finished scene's Kv is left as it is). status[2e] is 0 running,
  * 1 converged (preconditioned residual <= rtol times the initial one), 2 iteration limit,
  * -1 breakdown (a negative preconditioned norm); status[2e+1] its iterations. */
 typedef struct d2f_minres {
@@ -415,7 +416,7 @@ typedef struct d2f_minres {
   const double *mass;                 /* N */
   const double *fmask, *dinv, *b;     /* 2N: 1/0 free mask, Jacobi preconditioner (0 off free), rhs */
   double *x, *v, *r1, *r2, *w1, *w2;  /* 2N each: solution, Lanczos vector, recurrences */
-  double *Kv;                         /* 2N: h2 K v on entry to a step; scratch inside */
+  double *Kv;                         /* 2N: h2 K v on entry to a step; zero on return */
   double *scal;                       /* 8B Krylov scalars */
   int32_t *status;                    /* 2B */
   double rtol;

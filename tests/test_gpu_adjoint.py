@@ -255,3 +255,35 @@ def test_ipc_and_affine_rollout_checkpointing():
     err = np.abs(scale.grad.numpy() - g).max() / np.abs(g).max()
     print(f"checkpointed rollout vs whole trajectory: {err:.2e}")
     assert err < 1e-9
+
+
+def test_minres_small_and_large_scenes():
+    """The device MINRES keeps a scene's vectors in registers up to 4,096 degrees of freedom
+    and passes them through memory above: one adjoint solve over a batch with a 25 x 97-node
+    strip (4,850 dofs) beside a 7 x 73-node one, against scipy's MINRES on the oracle's
+    operator, and the residual of H lam = rhs on the free degrees of freedom."""
+    from diff_2d_fem_amd import Model
+    m = Model()
+    m.add_ground(0.0, mu=0.8)
+    for env, shape, size in ((0, (24, 96), (0.054, 0.0135)), (1, (6, 72), (0.054, 0.0045))):
+        m.add_soft_body(np.ones(shape, np.uint8), physical_size=size, center=(0.0, 0.009), E=0.5e6, nu=0.40,
+                        density=1100.0, env_id=env)
+    m.finalize()
+    m.gravity = (0.0, -9.81)
+    kw = dict(max_iter=200, eps=0.0, conv_check=0, friction=False)
+    integ = ImplicitIntegrator(path="scene", precision="double", **kw)
+    for _ in range(3):
+        integ.simulate(m, 2e-3)
+    x = m.get_x()
+    adj = integ.adjoint(m, 2e-3)
+    rhs = np.random.default_rng(4).standard_normal(x.shape)
+    lam = adj.adjoint_solve(x, rhs, rtol=1e-12, maxiter=20000)
+    assert getattr(adj, "minres_fail", 0) == 0
+    oi = OracleIntegrator(**kw)
+    oi.attach(m, 2e-3)
+    oa = OracleAdjoint(oi.solver, m.me)
+    lo = oa.adjoint_solve(x, rhs, rtol=1e-12, maxiter=20000)
+    free = adj.free_np.astype(bool)
+    assert np.abs(lam - lo).max() / np.abs(lo).max() < 1e-7
+    r = (oa.apply_H(x, lam) - rhs)[free]
+    assert np.abs(r).max() / np.abs(rhs[free]).max() < 1e-8
