@@ -3,6 +3,7 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3] [--path scene|plain]
                     [--precision mixed|double] [--no-plain] [--no-cpu] [--no-large]
                     [--no-adjoint] [--no-graph] [--no-tune] [--no-c5]
+    python bench.py --table [--precision double|mixed]
 
 The workload is the contact-heavy configuration: 256 scenes of two soft pads on kinematic
 jaws gripping and lifting a rigid block (diff_2d_fem_amd.workloads.pads_lift_block). W
@@ -110,6 +111,62 @@ def c5_leg(n_scenes=256, steps=10, warmup=3, plain=True):
     return out
 
 
+# env-steps/s of the reference's own table (README.md:206-210; RTX A5000, float64, the card about
+# 40 % loaded by another job), at 1, 16, 64, 256, 1024 and 4096 environments
+TABLE_COUNTS = (1, 16, 64, 256, 1024, 4096)
+TABLE_PUBLISHED = {"beam": (27, 353, 666, 909, 1037, 1061), "strip": (14, 187, 472, 678, 751, 744),
+                   "grip": (9, 97, 285, 429, 531, 521)}
+
+
+def env_table(precision="double", counts=TABLE_COUNTS, scenes=("beam", "strip", "grip")):
+    """The reference's throughput table on this build (protocol of tools/benchmark_envs.py:73-93):
+    build the scene copied into n environments, run its lead steps (reaching the named state),
+    synchronize, then time 20 steps (10 from 1024 environments on) including the host command.
+    Scene path, default plan. Per row: ms/step, iterations (mean over steps of the batch's
+    largest count), host command ms/step, env-steps/s and its ratio to the published figure."""
+    import torch
+    from diff_2d_fem_amd import ImplicitIntegrator
+    from diff_2d_fem_amd.workloads import TABLE_SCENES
+    rows = []
+    for name in scenes:
+        for n in counts:
+            t0 = time.perf_counter()
+            m, kw, command, dt, lead = TABLE_SCENES[name](n)
+            integ = ImplicitIntegrator(path="scene", precision=precision, **kw)
+            step = 0
+            for _ in range(lead):
+                step += 1
+                command(step)
+                integ.simulate(m, dt)
+            torch.cuda.synchronize()
+            build_s = time.perf_counter() - t0
+            steps = 10 if n >= 1024 else 20
+            its, t_host = [], 0.0
+            t1 = time.perf_counter()
+            for _ in range(steps):
+                step += 1
+                c0 = time.perf_counter()
+                command(step)
+                t_host += time.perf_counter() - c0
+                integ.simulate(m, dt)
+                its.append(m._solver.iters.max())              # on the device: read after the loop
+            torch.cuda.synchronize()
+            t = (time.perf_counter() - t1) / steps
+            it = float(np.mean([int(v) for v in its]))
+            st = integ.stats(m)
+            pub = TABLE_PUBLISHED[name][TABLE_COUNTS.index(n)] if n in TABLE_COUNTS else None
+            row = dict(scene=name, envs=n, nodes=m.n_nodes, ms_per_step=round(t * 1e3, 3), iterations=round(it, 1),
+                       max_iter=int(st["max_iter"]), host_ms_per_step=round(t_host / steps * 1e3, 3),
+                       env_steps_per_s=round(n / t, 1), ns_per_node_iteration=round(t / it / m.n_nodes * 1e9, 3),
+                       converged=int(np.sum(st["converged"])), lead_and_setup_s=round(build_s, 1),
+                       published_a5000=pub, ratio=None if pub is None else round(n / t / pub, 1))
+            rows.append(row)
+            print(f"table {name} {n} envs: {row['ms_per_step']} ms/step, {row['env_steps_per_s']} env-steps/s",
+                  file=sys.stderr, flush=True)
+            del m, integ
+    return rows
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +184,8 @@ def parse():
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay leg")
     ap.add_argument("--no-tune", action="store_true", help="scene path: keep the default plan")
     ap.add_argument("--no-c5", action="store_true", help="skip the 256-scene star-mesh leg")
+    ap.add_argument("--table", action="store_true",
+                    help="only the reference's throughput table (beam, strip, grip at 1-4096 environments)")
     return ap.parse_args()
 
 
@@ -520,6 +579,11 @@ def main():
         sys.exit(2)
     import torch
     from diff_2d_fem_amd import _lib
+    if args.table:
+        prec = args.precision or "double"
+        print(json.dumps({"table": env_table(prec), "precision": prec, "path": "scene",
+                          "published": "README.md:206-210, RTX A5000, float64"}), flush=True)
+        return
     has_scene = hasattr(_lib.lib(), "d2f_step_scene")
     path = args.path or ("scene" if has_scene else "plain")
     precision = args.precision or ("mixed" if path == "scene" else "double")

@@ -91,14 +91,28 @@ class Mesh:
     boundary_per_env: np.ndarray  # (B,)
     # self-contact exclusion as sorted lists (CSR over the boundary nodes in layout order):
     # boundary node b excludes the scene-local boundary indices excl_idx[excl_ptr[b]:excl_ptr[b+1]]
-    excl_ptr: np.ndarray          # (n_b + 1,) int64
-    excl_idx: np.ndarray          # int32 scene-local boundary indices
+    # (the properties excl_ptr, excl_idx; `excl` holds (ptr (n_b + 1,) int64, idx int32) as of
+    # the last exclusions applied)
+    excl: tuple
     tri_quad: np.ndarray = None   # (T,) body-local quad index of a pixel triangle, -1 for mesh bodies
     body_env: np.ndarray = None
     body_node_range: np.ndarray = None
     body_tri_range: np.ndarray = None
     body_grid: np.ndarray = None
     grid_to_live_node_per_body: list = field(default=None)
+    # exclusions added since `excl` was built: (env, local boundary indices), applied together
+    # the next time the lists are read
+    _excl_pending: list = field(default_factory=list, repr=False)
+
+    @property
+    def excl_ptr(self):
+        self._apply_exclusions()
+        return self.excl[0]
+
+    @property
+    def excl_idx(self):
+        self._apply_exclusions()
+        return self.excl[1]
 
     @property
     def sum_N(self):
@@ -137,19 +151,36 @@ class Mesh:
 
     def add_exclusion(self, env, local):
         """Exclude every pair of scene env's boundary nodes with local indices `local`
-        (each with itself too)."""
+        (each with itself too). Recorded here and applied, with every other exclusion added
+        since, the next time excl_ptr / excl_idx are read: one rebuild of the lists instead of
+        one per call."""
         local = np.unique(np.asarray(local, np.int64))
-        if local.size == 0:
+        if local.size:
+            self._excl_pending.append((int(env), local))
+
+    def _apply_exclusions(self):
+        if not self._excl_pending:
             return
-        b0 = int(self.boundary_starts[env])
-        p, idx = self.excl_ptr, self.excl_idx
-        rows = [idx[p[b]:p[b + 1]] for b in range(len(p) - 1)]
-        for j in local:
-            r = rows[b0 + j]
-            rows[b0 + j] = np.union1d(r, local).astype(np.int32)
-        counts = np.array([len(r) for r in rows], np.int64)
-        self.excl_ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
-        self.excl_idx = np.concatenate(rows).astype(np.int32) if rows else np.zeros(0, np.int32)
+        groups = {}
+        for env, local in self._excl_pending:
+            groups.setdefault(env, []).append(local)
+        self._excl_pending = []
+        p, idx = self.excl
+        counts = np.diff(p)
+        pieces, at = [], 0
+        for env in sorted(groups):
+            b0, b1 = int(self.boundary_starts[env]), int(self.boundary_starts[env + 1])
+            rows = [idx[p[b]:p[b + 1]] for b in range(b0, b1)]
+            for local in groups[env]:
+                for j in local:
+                    rows[j] = np.union1d(rows[j], local).astype(np.int32)
+            pieces.append(idx[p[at]:p[b0]])
+            pieces.extend(rows)
+            counts[b0:b1] = [len(r) for r in rows]
+            at = b1
+        pieces.append(idx[p[at]:])
+        ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        self.excl = (ptr, np.concatenate(pieces).astype(np.int32))
 
     def body_node_index(self, body, ix, iy):
         """Global id of the node at grid corner (ix, iy) of a pixel body, -1 if pruned."""
@@ -567,6 +598,6 @@ def build_scene(bodies, k_excl=2):
                 node_starts=starts([sum(per[h]["n_nodes"] for h in mem) for mem in members]),
                 tri_starts=starts([sum(len(per[h]["rest_area"]) for h in mem) for mem in members]),
                 boundary_starts=starts(b_pe), boundary_per_env=b_pe,
-                excl_ptr=excl_ptr, excl_idx=excl_idx,
+                excl=(excl_ptr, excl_idx),
                 tri_quad=tri_quad, body_env=envs, body_node_range=node_rng, body_tri_range=tri_rng,
                 body_grid=grid, grid_to_live_node_per_body=[per[h]["grid_to_live"] for h in range(nbod)])

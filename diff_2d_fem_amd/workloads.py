@@ -24,6 +24,17 @@ same whether it is built alone or inside a batch.
                     phi_k)), k <= 5, with a target triangle count U[2k, 20k]; 1-3 disjoint
                     round rigid regions; 2 tendons through interior waypoints, tension U[0, 50]
                     N/m; ground mu 0.6, gravity, dt 2 ms.
+
+The scenes of the reference's throughput table (tools/benchmark_envs.py:27-77, README.md:206-210),
+with the parameters of that tool, for the same protocol: TABLE_SCENES[name](n) returns
+(model, integrator keywords, command, dt, lead steps).
+  beam   an 8 x 80 px cantilever (0.040 x 0.004 m, E 1 MPa, nu 0.40, rho 1100, drag 50 /s), its
+         left column held, bent by a tendon through every pixel column at y = 1 mm, tension
+         10 + 20 i / (n - 1) N/m in environment i; no self-contact, no friction; dt 1 ms.
+  strip  strip_on_ground; dt 2 ms.
+  grip   a rigid 16 x 16 px block (pixel 0.5 mm, rho 600) on the ground (mu 0.5) between two
+         soft 16 x 8 px pads (E 1 MPa) whose outer columns are driven: 2.25 px inwards over 40
+         steps, then up 20 um per step; friction between bodies 0.8; dt 1 ms.
 """
 from __future__ import annotations
 
@@ -279,6 +290,66 @@ def star_meshes(n_env=4096, seed=0, R=10e-3, tri_range=(2000, 20000), progress=N
     return dict(model=m, dt=2e-3, command=lambda step: None, bodies=hs, tendons=tendons, size=2 * R,
                 name="star_meshes")
 
+
+def _table_beam(n):
+    m = Model()
+    m.damping = 50.0
+    handles = [m.add_soft_body(np.ones((8, 80), np.uint8), physical_size=(0.040, 0.004), center=(0.0, 0.0),
+                               E=1.0e6, nu=0.40, density=1100.0, env_id=i) for i in range(n)]
+    m.finalize()
+    cables = []
+    for h in handles:
+        X = m.rest_x(h)
+        m.set_kinematic_constraint(h, np.flatnonzero(X[:, 0] < X[:, 0].min() + 1e-9))
+        columns = np.unique(np.round(X[:, 0], 9))
+        cables.append(m.add_tendon(h, through=[(cx, 0.001) for cx in columns]))
+    tension = 10.0 + 20.0 * np.arange(n) / max(n - 1, 1)
+
+    def command(step):
+        for c, t in zip(cables, tension):
+            m.set_tendon_tension(c, float(t))
+    return m, dict(self_collision=False, friction=False), command, 1e-3, 5
+
+
+def _table_strip(n):
+    w = strip_on_ground(n)
+    return w["model"], {}, w["command"], w["dt"], 30
+
+
+def _table_grip(n):
+    px = 0.5e-3
+    m = Model()
+    m.body_friction = 0.8
+    m.add_ground(0.0, mu=0.5)
+    pads = []
+    for i in range(n):
+        m.add_affine_body(np.ones((16, 16), np.uint8), physical_size=(16 * px, 16 * px),
+                          center=(0.0, 8 * px + 0.9 * px / 8), density=600.0, env_id=i)
+        pads.append([m.add_soft_body(np.ones((16, 8), np.uint8), physical_size=(8 * px, 16 * px),
+                                     center=(side * 14 * px, 8 * px + 0.8e-3), E=1.0e6, nu=0.40, density=1100.0,
+                                     env_id=i) for side in (-1, 1)])
+    m.finalize()
+    m.gravity = (0.0, -9.81)
+    rest = []
+    for pair in pads:
+        for side, h in zip((-1, 1), pair):
+            X = m.rest_x(h)
+            back = X[:, 0].min() if side < 0 else X[:, 0].max()
+            m.set_kinematic_constraint(h, np.flatnonzero(np.abs(X[:, 0] - back) < 1e-9))
+            rest.append((side, h, X))
+    close = 40
+
+    def command(step):
+        a, up = min(step / close, 1.0), 20.0e-6 * max(step - close, 0)
+        for side, h, X in rest:
+            Y = X.copy()
+            Y[:, 0] -= side * 2.25 * px * a
+            Y[:, 1] += up
+            m.set_kinematic_target(h, Y)
+    return m, {}, command, 1e-3, close + 5
+
+
+TABLE_SCENES = {"beam": _table_beam, "strip": _table_strip, "grip": _table_grip}
 
 CONFIGS = {"c1": strip_on_ground, "c2": random_blobs, "c3": pads_lift_block, "c4": pneumatic_fingers,
            "c5": star_meshes}
