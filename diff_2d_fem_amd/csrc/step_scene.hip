@@ -1070,6 +1070,170 @@ struct Step {
     }
   }
 
+  // ------------------------------------------------------------------ tendons
+  // Tendon k's eyelets are ey_node[ey_start[k] .. ey_start[k + 1]); segment j joins eyelets j and
+  // j + 1. The tendon passes take their scratch from the boundary accumulators (free once the
+  // boundary pass has read them, until the next gradient pass clears them): per eyelet of the
+  // scene its node and tendon, its outgoing segment's vector and length, and its gradient and
+  // diagonal terms (8 doubles), and per tendon its tension and stiffness (2).
+  __device__ __forceinline__ bool tendon_lds_fits() const {
+    const int ne = s.ey_start[td1] - s.ey_start[td0];
+    return 8 * ne + 2 * (td1 - td0) <= 4 * s.max_nb;
+  }
+
+  // One thread walks the scene's cables in order (the fallback when the scratch does not fit).
+  __device__ __forceinline__ void tendons_grad_serial(double h2) {
+    for (int k = td0; k < td1; ++k) {
+      const int ja = s.ey_start[k], jz = s.ey_start[k + 1];
+      double L = 0.0;
+      for (int j = ja; j + 1 < jz; ++j) L += len(X(s.ey_node[j + 1]) - X(s.ey_node[j]));
+      s.td_L[k] = L;
+      double tau, ke;
+      if (s.td_stiff[k] > 0.0) {
+        const double ext = L - s.td_L0[k];
+        tau = ext > 0.0 ? s.td_stiff[k] * ext : 0.0;
+        ke = ext > 0.0 ? s.td_stiff[k] : 0.0;
+      } else {
+        tau = fmax(s.td_T[k], 0.0);
+        ke = 0.0;
+      }
+      s.td_tau[k] = tau;
+      s.td_keff[k] = ke;
+      for (int j = ja; j < jz; ++j) {
+        const int n = s.ey_node[j];
+        V2<double> gL = {0.0, 0.0}, geo = {0.0, 0.0};
+        const V2<double> xj = X(n);
+        if (j > ja) {
+          const V2<double> t = xj - X(s.ey_node[j - 1]);
+          const double l = fmax(len(t), 1.0e-12), il = 1.0 / l;
+          const V2<double> d = {t.x / l, t.y / l};
+          gL = gL + d;
+          geo = geo + V2<double>{(1.0 - d.x * d.x) * il, (1.0 - d.y * d.y) * il};
+        }
+        if (j + 1 < jz) {
+          const V2<double> t = X(s.ey_node[j + 1]) - xj;
+          const double l = fmax(len(t), 1.0e-12), il = 1.0 / l;
+          const V2<double> d = {t.x / l, t.y / l};
+          gL = gL - d;
+          geo = geo + V2<double>{(1.0 - d.x * d.x) * il, (1.0 - d.y * d.y) * il};
+        }
+        st2(s.grad, n, ld2(s.grad, n) + (h2 * tau) * gL);
+        st2(s.diag, n, ld2(s.diag, n) + V2<double>{h2 * (tau * geo.x + ke * gL.x * gL.x),
+                                                    h2 * (tau * geo.y + ke * gL.y * gL.y)});
+      }
+    }
+  }
+
+  // The same in parallel, with the same values: every thread takes segments, thread 0 sums
+  // each tendon's length in eyelet order and sets its tension, every thread takes eyelets'
+  // terms, then the thread of each node's first eyelet adds the node's terms in eyelet order (a
+  // node can be an eyelet more than once), so every sum is formed in the order of the walk above.
+  __device__ __forceinline__ void tendons_grad_parallel(double h2) {
+    const int E0 = s.ey_start[td0], ne = s.ey_start[td1] - E0;
+    // per eyelet (index j - E0): node, outgoing segment (x, y, length), gradient and diagonal terms
+    LdsD *nd = bacc, *sx = bacc + ne, *sy = bacc + 2 * ne, *sl = bacc + 3 * ne, *cgx = bacc + 4 * ne,
+         *cgy = bacc + 5 * ne, *cdx = bacc + 6 * ne, *cdy = bacc + 7 * ne, *tk = bacc + 8 * ne;
+    for (int k = td0; k < td1; ++k) {
+      const int jz = s.ey_start[k + 1];
+      for (int j = s.ey_start[k] + tid; j < jz; j += TPB) {
+        const int n = s.ey_node[j];
+        nd[j - E0] = double(n);
+        if (j + 1 < jz) {
+          const V2<double> t = X(s.ey_node[j + 1]) - X(n);
+          sx[j - E0] = t.x;
+          sy[j - E0] = t.y;
+          sl[j - E0] = len(t);
+        }
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      for (int k = td0; k < td1; ++k) {
+        const int ja = s.ey_start[k], jz = s.ey_start[k + 1];
+        double L = 0.0;
+        for (int j = ja; j + 1 < jz; ++j) L += sl[j - E0];
+        s.td_L[k] = L;
+        double tau, ke;
+        if (s.td_stiff[k] > 0.0) {
+          const double ext = L - s.td_L0[k];
+          tau = ext > 0.0 ? s.td_stiff[k] * ext : 0.0;
+          ke = ext > 0.0 ? s.td_stiff[k] : 0.0;
+        } else {
+          tau = fmax(s.td_T[k], 0.0);
+          ke = 0.0;
+        }
+        s.td_tau[k] = tau;
+        s.td_keff[k] = ke;
+        tk[2 * (k - td0)] = tau;
+        tk[2 * (k - td0) + 1] = ke;
+      }
+    }
+    __syncthreads();
+    for (int k = td0; k < td1; ++k) {
+      const int ja = s.ey_start[k], jz = s.ey_start[k + 1];
+      const double tau = tk[2 * (k - td0)], ke = tk[2 * (k - td0) + 1];
+      for (int j = ja + tid; j < jz; j += TPB) {
+        V2<double> gL = {0.0, 0.0}, geo = {0.0, 0.0};
+        if (j > ja) {
+          const V2<double> t = {sx[j - 1 - E0], sy[j - 1 - E0]};
+          const double l = fmax(sl[j - 1 - E0], 1.0e-12), il = 1.0 / l;
+          const V2<double> d = {t.x / l, t.y / l};
+          gL = gL + d;
+          geo = geo + V2<double>{(1.0 - d.x * d.x) * il, (1.0 - d.y * d.y) * il};
+        }
+        if (j + 1 < jz) {
+          const V2<double> t = {sx[j - E0], sy[j - E0]};
+          const double l = fmax(sl[j - E0], 1.0e-12), il = 1.0 / l;
+          const V2<double> d = {t.x / l, t.y / l};
+          gL = gL - d;
+          geo = geo + V2<double>{(1.0 - d.x * d.x) * il, (1.0 - d.y * d.y) * il};
+        }
+        const V2<double> cg = (h2 * tau) * gL;
+        cgx[j - E0] = cg.x;
+        cgy[j - E0] = cg.y;
+        cdx[j - E0] = h2 * (tau * geo.x + ke * gL.x * gL.x);
+        cdy[j - E0] = h2 * (tau * geo.y + ke * gL.y * gL.y);
+      }
+    }
+    __syncthreads();
+    for (int j = tid; j < ne; j += TPB) {
+      const double node = nd[j];
+      bool first = true;
+      for (int i = 0; i < j && first; ++i) first = nd[i] != node;
+      if (!first) continue;
+      const int n = int(node);
+      V2<double> g = ld2(s.grad, n), dg = ld2(s.diag, n);
+      for (int i = j; i < ne; ++i) {
+        if (nd[i] != node) continue;
+        g = g + V2<double>{cgx[i], cgy[i]};
+        dg = dg + V2<double>{cdx[i], cdy[i]};
+      }
+      st2(s.grad, n, g);
+      st2(s.diag, n, dg);
+    }
+  }
+
+  // The curvature's tendon terms per segment, for thread 0 to sum in order: along = dp . d,
+  // w = |dp|^2 - along^2 and 1 / l (3 doubles per eyelet in the same scratch).
+  __device__ __forceinline__ void tendon_curv_terms() {
+    const int E0 = s.ey_start[td0], ne = s.ey_start[td1] - E0;
+    LdsD *sa = bacc, *sw = bacc + ne, *si = bacc + 2 * ne;
+    for (int k = td0; k < td1; ++k) {
+      const int jz = s.ey_start[k + 1];
+      for (int j = s.ey_start[k] + tid; j + 1 < jz; j += TPB) {
+        const int na = s.ey_node[j], nbd = s.ey_node[j + 1];
+        const V2<double> t = X(nbd) - X(na);
+        const double l = fmax(len(t), 1.0e-12), il = 1.0 / l;
+        const V2<double> d = {t.x / l, t.y / l};
+        const V2<double> dp = ld2(s.p, nbd) - ld2(s.p, na);
+        const double along = dot(dp, d);
+        sa[j - E0] = along;
+        sw[j - E0] = dot(dp, dp) - along * along;
+        si[j - E0] = il;
+      }
+    }
+  }
+
   // Everything after elasticity: rigid-mode diagonal snapshot, contact, pressure, colliders,
   // friction, tendons, affine and rigid-mode gathers.
   __device__ __forceinline__ void grad_diag_rest() {
@@ -1131,47 +1295,11 @@ struct Step {
       st2(s.diag, n, dg);
     }
     __syncthreads();
-    // tendons: one thread walks the scene's cables (reference implicit/actuation.py:170-226)
-    if (td1 > td0 && tid == 0) {
-      for (int k = td0; k < td1; ++k) {
-        const int ja = s.ey_start[k], jz = s.ey_start[k + 1];
-        double L = 0.0;
-        for (int j = ja; j + 1 < jz; ++j) L += len(X(s.ey_node[j + 1]) - X(s.ey_node[j]));
-        s.td_L[k] = L;
-        double tau, ke;
-        if (s.td_stiff[k] > 0.0) {
-          const double ext = L - s.td_L0[k];
-          tau = ext > 0.0 ? s.td_stiff[k] * ext : 0.0;
-          ke = ext > 0.0 ? s.td_stiff[k] : 0.0;
-        } else {
-          tau = fmax(s.td_T[k], 0.0);
-          ke = 0.0;
-        }
-        s.td_tau[k] = tau;
-        s.td_keff[k] = ke;
-        for (int j = ja; j < jz; ++j) {
-          const int n = s.ey_node[j];
-          V2<double> gL = {0.0, 0.0}, geo = {0.0, 0.0};
-          const V2<double> xj = X(n);
-          if (j > ja) {
-            const V2<double> t = xj - X(s.ey_node[j - 1]);
-            const double l = fmax(len(t), 1.0e-12), il = 1.0 / l;
-            const V2<double> d = {t.x / l, t.y / l};
-            gL = gL + d;
-            geo = geo + V2<double>{(1.0 - d.x * d.x) * il, (1.0 - d.y * d.y) * il};
-          }
-          if (j + 1 < jz) {
-            const V2<double> t = X(s.ey_node[j + 1]) - xj;
-            const double l = fmax(len(t), 1.0e-12), il = 1.0 / l;
-            const V2<double> d = {t.x / l, t.y / l};
-            gL = gL - d;
-            geo = geo + V2<double>{(1.0 - d.x * d.x) * il, (1.0 - d.y * d.y) * il};
-          }
-          st2(s.grad, n, ld2(s.grad, n) + (h2 * tau) * gL);
-          st2(s.diag, n, ld2(s.diag, n) + V2<double>{h2 * (tau * geo.x + ke * gL.x * gL.x),
-                                                      h2 * (tau * geo.y + ke * gL.y * gL.y)});
-        }
-      }
+    // tendons (reference implicit/actuation.py:170-226)
+    if (td1 > td0) {
+      if (tendon_lds_fits()) tendons_grad_parallel(h2);
+      else if (tid == 0) tendons_grad_serial(h2);
+      if (sp1 > sp0) __syncthreads();   // the springs below may share nodes with the eyelets
     }
     // springs and pins: one thread, in order (reference implicit/actuation.py:395-413)
     if (sp1 > sp0 && tid == 0) {
@@ -1564,6 +1692,9 @@ struct Step {
       ipc_c = cm;
       amin = fmin(amin, am);
     }
+    // the tendons' terms per segment, for thread 0 after the reduction (its barriers make them
+    // visible)
+    if (td1 > td0 && tendon_lds_fits()) tendon_curv_terms();
     // one reduction: the sum of pHp, then the minima of -p_inf, alpha, clearance, IPC clearance
     double vr[5] = {ph, -pinf, amin, cmin, ipc_c};
     blk.template reduce<1>(vr);
@@ -1579,17 +1710,28 @@ struct Step {
         const NH<double> h = nh_eval(M2<double>{q[1], q[2], q[4], q[5]}, Mu, La);
         pH += h2 * fmax(nh_quad(h, M2<double>{pq[1], pq[2], pq[4], pq[5]}, Mu, La), 0.0);
       }
+      const bool tpar = td1 > td0 && tendon_lds_fits();
       for (int k = td0; k < td1; ++k) {
         double dL = 0.0;
-        for (int j = s.ey_start[k]; j + 1 < s.ey_start[k + 1]; ++j) {
-          const int na = s.ey_node[j], nbd = s.ey_node[j + 1];
-          const V2<double> t = X(nbd) - X(na);
-          const double l = fmax(len(t), 1.0e-12), il = 1.0 / l;
-          const V2<double> d = {t.x / l, t.y / l};
-          const V2<double> dp = ld2(s.p, nbd) - ld2(s.p, na);
-          const double along = dot(dp, d);
-          dL += along;
-          pH += h2 * s.td_tau[k] * (dot(dp, dp) - along * along) * il;
+        if (tpar) {
+          const int E0 = s.ey_start[td0], ne = s.ey_start[td1] - E0;
+          const LdsD *sa = bacc, *sw = bacc + ne, *si = bacc + 2 * ne;
+          const double tau = s.td_tau[k];
+          for (int j = s.ey_start[k]; j + 1 < s.ey_start[k + 1]; ++j) {
+            dL += sa[j - E0];
+            pH += h2 * tau * sw[j - E0] * si[j - E0];
+          }
+        } else {
+          for (int j = s.ey_start[k]; j + 1 < s.ey_start[k + 1]; ++j) {
+            const int na = s.ey_node[j], nbd = s.ey_node[j + 1];
+            const V2<double> t = X(nbd) - X(na);
+            const double l = fmax(len(t), 1.0e-12), il = 1.0 / l;
+            const V2<double> d = {t.x / l, t.y / l};
+            const V2<double> dp = ld2(s.p, nbd) - ld2(s.p, na);
+            const double along = dot(dp, d);
+            dL += along;
+            pH += h2 * s.td_tau[k] * (dot(dp, dp) - along * along) * il;
+          }
         }
         pH += h2 * s.td_keff[k] * dL * dL;
       }

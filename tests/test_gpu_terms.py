@@ -97,6 +97,35 @@ def test_tendons(path):
     _compare(tendon_strip, 5, 2e-3, path, FIXED, 0.03)
 
 
+def tendon_out_and_back(n_pts):
+    """Two tendons per scene along the held beam's lower edge, the second out and back (its
+    eyelets repeat nodes, and share nodes with the first): with 8 points the scene kernel's
+    parallel tendon pass runs, with 40 its scratch does not fit and one thread walks them."""
+    def make():
+        m = Model()
+        hs = [m.add_soft_body(np.ones((5, 40), np.uint8), physical_size=(0.03, 0.00375),
+                              center=(0.0, 0.02), E=0.3e6, nu=0.40, density=1100.0, env_id=i) for i in range(2)]
+        m.finalize()
+        m.gravity = (0.0, -9.81)
+        out = np.stack([np.linspace(-0.014, 0.0145, n_pts), np.full(n_pts, 0.0185)], 1)
+        for i, h in enumerate(hs):
+            _pin_left(m, h)
+            t = m.add_tendon(h, through=out[: n_pts // 2])
+            m.set_tendon_tension(t, 20.0)
+            t = m.add_tendon(h, through=np.concatenate([out, out[-2::-1]]), stiffness=1.5e3)
+            m.set_tendon_length(t, (0.97 - 0.02 * i) * m.tendon_length(t))
+        return m
+    return make
+
+
+@pytest.mark.parametrize("n_pts", [8, 40])
+def test_tendons_repeated_eyelets(n_pts):
+    make = tendon_out_and_back(n_pts)
+    ey = np.asarray(make().actuation.tendons[1].nodes)
+    assert len(np.unique(ey)) < len(ey)                 # the out-and-back cable repeats nodes
+    _compare(make, 5, 2e-3, "scene", FIXED, 0.03)
+
+
 def pushed_block(n_env=2):
     """A soft block on the ground pushed sideways by an external force, with gravity."""
     m = Model()
