@@ -118,11 +118,12 @@ TABLE_PUBLISHED = {"beam": (27, 353, 666, 909, 1037, 1061), "strip": (14, 187, 4
                    "grip": (9, 97, 285, 429, 531, 521)}
 
 
-def env_table(precision="double", counts=TABLE_COUNTS, scenes=("beam", "strip", "grip")):
+def env_table(precision="double", counts=TABLE_COUNTS, scenes=("beam", "strip", "grip"), tune=True):
     """The reference's throughput table on this build (protocol of tools/benchmark_envs.py:73-93):
     build the scene copied into n environments, run its lead steps (reaching the named state),
     synchronize, then time 20 steps (10 from 1024 environments on) including the host command.
-    Scene path, default plan. Per row: ms/step, iterations (mean over steps of the batch's
+    Scene path; with `tune` the autotuner picks the launch shape and LDS plan when the solver is
+    built (inside the untimed lead). Per row: ms/step, iterations (mean over steps of the batch's
     largest count), host command ms/step, env-steps/s and its ratio to the published figure."""
     import torch
     from diff_2d_fem_amd import ImplicitIntegrator
@@ -132,7 +133,7 @@ def env_table(precision="double", counts=TABLE_COUNTS, scenes=("beam", "strip", 
         for n in counts:
             t0 = time.perf_counter()
             m, kw, command, dt, lead = TABLE_SCENES[name](n)
-            integ = ImplicitIntegrator(path="scene", precision=precision, **kw)
+            integ = ImplicitIntegrator(path="scene", precision=precision, autotune=tune, **kw)
             step = 0
             for _ in range(lead):
                 step += 1
@@ -159,6 +160,7 @@ def env_table(precision="double", counts=TABLE_COUNTS, scenes=("beam", "strip", 
                        max_iter=int(st["max_iter"]), host_ms_per_step=round(t_host / steps * 1e3, 3),
                        env_steps_per_s=round(n / t, 1), ns_per_node_iteration=round(t / it / m.n_nodes * 1e9, 3),
                        converged=int(np.sum(st["converged"])), lead_and_setup_s=round(build_s, 1),
+                       block_size=int(m._solver.block_size),
                        published_a5000=pub, ratio=None if pub is None else round(n / t / pub, 1))
             rows.append(row)
             print(f"table {name} {n} envs: {row['ms_per_step']} ms/step, {row['env_steps_per_s']} env-steps/s",
@@ -581,7 +583,8 @@ def main():
     from diff_2d_fem_amd import _lib
     if args.table:
         prec = args.precision or "double"
-        print(json.dumps({"table": env_table(prec), "precision": prec, "path": "scene",
+        print(json.dumps({"table": env_table(prec, tune=not args.no_tune), "precision": prec, "path": "scene",
+                          "tuned": not args.no_tune,
                           "published": "README.md:206-210, RTX A5000, float64"}), flush=True)
         return
     has_scene = hasattr(_lib.lib(), "d2f_step_scene")

@@ -2012,7 +2012,7 @@ constexpr size_t PM_OFFSET = (sizeof(SH) + alignof(Pm) - 1) / alignof(Pm) * alig
 static_assert(alignof(SH) == 8 && alignof(Pm) == 8, "kernel-argument layout");
 
 template <typename R, int TPB>
-__global__ __launch_bounds__(TPB) void k_scene(SH s, Pm p) {
+__global__ __launch_bounds__(TPB, TPB == 256 ? 2 : 1) void k_scene(SH s, Pm p) {
   extern __shared__ double lds[];
   __shared__ SceneScalars sc;
   // The arguments are read in place from the kernel-argument segment (s at offset 0, p after
