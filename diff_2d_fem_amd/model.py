@@ -89,7 +89,7 @@ class Model:
         self._constrained = {}              # handle -> global ids of its prescribed nodes
         self.me = None                      # the flat scene mesh (finalize)
         self._solver = None                 # the device solver (ImplicitIntegrator.attach)
-        self._node_range, self._tri_range = [], []
+        self._node_range, self._tri_range, self._node_slice = [], [], []
         self._targets = self._x_host = self._v0 = None
         self._epoch = 0
         self.time = 0.0
@@ -271,6 +271,7 @@ class Model:
         reach = self.cfg.ipc_d_hat if self.cfg.ipc_d_hat is not None else dx / 8.0
         self.me = me = build_scene([b.scene_entry(max(2.0 * dx, 16.0 * reach)) for b in self._bodies])
         self._node_range = [np.arange(lo, hi) for lo, hi in me.body_node_range]
+        self._node_slice = [slice(int(lo), int(hi)) for lo, hi in me.body_node_range]
         self._tri_range = [np.arange(lo, hi) for lo, hi in me.body_tri_range]
         for h, b in enumerate(self._bodies):          # stiffness maps scale mu and lambda together
             if not b.uniform:
@@ -486,7 +487,7 @@ class Model:
 
     def set_kinematic_target(self, handle, targets):
         """Where a body's nodes are to be at the end of the next step ((n_body_nodes, 2))."""
-        self._targets[self._node_range[handle]] = np.asarray(targets, np.float64)
+        self._targets[self._node_slice[handle]] = targets
 
     def set_node_targets(self, nodes, targets):
         """Targets of any nodes, by global id ((n, 2)), in one call: set_kinematic_target for
