@@ -169,35 +169,39 @@ class AdjointPNCG2D:
             cap = 2 * n
         self._pair_cap, self._n_pairs = cap, n
 
-    def _kv(self, xw, vw, y, use_disc=False, use_sdf=False, use_self=False, use_ipc=None):
-        """y += h^2 K v at xw (elastic, selected colliders, node contact, IPC barrier between
-        bodies with the pairs of the last _set_pairs, pressure)."""
+    def _kv_calls(self, xw, vw, y, use_disc=False, use_sdf=False, use_self=False, use_ipc=None):
+        """The entry-point calls that form y += h^2 K v at xw (elastic, selected colliders, node
+        contact, IPC barrier between bodies with the pairs of the last _set_pairs, pressure), as
+        (function, bound ctypes arguments, name): the Krylov loop binds them once per solve."""
         L = _lib.lib()
         s = self.s
-        _lib.check(L.d2f_elastic_hvp_f64(self._p(xw), self._p(vw), self._p(s.tri), self._p(s.Bm),
-                                         self._p(s.area), self._p(self._mu_el), self._p(self._lam_el),
-                                         self.n_tri, self.h2, C.byref(self._tiles), self._p(y), self._stream()),
-                   "d2f_elastic_hvp_f64")
+        P, st = self._p, self._stream()
+        X, V, Y = P(xw), P(vw), P(y)
+        calls = [(L.d2f_elastic_hvp_f64, (X, V, P(s.tri), P(s.Bm), P(s.area), P(self._mu_el), P(self._lam_el),
+                                          self.n_tri, self.h2, C.byref(self._tiles), Y, st), "d2f_elastic_hvp_f64")]
         which = 2 if (use_disc and use_sdf) else (0 if use_disc else 1)
         if (use_disc or use_sdf) and self._n_col:
-            _lib.check(L.d2f_collider_hvp_f64(self._p(xw), self._p(vw), self._p(s.bnd), s.n_b, self._cols,
-                                              self._n_col, which, float(s.cfg.ipc_d_hat),
-                                              float(s.cfg.obstacle_stiffness), self.h2, self._p(y),
-                                              self._stream()), "d2f_collider_hvp_f64")
+            calls.append((L.d2f_collider_hvp_f64, (X, V, P(s.bnd), s.n_b, self._cols, self._n_col, which,
+                                                   float(s.cfg.ipc_d_hat), float(s.cfg.obstacle_stiffness), self.h2,
+                                                   Y, st), "d2f_collider_hvp_f64"))
         if self._node_on(use_self):           # reference :239-245, the pairs frozen at x
             nd = s.node
-            _lib.check(L.d2f_node_hvp_f64(self._p(xw), self._p(vw), self._p(s.bnd), self._p(s.bnd_start), s.B,
-                                          self._p(s.excl_bits), self._p(s.excl_bits_off), nd["r"], nd["k"], self.h2,
-                                          self._p(y), self._stream()), "d2f_node_hvp_f64")
+            calls.append((L.d2f_node_hvp_f64, (X, V, P(s.bnd), P(s.bnd_start), s.B, P(s.excl_bits),
+                                               P(s.excl_bits_off), nd["r"], nd["k"], self.h2, Y, st),
+                          "d2f_node_hvp_f64"))
         if self._ipc_on(use_ipc) and self._n_pairs:
             ip = s.ipc
-            _lib.check(L.d2f_ipc_hvp_f64(self._p(xw), self._p(vw), self._p(self._pairs), self._n_pairs,
-                                         self._p(s.ea), self._p(s.eb), self._p(s.ca), self._p(s.cb), ip["dhat"],
-                                         ip["kappa"], self.h2, self._p(y), self._stream()), "d2f_ipc_hvp_f64")
+            calls.append((L.d2f_ipc_hvp_f64, (X, V, P(self._pairs), self._n_pairs, P(s.ea), P(s.eb), P(s.ca),
+                                              P(s.cb), ip["dhat"], ip["kappa"], self.h2, Y, st), "d2f_ipc_hvp_f64"))
         if s.pressure is not None:
-            _lib.check(L.d2f_pressure_hvp_f64(self._p(vw), self._p(s.pe_a), self._p(s.pe_b), self._p(s.pe_ch),
-                                              self._p(s.ch_p), s.pressure["n_pe"], self.h2, self._p(y),
-                                              self._stream()), "d2f_pressure_hvp_f64")
+            calls.append((L.d2f_pressure_hvp_f64, (V, P(s.pe_a), P(s.pe_b), P(s.pe_ch), P(s.ch_p),
+                                                   s.pressure["n_pe"], self.h2, Y, st), "d2f_pressure_hvp_f64"))
+        return calls
+
+    def _kv(self, xw, vw, y, use_disc=False, use_sdf=False, use_self=False, use_ipc=None):
+        """y += h^2 K v at xw (the calls of _kv_calls)."""
+        for fn, args, name in self._kv_calls(xw, vw, y, use_disc, use_sdf, use_self, use_ipc):
+            _lib.check(fn(*args), name)
         return y
 
     # ---- the affine reduction (reduced vectors: soft entries in place, q in three slot nodes)
@@ -321,14 +325,22 @@ class AdjointPNCG2D:
             setattr(m, k, self._p(ws[k]))
         st = self._stream()
         _lib.check(L.d2f_minres_begin_f64(C.byref(m), st), "d2f_minres_begin_f64")
+        # Kv left cleared by the previous begin / step: the products add into it
+        calls = None if red else self._kv_calls(xw, ws["v"], ws["Kv"], **use)
+        step, mref = L.d2f_minres_step_f64, C.byref(m)
         done = 0
         while done < maxiter:
             for _ in range(min(check_every, maxiter - done)):
                 if red:
                     self._kv_reduced(xw, ws["v"], ws["Kv"], **use)
-                else:                        # Kv left cleared by the previous begin / step
-                    self._kv(xw, ws["v"], ws["Kv"], **use)
-                _lib.check(L.d2f_minres_step_f64(C.byref(m), st), "d2f_minres_step_f64")
+                else:
+                    for fn, args, name in calls:
+                        rc = fn(*args)
+                        if rc:
+                            _lib.check(rc, name)
+                rc = step(mref, st)
+                if rc:
+                    _lib.check(rc, "d2f_minres_step_f64")
             done += check_every
             if not bool((ws["status"][0::2] == 0).any()):
                 break
