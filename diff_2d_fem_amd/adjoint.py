@@ -85,6 +85,7 @@ class AdjointPNCG2D:
             self._setup_reduction()
         self._mu_el, self._lam_el = self._t(mu_el), self._t(lam_el)
         self._pairs = None                   # IPC pair list at the state being solved
+        self.minres_graph = True             # replay each solve's Krylov chunks as a HIP graph
         self._n_pairs = 0
 
     def _setup_reduction(self):
@@ -325,25 +326,42 @@ class AdjointPNCG2D:
             setattr(m, k, self._p(ws[k]))
         st = self._stream()
         _lib.check(L.d2f_minres_begin_f64(C.byref(m), st), "d2f_minres_begin_f64")
-        # Kv left cleared by the previous begin / step: the products add into it
-        calls = None if red else self._kv_calls(xw, ws["v"], ws["Kv"], **use)
         step, mref = L.d2f_minres_step_f64, C.byref(m)
-        done = 0
-        while done < maxiter:
-            for _ in range(min(check_every, maxiter - done)):
-                if red:
+
+        def chunk(stream, calls):
+            # check_every iterations: the products add into Kv (left cleared by the previous begin
+            # or step), then the Krylov step; a scene past its limit or converged is left alone
+            for _ in range(check_every):
+                if calls is None:
                     self._kv_reduced(xw, ws["v"], ws["Kv"], **use)
                 else:
                     for fn, args, name in calls:
                         rc = fn(*args)
                         if rc:
                             _lib.check(rc, name)
-                rc = step(mref, st)
+                rc = step(mref, stream)
                 if rc:
                     _lib.check(rc, "d2f_minres_step_f64")
+        graph = None
+        if not red and self.minres_graph:
+            # the chunk's launches captured once per solve (every pointer and scalar they take is
+            # fixed for the solve) and replayed: the loop is bound by kernel time, not by the
+            # host's launches
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph):
+                cs = self._stream()
+                chunk(cs, self._kv_calls(xw, ws["v"], ws["Kv"], **use))
+        calls = None if red else self._kv_calls(xw, ws["v"], ws["Kv"], **use)
+        done = 0
+        while done < maxiter:
+            if graph is not None:
+                graph.replay()
+            else:
+                chunk(st, calls)
             done += check_every
             if not bool((ws["status"][0::2] == 0).any()):
                 break
+        del graph
         status = ws["status"].view(-1, 2).cpu().numpy()
         return ws["x"].clone(), status[:, 0].copy(), status[:, 1].copy()
 
