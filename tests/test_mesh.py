@@ -123,3 +123,19 @@ def test_element_tiles_layout():
             assert owned[off[c] + k] == (chunks_at[n] == {c}) == (k < no)
         assert np.array_equal(lst[loc[c * ch:(c + 1) * ch]], sub)
     assert et["max_nodes"] == int(np.diff(off).max())
+
+
+def test_reference_table_scenes_match_the_published_sizes():
+    """The throughput-table scenes (workloads.TABLE_SCENES, after the reference's
+    tools/benchmark_envs.py) give the node counts and iteration budgets the reference's README
+    quotes for them (README.md:206-208: beam 729 nodes and 177 iterations per step, strip 511
+    and 167, the gripped block 595 nodes)."""
+    from diff_2d_fem_amd import ImplicitIntegrator
+    from diff_2d_fem_amd.workloads import TABLE_SCENES
+    want = {"beam": (729, 177), "strip": (511, 167), "grip": (595, 177)}
+    for name, build in TABLE_SCENES.items():
+        m, kw, command, dt, lead = build(2)
+        command(1)
+        command(lead)
+        assert m.n_nodes == 2 * want[name][0], name
+        assert ImplicitIntegrator(**kw)._iters_for(m, dt) == want[name][1], name
