@@ -2,7 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3] [--path scene|plain]
                     [--precision mixed|double] [--no-plain] [--no-cpu] [--no-large]
-                    [--no-adjoint] [--no-graph] [--no-tune] [--no-c5]
+                    [--no-adjoint] [--no-graph] [--no-tune] [--no-c5] [--no-table]
     python bench.py --table [--precision double|mixed]
 
 The workload is the contact-heavy configuration: 256 scenes of two soft pads on kinematic
@@ -27,7 +27,9 @@ launch) beside the eager per-step loop; `tuning` the scene kernel's plan that th
 chose from the warmed-up state (--no-tune keeps the default); `c5` the large uneven batch at 256
 star-mesh scenes (2k-20k triangles each; mixed precision, tuned, eager and graph-replayed, and
 the per-kernel float64 path on the same steps: a batch whose scenes converge at different
-iterations); `cpu_baseline` times the float64 CPU oracle on a bounded sample of the same workload.
+iterations); `reference_table` the reference's own throughput scenes (beam, strip, gripped
+block) at 1 and 4096 environments beside its published figures (--table runs every count);
+`cpu_baseline` times the float64 CPU oracle on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -188,6 +190,8 @@ def parse():
     ap.add_argument("--no-c5", action="store_true", help="skip the 256-scene star-mesh leg")
     ap.add_argument("--table", action="store_true",
                     help="only the reference's throughput table (beam, strip, grip at 1-4096 environments)")
+    ap.add_argument("--no-table", action="store_true",
+                    help="skip the reference table's rows at 1 and 4096 environments (mixed, tuned)")
     return ap.parse_args()
 
 
@@ -449,7 +453,8 @@ def adjoint_leg(n_fingers=512, T=10, every=5):
     to every chamber pressure and every element's modulus scale through the autograd rollout
     (checkpoints every `every` steps, recomputation in the backward pass, float64, per-scene
     device MINRES at the reference's rtol 1e-9). Times the forward pass and the backward pass
-    (recomputation + one adjoint solve per step) with synchronisation on both sides."""
+    (recomputation + one adjoint solve per step) with synchronisation on both sides, after an
+    untimed 2-step rollout from the same initial state."""
     import torch
     from diff_2d_fem_amd import ImplicitIntegrator
     from diff_2d_fem_amd import workloads as W
@@ -460,6 +465,14 @@ def adjoint_leg(n_fingers=512, T=10, every=5):
                            device="cuda")
     targets = torch.as_tensor(w["targets"], device="cuda")
     integ = ImplicitIntegrator(path="scene", precision="double", self_collision=False)
+    # untimed warm-up: a 2-step rollout through its forward and backward passes builds the scene
+    # layout, loads the kernels and sizes the allocator; then the model goes back to rest
+    warm = Rollout(m, integ, w["dt"], 2, every=1, adjoint_kw=dict(rtol=1e-9, maxiter=5000))
+    ws_ = torch.ones(m.n_elements, dtype=torch.float64, requires_grad=True)
+    wp_ = torch.tensor([c.pressure for c in m.actuation.chambers], requires_grad=True)
+    ((warm(ws_, wp_)[tips] - targets) ** 2).sum().backward()
+    del warm
+    m.reset()
     roll = Rollout(m, integ, w["dt"], T, every=every, adjoint_kw=dict(rtol=1e-9, maxiter=5000))
     scale = torch.ones(m.n_elements, dtype=torch.float64, requires_grad=True)
     p = torch.tensor([c.pressure for c in m.actuation.chambers], requires_grad=True)
@@ -609,6 +622,10 @@ def main():
         "max_iter": int(main_run["stats"]["max_iter"]),
         "converged_frac": round(float(np.mean(main_run["stats"]["converged"])), 4),
     }
+    if not args.no_adjoint:
+        import gc
+        gc.collect()
+        out["adjoint_c4"] = adjoint_leg()
     tuned = main_run["tuned"]
     if tuned is not None:
         out["tuning"] = {k: tuned[k] for k in ("block_size", "max_cells", "skin", "lds_order", "nacc_lds",
@@ -630,10 +647,14 @@ def main():
         pv = B * args.steps / plain["t"]
         out["plain_fp64"] = dict(value=round(pv, 2), ms_per_step=round(plain["t"] / args.steps * 1e3, 3))
         out["speedup_vs_plain_fp64"] = round(value / pv, 3)
-    if not args.no_adjoint:
-        out["adjoint_c4"] = adjoint_leg()
     if not args.no_c5:
         out["c5"] = c5_leg(plain=not args.no_plain)
+    if not args.no_table:
+        rows = env_table("mixed", counts=(1, 4096))
+        out["reference_table"] = dict(
+            precision="mixed", published="README.md:206-210, RTX A5000, float64",
+            rows=[{k: r[k] for k in ("scene", "envs", "ms_per_step", "iterations", "env_steps_per_s",
+                                      "published_a5000", "ratio")} for r in rows])
     if not args.no_cpu:
         out["cpu_baseline"], xs = cpu_baseline(args, args.cpu_steps)
         out["pos_err_vs_oracle"] = oracle_error(args, xs, path, precision)
