@@ -85,7 +85,6 @@ class AdjointPNCG2D:
             self._setup_reduction()
         self._mu_el, self._lam_el = self._t(mu_el), self._t(lam_el)
         self._pairs = None                   # IPC pair list at the state being solved
-        self.minres_graph = True             # replay each solve's Krylov chunks as a HIP graph
         self._n_pairs = 0
 
     def _setup_reduction(self):
@@ -342,26 +341,13 @@ class AdjointPNCG2D:
                 rc = step(mref, stream)
                 if rc:
                     _lib.check(rc, "d2f_minres_step_f64")
-        graph = None
-        if not red and self.minres_graph:
-            # the chunk's launches captured once per solve (every pointer and scalar they take is
-            # fixed for the solve) and replayed: the loop is bound by kernel time, not by the
-            # host's launches
-            graph = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(graph):
-                cs = self._stream()
-                chunk(cs, self._kv_calls(xw, ws["v"], ws["Kv"], **use))
         calls = None if red else self._kv_calls(xw, ws["v"], ws["Kv"], **use)
         done = 0
         while done < maxiter:
-            if graph is not None:
-                graph.replay()
-            else:
-                chunk(st, calls)
+            chunk(st, calls)
             done += check_every
             if not bool((ws["status"][0::2] == 0).any()):
                 break
-        del graph
         status = ws["status"].view(-1, 2).cpu().numpy()
         return ws["x"].clone(), status[:, 0].copy(), status[:, 1].copy()
 
