@@ -326,7 +326,8 @@ def scene_roofline(res, precision):
     bytes_ = it_per_env * (2.0 * (12 + 6 * se) * s.n_tri + 272.0 * s.N)
     dur = res["launch_ms"] * 1e-3
     ach = bytes_ / dur / 1e9
-    name = f"d2f::k_scene<{'float' if precision == 'mixed' else 'double'}, {s.block_size}>"
+    name = (f"d2f::k_scene<{'float' if precision == 'mixed' else 'double'}, {s.block_size}, "
+            f"{'true' if (getattr(s, 'tendon', None) or {}).get('n_td', 0) > 0 else 'false'}>")
     return dict(kernel=f"k_scene ({precision})", bound="hbm", achieved=round(ach, 1),
                 peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
                 traffic=measured_traffic(name, res),

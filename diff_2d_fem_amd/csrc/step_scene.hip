@@ -334,7 +334,9 @@ __device__ __forceinline__ V2<double> up(V2<R> v) {
 }
 
 // ----------------------------------------------------------------------------- the kernel
-template <typename R, int TPB>
+// TD: the launch has tendons. Without them td0 = td1 = 0 is a constant, and every tendon pass
+// compiles away (their code weighs on the register allocation of the whole kernel).
+template <typename R, int TPB, bool TD>
 struct Step {
   S &s;
   KPm &p;
@@ -402,8 +404,8 @@ struct Step {
     r1 = s.n_rg > 0 ? s.rg_start[e + 1] : 0;
     pe0 = s.n_pe > 0 ? s.pe_start[e] : 0;
     pe1 = s.n_pe > 0 ? s.pe_start[e + 1] : 0;
-    td0 = s.n_td > 0 ? s.td_start[e] : 0;
-    td1 = s.n_td > 0 ? s.td_start[e + 1] : 0;
+    td0 = TD && s.n_td > 0 ? s.td_start[e] : 0;
+    td1 = TD && s.n_td > 0 ? s.td_start[e + 1] : 0;
     sp0 = s.n_sp > 0 ? s.sp_start[e] : 0;
     sp1 = s.n_sp > 0 ? s.sp_start[e + 1] : 0;
     eoff = s.excl_bits_off ? s.excl_bits_off[e] : 0;
@@ -2011,7 +2013,7 @@ struct Step {
 constexpr size_t PM_OFFSET = (sizeof(SH) + alignof(Pm) - 1) / alignof(Pm) * alignof(Pm);
 static_assert(alignof(SH) == 8 && alignof(Pm) == 8, "kernel-argument layout");
 
-template <typename R, int TPB>
+template <typename R, int TPB, bool TD>
 __global__ __launch_bounds__(TPB, TPB == 256 ? 2 : 1) void k_scene(SH s, Pm p) {
   extern __shared__ double lds[];
   __shared__ SceneScalars sc;
@@ -2020,14 +2022,14 @@ __global__ __launch_bounds__(TPB, TPB == 256 ? 2 : 1) void k_scene(SH s, Pm p) {
   // private memory, and a copy in LDS would hide that its pointers are global.
   S *ks = (S *)__builtin_amdgcn_kernarg_segment_ptr();
   KPm *kp = (KPm *)((const __attribute__((address_space(4))) char *)ks + PM_OFFSET);
-  Step<R, TPB> st(*ks, *kp, *(LdsScalars *)&sc, (LdsD *)lds);
+  Step<R, TPB, TD> st(*ks, *kp, *(LdsScalars *)&sc, (LdsD *)lds);
   st.run();
 }
 
-template <typename R, int TPB>
-static int launch(const SH &s, const Pm &p, hipStream_t stream) {
+template <typename R, int TPB, bool TD>
+static int launch_td(const SH &s, const Pm &p, hipStream_t stream) {
   const size_t lds = sizeof(double) * (Block<TPB>::RED_SIZE + 8 * (size_t)s.max_nb +
-                                       Step<R, TPB>::GST * (size_t)s.max_ag +
+                                       Step<R, TPB, TD>::GST * (size_t)s.max_ag +
                                        (s.nacc_lds ? 4 * (size_t)s.max_nn : 0) +
                                        (s.xc_lds ? 2 * (size_t)s.max_nn : 0)) +
                      sizeof(int) * (2 * (size_t)s.max_cells + 1 + 2 * (size_t)s.max_nb + (size_t)s.excl_bits_lds +
@@ -2043,16 +2045,21 @@ static int launch(const SH &s, const Pm &p, hipStream_t stream) {
     int dev = 0;
     D2F_HIP(hipGetDevice(&dev));
     if (dev < 0 || dev >= 64 || lds_set[dev].load() < (int)lds) {
-      D2F_HIP(hipFuncSetAttribute((const void *)k_scene<R, TPB>,
+      D2F_HIP(hipFuncSetAttribute((const void *)k_scene<R, TPB, TD>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
       if (dev >= 0 && dev < 64) lds_set[dev].store((int)lds);
     }
   }
   const int n_wg = s.sc_list ? s.n_list : s.B;
   if (n_wg == 0) return 0;
-  hipLaunchKernelGGL((k_scene<R, TPB>), dim3(n_wg), dim3(TPB), lds, stream, s, p);
+  hipLaunchKernelGGL((k_scene<R, TPB, TD>), dim3(n_wg), dim3(TPB), lds, stream, s, p);
   D2F_HIP(hipGetLastError());
   return 0;
+}
+
+template <typename R, int TPB>
+static int launch(const SH &s, const Pm &p, hipStream_t stream) {
+  return s.n_td > 0 ? launch_td<R, TPB, true>(s, p, stream) : launch_td<R, TPB, false>(s, p, stream);
 }
 
 // The float and float64 kernels are compiled as two objects of this file (D2F_SCENE_PART 0 and
