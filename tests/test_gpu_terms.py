@@ -308,3 +308,31 @@ def test_cantilever_reaches_reference_static_equilibrium(path, precision, tol):
     err = np.abs(m.get_x() - f["cant_x"]).max() / 0.02
     print(f"{path} {precision}: |x - x_static| / length = {err:.3e}")
     assert err < tol
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+@pytest.mark.parametrize("name", ["beam", "grip"])
+def test_reference_table_scenes(name, precision):
+    """Two scenes of the reference's throughput table (workloads.TABLE_SCENES, after its
+    tools/benchmark_envs.py) on the scene path against the oracle: the beam bent by a tendon
+    through all 81 pixel columns (the kernel's parallel tendon pass; a different tension per
+    environment), and the pads gripping the rigid block (IPC with friction between bodies,
+    driven pads). Float64 within 1e-9 of the body length, mixed within 1e-5."""
+    from diff_2d_fem_amd.workloads import TABLE_SCENES
+    mo, kw, cmd_o, dt, _ = TABLE_SCENES[name](2)
+    mg, _, cmd_g, _, _ = TABLE_SCENES[name](2)
+    kw = dict(kw, **FIXED)
+    oi = OracleIntegrator(**kw)
+    gi = ImplicitIntegrator(path="scene", precision=precision, **kw)
+    for k in range(1, 4):
+        cmd_o(k)
+        cmd_g(k)
+        oi.simulate(mo, dt)
+        gi.simulate(mg, dt)
+    size = np.ptp(mo.me.rest_pos[:, 0])
+    err = np.abs(mg.get_x() - oi.solver.x).max() / size
+    moved = np.abs(oi.solver.x - mo.me.rest_pos).max() / size
+    print(f"{name} {precision}: max |dx| / size = {err:.3e} (motion {moved:.3e})", flush=True)
+    assert moved > 1e-5
+    assert err < (1e-9 if precision == "double" else 1e-5)
+    assert (gi.stats(mg)["iters"] == oi.stats()["iters"]).all()
