@@ -124,6 +124,7 @@ def lib():
     L.d2f_inversion_alpha_f64.argtypes = [P, P, P, P, i32, f64, P, P]
     L.d2f_elastic_energy_f64.argtypes = [P, P, P, P, P, P, P, i32, f64, P, P]
     L.d2f_elastic_hvp_f64.argtypes = [P, P, P, P, P, P, P, i32, f64, P, P, P]
+    L.d2f_elastic_pressure_hvp_f64.argtypes = [P, P, P, P, P, P, i32, f64, P, P, P, P, P, P, P]
     L.d2f_collider_force.argtypes = [C.POINTER(Scene), C.POINTER(Params), i32, i32, P, P]
     L.d2f_contact_force.argtypes = [C.POINTER(Scene), C.POINTER(Params), i32, i32, i32, P, P, P, P]
     L.d2f_collider_hvp_f64.argtypes = [P, P, P, i32, P, i32, i32, f64, f64, f64, P, P]

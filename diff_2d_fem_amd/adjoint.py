@@ -177,8 +177,16 @@ class AdjointPNCG2D:
         s = self.s
         P, st = self._p, self._stream()
         X, V, Y = P(xw), P(vw), P(y)
-        calls = [(L.d2f_elastic_hvp_f64, (X, V, P(s.tri), P(s.Bm), P(s.area), P(self._mu_el), P(self._lam_el),
-                                          self.n_tri, self.h2, C.byref(self._tiles), Y, st), "d2f_elastic_hvp_f64")]
+        pt = self._pressure_tiles() if s.pressure is not None else None
+        if pt is not None:                   # elasticity and pressure in one tiled pass
+            calls = [(L.d2f_elastic_pressure_hvp_f64,
+                      (X, V, P(s.Bm), P(s.area), P(self._mu_el), P(self._lam_el), self.n_tri, self.h2,
+                       C.byref(self._tiles), P(pt["pe_off"]), P(pt["pe_loc"]), P(pt["pe_ch"]), P(s.ch_p), Y, st),
+                      "d2f_elastic_pressure_hvp_f64")]
+        else:
+            calls = [(L.d2f_elastic_hvp_f64, (X, V, P(s.tri), P(s.Bm), P(s.area), P(self._mu_el), P(self._lam_el),
+                                              self.n_tri, self.h2, C.byref(self._tiles), Y, st),
+                      "d2f_elastic_hvp_f64")]
         which = 2 if (use_disc and use_sdf) else (0 if use_disc else 1)
         if (use_disc or use_sdf) and self._n_col:
             calls.append((L.d2f_collider_hvp_f64, (X, V, P(s.bnd), s.n_b, self._cols, self._n_col, which,
@@ -193,10 +201,25 @@ class AdjointPNCG2D:
             ip = s.ipc
             calls.append((L.d2f_ipc_hvp_f64, (X, V, P(self._pairs), self._n_pairs, P(s.ea), P(s.eb), P(s.ca),
                                               P(s.cb), ip["dhat"], ip["kappa"], self.h2, Y, st), "d2f_ipc_hvp_f64"))
-        if s.pressure is not None:
+        if s.pressure is not None and pt is None:
             calls.append((L.d2f_pressure_hvp_f64, (V, P(s.pe_a), P(s.pe_b), P(s.pe_ch), P(s.ch_p),
                                                    s.pressure["n_pe"], self.h2, Y, st), "d2f_pressure_hvp_f64"))
         return calls
+
+    def _pressure_tiles(self):
+        """The chamber edges dealt to the element tiles (layout.pressure_tiles) on the device,
+        built once; None when the tiles are not in use or an edge is no triangle's side."""
+        if getattr(self, "_ptiles", None) is None:
+            from .layout import ELEM_NODES_MAX, pressure_tiles
+            s, et = self.s, self.s.el_tiles
+            pt = None
+            if s.n_tri and et["max_nodes"] <= ELEM_NODES_MAX:
+                pt = pressure_tiles(s.tri.cpu().numpy(), et["tri_local"].cpu().numpy(), s.pe_a.cpu().numpy(),
+                                    s.pe_b.cpu().numpy(), s.pe_ch.cpu().numpy(), self.N)
+            self._ptiles = False if pt is None else {
+                k: torch.as_tensor(v.view(np.int32) if v.dtype == np.uint32 else v, device=self.dev)
+                for k, v in pt.items()}
+        return self._ptiles or None
 
     def _kv(self, xw, vw, y, use_disc=False, use_sdf=False, use_self=False, use_ipc=None):
         """y += h^2 K v at xw (the calls of _kv_calls)."""

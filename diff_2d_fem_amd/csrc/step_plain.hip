@@ -210,16 +210,28 @@ __global__ void k_elastic_halo(d2f_elem_tiles tl, double *grad, double *diag) {
   }
 }
 
+// Pressure chamber edges handed to the tiled Hessian-vector product: chunk c takes the edges
+// pe_off[c] .. pe_off[c + 1], whose two nodes are entries lo | hi << 10 of its node list, with
+// their chamber's index into ch_p. pe_off null: no pressure.
+struct PressureTiles {
+  const int32_t *pe_off;
+  const uint32_t *pe_loc;
+  const int32_t *pe_ch;
+  const double *ch_p;
+};
+
 // Elastic Hessian-vector product y += h2 (d2Psi/dx2) v (reference implicit/adjoint.py:70-85),
 // tiled as k_elastic_grad_diag_tiled: chunk c gathers its nodes' positions and direction into
 // LDS, sums every triangle's product per node there, then adds each owned node once (plain
 // read-modify-write) and its halo sums by device-scope atomics, or (PARTIAL) through the
-// tiles' partial slots and k_elastic_halo.
+// tiles' partial slots and k_elastic_halo. With pressure edges, their product (k_pressure_hvp's
+// term) is summed into the same LDS sums.
 template <bool PARTIAL>
 __global__ __launch_bounds__(ELEM_TPB) void k_elastic_hvp_tiled(const double *x, const double *vv,
                                                                 const double *Bm, const double *area,
                                                                 const double *mu, const double *lam, int T,
-                                                                double h2, d2f_elem_tiles tl, double *y) {
+                                                                double h2, d2f_elem_tiles tl, PressureTiles pt,
+                                                                double *y) {
   // planes of max_nodes each: x, y, v_x, v_y, (Hv)_x, (Hv)_y
   extern __shared__ double sh[];
   const int nbk = gridDim.x, q8 = nbk / 8, r8 = nbk % 8, xcd = blockIdx.x % 8;
@@ -260,6 +272,20 @@ __global__ __launch_bounds__(ELEM_TPB) void k_elastic_hvp_tiled(const double *x,
     for (int v = 0; v < 3; ++v) {
       atomicAdd(acc + li[v], f[v].x);
       atomicAdd(acc + M + li[v], f[v].y);
+    }
+  }
+  if (pt.pe_off) {
+    // -1/2 p J(v_b - v_a), J(u) = (u_y, -u_x), on both ends (h2 is applied below)
+    for (int q = pt.pe_off[c] + threadIdx.x; q < pt.pe_off[c + 1]; q += ELEM_TPB) {
+      const uint32_t l = pt.pe_loc[q];
+      const int la = int(l & 1023u), lb = int((l >> 10) & 1023u);
+      const double cp = -0.5 * pt.ch_p[pt.pe_ch[q]];
+      const V2<double> u = {vxs[lb] - vxs[la], vys[lb] - vys[la]};
+      const V2<double> w = {cp * u.y, -cp * u.x};
+      atomicAdd(acc + la, w.x);
+      atomicAdd(acc + M + la, w.y);
+      atomicAdd(acc + lb, w.x);
+      atomicAdd(acc + M + lb, w.y);
     }
   }
   __syncthreads();
@@ -2104,34 +2130,56 @@ int d2f_elastic_energy_f64(const double *x, const int32_t *tri, const double *Bm
   return 0;
 }
 
+static bool tiles_usable(const d2f_elem_tiles *tiles) {
+  return tiles && tiles->node_off && tiles->max_nodes > 0 && tiles->max_nodes <= D2F_ELEM_NODES_MAX;
+}
+
+static int launch_hvp_tiled(const double *x, const double *v, const double *Bm, const double *area,
+                            const double *mu, const double *lam, int32_t T, double h2, const d2f_elem_tiles *tiles,
+                            PressureTiles pt, double *y, hipStream_t st, const char *who) {
+  D2F_CHECK(tiles->nodes && tiles->halo_off && tiles->tri_local && tiles->part && tiles->n_hnodes >= 0 &&
+                (tiles->n_hnodes == 0 || (tiles->hnode && tiles->hnode_off && tiles->hdest)),
+            "%s: incomplete tiles", who);
+  const int blocks = (T + D2F_ELEM_CHUNK - 1) / D2F_ELEM_CHUNK;
+  const size_t lds = sizeof(double) * 6 * (size_t)tiles->max_nodes;   // at most 48 KiB
+  const bool partial = T >= ELEM_PARTIAL_MIN_T;
+  if (partial)
+    hipLaunchKernelGGL(k_elastic_hvp_tiled<true>, dim3(blocks), dim3(ELEM_TPB), lds, st, x, v, Bm, area, mu, lam, T,
+                       h2, *tiles, pt, y);
+  else
+    hipLaunchKernelGGL(k_elastic_hvp_tiled<false>, dim3(blocks), dim3(ELEM_TPB), lds, st, x, v, Bm, area, mu, lam,
+                       T, h2, *tiles, pt, y);
+  if (partial && tiles->n_hnodes > 0) {
+    const int hb = (int)std::min<long>((tiles->n_hnodes + 255) / 256, 65536L);
+    hipLaunchKernelGGL(k_elastic_halo, dim3(hb), dim3(256), 0, st, *tiles, y, (double *)nullptr);
+  }
+  D2F_HIP(hipGetLastError());
+  return 0;
+}
+
 int d2f_elastic_hvp_f64(const double *x, const double *v, const int32_t *tri, const double *Bm,
                         const double *area, const double *mu, const double *lam, int32_t T, double h2,
                         const d2f_elem_tiles *tiles, double *y, void *stream) {
   D2F_CHECK(T >= 0 && x && v && tri && Bm && area && mu && lam && y, "d2f_elastic_hvp_f64: bad argument");
   hipStream_t st = (hipStream_t)stream;
   if (T == 0) return 0;
-  if (tiles && tiles->node_off && tiles->max_nodes > 0 && tiles->max_nodes <= D2F_ELEM_NODES_MAX) {
-    D2F_CHECK(tiles->nodes && tiles->halo_off && tiles->tri_local && tiles->part && tiles->n_hnodes >= 0 &&
-                  (tiles->n_hnodes == 0 || (tiles->hnode && tiles->hnode_off && tiles->hdest)),
-              "d2f_elastic_hvp_f64: incomplete tiles");
-    const int blocks = (T + D2F_ELEM_CHUNK - 1) / D2F_ELEM_CHUNK;
-    const size_t lds = sizeof(double) * 6 * (size_t)tiles->max_nodes;   // at most 48 KiB
-    const bool partial = T >= ELEM_PARTIAL_MIN_T;
-    if (partial)
-      hipLaunchKernelGGL(k_elastic_hvp_tiled<true>, dim3(blocks), dim3(ELEM_TPB), lds, st, x, v, Bm, area, mu, lam,
-                         T, h2, *tiles, y);
-    else
-      hipLaunchKernelGGL(k_elastic_hvp_tiled<false>, dim3(blocks), dim3(ELEM_TPB), lds, st, x, v, Bm, area, mu,
-                         lam, T, h2, *tiles, y);
-    if (partial && tiles->n_hnodes > 0) {
-      const int hb = (int)std::min<long>((tiles->n_hnodes + 255) / 256, 65536L);
-      hipLaunchKernelGGL(k_elastic_halo, dim3(hb), dim3(256), 0, st, *tiles, y, (double *)nullptr);
-    }
-  } else {
-    LAUNCH(k_elastic_hvp, T, x, v, tri, Bm, area, mu, lam, T, h2, y);
-  }
+  if (tiles_usable(tiles))
+    return launch_hvp_tiled(x, v, Bm, area, mu, lam, T, h2, tiles, PressureTiles{}, y, st, "d2f_elastic_hvp_f64");
+  LAUNCH(k_elastic_hvp, T, x, v, tri, Bm, area, mu, lam, T, h2, y);
   D2F_HIP(hipGetLastError());
   return 0;
+}
+
+int d2f_elastic_pressure_hvp_f64(const double *x, const double *v, const double *Bm, const double *area,
+                                 const double *mu, const double *lam, int32_t T, double h2,
+                                 const d2f_elem_tiles *tiles, const int32_t *pe_off, const uint32_t *pe_loc,
+                                 const int32_t *pe_ch, const double *ch_p, double *y, void *stream) {
+  D2F_CHECK(T > 0 && x && v && Bm && area && mu && lam && y && pe_off && pe_loc && pe_ch && ch_p,
+            "d2f_elastic_pressure_hvp_f64: bad argument");
+  D2F_CHECK(tiles_usable(tiles), "d2f_elastic_pressure_hvp_f64: needs element tiles (max_nodes <= %d)",
+            D2F_ELEM_NODES_MAX);
+  return launch_hvp_tiled(x, v, Bm, area, mu, lam, T, h2, tiles, PressureTiles{pe_off, pe_loc, pe_ch, ch_p}, y,
+                          (hipStream_t)stream, "d2f_elastic_pressure_hvp_f64");
 }
 
 int d2f_ipc_min_distance(const d2f_scene *s, const d2f_params *p, void *stream) {

@@ -134,6 +134,36 @@ def element_tiles(tri, n_nodes, chunk=ELEM_CHUNK, device="cpu"):
                 owned=(~halo).to(torch.uint8), max_nodes=max_nodes, n_hnodes=int(hnode.numel()))
 
 
+def pressure_tiles(tri, tri_local, pe_a, pe_b, pe_ch, n_nodes, chunk=ELEM_CHUNK):
+    """The pressure chamber edges dealt to the element tiles (d2f_elastic_pressure_hvp_f64):
+    each edge goes to the chunk of a triangle having it as a side, with its two nodes as entries
+    of that chunk's node list. Returns numpy arrays pe_off (n_chunks + 1,) int32, pe_loc uint32
+    (lo | hi << 10) and pe_ch int32, edges grouped by chunk; None when an edge is no triangle's
+    side."""
+    tri = np.asarray(tri, np.int64).reshape(-1, 3)
+    T = len(tri)
+    a, b = np.asarray(pe_a, np.int64), np.asarray(pe_b, np.int64)
+    s0, s1 = tri, tri[:, [1, 2, 0]]
+    key = (np.minimum(s0, s1) * n_nodes + np.maximum(s0, s1)).reshape(-1)
+    order = np.argsort(key, kind="stable")
+    ks = key[order]
+    q = np.minimum(a, b) * n_nodes + np.maximum(a, b)
+    pos = np.searchsorted(ks, q)
+    if len(ks) == 0 or (pos >= len(ks)).any() or (ks[np.minimum(pos, len(ks) - 1)] != q).any():
+        return None
+    t = order[pos] // 3
+    loc = np.asarray(tri_local, np.int64)[t]
+    slots = np.stack([loc & 1023, (loc >> 10) & 1023, (loc >> 20) & 1023], 1)
+    rows = np.arange(len(t))
+    la = slots[rows, (tri[t] == a[:, None]).argmax(1)]
+    lb = slots[rows, (tri[t] == b[:, None]).argmax(1)]
+    c = t // chunk
+    o = np.argsort(c, kind="stable")
+    pe_off = np.searchsorted(c[o], np.arange((T + chunk - 1) // chunk + 1)).astype(np.int32)
+    return dict(pe_off=pe_off, pe_loc=(la | (lb << 10))[o].astype(np.uint32),
+                pe_ch=np.asarray(pe_ch, np.int64)[o].astype(np.int32))
+
+
 def tiles_struct(et):
     """The d2f_elem_tiles struct of element_tiles' tensors."""
     from ._lib import ElemTiles

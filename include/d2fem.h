@@ -242,6 +242,15 @@ int d2f_elastic_energy_f64(const double *x, const int32_t *tri, const double *Bm
 int d2f_elastic_hvp_f64(const double *x, const double *v, const int32_t *tri, const double *Bm,
                         const double *area, const double *mu, const double *lam, int32_t T,
                         double h2, const d2f_elem_tiles *tiles, double *y, void *stream);
+/* The same product plus the pressure chambers' (d2f_pressure_hvp_f64's term) in one tiled pass:
+ * chunk c of the tiles also takes the chamber edges pe_off[c] .. pe_off[c + 1], each with its two
+ * nodes as entries (lo | hi << 10) of the chunk's node list and its chamber's index into ch_p.
+ * No counterpart in the reference (its adjoint refuses pressure chambers); tiles required. */
+int d2f_elastic_pressure_hvp_f64(const double *x, const double *v, const double *Bm,
+                                 const double *area, const double *mu, const double *lam,
+                                 int32_t T, double h2, const d2f_elem_tiles *tiles,
+                                 const int32_t *pe_off, const uint32_t *pe_loc,
+                                 const int32_t *pe_ch, const double *ch_p, double *y, void *stream);
 
 /* ---- whole implicit steps ----------------------------------------------------------------- */
 /* One implicit step of every environment on the per-kernel path: one launch per reference

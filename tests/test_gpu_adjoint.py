@@ -287,3 +287,48 @@ def test_minres_small_and_large_scenes():
     assert np.abs(lam - lo).max() / np.abs(lo).max() < 1e-7
     r = (oa.apply_H(x, lam) - rhs)[free]
     assert np.abs(r).max() / np.abs(rhs[free]).max() < 1e-8
+
+
+def test_elastic_and_pressure_product_in_one_tiled_pass():
+    """d2f_elastic_pressure_hvp_f64 (elasticity and the chambers' pressure in one pass over the
+    element tiles, the chamber edges dealt to the chunks) against the elastic product plus
+    d2f_pressure_hvp_f64, on pneumatic fingers in a deformed state, and the adjoint's operator
+    (which takes the fused pass) against the oracle's H v."""
+    import ctypes as C
+    import torch
+    from diff_2d_fem_amd import _lib
+    from diff_2d_fem_amd import workloads as W
+    w = W.pneumatic_fingers(6)
+    m = w["model"]
+    integ = ImplicitIntegrator(path="scene", precision="double", self_collision=False)
+    for _ in range(2):
+        integ.simulate(m, w["dt"])
+    x = m.get_x()
+    adj = integ.adjoint(m, w["dt"])
+    s = adj.s
+    pt = adj._pressure_tiles()
+    assert pt is not None
+    rng = np.random.default_rng(9)
+    v = rng.standard_normal(x.shape) * 1e-4
+    d = torch.device("cuda")
+    xt, vt = torch.tensor(x, device=d), torch.tensor(v, device=d)
+    P = lambda t: C.c_void_p(t.data_ptr())
+    L = _lib.lib()
+    st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    y0 = torch.zeros_like(xt)
+    _lib.check(L.d2f_elastic_hvp_f64(P(xt), P(vt), P(s.tri), P(s.Bm), P(s.area), P(adj._mu_el), P(adj._lam_el),
+                                     adj.n_tri, adj.h2, C.byref(adj._tiles), P(y0), st), "hvp")
+    _lib.check(L.d2f_pressure_hvp_f64(P(vt), P(s.pe_a), P(s.pe_b), P(s.pe_ch), P(s.ch_p), s.pressure["n_pe"],
+                                      adj.h2, P(y0), st), "pressure hvp")
+    y1 = torch.zeros_like(xt)
+    _lib.check(L.d2f_elastic_pressure_hvp_f64(P(xt), P(vt), P(s.Bm), P(s.area), P(adj._mu_el), P(adj._lam_el),
+                                              adj.n_tri, adj.h2, C.byref(adj._tiles), P(pt["pe_off"]),
+                                              P(pt["pe_loc"]), P(pt["pe_ch"]), P(s.ch_p), P(y1), st), "fused")
+    a, b = y0.cpu().numpy(), y1.cpu().numpy()
+    assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
+    oi = OracleIntegrator(self_collision=False)
+    oi.attach(m, w["dt"])
+    oa = OracleAdjoint(oi.solver, m.me)
+    Hv = adj.apply_H(x, v)
+    Ho = oa.apply_H(x, v)
+    assert np.abs(Hv - Ho).max() <= 1e-10 * np.abs(Ho).max()

@@ -74,3 +74,36 @@ def test_lists_longer_than_the_limit_are_flagged():
     T = np.arange(3 * 2048).reshape(-1, 3)
     et = element_tiles(T, 3 * 2048)
     assert et["max_nodes"] == 3072 > ELEM_NODES_MAX
+
+
+@pytest.mark.parametrize("chunk", [1024, 256])
+def test_pressure_edges_dealt_to_tiles(chunk):
+    """layout.pressure_tiles on the pneumatic finger's chamber edges: every edge lands in a chunk
+    whose node list holds both its nodes at the local indices it records, the chunks' ranges
+    partition the edges, and each edge keeps its chamber; an edge that is no triangle's side is
+    refused."""
+    from diff_2d_fem_amd import workloads as W
+    from diff_2d_fem_amd.layout import pressure_tiles
+    w = W.pneumatic_fingers(3)
+    m = w["model"]
+    me = m.me
+    ea, eb, ech = [], [], []
+    for i, c in enumerate(m.actuation.chambers):
+        ea.append(np.asarray(c.ea, np.int64))
+        eb.append(np.asarray(c.eb, np.int64))
+        ech.append(np.full(len(c.ea), i))
+    ea, eb, ech = np.concatenate(ea), np.concatenate(eb), np.concatenate(ech)
+    N = me.rest_pos.shape[0]
+    et = {k: (v.numpy() if hasattr(v, "numpy") else v) for k, v in element_tiles(me.tri, N, chunk=chunk).items()}
+    pt = pressure_tiles(me.tri, et["tri_local"], ea, eb, ech, N, chunk=chunk)
+    off, loc, pch = pt["pe_off"], pt["pe_loc"].astype(np.int64), pt["pe_ch"]
+    assert off[0] == 0 and off[-1] == len(ea) and (np.diff(off) >= 0).all()
+    got = set()
+    for c in range(len(off) - 1):
+        lst = et["nodes"][et["node_off"][c]:et["node_off"][c + 1]]
+        for q in range(off[c], off[c + 1]):
+            a, b = lst[loc[q] & 1023], lst[(loc[q] >> 10) & 1023]
+            got.add((int(a), int(b), int(pch[q])))
+    assert got == set(zip(ea.tolist(), eb.tolist(), ech.tolist()))
+    far = np.array([0]), np.array([N - 1]), np.array([0])
+    assert pressure_tiles(me.tri, et["tri_local"], *far, N, chunk=chunk) is None
