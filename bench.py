@@ -326,11 +326,13 @@ def scene_roofline(res, precision):
     bytes_ = it_per_env * (2.0 * (12 + 6 * se) * s.n_tri + 272.0 * s.N)
     dur = res["launch_ms"] * 1e-3
     ach = bytes_ / dur / 1e9
-    name = (f"d2f::k_scene<{'float' if precision == 'mixed' else 'double'}, {s.block_size}, "
-            f"{'true' if (getattr(s, 'tendon', None) or {}).get('n_td', 0) > 0 else 'false'}>")
+    # the kernel's lean variant (no tendons, springs, pressure, rigid modes, node contact or line
+    # search) or its full one: the traffic file names the one the workload ran
+    base = f"d2f::k_scene<{'float' if precision == 'mixed' else 'double'}, {s.block_size}, "
+    traffic = measured_traffic(base + "false>", res) or measured_traffic(base + "true>", res)
     return dict(kernel=f"k_scene ({precision})", bound="hbm", achieved=round(ach, 1),
                 peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
-                traffic=measured_traffic(name, res),
+                traffic=traffic,
                 avg_us=round(dur * 1e6, 1), bytes_per_launch=int(bytes_),
                 iters_per_scene_per_step=round(it_per_env, 2))
 

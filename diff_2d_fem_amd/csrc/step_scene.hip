@@ -334,13 +334,18 @@ __device__ __forceinline__ V2<double> up(V2<R> v) {
 }
 
 // ----------------------------------------------------------------------------- the kernel
-// TD: the launch has tendons. Without them td0 = td1 = 0 is a constant, and every tendon pass
-// compiles away (their code weighs on the register allocation of the whole kernel).
-template <typename R, int TPB, bool TD>
+// FULL: the launch uses a term beyond elasticity, colliders, IPC contact and affine bodies
+// (tendons, springs, pressure chambers, rigid modes, "node" contact or the line search).
+// Without FULL their ranges are the constant 0 and their flags false, so their passes compile
+// away: the kernel's code then weighs less on the register allocation and on the instruction
+// cache.
+template <typename R, int TPB, bool FULL>
 struct Step {
   S &s;
   KPm &p;
   LdsScalars &sc;
+  __device__ __forceinline__ bool node_on() const { return FULL && p.node_on; }
+  __device__ __forceinline__ bool line_search() const { return FULL && p.line_search; }
   Block<TPB> blk;
   LdsD *bpos;     // 2 nb: boundary positions
   LdsD *bacc;     // 4 nb: boundary accumulators (grad x, grad y, diag x, diag y)
@@ -400,14 +405,14 @@ struct Step {
     bc1 = s.n_bc > 0 ? s.bc_start[e + 1] : 0;
     g0 = s.n_ag > 0 ? s.ag_start[e] : 0;
     g1 = s.n_ag > 0 ? s.ag_start[e + 1] : 0;
-    r0 = s.n_rg > 0 ? s.rg_start[e] : 0;
-    r1 = s.n_rg > 0 ? s.rg_start[e + 1] : 0;
-    pe0 = s.n_pe > 0 ? s.pe_start[e] : 0;
-    pe1 = s.n_pe > 0 ? s.pe_start[e + 1] : 0;
-    td0 = TD && s.n_td > 0 ? s.td_start[e] : 0;
-    td1 = TD && s.n_td > 0 ? s.td_start[e + 1] : 0;
-    sp0 = s.n_sp > 0 ? s.sp_start[e] : 0;
-    sp1 = s.n_sp > 0 ? s.sp_start[e + 1] : 0;
+    r0 = FULL && s.n_rg > 0 ? s.rg_start[e] : 0;
+    r1 = FULL && s.n_rg > 0 ? s.rg_start[e + 1] : 0;
+    pe0 = FULL && s.n_pe > 0 ? s.pe_start[e] : 0;
+    pe1 = FULL && s.n_pe > 0 ? s.pe_start[e + 1] : 0;
+    td0 = FULL && s.n_td > 0 ? s.td_start[e] : 0;
+    td1 = FULL && s.n_td > 0 ? s.td_start[e + 1] : 0;
+    sp0 = FULL && s.n_sp > 0 ? s.sp_start[e] : 0;
+    sp1 = FULL && s.n_sp > 0 ? s.sp_start[e + 1] : 0;
     eoff = s.excl_bits_off ? s.excl_bits_off[e] : 0;
     xcon = s.xc_lds && n1 - n0 <= s.max_nn;
   }
@@ -469,7 +474,7 @@ struct Step {
   // grid and the list are rebuilt. A pair within reach now was within reach + skin at the
   // build (each end moved at most skin / 2), so the list holds every pair a pass needs.
   // Contact distance the broadphase serves: the IPC reach or the "node" contact radius.
-  __device__ __forceinline__ double creach() const { return p.node_on ? p.node_r : p.ipc_reach; }
+  __device__ __forceinline__ double creach() const { return node_on() ? p.node_r : p.ipc_reach; }
 
   // A node that moved more than half the skin marks the list stale with this call's stamp
   // (no reduction needed: one barrier, then every thread reads the same flag).
@@ -485,7 +490,7 @@ struct Step {
     __syncthreads();
     if (!have || sc.stale == stamp) {
       build_grid();
-      if (p.node_on) {  // node pairs are found in the grid itself; plist marks it current
+      if (node_on()) {  // node pairs are found in the grid itself; plist marks it current
         if (tid == 0) sc.plist = 1;
         __syncthreads();
       } else {
@@ -1266,7 +1271,7 @@ struct Step {
       lds_add(bacc + 4 * jb + 1, g.y);
     }
     __syncthreads();
-    if (p.node_on && nb > 0) contact_prepare();
+    if (node_on() && nb > 0) contact_prepare();
     // colliders, friction and the accumulated contact terms, one thread per boundary node
     for (int j = tid; j < nb; j += TPB) {
       const int n = lbnd[j];
@@ -1290,7 +1295,7 @@ struct Step {
           dg = dg + V2<double>{kt * f.n.y * f.n.y, kt * f.n.x * f.n.x};
         }
       }
-      if (p.node_on) node_grad(j, n, x, h2, g, dg);
+      if (node_on()) node_grad(j, n, x, h2, g, dg);
       g = g + V2<double>{bacc[4 * j], bacc[4 * j + 1]};
       dg = dg + V2<double>{bacc[4 * j + 2], bacc[4 * j + 3]};
       st2(s.grad, n, g);
@@ -1678,7 +1683,7 @@ struct Step {
         }
       }
       for (int k = 0; k < s.n_col; ++k) amin = fmin(amin, collider_ccd(to_collider_k(s.col[k]), x, pp, p.ccd_eta));
-      if (p.node_on) ph += node_sum(1, j, n, x, h2);
+      if (node_on()) ph += node_sum(1, j, n, x, h2);
     }
     // pressure (reference implicit/actuation.py:102-110)
     for (int k = pe0 + tid; k < pe1; k += TPB) {
@@ -1803,12 +1808,12 @@ struct Step {
       const Elem<R> el = load_elem<R>(s, t);
       E += h2 * elem_psi(el, X(el.i0), X(el.i1), X(el.i2));
     }
-    if (p.node_on && nb > 0) contact_prepare();
+    if (node_on() && nb > 0) contact_prepare();
     for (int j = tid; j < nb; j += TPB) {
       const int n = lbnd[j];
       const V2<double> x = X(n);
       for (int k = 0; k < s.n_col; ++k) E += collider_energy(to_collider_k(s.col[k]), x, p.dhat, p.kappa, h2);
-      if (p.node_on) E += node_sum(2, j, n, x, h2);
+      if (node_on()) E += node_sum(2, j, n, x, h2);
       if (p.friction && s.free_[n]) {
         const V2<double> xn = ld2(s.xn, n);
         for (int k = 0; k < s.n_col; ++k) {
@@ -1864,7 +1869,7 @@ struct Step {
   __device__ __forceinline__ void update() {
     if (!sc.active) return;  // uniform: sc is shared
     const double a = sc.alpha;
-    if (!p.line_search) {
+    if (!line_search()) {
       // q += a pq touches only the groups' LDS state, which set_x does not read: thread 0 does
       // it first, and the slaved nodes are expanded from the new q after set_x's barrier.
       if (tid == 0)
@@ -1907,7 +1912,7 @@ struct Step {
       sc.plist = 0;
     }
     __syncthreads();
-    if (((p.ipc_on && k1 > k0) || p.node_on) && s.excl_bits) {
+    if (((p.ipc_on && k1 > k0) || node_on()) && s.excl_bits) {
       const int64_t w0 = s.excl_bits_off[e], nw = s.excl_bits_off[e + 1] - w0;
       if (nw <= s.excl_bits_lds) {
         for (int64_t i = tid; i < nw; i += TPB) xbits[i] = s.excl_bits[w0 + i];
@@ -2013,7 +2018,7 @@ struct Step {
 constexpr size_t PM_OFFSET = (sizeof(SH) + alignof(Pm) - 1) / alignof(Pm) * alignof(Pm);
 static_assert(alignof(SH) == 8 && alignof(Pm) == 8, "kernel-argument layout");
 
-template <typename R, int TPB, bool TD>
+template <typename R, int TPB, bool FULL>
 __global__ __launch_bounds__(TPB, TPB == 256 ? 2 : 1) void k_scene(SH s, Pm p) {
   extern __shared__ double lds[];
   __shared__ SceneScalars sc;
@@ -2022,14 +2027,14 @@ __global__ __launch_bounds__(TPB, TPB == 256 ? 2 : 1) void k_scene(SH s, Pm p) {
   // private memory, and a copy in LDS would hide that its pointers are global.
   S *ks = (S *)__builtin_amdgcn_kernarg_segment_ptr();
   KPm *kp = (KPm *)((const __attribute__((address_space(4))) char *)ks + PM_OFFSET);
-  Step<R, TPB, TD> st(*ks, *kp, *(LdsScalars *)&sc, (LdsD *)lds);
+  Step<R, TPB, FULL> st(*ks, *kp, *(LdsScalars *)&sc, (LdsD *)lds);
   st.run();
 }
 
-template <typename R, int TPB, bool TD>
-static int launch_td(const SH &s, const Pm &p, hipStream_t stream) {
+template <typename R, int TPB, bool FULL>
+static int launch_v(const SH &s, const Pm &p, hipStream_t stream) {
   const size_t lds = sizeof(double) * (Block<TPB>::RED_SIZE + 8 * (size_t)s.max_nb +
-                                       Step<R, TPB, TD>::GST * (size_t)s.max_ag +
+                                       Step<R, TPB, FULL>::GST * (size_t)s.max_ag +
                                        (s.nacc_lds ? 4 * (size_t)s.max_nn : 0) +
                                        (s.xc_lds ? 2 * (size_t)s.max_nn : 0)) +
                      sizeof(int) * (2 * (size_t)s.max_cells + 1 + 2 * (size_t)s.max_nb + (size_t)s.excl_bits_lds +
@@ -2045,21 +2050,22 @@ static int launch_td(const SH &s, const Pm &p, hipStream_t stream) {
     int dev = 0;
     D2F_HIP(hipGetDevice(&dev));
     if (dev < 0 || dev >= 64 || lds_set[dev].load() < (int)lds) {
-      D2F_HIP(hipFuncSetAttribute((const void *)k_scene<R, TPB, TD>,
+      D2F_HIP(hipFuncSetAttribute((const void *)k_scene<R, TPB, FULL>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
       if (dev >= 0 && dev < 64) lds_set[dev].store((int)lds);
     }
   }
   const int n_wg = s.sc_list ? s.n_list : s.B;
   if (n_wg == 0) return 0;
-  hipLaunchKernelGGL((k_scene<R, TPB, TD>), dim3(n_wg), dim3(TPB), lds, stream, s, p);
+  hipLaunchKernelGGL((k_scene<R, TPB, FULL>), dim3(n_wg), dim3(TPB), lds, stream, s, p);
   D2F_HIP(hipGetLastError());
   return 0;
 }
 
 template <typename R, int TPB>
 static int launch(const SH &s, const Pm &p, hipStream_t stream) {
-  return s.n_td > 0 ? launch_td<R, TPB, true>(s, p, stream) : launch_td<R, TPB, false>(s, p, stream);
+  const bool full = s.n_td > 0 || s.n_sp > 0 || s.n_pe > 0 || s.n_rg > 0 || p.node_on || p.line_search;
+  return full ? launch_v<R, TPB, true>(s, p, stream) : launch_v<R, TPB, false>(s, p, stream);
 }
 
 // The float and float64 kernels are compiled as two objects of this file (D2F_SCENE_PART 0 and
