@@ -121,3 +121,25 @@ def test_longest_first_launch_order():
     order = s1._order_t.cpu().numpy()
     assert sorted(order.tolist()) == list(range(12)) and order.tolist() != list(range(12))
     assert np.abs(x1 - x0).max() <= 1e-9 * w["size"]
+
+
+def test_lean_and_full_scene_kernels_agree():
+    """The scene kernel's lean variant (a batch without tendons, springs, pressure, rigid modes,
+    node contact or line search) and its full one give the same steps: the grip scene alone takes
+    the lean kernel; with a spring of negligible stiffness added it takes the full one."""
+    from tests.scenes import grip, close_jaws
+
+    def run(spring):
+        m, pads = grip(n_env=2)
+        if spring:                    # slack at rest and 1e-30 N/m: a no-op term
+            c = m.rest_x(pads[0][0]).mean(0)
+            m.add_spring(pads[0][0], c, point_b=c + [1e-4, 0.0], stiffness=1e-30, rest_length=1e-4)
+        integ = ImplicitIntegrator(path="scene", precision="mixed", max_iter=60, eps=0.0, conv_check=0)
+        for k in range(1, 6):
+            close_jaws(m, pads, k)
+            integ.simulate(m, 1e-3)
+        return m.get_x(), m._solver
+    x0, s0 = run(False)
+    x1, s1 = run(True)
+    assert s0.scene_struct().n_sp == 0 and s1.scene_struct().n_sp == 1
+    assert np.abs(x1 - x0).max() <= 1e-12
