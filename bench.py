@@ -318,12 +318,14 @@ def scene_roofline(res, precision):
     """The step kernel of the scene path (one launch per step): algorithmic bytes per launch
     = sum over scenes of iterations x per-iteration bytes, where one iteration reads each
     scene's element constants twice (gradient+diagonal and curvature: T (12 + 6 s_e) for the
-    three node indices, D_m^-1 and the area-weighted moduli) and touches node state for 272
-    bytes per node; s_e = 4 in mixed, 8 in float64."""
+    three node indices, D_m^-1 and the area-weighted moduli) and touches node state for 240
+    bytes per node (the per-kernel passes' 272 less the 32 of copying the gradient into the
+    previous-gradient array, which the kernel replaces by swapping the two arrays' roles each
+    iteration); s_e = 4 in mixed, 8 in float64."""
     s = res["m"]._solver
     se = 4 if precision == "mixed" else 8
     it_per_env = res["iters"] / res["m"].n_envs / res["n_steps"]
-    bytes_ = it_per_env * (2.0 * (12 + 6 * se) * s.n_tri + 272.0 * s.N)
+    bytes_ = it_per_env * (2.0 * (12 + 6 * se) * s.n_tri + 240.0 * s.N)
     dur = res["launch_ms"] * 1e-3
     ach = bytes_ / dur / 1e9
     # the kernel's lean variant (no tendons, springs, pressure, rigid modes, node contact or line

@@ -350,6 +350,8 @@ struct Step {
   LdsD *bpos;     // 2 nb: boundary positions
   LdsD *bacc;     // 4 nb: boundary accumulators (grad x, grad y, diag x, diag y)
   LdsD *bref;     // 2 nb: boundary positions the grid and pair list were built from
+  GD *gcur;       // this iteration's gradient (s.grad or s.grad_prev)
+  GD *gprev;      // the previous iteration's
   LdsD *nacc;     // 4 x max_nn, component-major: grad x, grad y, diag x, diag y of each node
   // The scene's affine groups in LDS for the whole step, GST doubles each: coordinates q (6),
   // their line-search start q0 (6), direction pq (6), preconditioned gradient zq (6), Mu, La.
@@ -379,6 +381,8 @@ struct Step {
       : s(s_), p(p_), sc(sc_) {
     e = s.sc_list ? s.sc_list[blockIdx.x] : blockIdx.x;
     tid = threadIdx.x;
+    gcur = s.grad;
+    gprev = s.grad_prev;
     blk.red = lds;
     bpos = lds + Block<TPB>::RED_SIZE;
     bacc = bpos + 2 * s.max_nb;
@@ -928,6 +932,12 @@ struct Step {
 
   // ------------------------------------------------------------------ gradient + diagonal
   __device__ __forceinline__ void grad_diag() {
+    // The gradient buffers change roles each iteration: the one holding the gradient of the
+    // iteration before (read by direction() as the previous gradient) is left as it is, and the
+    // new gradient is written over the older one, so no copy of it is made.
+    GD *const t = gprev;
+    gprev = gcur;
+    gcur = t;
     if (s.nacc_lds && n1 - n0 <= s.max_nn) {
       grad_diag_lds();
     } else {
@@ -938,7 +948,7 @@ struct Step {
 
   // Elastic gradient and diagonal (reference implicit/kernels.py:86-128) summed per node in
   // LDS with LDS atomics, all soft triangles in one pass; then one pass over the nodes writes
-  // grad_prev, grad = inertia - h^2 f_ext + h^2 elastic, diag = mass + h^2 elastic
+  // grad = inertia - h^2 f_ext + h^2 elastic, diag = mass + h^2 elastic
   // (reference grad_diag_mass_kernel, implicit/kernels.py:73-82).
   __device__ __forceinline__ void grad_diag_lds() {
     const double h2 = p.h2;
@@ -960,13 +970,12 @@ struct Step {
     if (t < tb) elem_to_lds(load_elem<R>(ep, t));
     __syncthreads();
     struct NodeIn {
-      V2<double> g, x, xh, fe;
+      V2<double> x, xh, fe;
       double m, f;
     };
     node_batches<2>(
         [&](int i) {
           NodeIn q;
-          q.g = ld2(s.grad, i);
           q.m = s.mass[i];
           q.f = fr(i);
           q.x = X(i);
@@ -975,13 +984,12 @@ struct Step {
           return q;
         },
         [&](int i, const NodeIn &q) {
-          st2(s.grad_prev, i, q.g);
           const V2<double> d = q.x - q.xh;
           const LdsD *a = nacc + (i - n0);
           V2<double> gn = {q.m * d.x * q.f, q.m * d.y * q.f};
           if (s.fext) gn = gn - h2 * q.fe;
           gn = gn + h2 * V2<double>{a[0], a[ns]};
-          st2(s.grad, i, gn);
+          st2(gcur, i, gn);
           st2(s.diag, i, V2<double>{q.m + h2 * a[2 * ns], q.m + h2 * a[3 * ns]});
         });
     __syncthreads();
@@ -1011,13 +1019,12 @@ struct Step {
   __device__ __forceinline__ void grad_diag_colored() {
     const double h2 = p.h2;
     struct NodeIn {
-      V2<double> g, x, xh, fe;
+      V2<double> x, xh, fe;
       double m, f;
     };
     node_batches<2>(
         [&](int i) {
           NodeIn q;
-          q.g = ld2(s.grad, i);
           q.m = s.mass[i];
           q.f = fr(i);
           q.x = X(i);
@@ -1026,11 +1033,10 @@ struct Step {
           return q;
         },
         [&](int i, const NodeIn &q) {
-          st2(s.grad_prev, i, q.g);
           const V2<double> d = q.x - q.xh;
           V2<double> gn = {q.m * d.x * q.f, q.m * d.y * q.f};
           if (s.fext) gn = gn - h2 * q.fe;
-          st2(s.grad, i, gn);
+          st2(gcur, i, gn);
           st2(s.diag, i, V2<double>{q.m, q.m});
         });
     for (int j = tid; j < 4 * nb; j += TPB) bacc[j] = 0.0;
@@ -1056,7 +1062,7 @@ struct Step {
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
               q.x[k] = X(id[k]);
-              q.g[k] = ld2(s.grad, id[k]);
+              q.g[k] = ld2(gcur, id[k]);
               q.d[k] = ld2(s.diag, id[k]);
             }
             return q;
@@ -1069,7 +1075,7 @@ struct Step {
             const int id[3] = {q.el.i0, q.el.i1, q.el.i2};
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-              st2(s.grad, id[k], q.g[k] + h2 * up(g[k]));
+              st2(gcur, id[k], q.g[k] + h2 * up(g[k]));
               st2(s.diag, id[k], q.d[k] + h2 * V2<double>{fmax(double(d[k].x), 0.0), fmax(double(d[k].y), 0.0)});
             }
           });
@@ -1124,7 +1130,7 @@ struct Step {
           gL = gL - d;
           geo = geo + V2<double>{(1.0 - d.x * d.x) * il, (1.0 - d.y * d.y) * il};
         }
-        st2(s.grad, n, ld2(s.grad, n) + (h2 * tau) * gL);
+        st2(gcur, n, ld2(gcur, n) + (h2 * tau) * gL);
         st2(s.diag, n, ld2(s.diag, n) + V2<double>{h2 * (tau * geo.x + ke * gL.x * gL.x),
                                                     h2 * (tau * geo.y + ke * gL.y * gL.y)});
       }
@@ -1209,13 +1215,13 @@ struct Step {
       for (int i = 0; i < j && first; ++i) first = nd[i] != node;
       if (!first) continue;
       const int n = int(node);
-      V2<double> g = ld2(s.grad, n), dg = ld2(s.diag, n);
+      V2<double> g = ld2(gcur, n), dg = ld2(s.diag, n);
       for (int i = j; i < ne; ++i) {
         if (nd[i] != node) continue;
         g = g + V2<double>{cgx[i], cgy[i]};
         dg = dg + V2<double>{cdx[i], cdy[i]};
       }
-      st2(s.grad, n, g);
+      st2(gcur, n, g);
       st2(s.diag, n, dg);
     }
   }
@@ -1276,7 +1282,7 @@ struct Step {
     for (int j = tid; j < nb; j += TPB) {
       const int n = lbnd[j];
       const V2<double> x = X(n);
-      V2<double> g = ld2(s.grad, n), dg = ld2(s.diag, n);
+      V2<double> g = ld2(gcur, n), dg = ld2(s.diag, n);
       for (int k = 0; k < s.n_col; ++k) {
         V2<double> gg, dd;
         if (collider_grad_diag(to_collider_k(s.col[k]), x, p.dhat, p.kappa, h2, 1, gg, dd)) {
@@ -1298,7 +1304,7 @@ struct Step {
       if (node_on()) node_grad(j, n, x, h2, g, dg);
       g = g + V2<double>{bacc[4 * j], bacc[4 * j + 1]};
       dg = dg + V2<double>{bacc[4 * j + 2], bacc[4 * j + 3]};
-      st2(s.grad, n, g);
+      st2(gcur, n, g);
       st2(s.diag, n, dg);
     }
     __syncthreads();
@@ -1316,10 +1322,10 @@ struct Step {
         const SpringGeo q = spring_geo(X(a), b >= 0 ? X(b) : ld2(s.sp_anchor, j), k, s.sp_L0[j]);
         const V2<double> g = (h2 * q.tension) * q.n;
         const V2<double> hd = spring_diag(q, k, h2);
-        st2(s.grad, a, ld2(s.grad, a) + g);
+        st2(gcur, a, ld2(gcur, a) + g);
         st2(s.diag, a, ld2(s.diag, a) + hd);
         if (b >= 0) {
-          st2(s.grad, b, ld2(s.grad, b) - g);
+          st2(gcur, b, ld2(gcur, b) - g);
           st2(s.diag, b, ld2(s.diag, b) + hd);
         }
       }
@@ -1334,8 +1340,8 @@ struct Step {
       for (int k = s.an_start[g] + tid; k < s.an_start[g + 1]; k += TPB) {
         const int i = s.an_node[k];
         const double w0 = s.an_w[3 * k], w1 = s.an_w[3 * k + 1], w2 = s.an_w[3 * k + 2];
-        const V2<double> gg = ld2(s.grad, i), dd = ld2(s.diag, i);
-        st2(s.grad, i, V2<double>{0.0, 0.0});   // each slaved node is gathered by one thread
+        const V2<double> gg = ld2(gcur, i), dd = ld2(s.diag, i);
+        st2(gcur, i, V2<double>{0.0, 0.0});   // each slaved node is gathered by one thread
         const double w[3] = {w0, w1, w2};
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
@@ -1385,7 +1391,7 @@ struct Step {
         const int i = s.rn_node[k];
         const V2<double> x = X(i);
         const V2<double> rr = {x.x - cx, x.y - cy};
-        const V2<double> gg = ld2(s.grad, i), gp = ld2(s.grad_prev, i);
+        const V2<double> gg = ld2(gcur, i), gp = ld2(gprev, i);
         const double m = s.mass[i];
         const V2<double> d = ld2(s.diag, i) - ld2(s.diag_el, i);
         const double dx = fmax(d.x, 0.0) + m, dy = fmax(d.y, 0.0) + m;
@@ -1443,16 +1449,16 @@ struct Step {
         [&](int i) {
           DirIn q;
           q.f = fr(i);
-          q.g = ld2(s.grad, i);
+          q.g = ld2(gcur, i);
           q.dh = ld2(s.diag, i);
           q.pp = ld2(s.p, i);
-          q.gp = ld2(s.grad_prev, i);
+          q.gp = ld2(gprev, i);
           return q;
         },
         [&](int i, const DirIn &q) {
           const double f = q.f;
           const V2<double> g = {q.g.x * f, q.g.y * f};
-          st2(s.grad, i, g);
+          st2(gcur, i, g);
           const double Px = f / fmax(q.dh.x, 1.0e-12), Py = f / fmax(q.dh.y, 1.0e-12);
           const V2<double> pp = q.pp;
           const V2<double> yy = g - q.gp;
@@ -1504,7 +1510,7 @@ struct Step {
           UpdIn q;
           q.f = fr(i);
           q.dh = ld2(s.diag, i);
-          q.g = ld2(s.grad, i);
+          q.g = ld2(gcur, i);
           q.pp = ld2(s.p, i);
           return q;
         },
@@ -1966,7 +1972,7 @@ struct Step {
     }
     for (int i = n0 + tid; i < n1; i += TPB) {
       st2(s.p, i, V2<double>{0.0, 0.0});
-      st2(s.grad, i, V2<double>{0.0, 0.0});
+      st2(gcur, i, V2<double>{0.0, 0.0});
     }
     __syncthreads();
     for (int it = 0; it < p.iter_max; ++it) {

@@ -74,6 +74,8 @@ typedef struct d2f_scene {
   int32_t B, N, T, n_b, n_e, n_bc;
   /* nodes [N] */
   double *x, *v, *xn, *xhat, *grad, *grad_prev, *p, *diag, *diag_el, *x0; /* 2N each */
+  /* grad and grad_prev are working arrays: d2f_step_scene swaps their roles every iteration,
+   * so after it either may hold the last gradient */
   double *mass;                       /* N */
   const uint8_t *free_;               /* N: 1 free, 0 kinematically driven */
   const int32_t *node2env;            /* N */
