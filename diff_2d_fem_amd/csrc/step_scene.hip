@@ -1428,6 +1428,12 @@ struct Step {
   // ------------------------------------------------------------------ Dai-Kou direction
   __device__ __forceinline__ void direction(int it) {
     double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    // With the element accumulators in LDS their planes are free until the next gradient pass:
+    // the dot loop parks each node's -P g and f p there, and the update of p reads them back
+    // instead of reloading the mask, diagonal, gradient and p from memory. (Lean variant only:
+    // in the full one the extra live values spill.)
+    const bool park = !FULL && s.nacc_lds && n1 - n0 <= s.max_nn;
+    const int ns = s.max_nn;
     struct DirIn {
       V2<double> g, dh, pp, gp;
       double f;
@@ -1468,6 +1474,13 @@ struct Step {
           v[2] += g.x * Py0 + g.y * Py1;
           v[3] += yy.x * pp.x + yy.y * pp.y;
           v[4] += yy.x * Py0 + yy.y * Py1;
+          if (park) {
+            LdsD *a = nacc + (i - n0);
+            a[0] = -Pg0;
+            a[ns] = -Pg1;
+            a[2 * ns] = pp.x * f;
+            a[3 * ns] = pp.y * f;
+          }
         });
     }
     blk.template sum<5>(v);
@@ -1505,7 +1518,13 @@ struct Step {
       V2<double> dh, g, pp;
       double f;
     };
-    node_batches<4>(
+    if (park) {
+      // p = (-P g + beta p) f, from the parked terms (f is 0 or 1: both terms carry it)
+      for (int i = n0 + tid; i < n1; i += TPB) {
+        const LdsD *a = nacc + (i - n0);
+        st2(s.p, i, V2<double>{a[0] + bb * a[2 * ns], a[ns] + bb * a[3 * ns]});
+      }
+    } else node_batches<4>(
         [&](int i) {
           UpdIn q;
           q.f = fr(i);
